@@ -1,0 +1,216 @@
+"""Benchmark of the element-matrix -> assembly -> solve path on one MI355X.
+
+Default workload: 3D linear-tet elasticity on a 64^3 unit cube (6 Kuhn tets per cell, 1,572,864 tets,
+274,625 nodes, 823,875 DOF, 36,168,777 nonzeros), lognormal random-field Young's modulus (mean 1e11,
+sigma 1.0, correlation length 0.1), nu = 0.3, z = 0 clamped, uniform traction on z = 1.  One step =
+fused assembly of K + Dirichlet elimination + preconditioner setup + PCG to true FP64 relative
+residual 1e-8, with the headline method (FP32 SpMV / block-Jacobi inside FP64 reliable updates).
+The plain FP64 Jacobi-PCG on the same GPU is timed beside it, and the CPU oracle (per-element Python
+loop + scipy COO->CSR + SuperLU with natural ordering, the reference's algorithm) on a bounded sample.
+
+Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cube64]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ('solves/s to FP64 rel-res 1e-8, learned vs plain PCG; assembly GB/s vs HBM peak')
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f'[bench] {msg}', file=sys.stderr, flush=True)
+
+
+def make_problem(name):
+    from finite_elements_amd import meshgen
+    if name == 'cube64':
+        return meshgen.cube_elasticity(n=64), '64^3 Kuhn-tet lognormal-field elasticity'
+    if name == 'cube32':
+        return meshgen.cube_elasticity(n=32), '32^3 Kuhn-tet lognormal-field elasticity'
+    if name == 'magnetic450':
+        return meshgen.magnetostatic_2d(n=450), '450^2 P1 magnetostatics, iron annulus + coils'
+    if name == 'cantilever':
+        return meshgen.cantilever_2d(), '40x10 plane-stress cantilever'
+    raise SystemExit(f'unknown workload {name}')
+
+
+def fused_assembly_bytes(sysm):
+    """Algorithmic bytes of one fused-assembly launch: every array it must touch, counted once."""
+    p = sysm.pattern
+    E, N, d, npe = sysm.n_elem, sysm.n_nodes, sysm.dof, sysm.npe
+    n_mat = {'tri_elastic': 3, 'tet_elastic': 2, 'tri_magnetic': 1}[sysm.kind]
+    coords = 8 * N * (3 if sysm.kind == 'tet_elastic' else 2)
+    conn = 4 * npe * E
+    mats = 8 * n_mat * E
+    graph = 4 * p.n_inc + 4 * (N + 1) * 2 + 4 * (p.nnzb + 1) + 2 * p.n_inc * npe + 4 * (len(p.blk))
+    vals = 8 * p.nnzb * d * d
+    compulsory = coords + conn + mats + vals
+    return coords + conn + mats + graph + vals, compulsory
+
+
+def pcg_spmv_bytes(sysm, value_bytes):
+    """Algorithmic bytes of one k_pcg_spmv launch: BSR values + block columns + row pointers, the
+    gathered z and p_old vectors once each, p_new and q written once."""
+    p = sysm.pattern
+    d = sysm.dof
+    n = sysm.n
+    return p.nnzb * (d * d * value_bytes + 4) + 4 * (sysm.n_nodes + 1) + 4 * n * value_bytes
+
+
+def time_assembly(sysm, reps=20):
+    import torch
+    from finite_elements_amd import _hip
+    out = sysm.stiffness(_hip.FE_LAYOUT_BSR)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        sysm.stiffness(_hip.FE_LAYOUT_BSR, out=out)
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / reps
+
+
+def one_solve(sysm, pr, method, fixed, g, f_t, profile=False):
+    """Assemble, eliminate, set up the preconditioner and solve; returns (u, info)."""
+    K = sysm.stiffness_bsr()
+    KD, b = sysm.prepare(K, fixed, g, f_t)
+    solver = sysm.solver(KD, precond=method['precond'], precision=method['precision'])
+    u, info = solver.solve(b, tol=1e-8, profile=profile)
+    return u, info
+
+
+def cpu_baseline(budget_s=30.0):
+    """The reference's algorithm (oracle restatement) on a bounded sample: 14^3 cube."""
+    from finite_elements_amd import meshgen
+    from oracle import reference as ref
+    import scipy.sparse.linalg
+    n = int(os.environ.get('FE_CPU_SAMPLE_N', '14'))
+    pr = meshgen.cube_elasticity(n=n, seed=4)
+    t0 = time.perf_counter()
+    prob = ref.problem_from_arrays(pr.kind, pr.coords, pr.conn, pr.materials, plane_strain=True,
+                                   plane_stress=False, fixed_dofs=pr.fixed_dofs, f=pr.f)
+    t1 = time.perf_counter()
+    K = ref.create_matrix(prob)
+    F = ref.create_source_matrix(prob)
+    t2 = time.perf_counter()
+    x = scipy.sparse.linalg.spsolve(K, F, permc_spec='NATURAL', use_umfpack=True)
+    t3 = time.perf_counter()
+    solve_s = t3 - t1
+    cores = len(os.sched_getaffinity(0))
+    return {'value': 1.0 / solve_s, 'unit': 'solves/s', 'cores': 1, 'kind': 'port',
+            'sample': (f'{n}^3 cube ({pr.n} DOF, {pr.conn.shape[0]} tets): oracle per-element Python '
+                       f'loop + scipy COO->CSR ({t2 - t1:.2f} s) + spsolve NATURAL on the Lagrange '
+                       f'system ({t3 - t2:.2f} s); single-threaded Python/SuperLU; {cores} host cores '
+                       'visible; the 64^3 case is infeasible for natural-order LU'),
+            'object_setup_s': t1 - t0, 'finite': bool(np.all(np.isfinite(x)))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--workload', default='cube64')
+    ap.add_argument('--plain-steps', type=int, default=2)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        raise SystemExit('bench.py measures one GPU: this build has no multi-GPU path (the sharding '
+                         'design is described in DESIGN.md); run with --gpus 1')
+    import torch
+    from finite_elements_amd.system import FESystem
+
+    log(f'building workload {args.workload}')
+    t0 = time.perf_counter()
+    pr, desc = make_problem(args.workload)
+    sysm = FESystem(pr.kind, pr.coords, pr.conn, pr.materials, plane_mode=pr.plane_mode)
+    fixed, g = sysm.dirichlet(pr.fixed_dofs, pr.fixed_values)
+    f_t = sysm.ctx.tensor(pr.f)
+    log(f'setup {time.perf_counter() - t0:.1f} s: {sysm.n} DOF, nnz {sysm.pattern.nnz}')
+
+    # ---- assembly roofline ----
+    asm_ms = time_assembly(sysm)
+    asm_bytes, asm_compulsory = fused_assembly_bytes(sysm)
+    asm = {'kernel': 'k_assemble_fused', 'ms': asm_ms, 'bytes': asm_bytes,
+           'gbps': asm_bytes / asm_ms / 1e6, 'frac': asm_bytes / asm_ms / 1e6 / HBM_PEAK_GBS,
+           'compulsory_bytes': asm_compulsory,
+           'compulsory_gbps': asm_compulsory / asm_ms / 1e6}
+    log(f'assembly {asm_ms:.3f} ms, {asm["gbps"]:.0f} GB/s algorithmic')
+
+    headline = {'precond': 'block_jacobi', 'precision': 'mixed'}
+    plain = {'precond': 'jacobi', 'precision': 'fp64'}
+
+    # ---- plain FP64 Jacobi-PCG (the in-GPU baseline) ----
+    one_solve(sysm, pr, plain, fixed, g, f_t)
+    torch.cuda.synchronize()
+    plain_times, plain_info = [], None
+    for _ in range(max(1, args.plain_steps)):
+        ts = time.perf_counter()
+        _, plain_info = one_solve(sysm, pr, plain, fixed, g, f_t)
+        torch.cuda.synchronize()
+        plain_times.append(time.perf_counter() - ts)
+    plain_s = float(np.median(plain_times))
+    log(f'plain FP64 Jacobi: {plain_s * 1e3:.1f} ms, {plain_info["iterations"]} it, relres {plain_info["relres"]:.2e}')
+
+    # ---- headline method: warmup then K timed steps ----
+    for _ in range(args.warmup):
+        one_solve(sysm, pr, headline, fixed, g, f_t)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    infos = []
+    for _ in range(args.steps):
+        _, info = one_solve(sysm, pr, headline, fixed, g, f_t, profile=True)
+        infos.append(info)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = elapsed / args.steps * 1e3
+    value = args.steps / elapsed
+    info = infos[-1]
+    log(f'headline: {ms_per_step:.1f} ms/solve, {info["iterations"]} it, relres {info["relres"]:.2e}')
+
+    spmv_ms = float(np.mean([i['spmv_ms'] for i in infos if i['spmv_samples'] > 0]))
+    spmv_bytes = pcg_spmv_bytes(sysm, 4)
+    roofline = {'kernel': 'k_pcg_spmv<float,3,16>' if sysm.dof == 3 else f'k_pcg_spmv<float,{sysm.dof},8>',
+                'bound': 'hbm', 'achieved': spmv_bytes / spmv_ms / 1e6, 'peak': HBM_PEAK_GBS,
+                'unit': 'GB/s', 'frac': spmv_bytes / spmv_ms / 1e6 / HBM_PEAK_GBS, 'traffic': None,
+                'bytes_per_launch': spmv_bytes, 'avg_ms': spmv_ms,
+                'samples': int(sum(i['spmv_samples'] for i in infos))}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log('CPU baseline (oracle restatement of the reference algorithm)')
+        cpu = cpu_baseline()
+        log(f'CPU baseline {cpu["value"]:.3f} solves/s on the sample')
+
+    out = {
+        'metric': METRIC, 'value': value, 'unit': 'solves/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': ms_per_step, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f32+f64', 'data': 'synthetic',
+        'config': {'workload': desc, 'dofs': int(sysm.n), 'nnz': int(sysm.pattern.nnz),
+                   'elements': int(sysm.n_elem), 'tol': 1e-8,
+                   'method': 'fused assembly + FP32 block-Jacobi PCG with FP64 reliable updates',
+                   'step': 'assembly + Dirichlet elimination + preconditioner setup + PCG'},
+        'headline': {'iterations': info['iterations'], 'relres_true_fp64': info['relres'],
+                     'reliable_updates': info['reliable_updates']},
+        'plain_fp64_jacobi': {'solves_per_s': 1.0 / plain_s, 'ms': plain_s * 1e3,
+                              'iterations': plain_info['iterations'], 'relres_true_fp64': plain_info['relres']},
+        'speedup_vs_plain': value * plain_s,
+        'assembly': asm,
+        'roofline': roofline,
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

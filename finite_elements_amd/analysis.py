@@ -1,0 +1,451 @@
+"""``FiniteElementAnalysis`` with the reference's API (finite_elements/analysis.py), GPU-backed.
+
+Host side (this module): lowering of loads and Dirichlet conditions with the reference's exact
+semantics -- per-(node, dimension) dictionaries where the last write wins and keys keep their first
+insertion position (analysis.py:28-110), node loads first, then element loads, then edge loads
+(457-489), element conditions scaled by ``element_to_node_factors`` and edge conditions by 0.5
+(201-239), condition rows appended after the DOF rows in dictionary order (241-279, 509-543).
+
+Device side (libfe_hip.so through ``FESystem``): element matrices, assembly, Dirichlet elimination
+and PCG.  Where the reference solves the Lagrange-augmented indefinite system with SuperLU
+(analysis.py:798-830), this class solves the equivalent eliminated SPD system and reconstructs the
+multipliers ``lambda_i = f_c - (K u)_c``, so ``solve()`` returns the same ``result_vector`` layout:
+n DOF values followed by one multiplier per condition.
+"""
+
+from __future__ import annotations
+
+from typing import List, Optional
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import conditions as fe_cond
+from . import core
+from .elements import (ElasticityElement, ElasticityTetrahedralElement3D,
+                       ElasticityTriangularElement2D, MagneticElement2D, _plane_mode)
+from .loads import NodeLoad
+from .mesh import ArrayMesh
+from .results import Result
+
+
+def node_boundary_conditions_to_dict(node_boundary_conditions):
+    """Last write wins per (node, dimension); first-insertion order (analysis.py:28-47)."""
+    out = {}
+    for bc in node_boundary_conditions:
+        out[(bc.application, bc.dimension)] = +bc.value
+    return out
+
+
+def node_boundary_from_dict(node_boundary_conditions_dict):
+    return [fe_cond.NodeBoundaryCondition(application=k[0], value=v, dimension=k[1])
+            for k, v in node_boundary_conditions_dict.items()]
+
+
+def node_loads_to_dict(node_loads):
+    """Last write wins per (node, dimension); first-insertion order (analysis.py:71-90)."""
+    out = {}
+    for load in node_loads:
+        out[(load.node, load.dimension)] = +load.value
+    return out
+
+
+def node_loads_from_dict(node_loads_dict):
+    return [NodeLoad(node=k[0], value=v, dimension=k[1]) for k, v in node_loads_dict.items()]
+
+
+def mesh_arrays(mesh):
+    """SoA view of a mesh: kind, coords (N, dim), conn (E, npe), per-element materials.
+
+    Elements are taken group by group in list order, the order of the reference's assembly loops
+    (analysis.py:326-338); node indices come from ``mesh.node_to_index``.
+    """
+    if isinstance(mesh, ArrayMesh):
+        mats = {k: mesh.material(k) for k in ('E', 'nu', 'thickness', 'rho', 'mu')
+                if mesh.materials.get(k) is not None}
+        return mesh.element_kind, mesh.coords, mesh.connectivity, mats
+    elements = [el for group in mesh.elements_groups for el in group.elements]
+    if not elements:
+        raise ValueError('mesh has no elements')
+    first = elements[0]
+    if isinstance(first, MagneticElement2D):
+        kind, cls = 'tri_magnetic', MagneticElement2D
+    elif isinstance(first, ElasticityTetrahedralElement3D):
+        kind, cls = 'tet_elastic', ElasticityTetrahedralElement3D
+    elif isinstance(first, ElasticityTriangularElement2D):
+        kind, cls = 'tri_elastic', ElasticityTriangularElement2D
+    else:
+        raise TypeError(f'unsupported element class {type(first).__name__}')
+    if not all(isinstance(el, cls) for el in elements):
+        raise TypeError('meshes mixing element classes are not supported')
+    idx = mesh.node_to_index
+    coords = np.array([tuple(node) for node in mesh.nodes], dtype=np.float64)
+    conn = np.array([[idx[p] for p in el.points] for el in elements], dtype=np.int32)
+    mats = {}
+    if kind == 'tri_magnetic':
+        mats['mu'] = np.array([el.mu_total for el in elements], dtype=np.float64)
+    else:
+        mats['E'] = np.array([el.elasticity_modulus for el in elements], dtype=np.float64)
+        mats['nu'] = np.array([el.poisson_ratio for el in elements], dtype=np.float64)
+        mats['rho'] = np.array([el.mass_density for el in elements], dtype=np.float64)
+        if kind == 'tri_elastic':
+            mats['thickness'] = np.array([el.thickness for el in elements], dtype=np.float64)
+    return kind, coords, conn, mats
+
+
+class FiniteElements:
+    """Problem definition and host-side lowering (finite_elements/analysis.py:113-577)."""
+
+    def __init__(self, mesh, element_loads: List, edge_loads: List, node_loads: List,
+                 magnet_loads: List, continuity_conditions: List, node_boundary_conditions: List,
+                 edge_boundary_conditions: List, element_boundary_conditions: List,
+                 plane_strain: bool = None, plane_stress: bool = None):
+        self.mesh = mesh
+        self.element_loads = element_loads
+        self.edge_loads = edge_loads
+        self.node_loads = node_loads
+        self.magnet_loads = magnet_loads
+        self.continuity_conditions = continuity_conditions
+        self.node_boundary_conditions = node_boundary_conditions
+        self.edge_boundary_conditions = edge_boundary_conditions
+        self.element_boundary_conditions = element_boundary_conditions
+        self.plane_strain = plane_strain
+        self.plane_stress = plane_stress
+        self._boundary_conditions = None
+        self._node_loads = None
+        self._positions = None
+        self._system = None
+        self._arrays = None
+        self.last_solve_info = None
+
+    # ---- geometry / DOF map ------------------------------------------------------------------
+    @property
+    def dimension(self):
+        if isinstance(self.mesh, ArrayMesh):
+            return {'tri_elastic': 2, 'tet_elastic': 3, 'tri_magnetic': 1}[self.mesh.element_kind]
+        return self.mesh.elements_groups[0].elements[0].dimension
+
+    @property
+    def elements_name(self):
+        return self.mesh.elements_groups[0].elements[0].__class__.__name__
+
+    def elements_permeability(self):
+        return [group.elements[0].mu_total for group in self.mesh.elements_groups]
+
+    @property
+    def positions(self):
+        if not self._positions:
+            self._positions = core.global_matrix_positions(dimension=self.dimension,
+                                                           nodes_number=len(self.mesh.nodes))
+        return self._positions
+
+    @property
+    def n_dofs(self):
+        return len(self.mesh.nodes) * self.dimension
+
+    def _dof(self, node, dimension):
+        return self.mesh.node_to_index[node] * self.dimension + (dimension - 1)
+
+    # ---- device system -------------------------------------------------------------------------
+    def _mesh_arrays(self):
+        if self._arrays is None:
+            self._arrays = mesh_arrays(self.mesh)
+        return self._arrays
+
+    def _plane_mode(self):
+        kind = self._mesh_arrays()[0]
+        if kind == 'tri_magnetic':
+            return 0
+        return _plane_mode(self.plane_strain, self.plane_stress)
+
+    def system(self):
+        """The device system of this mesh (built on first use)."""
+        if self._system is None:
+            from .system import FESystem
+            kind, coords, conn, mats = self._mesh_arrays()
+            self._system = FESystem(kind, coords, conn, mats, plane_mode=self._plane_mode())
+        return self._system
+
+    # ---- conditions ----------------------------------------------------------------------------
+    def c_matrix_continuity_conditions(self):
+        if self.continuity_conditions:
+            raise NotImplementedError(
+                'continuity conditions are not supported: the reference indexes their rows with '
+                'len(nodes)+len(node_loads)+i and node indices (analysis.py:190-197), which collide '
+                'with DOF rows')
+        return [], [], []
+
+    def boundary_conditions_element_to_node(self):
+        out = []
+        for cond in self.element_boundary_conditions:
+            factors = cond.application.element_to_node_factors()
+            for p_index, point in enumerate(cond.application.points):
+                out.append(fe_cond.NodeBoundaryCondition(point, cond.value * factors[p_index],
+                                                         cond.dimension))
+        return out
+
+    def boundary_conditions_edge_to_node(self):
+        out = []
+        for cond in self.edge_boundary_conditions:
+            for point in [cond.application.start, cond.application.end]:
+                out.append(fe_cond.NodeBoundaryCondition(point, cond.value * 0.5, cond.dimension))
+        return out
+
+    def _collapsed_boundary_conditions(self):
+        if not self._boundary_conditions:
+            bcs = self.node_boundary_conditions[:]
+            bcs.extend(self.boundary_conditions_element_to_node())
+            bcs.extend(self.boundary_conditions_edge_to_node())
+            self._boundary_conditions = node_boundary_from_dict(node_boundary_conditions_to_dict(bcs))
+        return self._boundary_conditions
+
+    def boundary_condition_dofs(self):
+        """(DOF positions, values) of the collapsed conditions, in condition-row order."""
+        bcs = self._collapsed_boundary_conditions()
+        pos = np.array([self._dof(bc.application, bc.dimension) for bc in bcs], dtype=np.int64)
+        val = np.array([bc.source_c_matrix() for bc in bcs], dtype=np.float64)
+        return pos, val
+
+    def c_matrix_boundary_conditions(self):
+        """(data, row, col) of the constraint entries (analysis.py:241-279)."""
+        pos, _ = self.boundary_condition_dofs()
+        n = self.n_dofs
+        data, row, col = [], [], []
+        for i, p in enumerate(pos.tolist()):
+            data.extend((1, 1))
+            row.extend((n + i, p))
+            col.extend((p, n + i))
+        return data, row, col
+
+    # ---- loads ---------------------------------------------------------------------------------
+    def loads_element_to_node(self):
+        out = []
+        for elements_load in self.element_loads:
+            per_element = elements_load.value_per_element
+            for j, element in enumerate(elements_load.elements):
+                factors = element.element_to_node_factors()
+                for p_index, point in enumerate(element.points):
+                    out.append(NodeLoad(point, per_element[j] * factors[p_index],
+                                        elements_load.dimension))
+        return out
+
+    def loads_edge_to_node(self):
+        out = []
+        for edge_load in self.edge_loads:
+            for point in [edge_load.edge.start, edge_load.edge.end]:
+                out.append(NodeLoad(point, edge_load.value * 0.5, edge_load.dimension))
+        return out
+
+    def source_c_matrix_loads(self):
+        node_loads = self.node_loads[:]
+        node_loads.extend(self.loads_element_to_node())
+        node_loads.extend(self.loads_edge_to_node())
+        node_loads = node_loads_from_dict(node_loads_to_dict(node_loads))
+        if not self._node_loads:
+            self._node_loads = node_loads
+        data = [load.source_c_matrix() for load in node_loads]
+        row = [self._dof(load.node, load.dimension) for load in node_loads]
+        return data, row
+
+    def source_c_matrix_boundary_conditions(self):
+        _, val = self.boundary_condition_dofs()
+        n = self.n_dofs
+        return val.tolist(), [n + i for i in range(len(val))]
+
+    def source_c_matrix_magnet_loads(self):
+        if self.magnet_loads:
+            raise NotImplementedError('magnet loads are not supported by finite_elements_amd')
+        return [], []
+
+    def load_vector(self):
+        """DOF part (length n) of the reference's source matrix."""
+        f = np.zeros(self.n_dofs, dtype=np.float64)
+        data, row = self.source_c_matrix_loads()
+        for d, r in zip(data, row):
+            f[r] += d
+        mdata, mrow = self.source_c_matrix_magnet_loads()
+        for d, r in zip(mdata, mrow):
+            f[r] += d
+        return f
+
+    # ---- element matrices ----------------------------------------------------------------------
+    def get_row_col_indices(self, element):
+        """Row/col DOF indices pairing with ``element.elementary_matrix()`` (analysis.py:714-735)."""
+        d = element.dimension
+        g = [self.positions[(self.mesh.node_to_index[p], i + 1)] for p in element.points
+             for i in range(d)]
+        m = len(g)
+        row = [v for v in g for _ in range(m)]
+        col = g * m
+        return row, col
+
+    def _matrix_data(self, what):
+        sysm = self.system()
+        if what == 'stiffness':
+            self._plane_mode()
+        ke = sysm.element_matrices(what, sym=False)
+        data = ke.t().contiguous().cpu().numpy().reshape(-1)
+        _, _, conn, _ = self._mesh_arrays()
+        d = self.dimension
+        g = (conn.astype(np.int64)[:, :, None] * d + np.arange(d)[None, None, :]).reshape(conn.shape[0], -1)
+        m = g.shape[1]
+        row = np.repeat(g, m, axis=1).reshape(-1)
+        col = np.tile(g, (1, m)).reshape(-1)
+        return data.tolist(), row.tolist(), col.tolist()
+
+    def k_matrix_data(self):
+        """(data, row, col) of all element stiffness matrices in element order (analysis.py:324-339)."""
+        return self._matrix_data('stiffness')
+
+    def m_matrix_data(self):
+        """(data, row, col) of all element mass matrices in element order (analysis.py:357-365)."""
+        return self._matrix_data('mass')
+
+    def _assembled_csr(self, what):
+        sysm = self.system()
+        if what == 'stiffness':
+            self._plane_mode()
+        from . import _hip
+        vals = (sysm.stiffness if what == 'stiffness' else sysm.mass)(_hip.FE_LAYOUT_CSR)
+        pat = sysm.pattern
+        n = self.n_dofs
+        return sp.csr_matrix((vals.cpu().numpy(), pat.colind.copy(), pat.rowptr.copy()), shape=(n, n))
+
+    def k_matrix(self, method_name):
+        if method_name == 'dense':
+            return self.k_matrix_dense()
+        if method_name == 'sparse':
+            return self.k_matrix_sparse()
+        raise NotImplementedError(
+            f'Class {self.__class__.__name__} does not implement {method_name} k matrix')
+
+    def m_matrix(self, method_name):
+        if method_name == 'dense':
+            return self.m_matrix_dense()
+        if method_name == 'sparse':
+            return self.m_matrix_sparse()
+        raise NotImplementedError(
+            f'Class {self.__class__.__name__} does not implement {method_name} m matrix')
+
+    def matrix_dense(self, method_name):
+        return self.matrix_sparse(method_name).toarray()
+
+    def matrix_sparse(self, method_name):
+        if method_name == 'k_matrix_data':
+            return self._assembled_csr('stiffness').tocsc()
+        if method_name == 'm_matrix_data':
+            return self._assembled_csr('mass').tocsc()
+        raise NotImplementedError(f'Class {self.__class__.__name__} does not implement {method_name}')
+
+    def k_matrix_dense(self):
+        return self.matrix_dense('k_matrix_data')
+
+    def k_matrix_sparse(self):
+        return self.matrix_sparse('k_matrix_data')
+
+    def m_matrix_dense(self):
+        return self.matrix_dense('m_matrix_data')
+
+    def m_matrix_sparse(self):
+        return self.matrix_sparse('m_matrix_data')
+
+
+class FiniteElementAnalysis(FiniteElements):
+    """Assembly and solve (finite_elements/analysis.py:580-830) on the MI355X path."""
+
+    def create_matrix(self):
+        """Augmented [[K, C^T], [C, 0]] as scipy CSR, same pattern as the reference's (analysis.py:661)."""
+        self.c_matrix_continuity_conditions()
+        K = self._assembled_csr('stiffness')
+        pos, _ = self.boundary_condition_dofs()
+        n = self.n_dofs
+        nbc = len(pos)
+        if nbc == 0:
+            rows_used = np.nonzero(np.diff(K.indptr))[0]
+            nr = int(rows_used[-1]) + 1 if rows_used.size else 0
+            nc = int(K.indices.max()) + 1 if K.indices.size else 0
+            return sp.csr_matrix((K.data, K.indices, K.indptr[:nr + 1]), shape=(nr, nc))
+        extra = np.zeros(n, dtype=np.int64)
+        extra[pos] = 1
+        row_len = np.diff(K.indptr).astype(np.int64) + extra
+        indptr = np.zeros(n + nbc + 1, dtype=np.int64)
+        indptr[1:n + 1] = np.cumsum(row_len)
+        indptr[n + 1:] = indptr[n] + np.arange(1, nbc + 1)
+        nnz = int(indptr[-1])
+        indices = np.empty(nnz, dtype=np.int32)
+        data = np.empty(nnz, dtype=np.float64)
+        # K entries keep their order; each constrained row gets its multiplier column appended last.
+        starts = indptr[:n]
+        k_len = np.diff(K.indptr)
+        k_rows = np.repeat(np.arange(n), k_len)
+        k_off = np.arange(K.indices.size) - np.repeat(K.indptr[:-1], k_len)
+        dest = starts[k_rows] + k_off
+        indices[dest] = K.indices
+        data[dest] = K.data
+        order = np.argsort(pos, kind='stable')
+        for i in order.tolist():
+            p = int(pos[i])
+            at = int(indptr[p + 1]) - 1
+            indices[at] = n + i
+            data[at] = 1.0
+        indices[indptr[n]:] = pos.astype(np.int32)
+        data[indptr[n]:] = 1.0
+        return sp.csr_matrix((data, indices, indptr), shape=(n + nbc, n + nbc))
+
+    def get_source_matrix_length(self):
+        return self.n_dofs + len(self.continuity_conditions) + len(self._collapsed_boundary_conditions())
+
+    def create_source_matrix(self):
+        """(n_total, 1) right-hand side (analysis.py:665-708)."""
+        out = np.zeros((self.get_source_matrix_length(), 1))
+        out[:self.n_dofs, 0] = self.load_vector()
+        data, row = self.source_c_matrix_boundary_conditions()
+        for d, r in zip(data, row):
+            out[r, 0] += d
+        return out
+
+    def solve(self, method: str = 'gpu', tol: float = 1e-8, precision: str = 'fp64',
+              precond: str = 'jacobi', max_iter: Optional[int] = None, learned=None):
+        """Solve and return ``Result(mesh, result_vector)`` (analysis.py:798-830).
+
+        ``result_vector`` = n DOF values followed by one Lagrange multiplier per collapsed condition,
+        in the reference's condition order.  method: 'gpu' (PCG on the MI355X).  precision: 'fp64' or
+        'mixed' (FP32 SpMV/preconditioner inside FP64 reliable updates).  precond: 'jacobi',
+        'block_jacobi' or 'learned' (needs ``learned``, a trained ``LearnedPreconditioner``).
+        Convergence is judged on the true FP64 residual of the eliminated system.
+        """
+        if method != 'gpu':
+            raise ValueError("finite_elements_amd solves on the GPU only (method='gpu')")
+        self.c_matrix_continuity_conditions()
+        sysm = self.system()
+        self._plane_mode()
+        pos, val = self.boundary_condition_dofs()
+        f = self.load_vector()
+        ctx = sysm.ctx
+        K = sysm.stiffness_bsr()
+        fixed, g = sysm.dirichlet(pos, val)
+        f_t = ctx.tensor(f)
+        KD, b = sysm.prepare(K, fixed, g, f_t)
+        M64 = None
+        pc = precond
+        if precond == 'learned':
+            if learned is None:
+                raise ValueError("precond='learned' needs a trained LearnedPreconditioner")
+            M64 = learned.preconditioner(sysm, KD, fixed)
+            pc = 'block_jacobi'
+        solver = sysm.solver(KD, pc, precision, M64=M64)
+        u, info = solver.solve(b, tol=tol, max_iter=max_iter)
+        from . import device
+        ku = device.spmv(K, u)
+        lam = f_t[pos] - ku[pos] if len(pos) else f_t[:0]
+        x = np.concatenate([u.cpu().numpy(), lam.cpu().numpy()])
+        self.last_solve_info = info
+        if info['status'] != 0:
+            import warnings
+            warnings.warn(f'PCG did not converge: {info}')
+        return Result(self.mesh, list(x))
+
+    def modal_analysis(self, order, k, **kwargs):
+        """Eigenpairs of K phi = lambda M phi (analysis.py:741-796) via the GPU LOBPCG."""
+        from .modal import modal_analysis
+        return modal_analysis(self, order, k, **kwargs)

@@ -1,0 +1,814 @@
+// Sparse matrix-vector products, preconditioner setup and the device-driven PCG.
+//
+// PCG layout: two kernels per iteration, both with a fixed grid so that every global reduction has a
+// fixed number of per-workgroup partials that the last workgroup to finish sums in a fixed order
+// (run-to-run bitwise reproducible):
+//   k_pcg_spmv   : p_new = z + beta * p_old (formed on the fly for every gathered column),
+//                  q = A p_new, partial p_new.q  -> alpha = rz / pq
+//   k_pcg_update : x += alpha p, r -= alpha q, z = M^-1 r, partial r.z and r.r -> beta, convergence
+// Every chunk of iterations ends with a conditional reliable update (true FP64 residual r = b - A x):
+// it runs when the recursive residual reached the tolerance (so convergence is always judged on the
+// true FP64 residual), when the iteration limit is hit, or (mixed precision) when the residual fell
+// by ru_delta since the last update.  The host only polls a status word between chunks.
+#include "fe_internal.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// --------------------------------------------------------------------------------------------------
+// reductions
+// --------------------------------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum NV values over the workgroup; result valid in thread 0.
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV]) {
+  __shared__ double red[NV][kBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) red[k][wid] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int w = 0; w < kBlock / 64; ++w) s += red[k][w];
+      v[k] = s;
+    }
+  __syncthreads();
+}
+
+// Write this workgroup's partials; return true in every thread of the last workgroup to finish,
+// which then holds the grid totals in v (thread 0).  The counter is reset for the next use.
+template <int NV>
+__device__ inline bool grid_sum(double (&v)[NV], double* partials, unsigned* counter) {
+  __shared__ bool am_last;
+  block_sum<NV>(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) partials[k * gridDim.x + blockIdx.x] = v[k];
+    __threadfence();
+    const unsigned prev = atomicAdd(counter, 1u);
+    am_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!am_last) return false;
+  __threadfence();
+  double t[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock)
+      s += __hip_atomic_load(&partials[k * gridDim.x + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    t[k] = s;
+  }
+  block_sum<NV>(t);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = t[k];
+    *counter = 0u;
+  }
+  return true;
+}
+
+// --------------------------------------------------------------------------------------------------
+// SpMV on node-block rows.  G lanes per block row; each lane walks the row's blocks with stride G.
+// --------------------------------------------------------------------------------------------------
+template <typename T, int D, int G>
+__device__ inline void bsr_row(const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
+                               const T* __restrict__ vals, const T* __restrict__ x, int64_t i,
+                               int lane, T (&acc)[D]) {
+#pragma unroll
+  for (int d = 0; d < D; ++d) acc[d] = T(0);
+  const int32_t s = bptr[i], e = bptr[i + 1];
+  for (int32_t q = s + lane; q < e; q += G) {
+    const int64_t j = bcol[q];
+    const T* v = vals + (int64_t)q * D * D;
+    T xj[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xj[d] = x[j * D + d];
+#pragma unroll
+    for (int di = 0; di < D; ++di)
+#pragma unroll
+      for (int dj = 0; dj < D; ++dj) acc[di] += v[di * D + dj] * xj[dj];
+  }
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+    for (int d = 0; d < D; ++d) acc[d] += __shfl_xor(acc[d], off, G);
+}
+
+template <typename T, int D, int G>
+__global__ __launch_bounds__(kBlock) void k_spmv_bsr(int64_t nb, const int32_t* __restrict__ bptr,
+                                                     const int32_t* __restrict__ bcol,
+                                                     const T* __restrict__ vals,
+                                                     const T* __restrict__ x, T* __restrict__ y) {
+  const int64_t gid = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  const int64_t i = gid / G;
+  const int lane = (int)(gid % G);
+  if (i >= nb) return;
+  T acc[D];
+  bsr_row<T, D, G>(bptr, bcol, vals, x, i, lane, acc);
+  if (lane < D) {
+    T out = acc[0];
+#pragma unroll
+    for (int d = 1; d < D; ++d)
+      if (lane == d) out = acc[d];
+    y[i * D + lane] = out;
+  }
+}
+
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_spmv_csr(int64_t n, const int32_t* __restrict__ rowptr,
+                                                     const int32_t* __restrict__ colind,
+                                                     const double* __restrict__ vals,
+                                                     const double* __restrict__ x,
+                                                     double* __restrict__ y) {
+  const int64_t gid = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  const int64_t i = gid / G;
+  const int lane = (int)(gid % G);
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int32_t k = rowptr[i] + lane; k < rowptr[i + 1]; k += G) acc += vals[k] * x[colind[k]];
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
+  if (lane == 0) y[i] = acc;
+}
+
+template <int D>
+constexpr int lanes_for() {
+  return D == 3 ? 16 : 8;
+}
+
+// --------------------------------------------------------------------------------------------------
+// preconditioner setup: inverse diagonal / inverse diagonal blocks (FP64)
+// --------------------------------------------------------------------------------------------------
+template <int D>
+__device__ inline const double* diag_block(const int32_t* bptr, const int32_t* bcol,
+                                           const double* vals, int64_t i) {
+  for (int32_t q = bptr[i]; q < bptr[i + 1]; ++q)
+    if (bcol[q] == i) return vals + (int64_t)q * D * D;
+  return nullptr;
+}
+
+template <int D>
+__global__ void k_precond_setup(int64_t nb, const int32_t* __restrict__ bptr,
+                                const int32_t* __restrict__ bcol, const double* __restrict__ vals,
+                                int kind, double* __restrict__ out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  const double* a = diag_block<D>(bptr, bcol, vals, i);
+  double blk[D * D];
+#pragma unroll
+  for (int k = 0; k < D * D; ++k) blk[k] = a ? a[k] : ((k % (D + 1)) == 0 ? 1.0 : 0.0);
+  if (kind == FE_PC_JACOBI) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double v = blk[d * D + d];
+      out[i * D + d] = (v != 0.0) ? 1.0 / v : 1.0;
+    }
+    return;
+  }
+  double inv[D * D];
+  if constexpr (D == 1) {
+    inv[0] = blk[0] != 0.0 ? 1.0 / blk[0] : 1.0;
+  } else if constexpr (D == 2) {
+    const double det = blk[0] * blk[3] - blk[1] * blk[2];
+    const double r = 1.0 / det;
+    inv[0] = blk[3] * r;
+    inv[1] = -blk[1] * r;
+    inv[2] = -blk[2] * r;
+    inv[3] = blk[0] * r;
+  } else {
+    const double* m = blk;
+    const double c00 = m[4] * m[8] - m[5] * m[7];
+    const double c01 = m[5] * m[6] - m[3] * m[8];
+    const double c02 = m[3] * m[7] - m[4] * m[6];
+    const double det = m[0] * c00 + m[1] * c01 + m[2] * c02;
+    const double r = 1.0 / det;
+    inv[0] = c00 * r;
+    inv[1] = (m[2] * m[7] - m[1] * m[8]) * r;
+    inv[2] = (m[1] * m[5] - m[2] * m[4]) * r;
+    inv[3] = c01 * r;
+    inv[4] = (m[0] * m[8] - m[2] * m[6]) * r;
+    inv[5] = (m[2] * m[3] - m[0] * m[5]) * r;
+    inv[6] = c02 * r;
+    inv[7] = (m[1] * m[6] - m[0] * m[7]) * r;
+    inv[8] = (m[0] * m[4] - m[1] * m[3]) * r;
+  }
+#pragma unroll
+  for (int k = 0; k < D * D; ++k) out[i * D * D + k] = inv[k];
+}
+
+// --------------------------------------------------------------------------------------------------
+// PCG state and kernels
+// --------------------------------------------------------------------------------------------------
+struct PcgState {
+  double rz, rz_prev, alpha, beta;
+  double rr, rr_ref, rr_true, bb;
+  double tol2bb, ru_delta2;
+  int32_t iter, max_iter;
+  int32_t done;     // 0 running, 1 converged, 2 iteration limit, 3 breakdown
+  int32_t need_ru;  // 1: reliable update / true-residual check requested
+  int32_t n_ru;
+  int32_t first;
+  int32_t mixed;
+  int32_t pad;
+  unsigned counter[4];
+};
+
+struct PcgStatus {
+  int32_t done, iter, n_ru, need_ru;
+  double rr, rr_true, bb, pad;
+};
+
+// Contiguous node range of logical workgroup v (XCD-aware ids).
+__device__ inline void block_range(int64_t nb, int64_t& lo, int64_t& hi) {
+  const int64_t v = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t per = ceil_div(nb, (int64_t)gridDim.x);
+  lo = min(nb, v * per);
+  hi = min(nb, lo + per);
+}
+
+template <typename T, int D, int G>
+__global__ __launch_bounds__(kBlock) void k_pcg_spmv(int64_t nb, const int32_t* __restrict__ bptr,
+                                                     const int32_t* __restrict__ bcol,
+                                                     const T* __restrict__ vals,
+                                                     const T* __restrict__ z, T* p0, T* p1,
+                                                     T* __restrict__ q, PcgState* st,
+                                                     double* partials) {
+  __shared__ int s_skip, s_sel;
+  __shared__ T s_beta;
+  if (threadIdx.x == 0) {
+    s_skip = st->done || st->need_ru;
+    s_beta = (T)st->beta;
+    s_sel = st->iter & 1;
+  }
+  __syncthreads();
+  if (s_skip) return;
+  const T beta = s_beta;
+  // The direction of the last completed iteration lives in p[iter & 1].
+  const T* __restrict__ p_old = s_sel ? p1 : p0;
+  T* __restrict__ p_new = s_sel ? p0 : p1;
+  int64_t lo, hi;
+  block_range(nb, lo, hi);
+  const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+  double dot = 0.0;
+  for (int64_t i = lo + grp; i < hi; i += kBlock / G) {
+    T acc[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) acc[d] = T(0);
+    const int32_t s = bptr[i], e = bptr[i + 1];
+    for (int32_t k = s + lane; k < e; k += G) {
+      const int64_t j = bcol[k];
+      const T* v = vals + (int64_t)k * D * D;
+      T pj[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) pj[d] = z[j * D + d] + beta * p_old[j * D + d];
+#pragma unroll
+      for (int di = 0; di < D; ++di)
+#pragma unroll
+        for (int dj = 0; dj < D; ++dj) acc[di] += v[di * D + dj] * pj[dj];
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+      for (int d = 0; d < D; ++d) acc[d] += __shfl_xor(acc[d], off, G);
+    if (lane < D) {
+      T qi = acc[0];
+#pragma unroll
+      for (int d = 1; d < D; ++d)
+        if (lane == d) qi = acc[d];
+      const int64_t r = i * D + lane;
+      const T pi = z[r] + beta * p_old[r];
+      p_new[r] = pi;
+      q[r] = qi;
+      dot += (double)pi * (double)qi;
+    }
+  }
+  double v[1] = {dot};
+  if (grid_sum<1>(v, partials, &st->counter[0]) && threadIdx.x == 0) {
+    const double pq = v[0];
+    if (!(pq > 0.0) || !isfinite(pq)) {
+      st->done = 3;
+      st->alpha = 0.0;
+    } else {
+      st->alpha = st->rz / pq;
+    }
+  }
+}
+
+// PC: 1 = point Jacobi (D values per node), 2 = block Jacobi (D*D values per node).
+template <typename T, int D, int PC>
+__device__ inline void apply_precond(const T* __restrict__ M, int64_t i, const T (&r)[D], T (&z)[D]) {
+  if (PC == FE_PC_JACOBI) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) z[d] = M[i * D + d] * r[d];
+  } else {
+    const T* m = M + i * D * D;
+#pragma unroll
+    for (int di = 0; di < D; ++di) {
+      T s = T(0);
+#pragma unroll
+      for (int dj = 0; dj < D; ++dj) s += m[di * D + dj] * r[dj];
+      z[di] = s;
+    }
+  }
+}
+
+template <typename T, int D, int PC>
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t nb, double* __restrict__ x,
+                                                       const T* p0, const T* p1,
+                                                       const T* __restrict__ q, T* __restrict__ r,
+                                                       T* __restrict__ z, const T* __restrict__ M,
+                                                       PcgState* st, double* partials) {
+  __shared__ int s_skip, s_sel;
+  __shared__ double s_alpha;
+  if (threadIdx.x == 0) {
+    s_skip = st->done || st->need_ru;
+    s_alpha = st->alpha;
+    s_sel = st->iter & 1;
+  }
+  __syncthreads();
+  if (s_skip) return;
+  // Same iteration as the preceding k_pcg_spmv: its p_new.
+  const T* __restrict__ p = s_sel ? p0 : p1;
+  const double alpha = s_alpha;
+  const T alpha_t = (T)alpha;
+  int64_t lo, hi;
+  block_range(nb, lo, hi);
+  double rz = 0.0, rr = 0.0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+    T ri[D], zi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int64_t k = i * D + d;
+      x[k] += alpha * (double)p[k];
+      ri[d] = r[k] - alpha_t * q[k];
+      r[k] = ri[d];
+    }
+    apply_precond<T, D, PC>(M, i, ri, zi);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      z[i * D + d] = zi[d];
+      rz += (double)ri[d] * (double)zi[d];
+      rr += (double)ri[d] * (double)ri[d];
+    }
+  }
+  double v[2] = {rz, rr};
+  if (grid_sum<2>(v, partials, &st->counter[1]) && threadIdx.x == 0) {
+    const double rz_new = v[0];
+    st->iter += 1;
+    st->rr = v[1];
+    if (!isfinite(rz_new) || !isfinite(v[1])) {
+      st->done = 3;
+      return;
+    }
+    st->rz_prev = st->rz;
+    st->beta = rz_new / st->rz;
+    st->rz = rz_new;
+    if (v[1] <= st->tol2bb || st->iter >= st->max_iter ||
+        (st->mixed && v[1] < st->ru_delta2 * st->rr_ref))
+      st->need_ru = 1;
+  }
+}
+
+// Reliable update, part 1: r64 = b - A64 x, partial ||r64||^2.
+template <int D, int G>
+__global__ __launch_bounds__(kBlock) void k_ru_residual(int64_t nb, const int32_t* __restrict__ bptr,
+                                                        const int32_t* __restrict__ bcol,
+                                                        const double* __restrict__ vals,
+                                                        const double* __restrict__ b,
+                                                        const double* __restrict__ x,
+                                                        double* __restrict__ r64, PcgState* st,
+                                                        double* partials) {
+  __shared__ int s_skip;
+  if (threadIdx.x == 0) s_skip = st->done || !st->need_ru;
+  __syncthreads();
+  if (s_skip) return;
+  int64_t lo, hi;
+  block_range(nb, lo, hi);
+  const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+  double rr = 0.0;
+  for (int64_t i = lo + grp; i < hi; i += kBlock / G) {
+    double acc[D];
+    bsr_row<double, D, G>(bptr, bcol, vals, x, i, lane, acc);
+    if (lane < D) {
+      double ai = acc[0];
+#pragma unroll
+      for (int d = 1; d < D; ++d)
+        if (lane == d) ai = acc[d];
+      const int64_t k = i * D + lane;
+      const double ri = b[k] - ai;
+      r64[k] = ri;
+      rr += ri * ri;
+    }
+  }
+  double v[1] = {rr};
+  if (grid_sum<1>(v, partials, &st->counter[2]) && threadIdx.x == 0) st->rr_true = v[0];
+}
+
+// Reliable update, part 2: convergence decision on the true residual; otherwise r = r64,
+// z = M^-1 r, rz = r.z, beta = rz / rz_prev (0 on the first update).
+template <typename T, int D, int PC>
+__global__ __launch_bounds__(kBlock) void k_ru_finish(int64_t nb, const double* __restrict__ r64,
+                                                      T* __restrict__ r, T* __restrict__ z,
+                                                      const T* __restrict__ M, PcgState* st,
+                                                      double* partials) {
+  __shared__ int s_skip;
+  if (threadIdx.x == 0) {
+    int skip = st->done || !st->need_ru;
+    if (!skip) {
+      if (st->rr_true <= st->tol2bb) {
+        skip = 1;
+      } else if (st->iter >= st->max_iter) {
+        skip = 1;
+      }
+    }
+    s_skip = skip;
+  }
+  __syncthreads();
+  if (s_skip) {
+    // Decision made by the last workgroup only, after every workgroup has read the state.
+    double v[1] = {0.0};
+    if (grid_sum<1>(v, partials, &st->counter[3]) && threadIdx.x == 0 && !st->done && st->need_ru) {
+      st->rr = st->rr_true;
+      st->n_ru += 1;
+      st->need_ru = 0;
+      st->done = (st->rr_true <= st->tol2bb) ? 1 : 2;
+    }
+    return;
+  }
+  int64_t lo, hi;
+  block_range(nb, lo, hi);
+  double rz = 0.0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+    T ri[D], zi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      ri[d] = (T)r64[i * D + d];
+      r[i * D + d] = ri[d];
+    }
+    apply_precond<T, D, PC>(M, i, ri, zi);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      z[i * D + d] = zi[d];
+      rz += (double)ri[d] * (double)zi[d];
+    }
+  }
+  double v[1] = {rz};
+  if (grid_sum<1>(v, partials, &st->counter[3]) && threadIdx.x == 0) {
+    const double rz_new = v[0];
+    if (!isfinite(rz_new) || rz_new < 0.0) {
+      st->done = 3;
+      return;
+    }
+    if (st->first) {
+      st->beta = 0.0;
+      st->first = 0;
+    } else {
+      st->beta = rz_new / st->rz_prev;
+    }
+    st->rz = rz_new;
+    st->rr = st->rr_true;
+    st->rr_ref = st->rr_true;
+    st->n_ru += 1;
+    st->need_ru = 0;
+  }
+}
+
+__global__ void k_pcg_init(int64_t n, const double* __restrict__ b, PcgState* st, double* partials,
+                           double tol, int max_iter, int mixed, double ru_delta) {
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    s += b[i] * b[i];
+  double v[1] = {s};
+  if (grid_sum<1>(v, partials, &st->counter[0]) && threadIdx.x == 0) {
+    st->bb = v[0];
+    st->tol2bb = tol * tol * v[0];
+    st->ru_delta2 = ru_delta * ru_delta;
+    st->max_iter = max_iter;
+    st->mixed = mixed;
+    st->need_ru = 1;
+    st->first = 1;
+    st->iter = 0;
+    st->done = 0;
+    st->n_ru = 0;
+    st->rz = 1.0;
+    st->rz_prev = 1.0;
+    st->beta = 0.0;
+  }
+}
+
+__global__ void k_pcg_status(const PcgState* st, PcgStatus* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    out->done = st->done;
+    out->iter = st->iter;
+    out->n_ru = st->n_ru;
+    out->need_ru = st->need_ru;
+    out->rr = st->rr;
+    out->rr_true = st->rr_true;
+    out->bb = st->bb;
+  }
+}
+
+// --------------------------------------------------------------------------------------------------
+// host side
+// --------------------------------------------------------------------------------------------------
+struct Work {
+  PcgState* st;
+  PcgStatus* status;
+  double* partials;
+  void* r;
+  void* z;
+  void* p0;
+  void* p1;
+  void* q;
+  double* r64;
+  int64_t bytes;
+};
+
+constexpr int64_t kAlign = 256;
+inline int64_t align_up(int64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+int pcg_grid(const fe_handle* h, int64_t nb) {
+  int64_t g = (int64_t)h->num_cus * 4;
+  if (g < 8) g = 8;
+  g = (g + 7) / 8 * 8;
+  const int64_t need = ceil_div(nb, 16);
+  while (g > 8 && g / 2 >= need) g /= 2;
+  return (int)g;
+}
+
+Work layout_work(char* base, int64_t n, int elem, int grid) {
+  Work w{};
+  int64_t off = 0;
+  auto take = [&](int64_t bytes) {
+    char* p = base ? base + off : nullptr;
+    off += align_up(bytes);
+    return p;
+  };
+  w.st = (PcgState*)take(sizeof(PcgState));
+  w.status = (PcgStatus*)take(sizeof(PcgStatus));
+  w.partials = (double*)take((int64_t)4 * grid * sizeof(double));
+  w.r = take(n * elem);
+  w.z = take(n * elem);
+  w.p0 = take(n * elem);
+  w.p1 = take(n * elem);
+  w.q = take(n * elem);
+  w.r64 = (double*)take(n * (int64_t)sizeof(double));
+  w.bytes = off;
+  return w;
+}
+
+template <typename T, int D, int PC>
+int run_pcg(fe_handle* h, const fe_bsr* A64, const fe_bsr* Ait, const T* M, const double* b,
+            double* x, const Work& w, int grid, const fe_pcg_opts* o, fe_pcg_stats* s) {
+  constexpr int G = lanes_for<D>();
+  const int64_t nb = A64->n_brows, n = nb * D;
+  hipStream_t st = h->stream;
+  FE_CHECK_HIP(hipMemsetAsync(w.st, 0, sizeof(PcgState), st));
+  FE_CHECK_HIP(hipMemsetAsync(w.p0, 0, n * sizeof(T), st));
+  FE_CHECK_HIP(hipMemsetAsync(w.p1, 0, n * sizeof(T), st));
+  hipLaunchKernelGGL(k_pcg_init, dim3(grid), dim3(kBlock), 0, st, n, b, w.st, w.partials, o->tol,
+                     o->max_iter, o->mixed, o->ru_delta);
+  FE_CHECK_LAUNCH();
+  const int32_t* bptr = A64->bptr;
+  const int32_t* bcol = A64->bcol;
+  const double* v64 = (const double*)A64->vals;
+  const T* vit = (const T*)Ait->vals;
+  T* r = (T*)w.r;
+  T* z = (T*)w.z;
+  T* q = (T*)w.q;
+  T* pb[2] = {(T*)w.p0, (T*)w.p1};
+  auto launch_ru = [&]() -> int {
+    hipLaunchKernelGGL((k_ru_residual<D, G>), dim3(grid), dim3(kBlock), 0, st, nb, bptr, bcol, v64, b,
+                       x, w.r64, w.st, w.partials);
+    hipLaunchKernelGGL((k_ru_finish<T, D, PC>), dim3(grid), dim3(kBlock), 0, st, nb, w.r64, r, z, M,
+                       w.st, w.partials);
+    FE_CHECK_LAUNCH();
+    return FE_OK;
+  };
+  int rc = launch_ru();
+  if (rc) return rc;
+  const int chunk = o->check_every > 0 ? o->check_every : 16;
+  PcgStatus* pinned = (PcgStatus*)h->pinned;
+  hipEvent_t ev[2];
+  hipEvent_t pev[2][2];  // profile: [slot][start/stop] around the chunk's first SpMV kernel
+  FE_CHECK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  FE_CHECK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  const bool prof = o->profile != 0;
+  if (prof)
+    for (int a = 0; a < 2; ++a)
+      for (int b2 = 0; b2 < 2; ++b2) FE_CHECK_HIP(hipEventCreate(&pev[a][b2]));
+  auto enqueue_chunk = [&](int slot) -> int {
+    for (int it = 0; it < chunk; ++it) {
+      if (prof && it == 0) FE_CHECK_HIP(hipEventRecord(pev[slot][0], st));
+      hipLaunchKernelGGL((k_pcg_spmv<T, D, G>), dim3(grid), dim3(kBlock), 0, st, nb, bptr, bcol, vit,
+                         z, pb[0], pb[1], q, w.st, w.partials);
+      if (prof && it == 0) FE_CHECK_HIP(hipEventRecord(pev[slot][1], st));
+      hipLaunchKernelGGL((k_pcg_update<T, D, PC>), dim3(grid), dim3(kBlock), 0, st, nb, x, pb[0], pb[1],
+                         q, r, z, M, w.st, w.partials);
+    }
+    int rc2 = launch_ru();
+    if (rc2) return rc2;
+    hipLaunchKernelGGL(k_pcg_status, dim3(1), dim3(64), 0, st, w.st, w.status);
+    FE_CHECK_HIP(hipMemcpyAsync(&pinned[slot], w.status, sizeof(PcgStatus), hipMemcpyDeviceToHost, st));
+    FE_CHECK_HIP(hipEventRecord(ev[slot], st));
+    return FE_OK;
+  };
+  // One chunk is always queued ahead of the one being polled, so the device never waits for the
+  // host; chunks queued after convergence find the done flag set and return immediately.
+  const int64_t max_chunks = 4 * ((int64_t)o->max_iter / chunk + 2) + 8;
+  PcgStatus last{};
+  int finished = 0;
+  double prof_ms = 0.0;
+  int prof_n = 0;
+  int prev_running = 1;  // the chunk before the polled one left the solver running (no pause)
+  rc = enqueue_chunk(0);
+  if (rc == FE_OK) rc = enqueue_chunk(1);
+  for (int64_t c = 0; rc == FE_OK && c < max_chunks; ++c) {
+    const int slot = (int)(c & 1);
+    FE_CHECK_HIP(hipEventSynchronize(ev[slot]));
+    last = pinned[slot];
+    if (prof && prev_running) {
+      // the sampled SpMV ran in full only if the solver was running when this chunk started
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, pev[slot][0], pev[slot][1]) == hipSuccess) {
+        prof_ms += ms;
+        prof_n += 1;
+      }
+    }
+    prev_running = !last.done && !last.need_ru;
+    if (last.done) {
+      finished = 1;
+      break;
+    }
+    rc = enqueue_chunk(slot);
+  }
+  if (prof)
+    for (int a = 0; a < 2; ++a)
+      for (int b2 = 0; b2 < 2; ++b2) (void)hipEventDestroy(pev[a][b2]);
+  s->spmv_ms = prof_n ? prof_ms / prof_n : 0.0;
+  s->spmv_samples = prof_n;
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  if (rc) return rc;
+  FE_CHECK_HIP(hipStreamSynchronize(st));
+  if (!finished) {
+    hipLaunchKernelGGL(k_pcg_status, dim3(1), dim3(64), 0, st, w.st, w.status);
+    FE_CHECK_HIP(hipMemcpyAsync(&pinned[0], w.status, sizeof(PcgStatus), hipMemcpyDeviceToHost, st));
+    FE_CHECK_HIP(hipStreamSynchronize(st));
+    last = pinned[0];
+  }
+  s->iterations = last.iter;
+  s->reliable_updates = last.n_ru;
+  s->bnorm = sqrt(last.bb);
+  s->relres = last.bb > 0 ? sqrt(last.rr_true / last.bb) : 0.0;
+  s->relres_rec = last.bb > 0 ? sqrt(last.rr / last.bb) : 0.0;
+  if (last.done == 1)
+    s->status = FE_OK;
+  else if (last.done == 3)
+    s->status = FE_ERR_BREAKDOWN;
+  else
+    s->status = FE_ERR_NOT_CONVERGED;
+  return FE_OK;
+}
+
+template <typename T, int D>
+int dispatch_pc(fe_handle* h, const fe_bsr* A64, const fe_bsr* Ait, const void* M, const double* b,
+                double* x, const Work& w, int grid, const fe_pcg_opts* o, fe_pcg_stats* s) {
+  if (o->precond == FE_PC_JACOBI)
+    return run_pcg<T, D, FE_PC_JACOBI>(h, A64, Ait, (const T*)M, b, x, w, grid, o, s);
+  return run_pcg<T, D, FE_PC_BLOCK_JACOBI>(h, A64, Ait, (const T*)M, b, x, w, grid, o, s);
+}
+
+template <typename T>
+int dispatch_dof(fe_handle* h, const fe_bsr* A64, const fe_bsr* Ait, const void* M, const double* b,
+                 double* x, const Work& w, int grid, const fe_pcg_opts* o, fe_pcg_stats* s) {
+  switch (A64->dof) {
+    case 1: return dispatch_pc<T, 1>(h, A64, Ait, M, b, x, w, grid, o, s);
+    case 2: return dispatch_pc<T, 2>(h, A64, Ait, M, b, x, w, grid, o, s);
+    case 3: return dispatch_pc<T, 3>(h, A64, Ait, M, b, x, w, grid, o, s);
+    default: return fe_set_error(FE_ERR_ARG, "fe_pcg: dof must be 1..3");
+  }
+}
+
+template <typename T, int D>
+void launch_spmv(fe_handle* h, const fe_bsr* A, const void* x, void* y) {
+  constexpr int G = lanes_for<D>();
+  const int64_t threads = A->n_brows * G;
+  hipLaunchKernelGGL((k_spmv_bsr<T, D, G>), dim3((unsigned)ceil_div(threads, kBlock)), dim3(kBlock), 0,
+                     h->stream, A->n_brows, A->bptr, A->bcol, (const T*)A->vals, (const T*)x, (T*)y);
+}
+
+}  // namespace
+
+extern "C" int fe_spmv(fe_handle* h, const fe_bsr* A, const void* x, void* y) {
+  FE_ENTER(h);
+  FE_REQUIRE(A && x && y, "fe_spmv: null argument");
+  FE_REQUIRE(A->dof >= 1 && A->dof <= 3, "fe_spmv: dof must be 1..3");
+  FE_REQUIRE(A->precision == 0 || A->precision == 1, "fe_spmv: precision must be 0 or 1");
+  if (A->n_brows == 0) return FE_OK;
+  FE_REQUIRE(A->bptr && A->bcol && A->vals, "fe_spmv: null matrix array");
+  FE_REQUIRE(x != y, "fe_spmv: x and y must not alias");
+  if (A->precision == 0) {
+    if (A->dof == 1) launch_spmv<double, 1>(h, A, x, y);
+    else if (A->dof == 2) launch_spmv<double, 2>(h, A, x, y);
+    else launch_spmv<double, 3>(h, A, x, y);
+  } else {
+    if (A->dof == 1) launch_spmv<float, 1>(h, A, x, y);
+    else if (A->dof == 2) launch_spmv<float, 2>(h, A, x, y);
+    else launch_spmv<float, 3>(h, A, x, y);
+  }
+  FE_CHECK_LAUNCH();
+  return FE_OK;
+}
+
+extern "C" int fe_spmv_csr(fe_handle* h, int64_t n_rows, const int32_t* rowptr,
+                           const int32_t* colind, const double* vals, const double* x, double* y) {
+  FE_ENTER(h);
+  FE_REQUIRE(n_rows >= 0, "fe_spmv_csr: negative n_rows");
+  if (n_rows == 0) return FE_OK;
+  FE_REQUIRE(rowptr && colind && vals && x && y && x != y, "fe_spmv_csr: bad pointers");
+  constexpr int G = 8;
+  hipLaunchKernelGGL(k_spmv_csr<G>, dim3((unsigned)ceil_div(n_rows * G, kBlock)), dim3(kBlock), 0,
+                     h->stream, n_rows, rowptr, colind, vals, x, y);
+  FE_CHECK_LAUNCH();
+  return FE_OK;
+}
+
+extern "C" int fe_precond_setup(fe_handle* h, const fe_bsr* A, int kind, double* out) {
+  FE_ENTER(h);
+  FE_REQUIRE(A && out, "fe_precond_setup: null argument");
+  FE_REQUIRE(A->precision == 0, "fe_precond_setup: FP64 matrix expected");
+  FE_REQUIRE(kind == FE_PC_JACOBI || kind == FE_PC_BLOCK_JACOBI, "fe_precond_setup: bad kind");
+  if (A->n_brows == 0) return FE_OK;
+  dim3 grid((unsigned)ceil_div(A->n_brows, 256)), block(256);
+  const double* v = (const double*)A->vals;
+  switch (A->dof) {
+    case 1:
+      hipLaunchKernelGGL(k_precond_setup<1>, grid, block, 0, h->stream, A->n_brows, A->bptr, A->bcol, v,
+                         kind, out);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_precond_setup<2>, grid, block, 0, h->stream, A->n_brows, A->bptr, A->bcol, v,
+                         kind, out);
+      break;
+    case 3:
+      hipLaunchKernelGGL(k_precond_setup<3>, grid, block, 0, h->stream, A->n_brows, A->bptr, A->bcol, v,
+                         kind, out);
+      break;
+    default:
+      return fe_set_error(FE_ERR_ARG, "fe_precond_setup: dof must be 1..3");
+  }
+  FE_CHECK_LAUNCH();
+  return FE_OK;
+}
+
+extern "C" int fe_pcg_workspace(fe_handle* h, const fe_bsr* A, const fe_pcg_opts* opts,
+                                int64_t* bytes) {
+  FE_ENTER(h);
+  FE_REQUIRE(A && opts && bytes, "fe_pcg_workspace: null argument");
+  const int grid = pcg_grid(h, A->n_brows);
+  const int elem = opts->mixed ? (int)sizeof(float) : (int)sizeof(double);
+  *bytes = layout_work(nullptr, A->n_brows * A->dof, elem, grid).bytes;
+  return FE_OK;
+}
+
+extern "C" int fe_pcg(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M64,
+                      const void* M32, const double* b, double* x, void* work, int64_t work_bytes,
+                      const fe_pcg_opts* o, fe_pcg_stats* s) {
+  FE_ENTER(h);
+  FE_REQUIRE(A64 && b && x && work && o && s, "fe_pcg: null argument");
+  FE_REQUIRE(A64->precision == 0, "fe_pcg: A64 must be FP64");
+  FE_REQUIRE(A64->dof >= 1 && A64->dof <= 3, "fe_pcg: dof must be 1..3");
+  FE_REQUIRE(o->precond == FE_PC_JACOBI || o->precond == FE_PC_BLOCK_JACOBI, "fe_pcg: bad precond");
+  FE_REQUIRE(o->tol > 0 && o->max_iter > 0, "fe_pcg: tol and max_iter must be positive");
+  if (o->mixed) {
+    FE_REQUIRE(A32 && M32, "fe_pcg: mixed precision needs the FP32 matrix and preconditioner");
+    FE_REQUIRE(A32->precision == 1 && A32->dof == A64->dof && A32->n_brows == A64->n_brows,
+               "fe_pcg: A32 must be the FP32 copy of A64");
+  } else {
+    FE_REQUIRE(M64, "fe_pcg: missing preconditioner");
+  }
+  *s = fe_pcg_stats{};
+  if (A64->n_brows == 0) return FE_OK;
+  const int grid = pcg_grid(h, A64->n_brows);
+  const int elem = o->mixed ? (int)sizeof(float) : (int)sizeof(double);
+  const int64_t n = A64->n_brows * A64->dof;
+  Work w = layout_work((char*)work, n, elem, grid);
+  FE_REQUIRE(work_bytes >= w.bytes, "fe_pcg: workspace too small (%lld < %lld)",
+             (long long)work_bytes, (long long)w.bytes);
+  if (o->mixed) return dispatch_dof<float>(h, A64, A32, M32, b, x, w, grid, o, s);
+  return dispatch_dof<double>(h, A64, A64, M64, b, x, w, grid, o, s);
+}

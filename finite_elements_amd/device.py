@@ -1,0 +1,436 @@
+"""Device plumbing: torch tensors on the ROCm device handed to libfe_hip.so through its C ABI.
+
+PyTorch only owns device memory and the stream here; every computation on the path (element matrices,
+assembly, Dirichlet elimination, SpMV, preconditioner setup, PCG) runs in the hand-written HIP kernels
+of libfe_hip.so.  There is no CPU fallback: without a ROCm device or without the built library these
+functions raise ``RuntimeError``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _hip
+
+KIND_INFO = {
+    # kind -> (C kind id, nodes per element, dof per node)
+    'tri_elastic': (_hip.FE_KIND_TRI_ELASTIC, 3, 2),
+    'tet_elastic': (_hip.FE_KIND_TET_ELASTIC, 4, 3),
+    'tri_magnetic': (_hip.FE_KIND_TRI_MAGNETIC, 3, 1),
+}
+
+
+def _vp(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError('finite_elements_amd: no ROCm GPU is visible; the HIP path has no CPU '
+                           'fallback')
+
+
+class Context:
+    """A libfe_hip handle bound to torch's current stream on one device."""
+
+    _by_device = {}
+
+    def __init__(self, device: int = 0):
+        require_gpu()
+        _hip.load()
+        self.device = device
+        self.torch_device = torch.device('cuda', device)
+        with torch.cuda.device(device):
+            self.stream = torch.cuda.current_stream(device)
+        h = ctypes.c_void_p()
+        _hip.call('fe_create', ctypes.byref(h), device, ctypes.c_void_p(self.stream.cuda_stream))
+        self.h = h
+
+    @classmethod
+    def get(cls, device: int = 0) -> 'Context':
+        ctx = cls._by_device.get(device)
+        if ctx is None:
+            ctx = cls(device)
+            cls._by_device[device] = ctx
+        return ctx
+
+    def synchronize(self):
+        _hip.call('fe_synchronize', self.h)
+
+    def tensor(self, arr, dtype=None):
+        t = torch.as_tensor(np.ascontiguousarray(arr))
+        if dtype is not None:
+            t = t.to(dtype)
+        return t.to(self.torch_device)
+
+    def empty(self, n, dtype=torch.float64):
+        return torch.empty(int(n), dtype=dtype, device=self.torch_device)
+
+    def zeros(self, n, dtype=torch.float64):
+        return torch.zeros(int(n), dtype=dtype, device=self.torch_device)
+
+
+# ------------------------------------------------------------------------------------------------
+# pattern (host C++ builder)
+# ------------------------------------------------------------------------------------------------
+class Pattern:
+    """Node graph, scalar CSR pattern and contribution maps of a mesh (built by fe_pattern_create).
+
+    ``conn`` is (E, npe) on the host.  ``with_gather`` also builds the per-entry contribution map of
+    the two-phase assembly (m*m int32 per element).
+    """
+
+    def __init__(self, conn: np.ndarray, dof: int, n_nodes: int, sym: bool = True,
+                 with_gather: bool = False, max_inc: Optional[int] = None):
+        _hip.load()
+        conn = np.ascontiguousarray(np.asarray(conn, dtype=np.int32))
+        self.n_elem, self.npe = conn.shape
+        self.dof = dof
+        self.n_nodes = int(n_nodes)
+        self.sym = bool(sym)
+        soa = np.ascontiguousarray(conn.T)
+        p = ctypes.c_void_p()
+        _hip.call('fe_pattern_create', soa.ctypes.data_as(ctypes.c_void_p), self.npe, dof,
+                  self.n_elem, self.n_nodes, int(sym), int(with_gather), ctypes.byref(p))
+        self._p = p
+        try:
+            if max_inc is not None:
+                _hip.call('fe_pattern_schedule', p, int(max_inc))
+            self._export(with_gather)
+        finally:
+            _hip.call('fe_pattern_destroy', p)
+            self._p = None
+
+    def _info(self):
+        info = (ctypes.c_int64 * _hip.FE_PAT_NINFO)()
+        _hip.call('fe_pattern_info', self._p, info)
+        return list(info)
+
+    def _copy(self, which, n, dtype):
+        out = np.empty(int(n), dtype=dtype)
+        if n:
+            _hip.call('fe_pattern_copy', self._p, which, out.ctypes.data_as(ctypes.c_void_p))
+        return out
+
+    def _export(self, with_gather):
+        info = self._info()
+        self.nnz = info[_hip.FE_PAT_NNZ]
+        self.nnzb = info[_hip.FE_PAT_NNZB]
+        self.n_inc = info[_hip.FE_PAT_NINC]
+        self.n_rows = info[_hip.FE_PAT_NROWS]
+        self.rowptr = self._copy(_hip.FE_ARR_ROWPTR, self.n_rows + 1, np.int32)
+        self.colind = self._copy(_hip.FE_ARR_COLIND, self.nnz, np.int32)
+        self.bptr = self._copy(_hip.FE_ARR_BPTR, self.n_nodes + 1, np.int32)
+        self.bcol = self._copy(_hip.FE_ARR_BCOL, self.nnzb, np.int32)
+        self.inc_ptr = self._copy(_hip.FE_ARR_INCPTR, self.n_nodes + 1, np.int32)
+        self.inc = self._copy(_hip.FE_ARR_INC, self.n_inc, np.int32)
+        self.bseg_ptr = self._copy(_hip.FE_ARR_BSEGPTR, self.nnzb + 1, np.int32)
+        self.bseg = self._copy(_hip.FE_ARR_BSEG, self.n_inc * self.npe, np.uint16)
+        self._set_schedule(info)
+        if with_gather:
+            ncontrib = info[_hip.FE_PAT_NCONTRIB]
+            self.gather = self._copy(_hip.FE_ARR_GATHER, ncontrib, np.int32)
+            self.seg_ptr = self._copy(_hip.FE_ARR_SEGPTR, self.nnz + 1, np.int32)
+        else:
+            self.gather = None
+            self.seg_ptr = None
+
+    def _set_schedule(self, info):
+        self.n_blk = info[_hip.FE_PAT_NBLK]
+        self.max_inc = info[_hip.FE_PAT_MAXINC]
+        self.blk = self._copy(_hip.FE_ARR_BLK, self.n_blk + 1, np.int32)
+
+    @staticmethod
+    def schedule_for(inc_ptr: np.ndarray, max_inc: int, max_nodes: int = 256):
+        """Row-block schedule (same greedy rule as fe_pattern_schedule) for a different max_inc."""
+        counts = np.diff(inc_ptr)
+        if counts.size and counts.max() > max_inc:
+            raise ValueError(f'a node has {counts.max()} incidences > max_inc {max_inc}')
+        blk = [0]
+        cur = nodes = 0
+        biggest = 0
+        for i, c in enumerate(counts.tolist()):
+            if nodes and (cur + c > max_inc or nodes >= max_nodes):
+                blk.append(i)
+                biggest = max(biggest, cur)
+                cur = nodes = 0
+            cur += c
+            nodes += 1
+        if nodes:
+            blk.append(len(counts))
+            biggest = max(biggest, cur)
+        return np.asarray(blk, dtype=np.int32), biggest
+
+
+# ------------------------------------------------------------------------------------------------
+# device mesh / graph / matrices
+# ------------------------------------------------------------------------------------------------
+class DeviceMesh:
+    """SoA mesh on the device: coordinates, connectivity [npe][E] and per-element materials."""
+
+    def __init__(self, ctx: Context, kind: str, coords: np.ndarray, conn: np.ndarray,
+                 materials: dict, plane_mode: int = 0):
+        self.ctx = ctx
+        self.kind = kind
+        self.kind_id, self.npe, self.dof = KIND_INFO[kind]
+        coords = np.asarray(coords, dtype=np.float64)
+        conn = np.asarray(conn, dtype=np.int32)
+        self.n_nodes = coords.shape[0]
+        self.n_elem = conn.shape[0]
+        self.plane_mode = plane_mode
+        self.x = ctx.tensor(coords[:, 0])
+        self.y = ctx.tensor(coords[:, 1])
+        self.z = ctx.tensor(coords[:, 2]) if coords.shape[1] > 2 else None
+        self.conn = ctx.tensor(np.ascontiguousarray(conn.T))
+
+        def mat(key):
+            v = materials.get(key)
+            if v is None:
+                return None
+            v = np.asarray(v, dtype=np.float64)
+            if v.ndim == 0:
+                v = np.full(self.n_elem, float(v))
+            return ctx.tensor(v)
+
+        if kind == 'tri_elastic':
+            self.mat0, self.mat1, self.mat2 = mat('E'), mat('nu'), mat('thickness')
+        elif kind == 'tet_elastic':
+            self.mat0, self.mat1, self.mat2 = mat('E'), mat('nu'), None
+        else:
+            self.mat0, self.mat1, self.mat2 = mat('mu'), None, None
+        self.rho = mat('rho')
+
+    def struct(self) -> _hip.FeMesh:
+        return _hip.FeMesh(self.kind_id, self.plane_mode, self.n_elem, self.n_nodes, _vp(self.x),
+                           _vp(self.y), _vp(self.z), _vp(self.conn), _vp(self.mat0), _vp(self.mat1),
+                           _vp(self.mat2), _vp(self.rho))
+
+
+class DeviceGraph:
+    """Pattern arrays on the device (for the fused assembly, SpMV and the solver)."""
+
+    def __init__(self, ctx: Context, pat: Pattern):
+        self.ctx = ctx
+        self.pattern = pat
+        self.n_nodes = pat.n_nodes
+        self.nnzb = pat.nnzb
+        self.dof = pat.dof
+        self.bptr = ctx.tensor(pat.bptr)
+        self.bcol = ctx.tensor(pat.bcol)
+        self.inc_ptr = ctx.tensor(pat.inc_ptr)
+        self.inc = ctx.tensor(pat.inc)
+        self.bseg_ptr = ctx.tensor(pat.bseg_ptr)
+        self.bseg = ctx.tensor(pat.bseg.view(np.int16))
+        self.set_schedule(pat.blk, pat.max_inc)
+
+    def set_schedule(self, blk: np.ndarray, max_inc: int):
+        self.blk = self.ctx.tensor(np.asarray(blk, dtype=np.int32))
+        self.n_blk = len(blk) - 1
+        self.max_inc = int(max_inc)
+
+    def struct(self) -> _hip.FeGraph:
+        return _hip.FeGraph(self.n_nodes, self.nnzb, int(self.inc.numel()), self.n_blk, self.max_inc, 0,
+                            _vp(self.bptr), _vp(self.bcol), _vp(self.inc_ptr), _vp(self.inc),
+                            _vp(self.bseg_ptr), _vp(self.bseg), _vp(self.blk))
+
+
+@dataclass
+class BsrMatrix:
+    """Node-block matrix on the device; ``vals`` is [nnzb * dof * dof] in BSR layout."""
+    graph: DeviceGraph
+    vals: torch.Tensor
+    dof: int
+
+    @property
+    def precision(self):
+        return 0 if self.vals.dtype == torch.float64 else 1
+
+    @property
+    def n(self):
+        return self.graph.n_nodes * self.dof
+
+    def struct(self) -> _hip.FeBsr:
+        return _hip.FeBsr(self.graph.n_nodes, self.dof, self.precision, _vp(self.graph.bptr),
+                          _vp(self.graph.bcol), _vp(self.vals))
+
+    def to_f32(self) -> 'BsrMatrix':
+        out = torch.empty(self.vals.numel(), dtype=torch.float32, device=self.vals.device)
+        cast_f64_to_f32(self.graph.ctx, self.vals, out)
+        return BsrMatrix(self.graph, out, self.dof)
+
+    def clone(self) -> 'BsrMatrix':
+        return BsrMatrix(self.graph, self.vals.clone(), self.dof)
+
+    def csr_values(self) -> torch.Tensor:
+        """Values permuted to the scalar-CSR layout of the pattern (device tensor)."""
+        out = torch.empty_like(self.vals)
+        relayout(self.graph.ctx, self.graph, self.dof, _hip.FE_LAYOUT_CSR, self.vals, out)
+        return out
+
+
+# ------------------------------------------------------------------------------------------------
+# kernels
+# ------------------------------------------------------------------------------------------------
+def element_matrices(mesh: DeviceMesh, what: str = 'stiffness', sym: bool = False) -> torch.Tensor:
+    """Entry-major element matrices ke[k, e] (k over m*m, or the m(m+1)/2 upper triangle)."""
+    ctx = mesh.ctx
+    m = mesh.npe * mesh.dof
+    rows = m * (m + 1) // 2 if sym else m * m
+    ke = torch.empty((rows, mesh.n_elem), dtype=torch.float64, device=ctx.torch_device)
+    if mesh.n_elem == 0:
+        return ke
+    h, s = ctx.h, int(sym)
+    if what == 'stiffness':
+        if mesh.kind == 'tri_elastic':
+            _hip.call('fe_ke_tri_elastic', h, _vp(mesh.x), _vp(mesh.y), _vp(mesh.conn), _vp(mesh.mat0),
+                      _vp(mesh.mat1), _vp(mesh.mat2), mesh.plane_mode, mesh.n_elem, s, _vp(ke))
+        elif mesh.kind == 'tet_elastic':
+            _hip.call('fe_ke_tet_elastic', h, _vp(mesh.x), _vp(mesh.y), _vp(mesh.z), _vp(mesh.conn),
+                      _vp(mesh.mat0), _vp(mesh.mat1), mesh.n_elem, s, _vp(ke))
+        else:
+            _hip.call('fe_ke_tri_magnetic', h, _vp(mesh.x), _vp(mesh.y), _vp(mesh.conn),
+                      _vp(mesh.mat0), mesh.n_elem, s, _vp(ke))
+    elif what == 'mass':
+        if mesh.rho is None:
+            raise ValueError('mass matrices need a mass density')
+        if mesh.kind == 'tri_elastic':
+            _hip.call('fe_me_tri', h, _vp(mesh.x), _vp(mesh.y), _vp(mesh.conn), _vp(mesh.rho),
+                      _vp(mesh.mat2), mesh.n_elem, s, _vp(ke))
+        elif mesh.kind == 'tet_elastic':
+            _hip.call('fe_me_tet', h, _vp(mesh.x), _vp(mesh.y), _vp(mesh.z), _vp(mesh.conn),
+                      _vp(mesh.rho), mesh.n_elem, s, _vp(ke))
+        else:
+            raise AttributeError("'MagneticElement2D' object has no attribute 'elementary_mass_matrix'")
+    else:
+        raise ValueError(f'unknown matrix {what!r}')
+    return ke
+
+
+def assemble_gather(ctx: Context, ke: torch.Tensor, gather: torch.Tensor, seg_ptr: torch.Tensor,
+                    nnz: int) -> torch.Tensor:
+    vals = ctx.empty(nnz)
+    _hip.call('fe_assemble_gather', ctx.h, _vp(ke), _vp(gather), _vp(seg_ptr), int(nnz), _vp(vals))
+    return vals
+
+
+def assemble_fused(mesh: DeviceMesh, graph: DeviceGraph, what: str = 'stiffness',
+                   layout: int = _hip.FE_LAYOUT_BSR, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    ctx = mesh.ctx
+    n = graph.nnzb * mesh.dof * mesh.dof
+    vals = ctx.empty(n) if out is None else out
+    w = _hip.FE_STIFFNESS if what == 'stiffness' else _hip.FE_MASS
+    ms, gs = mesh.struct(), graph.struct()
+    _hip.call('fe_assemble_fused', ctx.h, ctypes.byref(ms), ctypes.byref(gs), w, layout, _vp(vals))
+    return vals
+
+
+def relayout(ctx: Context, graph: DeviceGraph, dof: int, to_layout: int, src: torch.Tensor,
+             dst: torch.Tensor):
+    _hip.call('fe_relayout', ctx.h, _vp(graph.bptr), graph.n_nodes, dof, to_layout, _vp(src), _vp(dst))
+
+
+def cast_f64_to_f32(ctx: Context, src: torch.Tensor, dst: torch.Tensor):
+    _hip.call('fe_cast_f64_to_f32', ctx.h, src.numel(), _vp(src), _vp(dst))
+
+
+def spmv(A: BsrMatrix, x: torch.Tensor, y: Optional[torch.Tensor] = None) -> torch.Tensor:
+    ctx = A.graph.ctx
+    if y is None:
+        y = torch.empty_like(x)
+    s = A.struct()
+    _hip.call('fe_spmv', ctx.h, ctypes.byref(s), _vp(x), _vp(y))
+    return y
+
+
+def spmv_csr(ctx: Context, rowptr, colind, vals, x, y=None):
+    if y is None:
+        y = torch.empty_like(x)
+    _hip.call('fe_spmv_csr', ctx.h, rowptr.numel() - 1, _vp(rowptr), _vp(colind), _vp(vals), _vp(x),
+              _vp(y))
+    return y
+
+
+def apply_dirichlet(A: BsrMatrix, fixed: torch.Tensor) -> BsrMatrix:
+    """Copy of A with rows/columns of fixed DOFs replaced by identity."""
+    out = A.clone()
+    s = A.struct()
+    _hip.call('fe_apply_dirichlet_bsr', A.graph.ctx.h, ctypes.byref(s), _vp(fixed), _vp(out.vals))
+    return out
+
+
+def dirichlet_rhs(ctx: Context, fixed, g, f, kg):
+    b = torch.empty_like(f)
+    _hip.call('fe_dirichlet_rhs', ctx.h, f.numel(), _vp(fixed), _vp(g), _vp(f), _vp(kg), _vp(b))
+    return b
+
+
+def precond_setup(A: BsrMatrix, kind: str = 'block_jacobi') -> torch.Tensor:
+    ctx = A.graph.ctx
+    k = _hip.FE_PC_JACOBI if kind == 'jacobi' else _hip.FE_PC_BLOCK_JACOBI
+    size = A.n if kind == 'jacobi' else A.graph.n_nodes * A.dof * A.dof
+    out = ctx.empty(size)
+    s = A.struct()
+    _hip.call('fe_precond_setup', ctx.h, ctypes.byref(s), k, _vp(out))
+    return out
+
+
+class PcgSolver:
+    """PCG on a Dirichlet-applied SPD matrix, with its workspace kept between solves."""
+
+    def __init__(self, A64: BsrMatrix, precond: str = 'jacobi', mixed: bool = False,
+                 M64: Optional[torch.Tensor] = None, A32: Optional[BsrMatrix] = None,
+                 check_every: int = 16, ru_delta: float = 0.1):
+        self.ctx = A64.graph.ctx
+        self.A64 = A64
+        self.precond = precond
+        self.mixed = bool(mixed)
+        self.M64 = precond_setup(A64, 'jacobi' if precond == 'jacobi' else 'block_jacobi') \
+            if M64 is None else M64
+        if self.mixed:
+            self.A32 = A64.to_f32() if A32 is None else A32
+            self.M32 = self.M64.to(torch.float32)
+        else:
+            self.A32 = None
+            self.M32 = None
+        self.check_every = check_every
+        self.ru_delta = ru_delta
+        self._work = None
+
+    def set_preconditioner(self, M64: torch.Tensor):
+        self.M64 = M64
+        if self.mixed:
+            self.M32 = M64.to(torch.float32)
+
+    def _opts(self, tol, max_iter, profile=False):
+        pc = _hip.FE_PC_JACOBI if self.precond == 'jacobi' else _hip.FE_PC_BLOCK_JACOBI
+        return _hip.FePcgOpts(float(tol), int(max_iter), pc, int(self.mixed), int(self.check_every),
+                              float(self.ru_delta), int(profile), 0)
+
+    def solve(self, b: torch.Tensor, x: Optional[torch.Tensor] = None, tol: float = 1e-8,
+              max_iter: Optional[int] = None, profile: bool = False):
+        """Solve from initial guess x (zeros if None); returns (x, info dict)."""
+        ctx = self.ctx
+        if max_iter is None:
+            max_iter = max(1000, 10 * self.A64.n)
+        if x is None:
+            x = torch.zeros_like(b)
+        opts = self._opts(tol, max_iter, profile)
+        a64 = self.A64.struct()
+        a32 = self.A32.struct() if self.A32 is not None else None
+        nbytes = ctypes.c_int64()
+        _hip.call('fe_pcg_workspace', ctx.h, ctypes.byref(a64), ctypes.byref(opts), ctypes.byref(nbytes))
+        if self._work is None or self._work.numel() < nbytes.value:
+            self._work = torch.empty(nbytes.value, dtype=torch.uint8, device=ctx.torch_device)
+        stats = _hip.FePcgStats()
+        _hip.call('fe_pcg', ctx.h, ctypes.byref(a64), ctypes.byref(a32) if a32 is not None else None,
+                  _vp(self.M64), _vp(self.M32), _vp(b), _vp(x), _vp(self._work), self._work.numel(),
+                  ctypes.byref(opts), ctypes.byref(stats))
+        info = {'iterations': stats.iterations, 'reliable_updates': stats.reliable_updates,
+                'status': stats.status, 'relres': stats.relres, 'relres_rec': stats.relres_rec,
+                'bnorm': stats.bnorm, 'spmv_ms': stats.spmv_ms, 'spmv_samples': stats.spmv_samples}
+        return x, info

@@ -1,0 +1,118 @@
+"""Array-level finite-element system on the device.
+
+``FESystem`` owns one mesh's device data (SoA coordinates, connectivity, materials), its pattern and
+node graph, and runs the path: fused assembly -> Dirichlet elimination -> preconditioner -> PCG.  The
+reference-shaped ``FiniteElementAnalysis`` (analysis.py) lowers loads and conditions on the host and
+drives this class; benchmarks drive it directly with arrays.
+"""
+
+from __future__ import annotations
+
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _hip, device
+from .device import BsrMatrix, Context, DeviceGraph, DeviceMesh, Pattern, PcgSolver
+
+
+class FESystem:
+    """Device-resident mesh + pattern of one element kind.
+
+    kind: 'tri_elastic' | 'tet_elastic' | 'tri_magnetic'; coords (N, dim); conn (E, npe);
+    materials: dict of per-element arrays or scalars (E, nu, thickness, rho, mu).
+    """
+
+    def __init__(self, kind: str, coords, conn, materials: dict, plane_mode: int = 0,
+                 device_index: int = 0, with_gather: bool = False, max_inc: Optional[int] = None):
+        device.require_gpu()
+        self.ctx = Context.get(device_index)
+        self.kind = kind
+        _, self.npe, self.dof = device.KIND_INFO[kind]
+        coords = np.asarray(coords, dtype=np.float64)
+        conn = np.asarray(conn, dtype=np.int32)
+        self.n_nodes = coords.shape[0]
+        self.n_elem = conn.shape[0]
+        self.pattern = Pattern(conn, self.dof, self.n_nodes, sym=True, with_gather=with_gather,
+                               max_inc=max_inc)
+        self.graph = DeviceGraph(self.ctx, self.pattern)
+        self.mesh = DeviceMesh(self.ctx, kind, coords, conn, materials, plane_mode)
+        if with_gather:
+            self.gather = self.ctx.tensor(self.pattern.gather)
+            self.seg_ptr = self.ctx.tensor(self.pattern.seg_ptr)
+        else:
+            self.gather = self.seg_ptr = None
+
+    @property
+    def n(self):
+        return self.n_nodes * self.dof
+
+    # ---------------------------------------------------------------------------------------------
+    def set_schedule(self, max_inc: int):
+        """Re-tile the fused assembly (row-block schedule) for a given incidences-per-workgroup cap."""
+        blk, biggest = Pattern.schedule_for(self.pattern.inc_ptr, max_inc)
+        self.graph.set_schedule(blk, biggest)
+
+    def stiffness(self, layout: int = _hip.FE_LAYOUT_BSR, out=None) -> torch.Tensor:
+        return device.assemble_fused(self.mesh, self.graph, 'stiffness', layout, out)
+
+    def mass(self, layout: int = _hip.FE_LAYOUT_BSR, out=None) -> torch.Tensor:
+        return device.assemble_fused(self.mesh, self.graph, 'mass', layout, out)
+
+    def stiffness_bsr(self) -> BsrMatrix:
+        return BsrMatrix(self.graph, self.stiffness(_hip.FE_LAYOUT_BSR), self.dof)
+
+    def mass_bsr(self) -> BsrMatrix:
+        return BsrMatrix(self.graph, self.mass(_hip.FE_LAYOUT_BSR), self.dof)
+
+    def element_matrices(self, what: str = 'stiffness', sym: bool = False) -> torch.Tensor:
+        return device.element_matrices(self.mesh, what, sym)
+
+    def assemble_two_phase(self, what: str = 'stiffness') -> torch.Tensor:
+        """Element kernel (symmetric storage) + sorted segmented reduction -> CSR values."""
+        if self.gather is None:
+            raise ValueError('FESystem was built without the two-phase contribution map '
+                             '(with_gather=False)')
+        ke = self.element_matrices(what, sym=True)
+        return device.assemble_gather(self.ctx, ke, self.gather, self.seg_ptr, self.pattern.nnz)
+
+    # ---------------------------------------------------------------------------------------------
+    def dirichlet(self, fixed_dofs: np.ndarray, values: Optional[np.ndarray] = None):
+        """Device mask and value vector for fixed DOFs (indices into the interleaved DOF vector)."""
+        mask = np.zeros(self.n, dtype=np.uint8)
+        g = np.zeros(self.n, dtype=np.float64)
+        fixed_dofs = np.asarray(fixed_dofs, dtype=np.int64)
+        mask[fixed_dofs] = 1
+        if values is not None:
+            g[fixed_dofs] = np.asarray(values, dtype=np.float64)
+        return self.ctx.tensor(mask), self.ctx.tensor(g)
+
+    def prepare(self, K: BsrMatrix, fixed: torch.Tensor, g: torch.Tensor, f: torch.Tensor):
+        """Dirichlet elimination: returns (K_D, b) with K_D SPD on the free DOFs."""
+        kg = device.spmv(K, g)
+        b = device.dirichlet_rhs(self.ctx, fixed, g, f, kg)
+        KD = device.apply_dirichlet(K, fixed)
+        return KD, b
+
+    def solver(self, KD: BsrMatrix, precond: str = 'jacobi', precision: str = 'fp64',
+               M64: Optional[torch.Tensor] = None, check_every: int = 16,
+               ru_delta: float = 0.1) -> PcgSolver:
+        if precision not in ('fp64', 'mixed'):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        if precond not in ('jacobi', 'block_jacobi'):
+            raise ValueError("precond must be 'jacobi' or 'block_jacobi' (or a learned block "
+                             "preconditioner passed as M64)")
+        return PcgSolver(KD, precond=precond, mixed=(precision == 'mixed'), M64=M64,
+                         check_every=check_every, ru_delta=ru_delta)
+
+    def solve(self, f, fixed_dofs, values=None, tol: float = 1e-8, precond: str = 'jacobi',
+              precision: str = 'fp64', max_iter: Optional[int] = None):
+        """Assemble K, eliminate Dirichlet DOFs and solve K u = f; returns (u, K, info)."""
+        K = self.stiffness_bsr()
+        fixed, g = self.dirichlet(fixed_dofs, values)
+        f_t = f if isinstance(f, torch.Tensor) else self.ctx.tensor(np.asarray(f, dtype=np.float64))
+        KD, b = self.prepare(K, fixed, g, f_t)
+        solver = self.solver(KD, precond, precision)
+        u, info = solver.solve(b, tol=tol, max_iter=max_iter)
+        return u, K, info

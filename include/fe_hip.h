@@ -1,0 +1,243 @@
+/*
+ * fe_hip.h -- C ABI of libfe_hip.so, the MI355X (gfx950) element-matrix, assembly and solve path.
+ *
+ * The reference (Dessia-tech/finite_elements v0.2.0) is pure Python with no FFI; its only seams are
+ * Python methods.  Each entry point below names the reference function whose work it replaces.
+ *
+ * Conventions
+ *   - All array arguments are DEVICE pointers unless the comment says "host".  The caller owns every
+ *     buffer; the library never frees caller memory.  Workspaces use a size query then a caller buffer.
+ *   - Return 0 on success; > 0 is a hipError_t passed through; < 0 is a library code (FE_ERR_*).
+ *     fe_last_error() returns a thread-local message for the last failure.  No C++ exception crosses
+ *     the ABI.
+ *   - A handle is bound to one HIP stream and is not re-entrant; separate handles are independent.
+ *     Kernels are asynchronous on that stream; only calls returning statistics synchronise.
+ *   - Connectivity is SoA: conn[a * n_elem + e] is the node index of local vertex a of element e.
+ *   - Element matrices are entry-major: ke[k * n_elem + e].  With sym == 0 k runs over the m*m entries
+ *     of the element matrix in row-major order (the order of the reference's ndarray.flatten()); with
+ *     sym == 1 over the upper triangle (i <= j) in row-major order.  Local DOF order is
+ *     [u1, v1, (w1), u2, v2, ...] as in finite_elements/elements.py:410-412 and 742-747.
+ *   - DOF numbering is interleaved: dof = node * dof_per_node + d (finite_elements/core.py:89-108).
+ */
+#ifndef FE_HIP_H
+#define FE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FE_OK 0
+#define FE_ERR_ARG (-1)
+#define FE_ERR_NOT_CONVERGED (-2)
+#define FE_ERR_BREAKDOWN (-3)
+#define FE_ERR_UNSUPPORTED (-4)
+#define FE_ERR_NOMEM (-5)
+
+/* element kinds */
+#define FE_KIND_TRI_ELASTIC 1   /* ElasticityTriangularElement2D, 3 nodes, 2 dof/node */
+#define FE_KIND_TET_ELASTIC 2   /* ElasticityTetrahedralElement3D, 4 nodes, 3 dof/node */
+#define FE_KIND_TRI_MAGNETIC 3  /* MagneticElement2D, 3 nodes, 1 dof/node */
+
+/* plane modes for FE_KIND_TRI_ELASTIC */
+#define FE_PLANE_STRAIN 1
+#define FE_PLANE_STRESS 2
+
+/* what to assemble */
+#define FE_STIFFNESS 0
+#define FE_MASS 1
+
+/* value layouts of an assembled matrix */
+#define FE_LAYOUT_CSR 0   /* scalar CSR, columns sorted, the reference's csr_matrix layout */
+#define FE_LAYOUT_BSR 1   /* node-block rows: vals[(bptr[i] + q) * dof * dof + di * dof + dj] */
+
+/* preconditioners */
+#define FE_PC_JACOBI 1        /* inverse diagonal, one value per DOF */
+#define FE_PC_BLOCK_JACOBI 2  /* inverse of each node's dof x dof diagonal block (also the learned one) */
+
+typedef struct fe_handle fe_handle;
+typedef struct fe_pattern fe_pattern;
+
+/* Mesh on the device.  coords are SoA; mat0..mat2 are per-element material arrays:
+ *   FE_KIND_TRI_ELASTIC : mat0 = Young's modulus, mat1 = Poisson ratio, mat2 = thickness, rho = density
+ *   FE_KIND_TET_ELASTIC : mat0 = Young's modulus, mat1 = Poisson ratio,                  rho = density
+ *   FE_KIND_TRI_MAGNETIC: mat0 = mu_total (the element's permeability, reluctivity is 1/mu_total)   */
+typedef struct {
+  int32_t kind;
+  int32_t plane_mode;
+  int64_t n_elem;
+  int64_t n_nodes;
+  const double* x;
+  const double* y;
+  const double* z;
+  const int32_t* conn;
+  const double* mat0;
+  const double* mat1;
+  const double* mat2;
+  const double* rho;
+} fe_mesh;
+
+/* Node graph of a mesh on the device (built by fe_pattern_create, copied to the device by the caller).
+ *   bptr/bcol : node adjacency CSR (sorted, self included) = the block pattern of K
+ *   inc_ptr/inc : node -> (element, local vertex) incidences, inc = e * npe + a, ascending e per node
+ *   bseg_ptr/bseg : for block entry (i, q) the contributing incidences of node i, ascending element,
+ *                   bseg = (k_local << 2) | b  with k_local the incidence's rank within node i's list
+ *                   and b the local vertex of the column node
+ *   blk : node ranges per workgroup of the fused assembly (row-block schedule)                      */
+typedef struct {
+  int64_t n_nodes;
+  int64_t nnzb;
+  int64_t n_inc;
+  int64_t n_blk;
+  int32_t max_inc;  /* largest incidence count of one workgroup's node range */
+  int32_t pad;
+  const int32_t* bptr;
+  const int32_t* bcol;
+  const int32_t* inc_ptr;
+  const int32_t* inc;
+  const int32_t* bseg_ptr;
+  const uint16_t* bseg;
+  const int32_t* blk;
+} fe_graph;
+
+/* Block sparse matrix (node-block rows) on the device; precision 0 = FP64, 1 = FP32. */
+typedef struct {
+  int64_t n_brows;
+  int32_t dof;
+  int32_t precision;
+  const int32_t* bptr;
+  const int32_t* bcol;
+  const void* vals;
+} fe_bsr;
+
+typedef struct {
+  double tol;            /* target for the true FP64 relative residual ||b - A x|| / ||b|| */
+  int32_t max_iter;
+  int32_t precond;       /* FE_PC_* */
+  int32_t mixed;         /* 0: FP64 CG; 1: FP32 SpMV/preconditioner with FP64 reliable updates */
+  int32_t check_every;   /* iterations per launched chunk between host polls */
+  double ru_delta;       /* mixed: reliable update when ||r|| fell by this factor since the last one */
+  int32_t profile;       /* 1: time the first SpMV kernel of every chunk with HIP events */
+  int32_t pad;
+} fe_pcg_opts;
+
+typedef struct {
+  int32_t iterations;
+  int32_t reliable_updates;
+  int32_t status;        /* 0 converged, FE_ERR_NOT_CONVERGED, FE_ERR_BREAKDOWN */
+  int32_t pad;
+  double relres;         /* true FP64 relative residual at exit */
+  double relres_rec;     /* recursive residual estimate at exit */
+  double bnorm;
+  double spmv_ms;        /* profile: mean duration of the sampled SpMV kernels */
+  int32_t spmv_samples;  /* profile: number of samples */
+  int32_t pad2;
+} fe_pcg_stats;
+
+/* ---- context ---------------------------------------------------------------------------------- */
+int fe_version(void);
+const char* fe_last_error(void);
+int fe_create(fe_handle** h, int device, void* hip_stream);
+int fe_destroy(fe_handle* h);
+int fe_synchronize(fe_handle* h);
+
+/* ---- element kernels (one thread per element, entry-major output) ------------------------------- */
+/* replaces ElasticityTriangularElement2D.elementary_matrix, finite_elements/elements.py:466-511
+   (with _b_matrix 395-416 and _d_matrix_plane_strain/_stress 418-453) */
+int fe_ke_tri_elastic(fe_handle* h, const double* x, const double* y, const int32_t* conn,
+                      const double* young, const double* nu, const double* thickness, int plane_mode,
+                      int64_t n_elem, int sym, double* ke);
+/* replaces ElasticityTetrahedralElement3D.elementary_matrix, finite_elements/elements.py:809-828
+   (with _b_matrix 719-751 and _d_matrix_ 773-796) */
+int fe_ke_tet_elastic(fe_handle* h, const double* x, const double* y, const double* z,
+                      const int32_t* conn, const double* young, const double* nu, int64_t n_elem,
+                      int sym, double* ke);
+/* replaces MagneticElement2D.elementary_matrix, finite_elements/elements.py:93-118 */
+int fe_ke_tri_magnetic(fe_handle* h, const double* x, const double* y, const int32_t* conn,
+                       const double* mu_total, int64_t n_elem, int sym, double* ke);
+/* replaces ElasticityTriangularElement2D.elementary_mass_matrix, finite_elements/elements.py:513-536 */
+int fe_me_tri(fe_handle* h, const double* x, const double* y, const int32_t* conn, const double* rho,
+              const double* thickness, int64_t n_elem, int sym, double* me);
+/* replaces ElasticityTetrahedralElement3D.elementary_mass_matrix, finite_elements/elements.py:830-857 */
+int fe_me_tet(fe_handle* h, const double* x, const double* y, const double* z, const int32_t* conn,
+              const double* rho, int64_t n_elem, int sym, double* me);
+
+/* ---- pattern (host) ------------------------------------------------------------------------------ */
+/* replaces the COO index lists of get_row_col_indices / k_matrix_data (finite_elements/analysis.py:
+   714-735, 324-339) and the pattern half of scipy's COO->CSR at analysis.py:661.  conn is HOST memory. */
+#define FE_PAT_NNZ 0        /* scalar nonzeros of K */
+#define FE_PAT_NNZB 1       /* node blocks of K */
+#define FE_PAT_NINC 2       /* node-element incidences (= npe * n_elem) */
+#define FE_PAT_NCONTRIB 3   /* element-matrix contributions (= m*m * n_elem) */
+#define FE_PAT_NBLK 4       /* workgroups of the fused assembly schedule */
+#define FE_PAT_MAXINC 5     /* largest incidence count in one workgroup */
+#define FE_PAT_NROWS 6      /* n_nodes * dof */
+#define FE_PAT_NINFO 8
+#define FE_ARR_ROWPTR 0     /* int32 [nrows+1] */
+#define FE_ARR_COLIND 1     /* int32 [nnz] */
+#define FE_ARR_GATHER 2     /* int32 [ncontrib]: index into entry-major ke of each contribution,
+                               grouped by destination CSR entry, ascending element within a group */
+#define FE_ARR_SEGPTR 3     /* int32 [nnz+1] */
+#define FE_ARR_BPTR 4       /* int32 [n_nodes+1] */
+#define FE_ARR_BCOL 5       /* int32 [nnzb] */
+#define FE_ARR_INCPTR 6     /* int32 [n_nodes+1] */
+#define FE_ARR_INC 7        /* int32 [ninc] */
+#define FE_ARR_BSEGPTR 8    /* int32 [nnzb+1] */
+#define FE_ARR_BSEG 9       /* uint16 [ninc*npe] */
+#define FE_ARR_BLK 10       /* int32 [nblk+1] */
+int fe_pattern_create(const int32_t* conn, int npe, int dof, int64_t n_elem, int64_t n_nodes,
+                      int sym, int with_gather, fe_pattern** out);
+int fe_pattern_schedule(fe_pattern* p, int max_inc_per_block);
+int fe_pattern_info(const fe_pattern* p, int64_t* info);
+int fe_pattern_copy(const fe_pattern* p, int which, void* host_dst);
+int fe_pattern_destroy(fe_pattern* p);
+
+/* ---- assembly ------------------------------------------------------------------------------------ */
+/* Two-phase assembly from element matrices: vals[p] = sum over the sorted contribution segment of p.
+   LDS-staged segmented reduction, no atomics, bitwise deterministic.  Replaces the value half of
+   scipy's COO->CSR sum-duplicates at finite_elements/analysis.py:661 and the dict sum at 387-405. */
+int fe_assemble_gather(fe_handle* h, const double* ke, const int32_t* gather, const int32_t* seg_ptr,
+                       int64_t nnz, double* vals);
+/* Fused assembly: each workgroup owns a node range, computes the element-matrix row blocks of its
+   nodes' incidences from coordinates into LDS and reduces them per block entry in ascending element
+   order.  No element matrix touches HBM; no atomics; bitwise deterministic.  what = FE_STIFFNESS or
+   FE_MASS; layout = FE_LAYOUT_CSR or FE_LAYOUT_BSR.  Replaces k_matrix_data + csr_matrix
+   (finite_elements/analysis.py:324-339, 661) and m_matrix_data (357-365). */
+int fe_assemble_fused(fe_handle* h, const fe_mesh* mesh, const fe_graph* graph, int what, int layout,
+                      double* vals);
+/* Permute node-chunk values between the CSR and BSR layouts (out-of-place). */
+int fe_relayout(fe_handle* h, const int32_t* bptr, int64_t n_nodes, int dof, int to_layout,
+                const double* src, double* dst);
+
+/* ---- boundary conditions ------------------------------------------------------------------------- */
+/* Dirichlet elimination on a BSR copy of K: rows and columns of fixed DOFs become identity rows.
+   Replaces the Lagrange-multiplier augmentation of finite_elements/analysis.py:241-279, 509-543. */
+int fe_apply_dirichlet_bsr(fe_handle* h, const fe_bsr* A, const uint8_t* fixed, double* vals);
+/* b = f - Kg on free DOFs, b = g on fixed DOFs (Kg computed beforehand with fe_spmv). */
+int fe_dirichlet_rhs(fe_handle* h, int64_t n, const uint8_t* fixed, const double* g, const double* f,
+                     const double* kg, double* b);
+
+/* ---- sparse kernels -------------------------------------------------------------------------------- */
+int fe_spmv(fe_handle* h, const fe_bsr* A, const void* x, void* y);
+int fe_spmv_csr(fe_handle* h, int64_t n_rows, const int32_t* rowptr, const int32_t* colind,
+                const double* vals, const double* x, double* y);
+int fe_cast_f64_to_f32(fe_handle* h, int64_t n, const double* src, float* dst);
+
+/* ---- solver ------------------------------------------------------------------------------------- */
+/* Preconditioner setup from the FP64 (Dirichlet-applied) matrix: FE_PC_JACOBI writes n values,
+   FE_PC_BLOCK_JACOBI writes n_brows*dof*dof values (the inverse diagonal blocks). */
+int fe_precond_setup(fe_handle* h, const fe_bsr* A, int kind, double* out);
+/* PCG on the Dirichlet-applied SPD system (replaces spsolve at finite_elements/analysis.py:819-822).
+   A64/M64 are FP64; with opts->mixed the FP32 copies A32/M32 run the iteration and FP64 reliable
+   updates recompute the true residual.  x is the initial guess on entry. */
+int fe_pcg_workspace(fe_handle* h, const fe_bsr* A, const fe_pcg_opts* opts, int64_t* bytes);
+int fe_pcg(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M64, const void* M32,
+           const double* b, double* x, void* work, int64_t work_bytes, const fe_pcg_opts* opts,
+           fe_pcg_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FE_HIP_H */

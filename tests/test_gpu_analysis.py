@@ -1,0 +1,178 @@
+"""The reference-shaped API on the GPU path against the golden vectors the CPU oracle wrote from the
+reference's example problems (tests/golden/make_golden.py): augmented matrix pattern bit-exact and
+values to 1e-12 (row-scaled), source vector, and solve() result vectors (displacements and Lagrange
+multipliers) against spsolve(permc_spec='NATURAL')."""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import finite_elements_amd as fe
+from finite_elements_amd import conditions, loads, mesh
+from finite_elements_amd.analysis import FiniteElementAnalysis
+from oracle import reference as ref
+from tests.conftest import rowscaled_err
+
+pytestmark = pytest.mark.gpu
+MU0 = 4e-7 * np.pi
+
+
+def _tri_elastic_mesh(tris, E, nu, t, rho, groups=1):
+    els = [fe.elements.ElasticityTriangularElement2D(mesh.TriangularElement2D([mesh.Node2D(*p) for p in tri]),
+                                                     E, nu, rho, t) for tri in tris]
+    return mesh.Mesh([mesh.ElementsGroup(els, '')]), els
+
+
+def build(key):
+    """Product-side FiniteElementAnalysis of the golden problem ``key``."""
+    if key == 'ex1':
+        tris = [((3, 0), (3, 2), (0, 0)), ((0, 2), (0, 0), (3, 2))]
+        m, _ = _tri_elastic_mesh(tris, 30e6, 0.25, 0.5, 2.7)
+        N = mesh.Node2D
+        nl = [loads.NodeLoad(m.nodes[m.node_to_index[N(3, 2)]], -1000, 2)]
+        bcs = [conditions.NodeBoundaryCondition(N(3, 0), 0, 2), conditions.NodeBoundaryCondition(N(0, 2), 0, 1),
+               conditions.NodeBoundaryCondition(N(0, 2), 0, 2), conditions.NodeBoundaryCondition(N(0, 0), 0, 1),
+               conditions.NodeBoundaryCondition(N(0, 0), 0, 2)]
+        return FiniteElementAnalysis(m, [], [], nl, [], [], bcs, [], [], plane_strain=False, plane_stress=True)
+    if key == 'ex2':
+        tris = []
+        for i in range(9):
+            tris += [((i, 0), (i + 1, 0), (i, 1)), ((i + 1, 1), (i + 1, 0), (i, 1))]
+        m, _ = _tri_elastic_mesh(tris, 210e9, 0.25, 1.0, 7860)
+        nl = [loads.NodeLoad(m.nodes[m.node_to_index[mesh.Node2D(9, 1)]], -1e7, 2)]
+        bcs = []
+        for n in m.nodes:
+            if n.x == 0:
+                bcs += [conditions.NodeBoundaryCondition(n, 0, 1), conditions.NodeBoundaryCondition(n, 0, 2)]
+        return FiniteElementAnalysis(m, [], [], nl, [], [], bcs, [], [], plane_strain=True, plane_stress=False)
+    if key == 'tet1':
+        pts = [mesh.Node3D(2, 3, 4), mesh.Node3D(6, 3, 2), mesh.Node3D(2, 5, 1), mesh.Node3D(4, 3, 6)]
+        el = fe.elements.ElasticityTetrahedralElement3D(mesh.TetrahedralElement(pts), 210e9, 0.25, 7860)
+        m = mesh.Mesh([mesh.ElementsGroup([el], '')])
+        bcs = [conditions.NodeBoundaryCondition(p, 0, d) for p in pts[:3] for d in (1, 2, 3)]
+        nl = [loads.NodeLoad(pts[3], 1e5, 3)]
+        return FiniteElementAnalysis(m, [], [], nl, [], [], bcs, [], [], plane_strain=True, plane_stress=False)
+    if key == 'mag':
+        mus = [MU0 * 100000, MU0, MU0 * 50000]
+        groups = []
+        for phase in range(3):
+            els = []
+            for i in range(3 * phase, 3 * phase + 3):
+                t1 = mesh.TriangularElement2D([mesh.Point2D(i, 0), mesh.Point2D(i + 1, 0), mesh.Point2D(i, 1)])
+                t2 = mesh.TriangularElement2D([mesh.Point2D(i + 1, 1), mesh.Point2D(i + 1, 0), mesh.Point2D(i, 1)])
+                els += [fe.elements.MagneticElement2D(t1, mus[phase]), fe.elements.MagneticElement2D(t2, mus[phase])]
+            groups.append(mesh.ElementsGroup(els, ''))
+        m = mesh.Mesh(groups)
+        el_loads = [loads.ElementsLoad(groups[0].elements[0:2], 1e10, 1)]
+        bcs = [conditions.NodeBoundaryCondition(n, 0, 1) for n in m.nodes if abs(n[0] - 9) < 1e-6]
+        return FiniteElementAnalysis(m, el_loads, [], [], [], [], bcs, [], [])
+    if key == 'cantilever':
+        pr = fe.meshgen.cantilever_2d()
+        am = mesh.ArrayMesh(pr.coords, pr.conn, 'tri_elastic', pr.materials)
+        nodes = am.nodes
+        nl = [loads.NodeLoad(nodes[d // 2], pr.f[d], d % 2 + 1) for d in np.nonzero(pr.f)[0]]
+        bcs = [conditions.NodeBoundaryCondition(nodes[d // 2], 0.0, d % 2 + 1) for d in pr.fixed_dofs]
+        return FiniteElementAnalysis(am, [], [], nl, [], [], bcs, [], [], plane_strain=False, plane_stress=True)
+    raise KeyError(key)
+
+
+KEYS = ['ex1', 'ex2', 'tet1', 'mag', 'cantilever']
+
+
+@pytest.mark.parametrize('key', KEYS)
+def test_create_matrix_and_source(ctx, golden, key):
+    print(f'analysis create_matrix {key}', flush=True)
+    g = golden('script_problems.npz')
+    an = build(key)
+    assert np.array_equal(np.array([tuple(n) for n in an.mesh.nodes]), g[f'{key}_nodes'])
+    K = an.create_matrix()
+    Kg = sp.csr_matrix((g[f'{key}_data'], g[f'{key}_indices'], g[f'{key}_indptr']),
+                       shape=tuple(g[f'{key}_shape']))
+    assert K.shape == Kg.shape
+    assert np.array_equal(K.indptr, Kg.indptr) and np.array_equal(K.indices, Kg.indices)
+    assert rowscaled_err(Kg, K) < 1e-12
+    F = an.create_source_matrix()
+    np.testing.assert_allclose(F[:, 0], g[f'{key}_F'], rtol=0, atol=1e-12 * max(1.0, np.abs(g[f'{key}_F']).max()))
+
+
+@pytest.mark.parametrize('key', KEYS)
+def test_solve_result_vector(ctx, golden, key):
+    print(f'analysis solve {key}', flush=True)
+    g = golden('script_problems.npz')
+    an = build(key)
+    res = an.solve(tol=1e-12)
+    x = np.asarray(res.result_vector)
+    xg = g[f'{key}_x']
+    assert x.shape == xg.shape
+    n = an.n_dofs
+    info = an.last_solve_info
+    print(f'  {info}', flush=True)
+    assert info['status'] == 0 and info['relres'] <= 1e-12
+    u_err = np.linalg.norm(x[:n] - xg[:n]) / np.linalg.norm(xg[:n])
+    lam_err = np.linalg.norm(x[n:] - xg[n:]) / max(np.linalg.norm(xg[n:]), 1e-300)
+    print(f'  displacement rel err {u_err:.2e}, multiplier rel err {lam_err:.2e}', flush=True)
+    assert u_err < 1e-6 and lam_err < 1e-6
+
+
+def test_element_objects_and_matrix_data(ctx):
+    print('element objects', flush=True)
+    an = build('ex2')
+    prob = ref.Problem('tri_elastic', [tuple(n) for n in an.mesh.nodes],
+                       [(tuple(tuple(p) for p in el.points),
+                         {'E': 210e9, 'nu': 0.25, 'thickness': 1.0, 'rho': 7860})
+                        for el in an.mesh.elements_groups[0].elements],
+                       plane_strain=True, plane_stress=False)
+    d, r, c = an.k_matrix_data()
+    do, ro, co = ref.matrix_data(prob, 'k')
+    assert r == ro and c == co
+    np.testing.assert_allclose(d, do, rtol=1e-12, atol=1e-12 * np.abs(do).max())
+    dm, rm, cm = an.m_matrix_data()
+    dmo, rmo, cmo = ref.matrix_data(prob, 'm')
+    assert rm == rmo and cm == cmo
+    np.testing.assert_allclose(dm, dmo, rtol=1e-13, atol=1e-13 * np.abs(dmo).max())
+    Ms = an.m_matrix('sparse')
+    Mo = ref.matrix_sparse(prob, 'm')
+    assert abs(Ms - Mo).max() <= 1e-12 * abs(Mo).max()
+    Kd = an.k_matrix('dense')
+    assert np.abs(Kd - ref.matrix_sparse(prob, 'k').toarray()).max() <= 1e-12 * np.abs(Kd).max()
+    el = an.mesh.elements_groups[0].elements[3]
+    ke = el.elementary_matrix(plane_strain=True, plane_stress=False)
+    ko = ref.tri_elastic_ke([tuple(p) for p in el.points], 210e9, 0.25, 1.0, True, False)
+    np.testing.assert_allclose(ke, ko, rtol=1e-12, atol=1e-12 * np.abs(ko).max())
+    with pytest.raises(ValueError):
+        el.elementary_matrix(plane_strain=True, plane_stress=True)
+    tet = build('tet1').mesh.elements_groups[0].elements[0]
+    kt = tet.elementary_matrix(plane_strain=True, plane_stress=False)
+    kto = ref.tet_elastic_ke([tuple(p) for p in tet.points], 210e9, 0.25)
+    np.testing.assert_allclose(kt, kto, rtol=1e-12, atol=1e-12 * np.abs(kto).max())
+    mag = build('mag').mesh.elements_groups[1].elements[0]
+    km = mag.elementary_matrix()
+    assert len(km) == 9
+    np.testing.assert_allclose(km, ref.magnetic_ke([tuple(p) for p in mag.points], mag.mu_total), rtol=1e-12)
+
+
+def test_gmsh_beams_and_cantilever_trend(ctx, golden):
+    print('gmsh beams', flush=True)
+    meshes = golden('gmsh_meshes.npz')
+    gb = golden('gmsh_beam2d.npz')
+    errs = []
+    E, t, P, L, b = 30e6, 1.0, -1000.0, 10.0, 1.0
+    delta = P * L ** 3 / (3 * E * (b ** 3 * t / 12))  # scripts/Elasticity/beam2d_example_3.py:139-143
+    for h in ('0.8', '0.5', '0.3', '0.18', '0.1'):
+        coords = meshes[f'beam_2d_{h}_coords'][:, :2]
+        conn = meshes[f'beam_2d_{h}_triangles']
+        ne = conn.shape[0]
+        am = mesh.ArrayMesh(coords, conn, 'tri_elastic',
+                            {'E': 30e6, 'nu': 0.25, 'thickness': 1.0, 'rho': 2.7})
+        nodes = am.nodes
+        tip = int(gb[f'h{h}_tip'])
+        nl = [loads.NodeLoad(nodes[tip], -1000.0, 2)]
+        bcs = [conditions.NodeBoundaryCondition(nodes[d // 2], 0.0, d % 2 + 1) for d in gb[f'h{h}_fixed']]
+        an = FiniteElementAnalysis(am, [], [], nl, [], [], bcs, [], [], plane_strain=False, plane_stress=True)
+        x = np.asarray(an.solve(tol=1e-12).result_vector)
+        xg = gb[f'h{h}_x']
+        n = coords.shape[0] * 2
+        assert np.linalg.norm(x[:n] - xg[:n]) / np.linalg.norm(xg[:n]) < 1e-6
+        errs.append(abs(x[2 * tip + 1] - delta))
+    # the tip deflection approaches the Euler-Bernoulli value as the mesh is refined
+    assert errs[-1] < errs[0]
