@@ -69,12 +69,12 @@ def pcg_spmv_bytes(sysm, value_bytes):
 def time_assembly(sysm, reps=20):
     import torch
     from finite_elements_amd import _hip
-    out = sysm.stiffness(_hip.FE_LAYOUT_BSR)
+    out = sysm.stiffness(_hip.FE_LAYOUT_PLANES)
     torch.cuda.synchronize()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     start.record()
     for _ in range(reps):
-        sysm.stiffness(_hip.FE_LAYOUT_BSR, out=out)
+        sysm.stiffness(_hip.FE_LAYOUT_PLANES, out=out)
     stop.record()
     torch.cuda.synchronize()
     return start.elapsed_time(stop) / reps
@@ -85,7 +85,7 @@ def one_solve(sysm, pr, method, fixed, g, f_t, profile=False):
     K = sysm.stiffness_bsr()
     KD, b = sysm.prepare(K, fixed, g, f_t)
     solver = sysm.solver(KD, precond=method['precond'], precision=method['precision'])
-    u, info = solver.solve(b, tol=1e-8, profile=profile)
+    u, info = solver.solve(b, tol=1e-8, max_iter=40000, profile=profile)
     return u, info
 
 
@@ -138,10 +138,14 @@ def main():
     f_t = sysm.ctx.tensor(pr.f)
     log(f'setup {time.perf_counter() - t0:.1f} s: {sysm.n} DOF, nnz {sysm.pattern.nnz}')
 
-    # ---- assembly roofline ----
+    # ---- assembly: autotune the row-block schedule, then time the best ----
+    best, tune_times = sysm.tune_assembly()
+    log(f'assembly autotune: best (threads, max_inc) = {best}; ' +
+        ', '.join(f'{k}:{v:.3f}ms' for k, v in sorted(tune_times.items())))
     asm_ms = time_assembly(sysm)
     asm_bytes, asm_compulsory = fused_assembly_bytes(sysm)
-    asm = {'kernel': 'k_assemble_fused', 'ms': asm_ms, 'bytes': asm_bytes,
+    asm = {'kernel': 'k_assemble_fused', 'schedule': {'threads': best[0], 'max_inc': best[1]},
+           'ms': asm_ms, 'bytes': asm_bytes,
            'gbps': asm_bytes / asm_ms / 1e6, 'frac': asm_bytes / asm_ms / 1e6 / HBM_PEAK_GBS,
            'compulsory_bytes': asm_compulsory,
            'compulsory_gbps': asm_compulsory / asm_ms / 1e6}

@@ -27,6 +27,7 @@ FE_STIFFNESS = 0
 FE_MASS = 1
 FE_LAYOUT_CSR = 0
 FE_LAYOUT_BSR = 1
+FE_LAYOUT_PLANES = 2
 FE_PC_JACOBI = 1
 FE_PC_BLOCK_JACOBI = 2
 
@@ -50,13 +51,13 @@ class FeMesh(ctypes.Structure):
 
 class FeGraph(ctypes.Structure):
     _fields_ = [('n_nodes', _i64), ('nnzb', _i64), ('n_inc', _i64), ('n_blk', _i64),
-                ('max_inc', _i32), ('pad', _i32), ('bptr', _vp), ('bcol', _vp), ('inc_ptr', _vp),
+                ('max_inc', _i32), ('threads', _i32), ('bptr', _vp), ('bcol', _vp), ('inc_ptr', _vp),
                 ('inc', _vp), ('bseg_ptr', _vp), ('bseg', _vp), ('blk', _vp)]
 
 
 class FeBsr(ctypes.Structure):
-    _fields_ = [('n_brows', _i64), ('dof', _i32), ('precision', _i32), ('bptr', _vp), ('bcol', _vp),
-                ('vals', _vp)]
+    _fields_ = [('n_brows', _i64), ('nnzb', _i64), ('dof', _i32), ('precision', _i32),
+                ('layout', _i32), ('pad', _i32), ('bptr', _vp), ('bcol', _vp), ('vals', _vp)]
 
 
 class FePcgOpts(ctypes.Structure):
@@ -65,7 +66,7 @@ class FePcgOpts(ctypes.Structure):
 
 
 class FePcgStats(ctypes.Structure):
-    _fields_ = [('iterations', _i32), ('reliable_updates', _i32), ('status', _i32), ('pad', _i32),
+    _fields_ = [('iterations', _i32), ('reliable_updates', _i32), ('status', _i32), ('restarts', _i32),
                 ('relres', _dbl), ('relres_rec', _dbl), ('bnorm', _dbl), ('spmv_ms', _dbl),
                 ('spmv_samples', _i32), ('pad2', _i32)]
 
@@ -96,6 +97,7 @@ SIGNATURES = {
     'fe_spmv': (_int, [_vp, _P(FeBsr), _vp, _vp]),
     'fe_spmv_csr': (_int, [_vp, _i64, _vp, _vp, _vp, _vp, _vp]),
     'fe_cast_f64_to_f32': (_int, [_vp, _i64, _vp, _vp]),
+    'fe_bsr_convert': (_int, [_vp, _P(FeBsr), _P(FeBsr)]),
     'fe_precond_setup': (_int, [_vp, _P(FeBsr), _int, _vp]),
     'fe_pcg_workspace': (_int, [_vp, _P(FeBsr), _P(FePcgOpts), _P(_i64)]),
     'fe_pcg': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp, _vp, _vp, _vp, _vp, _i64, _P(FePcgOpts),

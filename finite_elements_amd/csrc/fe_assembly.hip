@@ -48,7 +48,6 @@ __global__ __launch_bounds__(kGatherBlock) void k_assemble_gather(
 // value of the range (contiguous in HBM, so stores are coalesced) sums the LDS values listed by the
 // block-level contribution map, in ascending element order.
 // ---------------------------------------------------------------------------------------------------
-constexpr int kFusedBlock = 256;
 constexpr int kFusedMaxNodes = 256;
 
 template <int KIND>
@@ -146,27 +145,44 @@ __device__ inline void element_row_block(const fe_mesh& m, int64_t e, int a, dou
   }
 }
 
-template <int KIND, int WHAT, int LAYOUT>
-__global__ __launch_bounds__(kFusedBlock) void k_assemble_fused(fe_mesh m, fe_graph g,
-                                                                double* __restrict__ vals) {
+// Contribution-map entries staged in LDS per workgroup (larger ranges read HBM directly).
+constexpr int kStageBq = 512;
+constexpr int kStageSeg = 1024;
+
+template <int KIND, int WHAT, int LAYOUT, int NT>
+__global__ __launch_bounds__(NT) void k_assemble_fused(fe_mesh m, fe_graph g,
+                                                       double* __restrict__ vals) {
   constexpr int NPE = KindTraits<KIND>::NPE, D = KindTraits<KIND>::D;
   constexpr int BS = NPE * D * D;
   extern __shared__ double S[];  // [BS][stride]
   __shared__ int32_t s_bptr[kFusedMaxNodes + 1];
   __shared__ int32_t s_iptr[kFusedMaxNodes + 1];
+  __shared__ int32_t s_bsp[kStageBq + 1];
+  __shared__ uint16_t s_bseg[kStageSeg];
   const int64_t w = xcd_remap(blockIdx.x, gridDim.x);
   const int32_t n0 = g.blk[w], n1 = g.blk[w + 1];
   const int32_t nn = n1 - n0;
-  for (int t = threadIdx.x; t <= nn; t += kFusedBlock) {
+  for (int t = threadIdx.x; t <= nn; t += NT) {
     s_bptr[t] = g.bptr[n0 + t];
     s_iptr[t] = g.inc_ptr[n0 + t];
   }
   __syncthreads();
   const int32_t I0 = s_iptr[0];
   const int32_t ninc = s_iptr[nn] - I0;
+  const int32_t B0 = s_bptr[0], B1 = s_bptr[nn];
+  const int32_t nbq = B1 - B0;
   const int stride = g.max_inc;
+  // Stage the block-level contribution map of the range (independent of phase 1's loads).
+  const bool staged_bq = nbq + 1 <= kStageBq + 1;
+  if (staged_bq)
+    for (int t = threadIdx.x; t <= nbq; t += NT) s_bsp[t] = g.bseg_ptr[B0 + t];
+  const int32_t c_lo = g.bseg_ptr[B0];
+  const int32_t nseg = g.bseg_ptr[B1] - c_lo;
+  const bool staged_seg = nseg <= kStageSeg;
+  if (staged_seg)
+    for (int t = threadIdx.x; t < nseg; t += NT) s_bseg[t] = g.bseg[c_lo + t];
   // Phase 1: element-matrix row blocks of every incidence of the range.
-  for (int t = threadIdx.x; t < ninc; t += kFusedBlock) {
+  for (int t = threadIdx.x; t < ninc; t += NT) {
     const int32_t packed = g.inc[I0 + t];
     const int64_t e = packed / NPE;
     const int a = packed - (int)e * NPE;
@@ -176,40 +192,58 @@ __global__ __launch_bounds__(kFusedBlock) void k_assemble_fused(fe_mesh m, fe_gr
     for (int k = 0; k < BS; ++k) S[k * stride + t] = blk[k];
   }
   __syncthreads();
-  // Phase 2: one thread per output value, ascending-element sums.
-  const int64_t o0 = (int64_t)s_bptr[0] * D * D;
-  const int64_t o1 = (int64_t)s_bptr[nn] * D * D;
-  for (int64_t o = o0 + threadIdx.x; o < o1; o += kFusedBlock) {
-    // node of o: largest j with s_bptr[j]*D*D <= o
+  // Phase 2: one thread per output value of the range, ascending-element sums.  Threads walk the
+  // range in the destination layout's order so that the stores are coalesced.
+  const int64_t nout = (int64_t)nbq * D * D;
+  for (int64_t t = threadIdx.x; t < nout; t += NT) {
+    int32_t bq;
+    int di, dj;
+    int64_t o;
+    if (LAYOUT == FE_LAYOUT_PLANES) {
+      const int k = (int)(t / nbq);
+      bq = B0 + (int32_t)(t - (int64_t)k * nbq);
+      di = k / D;
+      dj = k - di * D;
+      o = (int64_t)k * g.nnzb + bq;
+    } else {
+      o = (int64_t)B0 * D * D + t;
+      bq = -1;
+      di = dj = 0;
+    }
+    // node j of the output: largest j with s_bptr[j] <= (block of o)
+    const int32_t key = (LAYOUT == FE_LAYOUT_PLANES) ? bq : (int32_t)(o / (D * D));
     int lo = 0, hi = nn - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if ((int64_t)s_bptr[mid] * D * D <= o)
+      if (s_bptr[mid] <= key)
         lo = mid;
       else
         hi = mid - 1;
     }
     const int j = lo;
-    const int32_t deg = s_bptr[j + 1] - s_bptr[j];
-    const int32_t l = (int32_t)(o - (int64_t)s_bptr[j] * D * D);
-    int q, di, dj;
-    if (LAYOUT == FE_LAYOUT_CSR) {
-      di = l / (deg * D);
-      const int rem = l - di * deg * D;
-      q = rem / D;
-      dj = rem - q * D;
-    } else {
-      q = l / (D * D);
-      const int r2 = l - q * D * D;
-      di = r2 / D;
-      dj = r2 - di * D;
+    if (LAYOUT != FE_LAYOUT_PLANES) {
+      const int32_t deg = s_bptr[j + 1] - s_bptr[j];
+      const int32_t l = (int32_t)(o - (int64_t)s_bptr[j] * D * D);
+      int q;
+      if (LAYOUT == FE_LAYOUT_CSR) {
+        di = l / (deg * D);
+        const int rem = l - di * deg * D;
+        q = rem / D;
+        dj = rem - q * D;
+      } else {
+        q = l / (D * D);
+        const int r2 = l - q * D * D;
+        di = r2 / D;
+        dj = r2 - di * D;
+      }
+      bq = s_bptr[j] + q;
     }
-    const int32_t bq = s_bptr[j] + q;
-    const int32_t c0 = g.bseg_ptr[bq], c1 = g.bseg_ptr[bq + 1];
+    const int32_t c0 = staged_bq ? s_bsp[bq - B0] : g.bseg_ptr[bq];
+    const int32_t c1 = staged_bq ? s_bsp[bq - B0 + 1] : g.bseg_ptr[bq + 1];
     const int32_t base = s_iptr[j] - I0;
     double sum = 0.0;
     for (int32_t c = c0; c < c1; ++c) {
-      const int v = g.bseg[c];
+      const int v = staged_seg ? s_bseg[c - c_lo] : g.bseg[c];
       const int kl = v >> 2, b = v & 3;
       sum += S[((b * D + di) * D + dj) * stride + base + kl];
     }
@@ -217,40 +251,50 @@ __global__ __launch_bounds__(kFusedBlock) void k_assemble_fused(fe_mesh m, fe_gr
   }
 }
 
+template <int KIND, int WHAT, int LAYOUT>
+int launch_fused_nt(fe_handle* h, const fe_mesh* m, const fe_graph* g, double* vals, size_t lds) {
+  dim3 grid((unsigned)g->n_blk);
+  const int nt = g->threads > 0 ? g->threads : 256;
+#define FE_LAUNCH_NT(NT)                                                                        \
+  do {                                                                                          \
+    auto kern = k_assemble_fused<KIND, WHAT, LAYOUT, NT>;                                       \
+    if (lds > 64 * 1024)                                                                        \
+      FE_CHECK_HIP(hipFuncSetAttribute((const void*)kern,                                       \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));  \
+    hipLaunchKernelGGL(kern, grid, dim3(NT), lds, h->stream, *m, *g, vals);                     \
+  } while (0)
+  if (nt == 64)
+    FE_LAUNCH_NT(64);
+  else if (nt == 128)
+    FE_LAUNCH_NT(128);
+  else if (nt == 256)
+    FE_LAUNCH_NT(256);
+  else
+    return fe_set_error(FE_ERR_ARG, "fe_assemble_fused: threads must be 64, 128 or 256");
+#undef FE_LAUNCH_NT
+  FE_CHECK_LAUNCH();
+  return FE_OK;
+}
+
 template <int KIND>
 int launch_fused(fe_handle* h, const fe_mesh* m, const fe_graph* g, int what, int layout,
                  double* vals) {
   constexpr int NPE = KindTraits<KIND>::NPE, D = KindTraits<KIND>::D;
   const size_t lds = (size_t)NPE * D * D * g->max_inc * sizeof(double);
-  if (lds > 150 * 1024)
+  if (lds > 140 * 1024)
     return fe_set_error(FE_ERR_ARG, "fe_assemble_fused: schedule needs %zu bytes of LDS", lds);
-  dim3 grid((unsigned)g->n_blk), block(kFusedBlock);
-#define FE_LAUNCH_FUSED(W, L)                                                              \
-  do {                                                                                     \
-    auto kern = k_assemble_fused<KIND, W, L>;                                              \
-    if (lds > 64 * 1024)                                                                   \
-      FE_CHECK_HIP(hipFuncSetAttribute((const void*)kern,                                  \
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
-    hipLaunchKernelGGL(kern, grid, block, lds, h->stream, *m, *g, vals);                   \
-  } while (0)
-  if (what == FE_STIFFNESS) {
-    if (layout == FE_LAYOUT_CSR)
-      FE_LAUNCH_FUSED(FE_STIFFNESS, FE_LAYOUT_CSR);
-    else
-      FE_LAUNCH_FUSED(FE_STIFFNESS, FE_LAYOUT_BSR);
-  } else {
-    if constexpr (KIND == FE_KIND_TRI_MAGNETIC) {
-      return fe_set_error(FE_ERR_UNSUPPORTED, "fe_assemble_fused: magnetic elements have no mass matrix");
-    } else {
-      if (layout == FE_LAYOUT_CSR)
-        FE_LAUNCH_FUSED(FE_MASS, FE_LAYOUT_CSR);
-      else
-        FE_LAUNCH_FUSED(FE_MASS, FE_LAYOUT_BSR);
+  if (what == FE_MASS && KIND == FE_KIND_TRI_MAGNETIC)
+    return fe_set_error(FE_ERR_UNSUPPORTED, "fe_assemble_fused: magnetic elements have no mass matrix");
+  if constexpr (KIND != FE_KIND_TRI_MAGNETIC) {
+    if (what == FE_MASS) {
+      if (layout == FE_LAYOUT_CSR) return launch_fused_nt<KIND, FE_MASS, FE_LAYOUT_CSR>(h, m, g, vals, lds);
+      if (layout == FE_LAYOUT_BSR) return launch_fused_nt<KIND, FE_MASS, FE_LAYOUT_BSR>(h, m, g, vals, lds);
+      return launch_fused_nt<KIND, FE_MASS, FE_LAYOUT_PLANES>(h, m, g, vals, lds);
     }
   }
-#undef FE_LAUNCH_FUSED
-  FE_CHECK_LAUNCH();
-  return FE_OK;
+  if (layout == FE_LAYOUT_CSR) return launch_fused_nt<KIND, FE_STIFFNESS, FE_LAYOUT_CSR>(h, m, g, vals, lds);
+  if (layout == FE_LAYOUT_BSR) return launch_fused_nt<KIND, FE_STIFFNESS, FE_LAYOUT_BSR>(h, m, g, vals, lds);
+  return launch_fused_nt<KIND, FE_STIFFNESS, FE_LAYOUT_PLANES>(h, m, g, vals, lds);
 }
 
 // ---------------------------------------------------------------------------------------------------
@@ -278,8 +322,8 @@ __global__ void k_relayout(const int32_t* __restrict__ bptr, int64_t nn, int to_
 // ---------------------------------------------------------------------------------------------------
 // Dirichlet elimination on BSR values: entries in a fixed row or fixed column become identity.
 // ---------------------------------------------------------------------------------------------------
-template <int D>
-__global__ void k_dirichlet_bsr(int64_t nn, const int32_t* __restrict__ bptr,
+template <int D, int LAYOUT>
+__global__ void k_dirichlet_bsr(int64_t nn, int64_t nnzb, const int32_t* __restrict__ bptr,
                                 const int32_t* __restrict__ bcol, const uint8_t* __restrict__ fixed,
                                 double* __restrict__ vals) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -289,7 +333,9 @@ __global__ void k_dirichlet_bsr(int64_t nn, const int32_t* __restrict__ bptr,
     for (int di = 0; di < D; ++di)
       for (int dj = 0; dj < D; ++dj) {
         const int64_t r = i * D + di, c = j * D + dj;
-        if (fixed[r] || fixed[c]) vals[(int64_t)q * D * D + di * D + dj] = (r == c) ? 1.0 : 0.0;
+        const int64_t at = LAYOUT == FE_LAYOUT_PLANES ? (int64_t)(di * D + dj) * nnzb + q
+                                                      : (int64_t)q * D * D + di * D + dj;
+        if (fixed[r] || fixed[c]) vals[at] = (r == c) ? 1.0 : 0.0;
       }
   }
 }
@@ -321,7 +367,8 @@ extern "C" int fe_assemble_fused(fe_handle* h, const fe_mesh* m, const fe_graph*
   FE_ENTER(h);
   FE_REQUIRE(m && g && vals, "fe_assemble_fused: null argument");
   FE_REQUIRE(what == FE_STIFFNESS || what == FE_MASS, "fe_assemble_fused: bad 'what'");
-  FE_REQUIRE(layout == FE_LAYOUT_CSR || layout == FE_LAYOUT_BSR, "fe_assemble_fused: bad layout");
+  FE_REQUIRE(layout == FE_LAYOUT_CSR || layout == FE_LAYOUT_BSR || layout == FE_LAYOUT_PLANES,
+             "fe_assemble_fused: bad layout");
   FE_REQUIRE(g->n_blk >= 0 && g->max_inc >= 0, "fe_assemble_fused: bad schedule");
   if (g->n_blk == 0 || g->nnzb == 0) return FE_OK;
   FE_REQUIRE(m->conn && m->x && m->y && g->bptr && g->inc_ptr && g->inc && g->bseg_ptr && g->bseg &&
@@ -372,19 +419,26 @@ extern "C" int fe_apply_dirichlet_bsr(fe_handle* h, const fe_bsr* A, const uint8
   FE_ENTER(h);
   FE_REQUIRE(A && fixed && vals, "fe_apply_dirichlet_bsr: null argument");
   FE_REQUIRE(A->precision == 0, "fe_apply_dirichlet_bsr: FP64 matrix expected");
+  FE_REQUIRE(A->dof >= 1 && A->dof <= 3, "fe_apply_dirichlet_bsr: dof must be 1..3");
+  FE_REQUIRE(A->layout == FE_LAYOUT_BSR || A->layout == FE_LAYOUT_PLANES,
+             "fe_apply_dirichlet_bsr: layout must be FE_LAYOUT_BSR or FE_LAYOUT_PLANES");
   if (A->n_brows == 0) return FE_OK;
   dim3 grid((unsigned)ceil_div(A->n_brows, 256)), block(256);
-  if (A->dof == 1)
-    hipLaunchKernelGGL(k_dirichlet_bsr<1>, grid, block, 0, h->stream, A->n_brows, A->bptr, A->bcol,
-                       fixed, vals);
-  else if (A->dof == 2)
-    hipLaunchKernelGGL(k_dirichlet_bsr<2>, grid, block, 0, h->stream, A->n_brows, A->bptr, A->bcol,
-                       fixed, vals);
-  else if (A->dof == 3)
-    hipLaunchKernelGGL(k_dirichlet_bsr<3>, grid, block, 0, h->stream, A->n_brows, A->bptr, A->bcol,
-                       fixed, vals);
-  else
-    return fe_set_error(FE_ERR_ARG, "fe_apply_dirichlet_bsr: dof must be 1..3");
+#define FE_DIR_LAUNCH(D)                                                                          \
+  if (A->layout == FE_LAYOUT_PLANES)                                                              \
+    hipLaunchKernelGGL((k_dirichlet_bsr<D, FE_LAYOUT_PLANES>), grid, block, 0, h->stream,         \
+                       A->n_brows, A->nnzb, A->bptr, A->bcol, fixed, vals);                       \
+  else                                                                                            \
+    hipLaunchKernelGGL((k_dirichlet_bsr<D, FE_LAYOUT_BSR>), grid, block, 0, h->stream, A->n_brows, \
+                       A->nnzb, A->bptr, A->bcol, fixed, vals);
+  if (A->dof == 1) {
+    FE_DIR_LAUNCH(1)
+  } else if (A->dof == 2) {
+    FE_DIR_LAUNCH(2)
+  } else {
+    FE_DIR_LAUNCH(3)
+  }
+#undef FE_DIR_LAUNCH
   FE_CHECK_LAUNCH();
   return FE_OK;
 }

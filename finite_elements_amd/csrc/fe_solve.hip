@@ -16,6 +16,12 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// Value index of entry k (= di*D + dj) of block q.
+template <int D, int LAYOUT>
+__device__ __host__ inline int64_t vix(int64_t q, int k, int64_t nnzb) {
+  return LAYOUT == FE_LAYOUT_PLANES ? (int64_t)k * nnzb + q : q * D * D + k;
+}
+
 // --------------------------------------------------------------------------------------------------
 // reductions
 // --------------------------------------------------------------------------------------------------
@@ -82,23 +88,22 @@ __device__ inline bool grid_sum(double (&v)[NV], double* partials, unsigned* cou
 // --------------------------------------------------------------------------------------------------
 // SpMV on node-block rows.  G lanes per block row; each lane walks the row's blocks with stride G.
 // --------------------------------------------------------------------------------------------------
-template <typename T, int D, int G>
+template <typename T, int D, int G, int LAYOUT>
 __device__ inline void bsr_row(const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
-                               const T* __restrict__ vals, const T* __restrict__ x, int64_t i,
-                               int lane, T (&acc)[D]) {
+                               const T* __restrict__ vals, int64_t nnzb, const T* __restrict__ x,
+                               int64_t i, int lane, T (&acc)[D]) {
 #pragma unroll
   for (int d = 0; d < D; ++d) acc[d] = T(0);
   const int32_t s = bptr[i], e = bptr[i + 1];
   for (int32_t q = s + lane; q < e; q += G) {
     const int64_t j = bcol[q];
-    const T* v = vals + (int64_t)q * D * D;
     T xj[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) xj[d] = x[j * D + d];
 #pragma unroll
     for (int di = 0; di < D; ++di)
 #pragma unroll
-      for (int dj = 0; dj < D; ++dj) acc[di] += v[di * D + dj] * xj[dj];
+      for (int dj = 0; dj < D; ++dj) acc[di] += vals[vix<D, LAYOUT>(q, di * D + dj, nnzb)] * xj[dj];
   }
 #pragma unroll
   for (int off = G / 2; off > 0; off >>= 1)
@@ -106,8 +111,18 @@ __device__ inline void bsr_row(const int32_t* __restrict__ bptr, const int32_t* 
     for (int d = 0; d < D; ++d) acc[d] += __shfl_xor(acc[d], off, G);
 }
 
-template <typename T, int D, int G>
-__global__ __launch_bounds__(kBlock) void k_spmv_bsr(int64_t nb, const int32_t* __restrict__ bptr,
+template <typename T, int D>
+__device__ inline T pick(const T (&acc)[D], int lane) {
+  T out = acc[0];
+#pragma unroll
+  for (int d = 1; d < D; ++d)
+    if (lane == d) out = acc[d];
+  return out;
+}
+
+template <typename T, int D, int G, int LAYOUT>
+__global__ __launch_bounds__(kBlock) void k_spmv_bsr(int64_t nb, int64_t nnzb,
+                                                     const int32_t* __restrict__ bptr,
                                                      const int32_t* __restrict__ bcol,
                                                      const T* __restrict__ vals,
                                                      const T* __restrict__ x, T* __restrict__ y) {
@@ -116,14 +131,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_bsr(int64_t nb, const int32_t* 
   const int lane = (int)(gid % G);
   if (i >= nb) return;
   T acc[D];
-  bsr_row<T, D, G>(bptr, bcol, vals, x, i, lane, acc);
-  if (lane < D) {
-    T out = acc[0];
-#pragma unroll
-    for (int d = 1; d < D; ++d)
-      if (lane == d) out = acc[d];
-    y[i * D + lane] = out;
-  }
+  bsr_row<T, D, G, LAYOUT>(bptr, bcol, vals, nnzb, x, i, lane, acc);
+  if (lane < D) y[i * D + lane] = pick<T, D>(acc, lane);
 }
 
 template <int G>
@@ -149,26 +158,41 @@ constexpr int lanes_for() {
 }
 
 // --------------------------------------------------------------------------------------------------
-// preconditioner setup: inverse diagonal / inverse diagonal blocks (FP64)
+// value conversion between precisions and layouts
 // --------------------------------------------------------------------------------------------------
-template <int D>
-__device__ inline const double* diag_block(const int32_t* bptr, const int32_t* bcol,
-                                           const double* vals, int64_t i) {
-  for (int32_t q = bptr[i]; q < bptr[i + 1]; ++q)
-    if (bcol[q] == i) return vals + (int64_t)q * D * D;
-  return nullptr;
+template <typename S, typename T, int D, int LS, int LT>
+__global__ void k_bsr_convert(int64_t nnzb, const S* __restrict__ src, T* __restrict__ dst) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nnzb * D * D) return;
+  // enumerate in destination order so that the stores are coalesced
+  int64_t q;
+  int k;
+  if (LT == FE_LAYOUT_PLANES) {
+    k = (int)(t / nnzb);
+    q = t - (int64_t)k * nnzb;
+  } else {
+    q = t / (D * D);
+    k = (int)(t - q * D * D);
+  }
+  dst[vix<D, LT>(q, k, nnzb)] = (T)src[vix<D, LS>(q, k, nnzb)];
 }
 
-template <int D>
-__global__ void k_precond_setup(int64_t nb, const int32_t* __restrict__ bptr,
+// --------------------------------------------------------------------------------------------------
+// preconditioner setup: inverse diagonal / inverse diagonal blocks (FP64)
+// --------------------------------------------------------------------------------------------------
+template <int D, int LAYOUT>
+__global__ void k_precond_setup(int64_t nb, int64_t nnzb, const int32_t* __restrict__ bptr,
                                 const int32_t* __restrict__ bcol, const double* __restrict__ vals,
                                 int kind, double* __restrict__ out) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= nb) return;
-  const double* a = diag_block<D>(bptr, bcol, vals, i);
+  int64_t qd = -1;
+  for (int32_t q = bptr[i]; q < bptr[i + 1]; ++q)
+    if (bcol[q] == i) qd = q;
   double blk[D * D];
 #pragma unroll
-  for (int k = 0; k < D * D; ++k) blk[k] = a ? a[k] : ((k % (D + 1)) == 0 ? 1.0 : 0.0);
+  for (int k = 0; k < D * D; ++k)
+    blk[k] = qd >= 0 ? vals[vix<D, LAYOUT>(qd, k, nnzb)] : ((k % (D + 1)) == 0 ? 1.0 : 0.0);
   if (kind == FE_PC_JACOBI) {
 #pragma unroll
     for (int d = 0; d < D; ++d) {
@@ -214,21 +238,26 @@ __global__ void k_precond_setup(int64_t nb, const int32_t* __restrict__ bptr,
 struct PcgState {
   double rz, rz_prev, alpha, beta;
   double rr, rr_ref, rr_true, bb;
-  double tol2bb, ru_delta2;
+  double tol2bb, ru_delta2, rr_best, pad0;
   int32_t iter, max_iter;
-  int32_t done;     // 0 running, 1 converged, 2 iteration limit, 3 breakdown
+  int32_t done;     // 0 running, 1 converged, 2 iteration limit / stagnation, 3 breakdown
   int32_t need_ru;  // 1: reliable update / true-residual check requested
   int32_t n_ru;
   int32_t first;
   int32_t mixed;
-  int32_t pad;
+  int32_t stall;    // reliable updates since the true residual last improved
+  int32_t restarts;
+  int32_t pad1;
   unsigned counter[4];
 };
 
 struct PcgStatus {
   int32_t done, iter, n_ru, need_ru;
-  double rr, rr_true, bb, pad;
+  double rr, rr_true, bb;
+  int32_t restarts, pad;
 };
+
+constexpr int kStallLimit = 10;
 
 // Contiguous node range of logical workgroup v (XCD-aware ids).
 __device__ inline void block_range(int64_t nb, int64_t& lo, int64_t& hi) {
@@ -239,7 +268,8 @@ __device__ inline void block_range(int64_t nb, int64_t& lo, int64_t& hi) {
 }
 
 template <typename T, int D, int G>
-__global__ __launch_bounds__(kBlock) void k_pcg_spmv(int64_t nb, const int32_t* __restrict__ bptr,
+__global__ __launch_bounds__(kBlock) void k_pcg_spmv(int64_t nb, int64_t nnzb,
+                                                     const int32_t* __restrict__ bptr,
                                                      const int32_t* __restrict__ bcol,
                                                      const T* __restrict__ vals,
                                                      const T* __restrict__ z, T* p0, T* p1,
@@ -269,24 +299,21 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int64_t nb, const int32_t* 
     const int32_t s = bptr[i], e = bptr[i + 1];
     for (int32_t k = s + lane; k < e; k += G) {
       const int64_t j = bcol[k];
-      const T* v = vals + (int64_t)k * D * D;
       T pj[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) pj[d] = z[j * D + d] + beta * p_old[j * D + d];
 #pragma unroll
       for (int di = 0; di < D; ++di)
 #pragma unroll
-        for (int dj = 0; dj < D; ++dj) acc[di] += v[di * D + dj] * pj[dj];
+        for (int dj = 0; dj < D; ++dj)
+          acc[di] += vals[vix<D, FE_LAYOUT_PLANES>(k, di * D + dj, nnzb)] * pj[dj];
     }
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1)
 #pragma unroll
       for (int d = 0; d < D; ++d) acc[d] += __shfl_xor(acc[d], off, G);
     if (lane < D) {
-      T qi = acc[0];
-#pragma unroll
-      for (int d = 1; d < D; ++d)
-        if (lane == d) qi = acc[d];
+      const T qi = pick<T, D>(acc, lane);
       const int64_t r = i * D + lane;
       const T pi = z[r] + beta * p_old[r];
       p_new[r] = pi;
@@ -383,7 +410,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t nb, double* __res
 
 // Reliable update, part 1: r64 = b - A64 x, partial ||r64||^2.
 template <int D, int G>
-__global__ __launch_bounds__(kBlock) void k_ru_residual(int64_t nb, const int32_t* __restrict__ bptr,
+__global__ __launch_bounds__(kBlock) void k_ru_residual(int64_t nb, int64_t nnzb,
+                                                        const int32_t* __restrict__ bptr,
                                                         const int32_t* __restrict__ bcol,
                                                         const double* __restrict__ vals,
                                                         const double* __restrict__ b,
@@ -400,14 +428,10 @@ __global__ __launch_bounds__(kBlock) void k_ru_residual(int64_t nb, const int32_
   double rr = 0.0;
   for (int64_t i = lo + grp; i < hi; i += kBlock / G) {
     double acc[D];
-    bsr_row<double, D, G>(bptr, bcol, vals, x, i, lane, acc);
+    bsr_row<double, D, G, FE_LAYOUT_PLANES>(bptr, bcol, vals, nnzb, x, i, lane, acc);
     if (lane < D) {
-      double ai = acc[0];
-#pragma unroll
-      for (int d = 1; d < D; ++d)
-        if (lane == d) ai = acc[d];
       const int64_t k = i * D + lane;
-      const double ri = b[k] - ai;
+      const double ri = b[k] - pick<double, D>(acc, lane);
       r64[k] = ri;
       rr += ri * ri;
     }
@@ -417,7 +441,8 @@ __global__ __launch_bounds__(kBlock) void k_ru_residual(int64_t nb, const int32_
 }
 
 // Reliable update, part 2: convergence decision on the true residual; otherwise r = r64,
-// z = M^-1 r, rz = r.z, beta = rz / rz_prev (0 on the first update).
+// z = M^-1 r, rz = r.z, beta = rz / rz_prev -- or beta = 0 (restart from the true residual) on the
+// first update and whenever the recursive residual had drifted far below the true one.
 template <typename T, int D, int PC>
 __global__ __launch_bounds__(kBlock) void k_ru_finish(int64_t nb, const double* __restrict__ r64,
                                                       T* __restrict__ r, T* __restrict__ z,
@@ -426,13 +451,7 @@ __global__ __launch_bounds__(kBlock) void k_ru_finish(int64_t nb, const double* 
   __shared__ int s_skip;
   if (threadIdx.x == 0) {
     int skip = st->done || !st->need_ru;
-    if (!skip) {
-      if (st->rr_true <= st->tol2bb) {
-        skip = 1;
-      } else if (st->iter >= st->max_iter) {
-        skip = 1;
-      }
-    }
+    if (!skip && (st->rr_true <= st->tol2bb || st->iter >= st->max_iter)) skip = 1;
     s_skip = skip;
   }
   __syncthreads();
@@ -471,11 +490,19 @@ __global__ __launch_bounds__(kBlock) void k_ru_finish(int64_t nb, const double* 
       st->done = 3;
       return;
     }
-    if (st->first) {
+    const bool drifted = !st->first && st->rr_true > 4.0 * st->rr;
+    if (st->first || drifted) {
       st->beta = 0.0;
+      if (drifted) st->restarts += 1;
       st->first = 0;
     } else {
       st->beta = rz_new / st->rz_prev;
+    }
+    if (st->rr_true < 0.81 * st->rr_best) {
+      st->rr_best = st->rr_true;
+      st->stall = 0;
+    } else if (++st->stall >= kStallLimit) {
+      st->done = 2;
     }
     st->rz = rz_new;
     st->rr = st->rr_true;
@@ -506,6 +533,9 @@ __global__ void k_pcg_init(int64_t n, const double* __restrict__ b, PcgState* st
     st->rz = 1.0;
     st->rz_prev = 1.0;
     st->beta = 0.0;
+    st->rr_best = 1.0e308;
+    st->stall = 0;
+    st->restarts = 0;
   }
 }
 
@@ -518,6 +548,7 @@ __global__ void k_pcg_status(const PcgState* st, PcgStatus* out) {
     out->rr = st->rr;
     out->rr_true = st->rr_true;
     out->bb = st->bb;
+    out->restarts = st->restarts;
   }
 }
 
@@ -574,7 +605,7 @@ template <typename T, int D, int PC>
 int run_pcg(fe_handle* h, const fe_bsr* A64, const fe_bsr* Ait, const T* M, const double* b,
             double* x, const Work& w, int grid, const fe_pcg_opts* o, fe_pcg_stats* s) {
   constexpr int G = lanes_for<D>();
-  const int64_t nb = A64->n_brows, n = nb * D;
+  const int64_t nb = A64->n_brows, n = nb * D, nnzb = A64->nnzb;
   hipStream_t st = h->stream;
   FE_CHECK_HIP(hipMemsetAsync(w.st, 0, sizeof(PcgState), st));
   FE_CHECK_HIP(hipMemsetAsync(w.p0, 0, n * sizeof(T), st));
@@ -591,8 +622,8 @@ int run_pcg(fe_handle* h, const fe_bsr* A64, const fe_bsr* Ait, const T* M, cons
   T* q = (T*)w.q;
   T* pb[2] = {(T*)w.p0, (T*)w.p1};
   auto launch_ru = [&]() -> int {
-    hipLaunchKernelGGL((k_ru_residual<D, G>), dim3(grid), dim3(kBlock), 0, st, nb, bptr, bcol, v64, b,
-                       x, w.r64, w.st, w.partials);
+    hipLaunchKernelGGL((k_ru_residual<D, G>), dim3(grid), dim3(kBlock), 0, st, nb, nnzb, bptr, bcol,
+                       v64, b, x, w.r64, w.st, w.partials);
     hipLaunchKernelGGL((k_ru_finish<T, D, PC>), dim3(grid), dim3(kBlock), 0, st, nb, w.r64, r, z, M,
                        w.st, w.partials);
     FE_CHECK_LAUNCH();
@@ -613,8 +644,8 @@ int run_pcg(fe_handle* h, const fe_bsr* A64, const fe_bsr* Ait, const T* M, cons
   auto enqueue_chunk = [&](int slot) -> int {
     for (int it = 0; it < chunk; ++it) {
       if (prof && it == 0) FE_CHECK_HIP(hipEventRecord(pev[slot][0], st));
-      hipLaunchKernelGGL((k_pcg_spmv<T, D, G>), dim3(grid), dim3(kBlock), 0, st, nb, bptr, bcol, vit,
-                         z, pb[0], pb[1], q, w.st, w.partials);
+      hipLaunchKernelGGL((k_pcg_spmv<T, D, G>), dim3(grid), dim3(kBlock), 0, st, nb, nnzb, bptr, bcol,
+                         vit, z, pb[0], pb[1], q, w.st, w.partials);
       if (prof && it == 0) FE_CHECK_HIP(hipEventRecord(pev[slot][1], st));
       hipLaunchKernelGGL((k_pcg_update<T, D, PC>), dim3(grid), dim3(kBlock), 0, st, nb, x, pb[0], pb[1],
                          q, r, z, M, w.st, w.partials);
@@ -672,6 +703,7 @@ int run_pcg(fe_handle* h, const fe_bsr* A64, const fe_bsr* Ait, const T* M, cons
   }
   s->iterations = last.iter;
   s->reliable_updates = last.n_ru;
+  s->restarts = last.restarts;
   s->bnorm = sqrt(last.bb);
   s->relres = last.bb > 0 ? sqrt(last.rr_true / last.bb) : 0.0;
   s->relres_rec = last.bb > 0 ? sqrt(last.rr / last.bb) : 0.0;
@@ -703,32 +735,82 @@ int dispatch_dof(fe_handle* h, const fe_bsr* A64, const fe_bsr* Ait, const void*
   }
 }
 
-template <typename T, int D>
+template <typename T, int D, int LAYOUT>
 void launch_spmv(fe_handle* h, const fe_bsr* A, const void* x, void* y) {
   constexpr int G = lanes_for<D>();
   const int64_t threads = A->n_brows * G;
-  hipLaunchKernelGGL((k_spmv_bsr<T, D, G>), dim3((unsigned)ceil_div(threads, kBlock)), dim3(kBlock), 0,
-                     h->stream, A->n_brows, A->bptr, A->bcol, (const T*)A->vals, (const T*)x, (T*)y);
+  hipLaunchKernelGGL((k_spmv_bsr<T, D, G, LAYOUT>), dim3((unsigned)ceil_div(threads, kBlock)),
+                     dim3(kBlock), 0, h->stream, A->n_brows, A->nnzb, A->bptr, A->bcol,
+                     (const T*)A->vals, (const T*)x, (T*)y);
+}
+
+template <typename T, int D>
+void launch_spmv_l(fe_handle* h, const fe_bsr* A, const void* x, void* y) {
+  if (A->layout == FE_LAYOUT_PLANES)
+    launch_spmv<T, D, FE_LAYOUT_PLANES>(h, A, x, y);
+  else
+    launch_spmv<T, D, FE_LAYOUT_BSR>(h, A, x, y);
+}
+
+int check_bsr(const fe_bsr* A, const char* who) {
+  if (!A) return fe_set_error(FE_ERR_ARG, "%s: null matrix", who);
+  if (A->dof < 1 || A->dof > 3) return fe_set_error(FE_ERR_ARG, "%s: dof must be 1..3", who);
+  if (A->precision != 0 && A->precision != 1)
+    return fe_set_error(FE_ERR_ARG, "%s: precision must be 0 (FP64) or 1 (FP32)", who);
+  if (A->layout != FE_LAYOUT_BSR && A->layout != FE_LAYOUT_PLANES)
+    return fe_set_error(FE_ERR_ARG, "%s: layout must be FE_LAYOUT_BSR or FE_LAYOUT_PLANES", who);
+  if (A->n_brows < 0 || A->nnzb < 0) return fe_set_error(FE_ERR_ARG, "%s: negative sizes", who);
+  if (A->n_brows > 0 && (!A->bptr || !A->bcol || !A->vals))
+    return fe_set_error(FE_ERR_ARG, "%s: null matrix array", who);
+  return FE_OK;
+}
+
+template <typename S, typename T, int D>
+void launch_convert(fe_handle* h, const fe_bsr* A, const fe_bsr* B) {
+  const int64_t n = A->nnzb * D * D;
+  dim3 grid((unsigned)ceil_div(n, 256)), block(256);
+  const S* src = (const S*)A->vals;
+  T* dst = (T*)B->vals;
+  if (A->layout == FE_LAYOUT_PLANES) {
+    if (B->layout == FE_LAYOUT_PLANES)
+      hipLaunchKernelGGL((k_bsr_convert<S, T, D, FE_LAYOUT_PLANES, FE_LAYOUT_PLANES>), grid, block, 0,
+                         h->stream, A->nnzb, src, dst);
+    else
+      hipLaunchKernelGGL((k_bsr_convert<S, T, D, FE_LAYOUT_PLANES, FE_LAYOUT_BSR>), grid, block, 0,
+                         h->stream, A->nnzb, src, dst);
+  } else {
+    if (B->layout == FE_LAYOUT_PLANES)
+      hipLaunchKernelGGL((k_bsr_convert<S, T, D, FE_LAYOUT_BSR, FE_LAYOUT_PLANES>), grid, block, 0,
+                         h->stream, A->nnzb, src, dst);
+    else
+      hipLaunchKernelGGL((k_bsr_convert<S, T, D, FE_LAYOUT_BSR, FE_LAYOUT_BSR>), grid, block, 0,
+                         h->stream, A->nnzb, src, dst);
+  }
+}
+
+template <typename S, typename T>
+void launch_convert_d(fe_handle* h, const fe_bsr* A, const fe_bsr* B) {
+  if (A->dof == 1) launch_convert<S, T, 1>(h, A, B);
+  else if (A->dof == 2) launch_convert<S, T, 2>(h, A, B);
+  else launch_convert<S, T, 3>(h, A, B);
 }
 
 }  // namespace
 
 extern "C" int fe_spmv(fe_handle* h, const fe_bsr* A, const void* x, void* y) {
   FE_ENTER(h);
-  FE_REQUIRE(A && x && y, "fe_spmv: null argument");
-  FE_REQUIRE(A->dof >= 1 && A->dof <= 3, "fe_spmv: dof must be 1..3");
-  FE_REQUIRE(A->precision == 0 || A->precision == 1, "fe_spmv: precision must be 0 or 1");
+  int rc = check_bsr(A, "fe_spmv");
+  if (rc) return rc;
   if (A->n_brows == 0) return FE_OK;
-  FE_REQUIRE(A->bptr && A->bcol && A->vals, "fe_spmv: null matrix array");
-  FE_REQUIRE(x != y, "fe_spmv: x and y must not alias");
+  FE_REQUIRE(x && y && x != y, "fe_spmv: x and y must be distinct non-null vectors");
   if (A->precision == 0) {
-    if (A->dof == 1) launch_spmv<double, 1>(h, A, x, y);
-    else if (A->dof == 2) launch_spmv<double, 2>(h, A, x, y);
-    else launch_spmv<double, 3>(h, A, x, y);
+    if (A->dof == 1) launch_spmv_l<double, 1>(h, A, x, y);
+    else if (A->dof == 2) launch_spmv_l<double, 2>(h, A, x, y);
+    else launch_spmv_l<double, 3>(h, A, x, y);
   } else {
-    if (A->dof == 1) launch_spmv<float, 1>(h, A, x, y);
-    else if (A->dof == 2) launch_spmv<float, 2>(h, A, x, y);
-    else launch_spmv<float, 3>(h, A, x, y);
+    if (A->dof == 1) launch_spmv_l<float, 1>(h, A, x, y);
+    else if (A->dof == 2) launch_spmv_l<float, 2>(h, A, x, y);
+    else launch_spmv_l<float, 3>(h, A, x, y);
   }
   FE_CHECK_LAUNCH();
   return FE_OK;
@@ -747,30 +829,49 @@ extern "C" int fe_spmv_csr(fe_handle* h, int64_t n_rows, const int32_t* rowptr,
   return FE_OK;
 }
 
+extern "C" int fe_bsr_convert(fe_handle* h, const fe_bsr* src, const fe_bsr* dst) {
+  FE_ENTER(h);
+  int rc = check_bsr(src, "fe_bsr_convert(src)");
+  if (rc) return rc;
+  rc = check_bsr(dst, "fe_bsr_convert(dst)");
+  if (rc) return rc;
+  FE_REQUIRE(src->dof == dst->dof && src->nnzb == dst->nnzb && src->n_brows == dst->n_brows,
+             "fe_bsr_convert: src and dst must share the pattern");
+  FE_REQUIRE(src->vals != dst->vals, "fe_bsr_convert: must be out-of-place");
+  if (src->nnzb == 0) return FE_OK;
+  if (src->precision == 0 && dst->precision == 0) launch_convert_d<double, double>(h, src, dst);
+  else if (src->precision == 0) launch_convert_d<double, float>(h, src, dst);
+  else if (dst->precision == 0) launch_convert_d<float, double>(h, src, dst);
+  else launch_convert_d<float, float>(h, src, dst);
+  FE_CHECK_LAUNCH();
+  return FE_OK;
+}
+
 extern "C" int fe_precond_setup(fe_handle* h, const fe_bsr* A, int kind, double* out) {
   FE_ENTER(h);
-  FE_REQUIRE(A && out, "fe_precond_setup: null argument");
+  int rc = check_bsr(A, "fe_precond_setup");
+  if (rc) return rc;
+  FE_REQUIRE(out, "fe_precond_setup: null output");
   FE_REQUIRE(A->precision == 0, "fe_precond_setup: FP64 matrix expected");
   FE_REQUIRE(kind == FE_PC_JACOBI || kind == FE_PC_BLOCK_JACOBI, "fe_precond_setup: bad kind");
   if (A->n_brows == 0) return FE_OK;
   dim3 grid((unsigned)ceil_div(A->n_brows, 256)), block(256);
   const double* v = (const double*)A->vals;
-  switch (A->dof) {
-    case 1:
-      hipLaunchKernelGGL(k_precond_setup<1>, grid, block, 0, h->stream, A->n_brows, A->bptr, A->bcol, v,
-                         kind, out);
-      break;
-    case 2:
-      hipLaunchKernelGGL(k_precond_setup<2>, grid, block, 0, h->stream, A->n_brows, A->bptr, A->bcol, v,
-                         kind, out);
-      break;
-    case 3:
-      hipLaunchKernelGGL(k_precond_setup<3>, grid, block, 0, h->stream, A->n_brows, A->bptr, A->bcol, v,
-                         kind, out);
-      break;
-    default:
-      return fe_set_error(FE_ERR_ARG, "fe_precond_setup: dof must be 1..3");
+#define FE_PC_LAUNCH(D)                                                                           \
+  if (A->layout == FE_LAYOUT_PLANES)                                                              \
+    hipLaunchKernelGGL((k_precond_setup<D, FE_LAYOUT_PLANES>), grid, block, 0, h->stream,         \
+                       A->n_brows, A->nnzb, A->bptr, A->bcol, v, kind, out);                      \
+  else                                                                                            \
+    hipLaunchKernelGGL((k_precond_setup<D, FE_LAYOUT_BSR>), grid, block, 0, h->stream, A->n_brows, \
+                       A->nnzb, A->bptr, A->bcol, v, kind, out);
+  if (A->dof == 1) {
+    FE_PC_LAUNCH(1)
+  } else if (A->dof == 2) {
+    FE_PC_LAUNCH(2)
+  } else {
+    FE_PC_LAUNCH(3)
   }
+#undef FE_PC_LAUNCH
   FE_CHECK_LAUNCH();
   return FE_OK;
 }
@@ -789,15 +890,20 @@ extern "C" int fe_pcg(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const 
                       const void* M32, const double* b, double* x, void* work, int64_t work_bytes,
                       const fe_pcg_opts* o, fe_pcg_stats* s) {
   FE_ENTER(h);
-  FE_REQUIRE(A64 && b && x && work && o && s, "fe_pcg: null argument");
-  FE_REQUIRE(A64->precision == 0, "fe_pcg: A64 must be FP64");
-  FE_REQUIRE(A64->dof >= 1 && A64->dof <= 3, "fe_pcg: dof must be 1..3");
+  FE_REQUIRE(b && x && work && o && s, "fe_pcg: null argument");
+  int rc = check_bsr(A64, "fe_pcg(A64)");
+  if (rc) return rc;
+  FE_REQUIRE(A64->precision == 0 && A64->layout == FE_LAYOUT_PLANES,
+             "fe_pcg: A64 must be FP64 in FE_LAYOUT_PLANES");
   FE_REQUIRE(o->precond == FE_PC_JACOBI || o->precond == FE_PC_BLOCK_JACOBI, "fe_pcg: bad precond");
   FE_REQUIRE(o->tol > 0 && o->max_iter > 0, "fe_pcg: tol and max_iter must be positive");
   if (o->mixed) {
-    FE_REQUIRE(A32 && M32, "fe_pcg: mixed precision needs the FP32 matrix and preconditioner");
-    FE_REQUIRE(A32->precision == 1 && A32->dof == A64->dof && A32->n_brows == A64->n_brows,
-               "fe_pcg: A32 must be the FP32 copy of A64");
+    rc = check_bsr(A32, "fe_pcg(A32)");
+    if (rc) return rc;
+    FE_REQUIRE(M32, "fe_pcg: mixed precision needs the FP32 preconditioner");
+    FE_REQUIRE(A32->precision == 1 && A32->layout == FE_LAYOUT_PLANES && A32->dof == A64->dof &&
+                   A32->n_brows == A64->n_brows && A32->nnzb == A64->nnzb,
+               "fe_pcg: A32 must be the FP32 FE_LAYOUT_PLANES copy of A64");
   } else {
     FE_REQUIRE(M64, "fe_pcg: missing preconditioner");
   }

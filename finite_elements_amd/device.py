@@ -228,23 +228,27 @@ class DeviceGraph:
         self.bseg = ctx.tensor(pat.bseg.view(np.int16))
         self.set_schedule(pat.blk, pat.max_inc)
 
-    def set_schedule(self, blk: np.ndarray, max_inc: int):
+    def set_schedule(self, blk: np.ndarray, max_inc: int, threads: int = 256):
         self.blk = self.ctx.tensor(np.asarray(blk, dtype=np.int32))
         self.n_blk = len(blk) - 1
         self.max_inc = int(max_inc)
+        self.threads = int(threads)
 
     def struct(self) -> _hip.FeGraph:
-        return _hip.FeGraph(self.n_nodes, self.nnzb, int(self.inc.numel()), self.n_blk, self.max_inc, 0,
+        return _hip.FeGraph(self.n_nodes, self.nnzb, int(self.inc.numel()), self.n_blk, self.max_inc,
+                            self.threads,
                             _vp(self.bptr), _vp(self.bcol), _vp(self.inc_ptr), _vp(self.inc),
                             _vp(self.bseg_ptr), _vp(self.bseg), _vp(self.blk))
 
 
 @dataclass
 class BsrMatrix:
-    """Node-block matrix on the device; ``vals`` is [nnzb * dof * dof] in BSR layout."""
+    """Node-block matrix on the device: ``vals`` holds nnzb * dof * dof values in ``layout``
+    (FE_LAYOUT_BSR: [q][di][dj]; FE_LAYOUT_PLANES: [di*dof+dj][q], the solver's layout)."""
     graph: DeviceGraph
     vals: torch.Tensor
     dof: int
+    layout: int = _hip.FE_LAYOUT_BSR
 
     @property
     def precision(self):
@@ -255,21 +259,28 @@ class BsrMatrix:
         return self.graph.n_nodes * self.dof
 
     def struct(self) -> _hip.FeBsr:
-        return _hip.FeBsr(self.graph.n_nodes, self.dof, self.precision, _vp(self.graph.bptr),
-                          _vp(self.graph.bcol), _vp(self.vals))
+        return _hip.FeBsr(self.graph.n_nodes, self.graph.nnzb, self.dof, self.precision, self.layout, 0,
+                          _vp(self.graph.bptr), _vp(self.graph.bcol), _vp(self.vals))
+
+    def convert(self, layout: int, dtype=torch.float64) -> 'BsrMatrix':
+        out = BsrMatrix(self.graph, torch.empty(self.vals.numel(), dtype=dtype, device=self.vals.device),
+                        self.dof, layout)
+        a, b = self.struct(), out.struct()
+        _hip.call('fe_bsr_convert', self.graph.ctx.h, ctypes.byref(a), ctypes.byref(b))
+        return out
 
     def to_f32(self) -> 'BsrMatrix':
-        out = torch.empty(self.vals.numel(), dtype=torch.float32, device=self.vals.device)
-        cast_f64_to_f32(self.graph.ctx, self.vals, out)
-        return BsrMatrix(self.graph, out, self.dof)
+        return self.convert(self.layout, torch.float32)
 
     def clone(self) -> 'BsrMatrix':
-        return BsrMatrix(self.graph, self.vals.clone(), self.dof)
+        return BsrMatrix(self.graph, self.vals.clone(), self.dof, self.layout)
 
     def csr_values(self) -> torch.Tensor:
-        """Values permuted to the scalar-CSR layout of the pattern (device tensor)."""
-        out = torch.empty_like(self.vals)
-        relayout(self.graph.ctx, self.graph, self.dof, _hip.FE_LAYOUT_CSR, self.vals, out)
+        """Values permuted to the scalar-CSR layout of the pattern (device tensor, FP64)."""
+        src = self if (self.layout == _hip.FE_LAYOUT_BSR and self.precision == 0) else \
+            self.convert(_hip.FE_LAYOUT_BSR, torch.float64)
+        out = torch.empty_like(src.vals)
+        relayout(self.graph.ctx, self.graph, self.dof, _hip.FE_LAYOUT_CSR, src.vals, out)
         return out
 
 
@@ -386,13 +397,15 @@ class PcgSolver:
                  M64: Optional[torch.Tensor] = None, A32: Optional[BsrMatrix] = None,
                  check_every: int = 16, ru_delta: float = 0.1):
         self.ctx = A64.graph.ctx
+        if A64.layout != _hip.FE_LAYOUT_PLANES or A64.precision != 0:
+            A64 = A64.convert(_hip.FE_LAYOUT_PLANES, torch.float64)
         self.A64 = A64
         self.precond = precond
         self.mixed = bool(mixed)
         self.M64 = precond_setup(A64, 'jacobi' if precond == 'jacobi' else 'block_jacobi') \
             if M64 is None else M64
         if self.mixed:
-            self.A32 = A64.to_f32() if A32 is None else A32
+            self.A32 = A64.convert(_hip.FE_LAYOUT_PLANES, torch.float32) if A32 is None else A32
             self.M32 = self.M64.to(torch.float32)
         else:
             self.A32 = None
@@ -416,7 +429,7 @@ class PcgSolver:
         """Solve from initial guess x (zeros if None); returns (x, info dict)."""
         ctx = self.ctx
         if max_iter is None:
-            max_iter = max(1000, 10 * self.A64.n)
+            max_iter = max(2000, min(10 * self.A64.n, 50000))
         if x is None:
             x = torch.zeros_like(b)
         opts = self._opts(tol, max_iter, profile)
@@ -431,6 +444,7 @@ class PcgSolver:
                   _vp(self.M64), _vp(self.M32), _vp(b), _vp(x), _vp(self._work), self._work.numel(),
                   ctypes.byref(opts), ctypes.byref(stats))
         info = {'iterations': stats.iterations, 'reliable_updates': stats.reliable_updates,
+                'restarts': stats.restarts,
                 'status': stats.status, 'relres': stats.relres, 'relres_rec': stats.relres_rec,
                 'bnorm': stats.bnorm, 'spmv_ms': stats.spmv_ms, 'spmv_samples': stats.spmv_samples}
         return x, info

@@ -49,10 +49,42 @@ class FESystem:
         return self.n_nodes * self.dof
 
     # ---------------------------------------------------------------------------------------------
-    def set_schedule(self, max_inc: int):
-        """Re-tile the fused assembly (row-block schedule) for a given incidences-per-workgroup cap."""
+    def set_schedule(self, max_inc: int, threads: int = 256):
+        """Re-tile the fused assembly: row-block schedule with at most ``max_inc`` node-element
+        incidences per workgroup, ``threads`` lanes per workgroup (64, 128 or 256)."""
         blk, biggest = Pattern.schedule_for(self.pattern.inc_ptr, max_inc)
-        self.graph.set_schedule(blk, biggest)
+        self.graph.set_schedule(blk, biggest, threads)
+
+    def tune_assembly(self, candidates=None, reps: int = 5, layout: int = _hip.FE_LAYOUT_PLANES):
+        """Autotuner of the fused assembly: time every (threads, max_inc) candidate with HIP events
+        on the device stream and keep the fastest.  Returns (best, {candidate: ms})."""
+        if candidates is None:
+            base = int(np.diff(self.pattern.inc_ptr).max()) if self.n_nodes else 1
+            cands = []
+            for threads in (64, 128, 256):
+                for mi in (32, 48, 64, 96, 128, 192, 256):
+                    if mi >= base and mi <= 4 * threads:
+                        cands.append((threads, mi))
+            candidates = cands
+        out = self.stiffness(layout)
+        times = {}
+        for threads, mi in candidates:
+            try:
+                self.set_schedule(mi, threads)
+                self.stiffness(layout, out=out)
+            except (ValueError, RuntimeError):
+                continue
+            start = torch.cuda.Event(enable_timing=True)
+            stop = torch.cuda.Event(enable_timing=True)
+            start.record()
+            for _ in range(reps):
+                self.stiffness(layout, out=out)
+            stop.record()
+            stop.synchronize()
+            times[(threads, mi)] = start.elapsed_time(stop) / reps
+        best = min(times, key=times.get)
+        self.set_schedule(best[1], best[0])
+        return best, times
 
     def stiffness(self, layout: int = _hip.FE_LAYOUT_BSR, out=None) -> torch.Tensor:
         return device.assemble_fused(self.mesh, self.graph, 'stiffness', layout, out)
@@ -60,11 +92,11 @@ class FESystem:
     def mass(self, layout: int = _hip.FE_LAYOUT_BSR, out=None) -> torch.Tensor:
         return device.assemble_fused(self.mesh, self.graph, 'mass', layout, out)
 
-    def stiffness_bsr(self) -> BsrMatrix:
-        return BsrMatrix(self.graph, self.stiffness(_hip.FE_LAYOUT_BSR), self.dof)
+    def stiffness_bsr(self, layout: int = _hip.FE_LAYOUT_PLANES) -> BsrMatrix:
+        return BsrMatrix(self.graph, self.stiffness(layout), self.dof, layout)
 
-    def mass_bsr(self) -> BsrMatrix:
-        return BsrMatrix(self.graph, self.mass(_hip.FE_LAYOUT_BSR), self.dof)
+    def mass_bsr(self, layout: int = _hip.FE_LAYOUT_PLANES) -> BsrMatrix:
+        return BsrMatrix(self.graph, self.mass(layout), self.dof, layout)
 
     def element_matrices(self, what: str = 'stiffness', sym: bool = False) -> torch.Tensor:
         return device.element_matrices(self.mesh, what, sym)

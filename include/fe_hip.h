@@ -51,6 +51,7 @@ extern "C" {
 /* value layouts of an assembled matrix */
 #define FE_LAYOUT_CSR 0   /* scalar CSR, columns sorted, the reference's csr_matrix layout */
 #define FE_LAYOUT_BSR 1   /* node-block rows: vals[(bptr[i] + q) * dof * dof + di * dof + dj] */
+#define FE_LAYOUT_PLANES 2 /* node-block rows, one plane per block entry: vals[(di*dof + dj) * nnzb + q] */
 
 /* preconditioners */
 #define FE_PC_JACOBI 1        /* inverse diagonal, one value per DOF */
@@ -91,7 +92,7 @@ typedef struct {
   int64_t n_inc;
   int64_t n_blk;
   int32_t max_inc;  /* largest incidence count of one workgroup's node range */
-  int32_t pad;
+  int32_t threads;  /* workgroup size of the fused assembly: 64, 128 or 256 (0 = 256) */
   const int32_t* bptr;
   const int32_t* bcol;
   const int32_t* inc_ptr;
@@ -101,11 +102,16 @@ typedef struct {
   const int32_t* blk;
 } fe_graph;
 
-/* Block sparse matrix (node-block rows) on the device; precision 0 = FP64, 1 = FP32. */
+/* Block sparse matrix (node-block rows) on the device; precision 0 = FP64, 1 = FP32; layout
+   FE_LAYOUT_BSR or FE_LAYOUT_PLANES (the solver's layout: every value load is coalesced across the
+   lanes that walk a block row). */
 typedef struct {
   int64_t n_brows;
+  int64_t nnzb;
   int32_t dof;
   int32_t precision;
+  int32_t layout;
+  int32_t pad;
   const int32_t* bptr;
   const int32_t* bcol;
   const void* vals;
@@ -126,7 +132,7 @@ typedef struct {
   int32_t iterations;
   int32_t reliable_updates;
   int32_t status;        /* 0 converged, FE_ERR_NOT_CONVERGED, FE_ERR_BREAKDOWN */
-  int32_t pad;
+  int32_t restarts;      /* restarts from the true residual after recursive-residual drift */
   double relres;         /* true FP64 relative residual at exit */
   double relres_rec;     /* recursive residual estimate at exit */
   double bnorm;
@@ -223,6 +229,8 @@ int fe_spmv(fe_handle* h, const fe_bsr* A, const void* x, void* y);
 int fe_spmv_csr(fe_handle* h, int64_t n_rows, const int32_t* rowptr, const int32_t* colind,
                 const double* vals, const double* x, double* y);
 int fe_cast_f64_to_f32(fe_handle* h, int64_t n, const double* src, float* dst);
+/* Copy the values of src into dst->vals converting precision and layout (same pattern). */
+int fe_bsr_convert(fe_handle* h, const fe_bsr* src, const fe_bsr* dst);
 
 /* ---- solver ------------------------------------------------------------------------------------- */
 /* Preconditioner setup from the FP64 (Dirichlet-applied) matrix: FE_PC_JACOBI writes n values,
@@ -230,7 +238,9 @@ int fe_cast_f64_to_f32(fe_handle* h, int64_t n, const double* src, float* dst);
 int fe_precond_setup(fe_handle* h, const fe_bsr* A, int kind, double* out);
 /* PCG on the Dirichlet-applied SPD system (replaces spsolve at finite_elements/analysis.py:819-822).
    A64/M64 are FP64; with opts->mixed the FP32 copies A32/M32 run the iteration and FP64 reliable
-   updates recompute the true residual.  x is the initial guess on entry. */
+   updates recompute the true residual.  Matrices must be in FE_LAYOUT_PLANES.  x is the initial guess
+   on entry.  When the true residual disagrees with the recursive one the iteration restarts from the
+   true residual; ten reliable updates without progress end the solve as not converged. */
 int fe_pcg_workspace(fe_handle* h, const fe_bsr* A, const fe_pcg_opts* opts, int64_t* bytes);
 int fe_pcg(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M64, const void* M32,
            const double* b, double* x, void* work, int64_t work_bytes, const fe_pcg_opts* opts,
