@@ -102,6 +102,15 @@ SIGNATURES = {
     'fe_pcg_workspace': (_int, [_vp, _P(FeBsr), _P(FePcgOpts), _P(_i64)]),
     'fe_pcg': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp, _vp, _vp, _vp, _vp, _i64, _P(FePcgOpts),
                       _P(FePcgStats)]),
+    'fe_gnn_num_params': (_int, [_int]),
+    'fe_gnn_features': (_int, [_vp, _P(FeBsr), _vp, _vp, _vp]),
+    'fe_gnn_forward': (_int, [_vp, _i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'fe_gnn_loss_workspace': (_int, [_i64, _P(_i64)]),
+    'fe_gnn_loss_grad': (_int, [_vp, _P(FeBsr), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'fe_gnn_backward_workspace': (_int, [_i64, _P(_i64)]),
+    'fe_gnn_backward': (_int, [_vp, _i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'fe_gnn_adam': (_int, [_vp, _int, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                           ctypes.c_float, _int, ctypes.c_float]),
 }
 
 _lib = None

@@ -1,11 +1,8 @@
 // Sparse matrix-vector products, preconditioner setup and the device-driven PCG.
 //
-// PCG layout: two kernels per iteration, both with a fixed grid so that every global reduction has a
-// fixed number of per-workgroup partials that the last workgroup to finish sums in a fixed order
-// (run-to-run bitwise reproducible):
-//   k_pcg_spmv   : p_new = z + beta * p_old (formed on the fly for every gathered column),
-//                  q = A p_new, partial p_new.q  -> alpha = rz / pq
-//   k_pcg_update : x += alpha p, r -= alpha q, z = M^-1 r, partial r.z and r.r -> beta, convergence
+// PCG layout: two kernels per iteration (Chronopoulos-Gear recurrences, see below).  Every global
+// reduction runs on a fixed grid whose per-workgroup partials the last workgroup to finish sums in a
+// fixed order (run-to-run bitwise reproducible).
 // Every chunk of iterations ends with a conditional reliable update (true FP64 residual r = b - A x):
 // it runs when the recursive residual reached the tolerance (so convergence is always judged on the
 // true FP64 residual), when the iteration limit is hit, or (mixed precision) when the residual fell
@@ -233,10 +230,17 @@ __global__ void k_precond_setup(int64_t nb, int64_t nnzb, const int32_t* __restr
 }
 
 // --------------------------------------------------------------------------------------------------
-// PCG state and kernels
+// PCG (Chronopoulos-Gear form): one global reduction point per iteration.
+//   K_a  k_cg_vec  : p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s, u = M r
+//                    (streaming, one thread per DOF, coalesced; the node's r staged in LDS for M)
+//   K_b  k_cg_spmv : w = A u, gamma = r.u, delta = w.u, rr = r.r  -> alpha, beta for the next K_a
+// s = A p is carried by recurrence, so each iteration reads the matrix once and gathers one vector.
+// Reliable updates (k_ru_residual, k_ru_prep, k_ru_spmv): r = b - A64 x in FP64, u = M r, then one
+// matrix pass computes w = A u and s = A p together and the exact alpha of the continued direction
+// from (u,w), (u,s), (p,s) -- the Krylov direction is kept, not restarted.
 // --------------------------------------------------------------------------------------------------
 struct PcgState {
-  double rz, rz_prev, alpha, beta;
+  double alpha, beta, gamma, gamma_prev;
   double rr, rr_ref, rr_true, bb;
   double tol2bb, ru_delta2, rr_best, pad0;
   int32_t iter, max_iter;
@@ -267,144 +271,114 @@ __device__ inline void block_range(int64_t nb, int64_t& lo, int64_t& hi) {
   hi = min(nb, lo + per);
 }
 
-template <typename T, int D, int G>
-__global__ __launch_bounds__(kBlock) void k_pcg_spmv(int64_t nb, int64_t nnzb,
-                                                     const int32_t* __restrict__ bptr,
-                                                     const int32_t* __restrict__ bcol,
-                                                     const T* __restrict__ vals,
-                                                     const T* __restrict__ z, T* p0, T* p1,
-                                                     T* __restrict__ q, PcgState* st,
-                                                     double* partials) {
-  __shared__ int s_skip, s_sel;
-  __shared__ T s_beta;
+// u_k = (M r)_k for DOF k = node*D + d, with the node's D residuals in LDS (s_r) for block Jacobi.
+template <typename T, int D, int PC>
+__device__ inline T precond_dof(const T* __restrict__ M, int64_t node, int d, int64_t k,
+                                const T* s_rnode, T rk) {
+  if (PC == FE_PC_JACOBI) return M[k] * rk;
+  const T* m = M + node * D * D + d * D;
+  T s = T(0);
+#pragma unroll
+  for (int j = 0; j < D; ++j) s += m[j] * s_rnode[j];
+  return s;
+}
+
+constexpr int kVecNodes = 64;  // nodes per workgroup of the streaming kernels (64*D threads)
+
+template <typename T, int D, int PC>
+__global__ __launch_bounds__(kVecNodes * D) void k_cg_vec(int64_t nb, double* __restrict__ x,
+                                                          T* __restrict__ p, T* __restrict__ s,
+                                                          T* __restrict__ r, T* __restrict__ u,
+                                                          const T* __restrict__ w,
+                                                          const T* __restrict__ M,
+                                                          const PcgState* st) {
+  __shared__ T s_r[kVecNodes * D];
+  __shared__ int s_skip;
+  __shared__ double s_ab[2];
   if (threadIdx.x == 0) {
     s_skip = st->done || st->need_ru;
-    s_beta = (T)st->beta;
-    s_sel = st->iter & 1;
+    s_ab[0] = st->alpha;
+    s_ab[1] = st->beta;
   }
   __syncthreads();
   if (s_skip) return;
-  const T beta = s_beta;
-  // The direction of the last completed iteration lives in p[iter & 1].
-  const T* __restrict__ p_old = s_sel ? p1 : p0;
-  T* __restrict__ p_new = s_sel ? p0 : p1;
+  const double alpha = s_ab[0];
+  const T alpha_t = (T)alpha, beta = (T)s_ab[1];
+  const int64_t n = nb * D;
+  const int64_t k = (int64_t)blockIdx.x * kVecNodes * D + threadIdx.x;
+  T rk = T(0);
+  if (k < n) {
+    const T pk = u[k] + beta * p[k];
+    const T sk = w[k] + beta * s[k];
+    p[k] = pk;
+    s[k] = sk;
+    x[k] += alpha * (double)pk;
+    rk = r[k] - alpha_t * sk;
+    r[k] = rk;
+  }
+  s_r[threadIdx.x] = rk;
+  __syncthreads();
+  if (k < n) {
+    const int d = threadIdx.x % D;
+    u[k] = precond_dof<T, D, PC>(M, k / D, d, k, &s_r[threadIdx.x - d], rk);
+  }
+}
+
+template <typename T, int D, int G>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv(int64_t nb, int64_t nnzb,
+                                                    const int32_t* __restrict__ bptr,
+                                                    const int32_t* __restrict__ bcol,
+                                                    const T* __restrict__ vals,
+                                                    const T* __restrict__ u, const T* __restrict__ r,
+                                                    T* __restrict__ w, PcgState* st,
+                                                    double* partials) {
+  __shared__ int s_skip;
+  if (threadIdx.x == 0) s_skip = st->done || st->need_ru;
+  __syncthreads();
+  if (s_skip) return;
   int64_t lo, hi;
   block_range(nb, lo, hi);
   const int grp = threadIdx.x / G, lane = threadIdx.x % G;
-  double dot = 0.0;
+  double gam = 0.0, del = 0.0, rr = 0.0;
   for (int64_t i = lo + grp; i < hi; i += kBlock / G) {
     T acc[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) acc[d] = T(0);
-    const int32_t s = bptr[i], e = bptr[i + 1];
-    for (int32_t k = s + lane; k < e; k += G) {
-      const int64_t j = bcol[k];
-      T pj[D];
-#pragma unroll
-      for (int d = 0; d < D; ++d) pj[d] = z[j * D + d] + beta * p_old[j * D + d];
-#pragma unroll
-      for (int di = 0; di < D; ++di)
-#pragma unroll
-        for (int dj = 0; dj < D; ++dj)
-          acc[di] += vals[vix<D, FE_LAYOUT_PLANES>(k, di * D + dj, nnzb)] * pj[dj];
-    }
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1)
-#pragma unroll
-      for (int d = 0; d < D; ++d) acc[d] += __shfl_xor(acc[d], off, G);
+    bsr_row<T, D, G, FE_LAYOUT_PLANES>(bptr, bcol, vals, nnzb, u, i, lane, acc);
     if (lane < D) {
-      const T qi = pick<T, D>(acc, lane);
-      const int64_t r = i * D + lane;
-      const T pi = z[r] + beta * p_old[r];
-      p_new[r] = pi;
-      q[r] = qi;
-      dot += (double)pi * (double)qi;
+      const int64_t k = i * D + lane;
+      const T wk = pick<T, D>(acc, lane);
+      w[k] = wk;
+      const double uk = u[k], rk = r[k];
+      gam += rk * uk;
+      del += (double)wk * uk;
+      rr += rk * rk;
     }
   }
-  double v[1] = {dot};
-  if (grid_sum<1>(v, partials, &st->counter[0]) && threadIdx.x == 0) {
-    const double pq = v[0];
-    if (!(pq > 0.0) || !isfinite(pq)) {
-      st->done = 3;
-      st->alpha = 0.0;
-    } else {
-      st->alpha = st->rz / pq;
-    }
-  }
-}
-
-// PC: 1 = point Jacobi (D values per node), 2 = block Jacobi (D*D values per node).
-template <typename T, int D, int PC>
-__device__ inline void apply_precond(const T* __restrict__ M, int64_t i, const T (&r)[D], T (&z)[D]) {
-  if (PC == FE_PC_JACOBI) {
-#pragma unroll
-    for (int d = 0; d < D; ++d) z[d] = M[i * D + d] * r[d];
-  } else {
-    const T* m = M + i * D * D;
-#pragma unroll
-    for (int di = 0; di < D; ++di) {
-      T s = T(0);
-#pragma unroll
-      for (int dj = 0; dj < D; ++dj) s += m[di * D + dj] * r[dj];
-      z[di] = s;
-    }
-  }
-}
-
-template <typename T, int D, int PC>
-__global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t nb, double* __restrict__ x,
-                                                       const T* p0, const T* p1,
-                                                       const T* __restrict__ q, T* __restrict__ r,
-                                                       T* __restrict__ z, const T* __restrict__ M,
-                                                       PcgState* st, double* partials) {
-  __shared__ int s_skip, s_sel;
-  __shared__ double s_alpha;
-  if (threadIdx.x == 0) {
-    s_skip = st->done || st->need_ru;
-    s_alpha = st->alpha;
-    s_sel = st->iter & 1;
-  }
-  __syncthreads();
-  if (s_skip) return;
-  // Same iteration as the preceding k_pcg_spmv: its p_new.
-  const T* __restrict__ p = s_sel ? p0 : p1;
-  const double alpha = s_alpha;
-  const T alpha_t = (T)alpha;
-  int64_t lo, hi;
-  block_range(nb, lo, hi);
-  double rz = 0.0, rr = 0.0;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-    T ri[D], zi[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      const int64_t k = i * D + d;
-      x[k] += alpha * (double)p[k];
-      ri[d] = r[k] - alpha_t * q[k];
-      r[k] = ri[d];
-    }
-    apply_precond<T, D, PC>(M, i, ri, zi);
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      z[i * D + d] = zi[d];
-      rz += (double)ri[d] * (double)zi[d];
-      rr += (double)ri[d] * (double)ri[d];
-    }
-  }
-  double v[2] = {rz, rr};
-  if (grid_sum<2>(v, partials, &st->counter[1]) && threadIdx.x == 0) {
-    const double rz_new = v[0];
+  double v[3] = {gam, del, rr};
+  if (grid_sum<3>(v, partials, &st->counter[0]) && threadIdx.x == 0) {
+    const double g = v[0], dl = v[1];
     st->iter += 1;
-    st->rr = v[1];
-    if (!isfinite(rz_new) || !isfinite(v[1])) {
+    st->rr = v[2];
+    if (!isfinite(g) || !isfinite(dl) || !isfinite(v[2]) || g < 0.0) {
       st->done = 3;
       return;
     }
-    st->rz_prev = st->rz;
-    st->beta = rz_new / st->rz;
-    st->rz = rz_new;
-    if (v[1] <= st->tol2bb || st->iter >= st->max_iter ||
-        (st->mixed && v[1] < st->ru_delta2 * st->rr_ref))
+    st->gamma_prev = st->gamma;
+    st->gamma = g;
+    // convergence claims, the iteration limit and (mixed precision) the reliable-update threshold
+    // hand over to the true-residual check at the end of the chunk
+    if (v[2] <= st->tol2bb || st->iter >= st->max_iter ||
+        (st->mixed && v[2] < st->ru_delta2 * st->rr_ref)) {
       st->need_ru = 1;
+      return;
+    }
+    const double beta = g / st->gamma_prev;
+    const double den = dl - beta * g / st->alpha;
+    if (!(den > 0.0)) {
+      st->done = 3;
+      return;
+    }
+    st->beta = beta;
+    st->alpha = g / den;
   }
 }
 
@@ -437,75 +411,148 @@ __global__ __launch_bounds__(kBlock) void k_ru_residual(int64_t nb, int64_t nnzb
     }
   }
   double v[1] = {rr};
-  if (grid_sum<1>(v, partials, &st->counter[2]) && threadIdx.x == 0) st->rr_true = v[0];
-}
-
-// Reliable update, part 2: convergence decision on the true residual; otherwise r = r64,
-// z = M^-1 r, rz = r.z, beta = rz / rz_prev -- or beta = 0 (restart from the true residual) on the
-// first update and whenever the recursive residual had drifted far below the true one.
-template <typename T, int D, int PC>
-__global__ __launch_bounds__(kBlock) void k_ru_finish(int64_t nb, const double* __restrict__ r64,
-                                                      T* __restrict__ r, T* __restrict__ z,
-                                                      const T* __restrict__ M, PcgState* st,
-                                                      double* partials) {
-  __shared__ int s_skip;
-  if (threadIdx.x == 0) {
-    int skip = st->done || !st->need_ru;
-    if (!skip && (st->rr_true <= st->tol2bb || st->iter >= st->max_iter)) skip = 1;
-    s_skip = skip;
-  }
-  __syncthreads();
-  if (s_skip) {
-    // Decision made by the last workgroup only, after every workgroup has read the state.
-    double v[1] = {0.0};
-    if (grid_sum<1>(v, partials, &st->counter[3]) && threadIdx.x == 0 && !st->done && st->need_ru) {
-      st->rr = st->rr_true;
+  if (grid_sum<1>(v, partials, &st->counter[1]) && threadIdx.x == 0) {
+    st->rr_true = v[0];
+    // decide here: converged, out of iterations, or continue with the replaced residual
+    if (v[0] <= st->tol2bb) {
+      st->done = 1;
+    } else if (st->iter >= st->max_iter) {
+      st->done = 2;
+    } else {
+      if (v[0] < 0.81 * st->rr_best) {
+        st->rr_best = v[0];
+        st->stall = 0;
+      } else if (++st->stall >= kStallLimit) {
+        st->done = 2;
+      }
+      // the recursive residual drifted far below the true one: the direction is stale, restart
+      if (!st->first && v[0] > 4.0 * st->rr) {
+        st->first = 1;
+        st->restarts += 1;
+      }
+    }
+    if (st->done) {
+      st->rr = v[0];
       st->n_ru += 1;
       st->need_ru = 0;
-      st->done = (st->rr_true <= st->tol2bb) ? 1 : 2;
     }
-    return;
   }
+}
+
+// Reliable update, part 2: r = r64, u = M r (and p, s cleared on a (re)start).
+template <typename T, int D, int PC>
+__global__ __launch_bounds__(kVecNodes * D) void k_ru_prep(int64_t nb, const double* __restrict__ r64,
+                                                           T* __restrict__ r, T* __restrict__ u,
+                                                           T* __restrict__ p, T* __restrict__ s,
+                                                           const T* __restrict__ M,
+                                                           const PcgState* st) {
+  __shared__ T s_r[kVecNodes * D];
+  __shared__ int s_skip, s_first;
+  if (threadIdx.x == 0) {
+    s_skip = st->done || !st->need_ru;
+    s_first = st->first;
+  }
+  __syncthreads();
+  if (s_skip) return;
+  const int64_t n = nb * D;
+  const int64_t k = (int64_t)blockIdx.x * kVecNodes * D + threadIdx.x;
+  T rk = T(0);
+  if (k < n) {
+    rk = (T)r64[k];
+    r[k] = rk;
+    if (s_first) {
+      p[k] = T(0);
+      s[k] = T(0);
+    }
+  }
+  s_r[threadIdx.x] = rk;
+  __syncthreads();
+  if (k < n) {
+    const int d = threadIdx.x % D;
+    u[k] = precond_dof<T, D, PC>(M, k / D, d, k, &s_r[threadIdx.x - d], rk);
+  }
+}
+
+// Reliable update, part 3: one matrix pass for w = A u and s = A p; dots (r,u), (u,w), (u,s),
+// (p,s), (r,r); beta = gamma/gamma_prev (0 on a (re)start) and the exact alpha of the direction
+// p' = u + beta p: p'.Ap' = (u,w) + 2 beta (u,s) + beta^2 (p,s).
+template <typename T, int D, int G>
+__global__ __launch_bounds__(kBlock) void k_ru_spmv(int64_t nb, int64_t nnzb,
+                                                    const int32_t* __restrict__ bptr,
+                                                    const int32_t* __restrict__ bcol,
+                                                    const T* __restrict__ vals,
+                                                    const T* __restrict__ u, const T* __restrict__ p,
+                                                    const T* __restrict__ r, T* __restrict__ w,
+                                                    T* __restrict__ s, PcgState* st,
+                                                    double* partials) {
+  __shared__ int s_skip;
+  if (threadIdx.x == 0) s_skip = st->done || !st->need_ru;
+  __syncthreads();
+  if (s_skip) return;
   int64_t lo, hi;
   block_range(nb, lo, hi);
-  double rz = 0.0;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-    T ri[D], zi[D];
+  const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+  double acc5[5] = {0, 0, 0, 0, 0};
+  for (int64_t i = lo + grp; i < hi; i += kBlock / G) {
+    T au[D], ap[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      ri[d] = (T)r64[i * D + d];
-      r[i * D + d] = ri[d];
+    for (int d = 0; d < D; ++d) au[d] = ap[d] = T(0);
+    const int32_t b0 = bptr[i], b1 = bptr[i + 1];
+    for (int32_t q = b0 + lane; q < b1; q += G) {
+      const int64_t j = bcol[q];
+      T uj[D], pj[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        uj[d] = u[j * D + d];
+        pj[d] = p[j * D + d];
+      }
+#pragma unroll
+      for (int di = 0; di < D; ++di)
+#pragma unroll
+        for (int dj = 0; dj < D; ++dj) {
+          const T a = vals[vix<D, FE_LAYOUT_PLANES>(q, di * D + dj, nnzb)];
+          au[di] += a * uj[dj];
+          ap[di] += a * pj[dj];
+        }
     }
-    apply_precond<T, D, PC>(M, i, ri, zi);
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      z[i * D + d] = zi[d];
-      rz += (double)ri[d] * (double)zi[d];
+    for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        au[d] += __shfl_xor(au[d], off, G);
+        ap[d] += __shfl_xor(ap[d], off, G);
+      }
+    if (lane < D) {
+      const int64_t k = i * D + lane;
+      const T wk = pick<T, D>(au, lane), sk = pick<T, D>(ap, lane);
+      w[k] = wk;
+      s[k] = sk;
+      const double uk = u[k], pk = p[k], rk = r[k];
+      acc5[0] += rk * uk;
+      acc5[1] += uk * (double)wk;
+      acc5[2] += uk * (double)sk;
+      acc5[3] += pk * (double)sk;
+      acc5[4] += rk * rk;
     }
   }
-  double v[1] = {rz};
-  if (grid_sum<1>(v, partials, &st->counter[3]) && threadIdx.x == 0) {
-    const double rz_new = v[0];
-    if (!isfinite(rz_new) || rz_new < 0.0) {
+  if (grid_sum<5>(acc5, partials, &st->counter[2]) && threadIdx.x == 0) {
+    const double g = acc5[0];
+    if (!isfinite(g) || g < 0.0 || !isfinite(acc5[1])) {
       st->done = 3;
       return;
     }
-    const bool drifted = !st->first && st->rr_true > 4.0 * st->rr;
-    if (st->first || drifted) {
-      st->beta = 0.0;
-      if (drifted) st->restarts += 1;
-      st->first = 0;
-    } else {
-      st->beta = rz_new / st->rz_prev;
+    // gamma_prev holds r.u of the iteration before the one whose residual was replaced
+    const double beta = st->first ? 0.0 : g / st->gamma_prev;
+    const double pap = acc5[1] + 2.0 * beta * acc5[2] + beta * beta * acc5[3];
+    if (!(pap > 0.0)) {
+      st->done = 3;
+      return;
     }
-    if (st->rr_true < 0.81 * st->rr_best) {
-      st->rr_best = st->rr_true;
-      st->stall = 0;
-    } else if (++st->stall >= kStallLimit) {
-      st->done = 2;
-    }
-    st->rz = rz_new;
-    st->rr = st->rr_true;
+    st->beta = beta;
+    st->alpha = g / pap;
+    st->gamma = g;
+    st->first = 0;
+    st->rr = acc5[4];
     st->rr_ref = st->rr_true;
     st->n_ru += 1;
     st->need_ru = 0;
@@ -519,7 +566,7 @@ __global__ void k_pcg_init(int64_t n, const double* __restrict__ b, PcgState* st
        i += (int64_t)gridDim.x * kBlock)
     s += b[i] * b[i];
   double v[1] = {s};
-  if (grid_sum<1>(v, partials, &st->counter[0]) && threadIdx.x == 0) {
+  if (grid_sum<1>(v, partials, &st->counter[3]) && threadIdx.x == 0) {
     st->bb = v[0];
     st->tol2bb = tol * tol * v[0];
     st->ru_delta2 = ru_delta * ru_delta;
@@ -530,9 +577,11 @@ __global__ void k_pcg_init(int64_t n, const double* __restrict__ b, PcgState* st
     st->iter = 0;
     st->done = 0;
     st->n_ru = 0;
-    st->rz = 1.0;
-    st->rz_prev = 1.0;
+    st->alpha = 1.0;
     st->beta = 0.0;
+    st->gamma = 1.0;
+    st->gamma_prev = 1.0;
+    st->rr = 0.0;
     st->rr_best = 1.0e308;
     st->stall = 0;
     st->restarts = 0;
@@ -560,10 +609,10 @@ struct Work {
   PcgStatus* status;
   double* partials;
   void* r;
-  void* z;
-  void* p0;
-  void* p1;
-  void* q;
+  void* u;
+  void* w;
+  void* p;
+  void* s;
   double* r64;
   int64_t bytes;
 };
@@ -571,8 +620,10 @@ struct Work {
 constexpr int64_t kAlign = 256;
 inline int64_t align_up(int64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
+// Fixed grid of the reduction kernels: 8 workgroups per CU (a multiple of the 8 XCDs), fewer for
+// small systems.
 int pcg_grid(const fe_handle* h, int64_t nb) {
-  int64_t g = (int64_t)h->num_cus * 4;
+  int64_t g = (int64_t)h->num_cus * 8;
   if (g < 8) g = 8;
   g = (g + 7) / 8 * 8;
   const int64_t need = ceil_div(nb, 16);
@@ -590,12 +641,12 @@ Work layout_work(char* base, int64_t n, int elem, int grid) {
   };
   w.st = (PcgState*)take(sizeof(PcgState));
   w.status = (PcgStatus*)take(sizeof(PcgStatus));
-  w.partials = (double*)take((int64_t)4 * grid * sizeof(double));
+  w.partials = (double*)take((int64_t)5 * grid * sizeof(double));
   w.r = take(n * elem);
-  w.z = take(n * elem);
-  w.p0 = take(n * elem);
-  w.p1 = take(n * elem);
-  w.q = take(n * elem);
+  w.u = take(n * elem);
+  w.w = take(n * elem);
+  w.p = take(n * elem);
+  w.s = take(n * elem);
   w.r64 = (double*)take(n * (int64_t)sizeof(double));
   w.bytes = off;
   return w;
@@ -608,8 +659,8 @@ int run_pcg(fe_handle* h, const fe_bsr* A64, const fe_bsr* Ait, const T* M, cons
   const int64_t nb = A64->n_brows, n = nb * D, nnzb = A64->nnzb;
   hipStream_t st = h->stream;
   FE_CHECK_HIP(hipMemsetAsync(w.st, 0, sizeof(PcgState), st));
-  FE_CHECK_HIP(hipMemsetAsync(w.p0, 0, n * sizeof(T), st));
-  FE_CHECK_HIP(hipMemsetAsync(w.p1, 0, n * sizeof(T), st));
+  FE_CHECK_HIP(hipMemsetAsync(w.p, 0, n * sizeof(T), st));
+  FE_CHECK_HIP(hipMemsetAsync(w.s, 0, n * sizeof(T), st));
   hipLaunchKernelGGL(k_pcg_init, dim3(grid), dim3(kBlock), 0, st, n, b, w.st, w.partials, o->tol,
                      o->max_iter, o->mixed, o->ru_delta);
   FE_CHECK_LAUNCH();
@@ -618,14 +669,17 @@ int run_pcg(fe_handle* h, const fe_bsr* A64, const fe_bsr* Ait, const T* M, cons
   const double* v64 = (const double*)A64->vals;
   const T* vit = (const T*)Ait->vals;
   T* r = (T*)w.r;
-  T* z = (T*)w.z;
-  T* q = (T*)w.q;
-  T* pb[2] = {(T*)w.p0, (T*)w.p1};
+  T* u = (T*)w.u;
+  T* wv = (T*)w.w;
+  T* p = (T*)w.p;
+  T* sv = (T*)w.s;
+  const dim3 gvec((unsigned)ceil_div(nb, kVecNodes)), bvec(kVecNodes * D);
   auto launch_ru = [&]() -> int {
     hipLaunchKernelGGL((k_ru_residual<D, G>), dim3(grid), dim3(kBlock), 0, st, nb, nnzb, bptr, bcol,
                        v64, b, x, w.r64, w.st, w.partials);
-    hipLaunchKernelGGL((k_ru_finish<T, D, PC>), dim3(grid), dim3(kBlock), 0, st, nb, w.r64, r, z, M,
-                       w.st, w.partials);
+    hipLaunchKernelGGL((k_ru_prep<T, D, PC>), gvec, bvec, 0, st, nb, w.r64, r, u, p, sv, M, w.st);
+    hipLaunchKernelGGL((k_ru_spmv<T, D, G>), dim3(grid), dim3(kBlock), 0, st, nb, nnzb, bptr, bcol, vit,
+                       u, p, r, wv, sv, w.st, w.partials);
     FE_CHECK_LAUNCH();
     return FE_OK;
   };
@@ -643,12 +697,11 @@ int run_pcg(fe_handle* h, const fe_bsr* A64, const fe_bsr* Ait, const T* M, cons
       for (int b2 = 0; b2 < 2; ++b2) FE_CHECK_HIP(hipEventCreate(&pev[a][b2]));
   auto enqueue_chunk = [&](int slot) -> int {
     for (int it = 0; it < chunk; ++it) {
+      hipLaunchKernelGGL((k_cg_vec<T, D, PC>), gvec, bvec, 0, st, nb, x, p, sv, r, u, wv, M, w.st);
       if (prof && it == 0) FE_CHECK_HIP(hipEventRecord(pev[slot][0], st));
-      hipLaunchKernelGGL((k_pcg_spmv<T, D, G>), dim3(grid), dim3(kBlock), 0, st, nb, nnzb, bptr, bcol,
-                         vit, z, pb[0], pb[1], q, w.st, w.partials);
+      hipLaunchKernelGGL((k_cg_spmv<T, D, G>), dim3(grid), dim3(kBlock), 0, st, nb, nnzb, bptr, bcol,
+                         vit, u, r, wv, w.st, w.partials);
       if (prof && it == 0) FE_CHECK_HIP(hipEventRecord(pev[slot][1], st));
-      hipLaunchKernelGGL((k_pcg_update<T, D, PC>), dim3(grid), dim3(kBlock), 0, st, nb, x, pb[0], pb[1],
-                         q, r, z, M, w.st, w.partials);
     }
     int rc2 = launch_ru();
     if (rc2) return rc2;

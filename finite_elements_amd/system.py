@@ -38,6 +38,7 @@ class FESystem:
                                max_inc=max_inc)
         self.graph = DeviceGraph(self.ctx, self.pattern)
         self.mesh = DeviceMesh(self.ctx, kind, coords, conn, materials, plane_mode)
+        self._coords, self._conn = coords, conn
         if with_gather:
             self.gather = self.ctx.tensor(self.pattern.gather)
             self.seg_ptr = self.ctx.tensor(self.pattern.seg_ptr)
@@ -47,6 +48,11 @@ class FESystem:
     @property
     def n(self):
         return self.n_nodes * self.dof
+
+    def set_materials(self, materials: dict):
+        """Replace the per-element material arrays (same mesh and pattern)."""
+        self.mesh = DeviceMesh(self.ctx, self.kind, self._coords, self._conn, materials,
+                               self.mesh.plane_mode)
 
     # ---------------------------------------------------------------------------------------------
     def set_schedule(self, max_inc: int, threads: int = 256):

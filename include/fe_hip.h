@@ -246,6 +246,30 @@ int fe_pcg(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M64, 
            const double* b, double* x, void* work, int64_t work_bytes, const fe_pcg_opts* opts,
            fe_pcg_stats* stats);
 
+/* ---- learned preconditioner ----------------------------------------------------------------------
+   No counterpart in the reference (its solve is a direct spsolve): a per-node graph network over the
+   node graph of K whose output scales each block-Jacobi block, M_i = diag(s_i) K_ii^-1 diag(s_i),
+   s_i = exp(clamp(o_i, -4, 4)).  3 message-passing layers of width 32 on 8 scale-free node features.
+   Parameters are one FP32 vector of fe_gnn_num_params(dof) entries.
+   acts: [3][n_nodes][32] layer outputs; aggs: [n_nodes][8 + 32 + 32] neighbour means (for backward).
+   fe_gnn_loss_grad accumulates, for one residual/error pair (r, e) with A e = r and z = M r, the loss
+   L = 1 - (r.z)^2 / ((z.Az)(r.e)) into *loss_acc and dL/do into dlo. */
+int fe_gnn_num_params(int dof); /* returns the count (>= 0) or FE_ERR_ARG */
+int fe_gnn_features(fe_handle* h, const fe_bsr* A, const uint8_t* fixed, float* feat, float* scratch);
+int fe_gnn_forward(fe_handle* h, int64_t n_nodes, int dof, const int32_t* bptr, const int32_t* bcol,
+                   const float* params, const float* feat, float* acts, float* aggs,
+                   const double* binv, float* o, double* M);
+int fe_gnn_loss_workspace(int64_t n_dofs, int64_t* doubles);
+int fe_gnn_loss_grad(fe_handle* h, const fe_bsr* A, const double* binv, const double* M,
+                     const float* o, const double* r, const double* e, double* work,
+                     double* loss_acc, float* dlo);
+int fe_gnn_backward_workspace(int64_t n_nodes, int64_t* floats);
+int fe_gnn_backward(fe_handle* h, int64_t n_nodes, int dof, const int32_t* bptr, const int32_t* bcol,
+                    const float* params, const float* feat, const float* acts, const float* aggs,
+                    const float* dlo, float* work, float* grad);
+int fe_gnn_adam(fe_handle* h, int n, float* params, const float* grad, float* m, float* v, float lr,
+                float beta1, float beta2, float eps, int step, float grad_scale);
+
 #ifdef __cplusplus
 }
 #endif
