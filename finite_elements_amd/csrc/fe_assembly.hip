@@ -44,9 +44,9 @@ __global__ __launch_bounds__(kGatherBlock) void k_assemble_gather(
 // ---------------------------------------------------------------------------------------------------
 // Fused assembly.  A workgroup owns the node range [blk[w], blk[w+1]).  Phase 1: one thread per
 // node-element incidence (node i = local vertex a of element e) computes row block a of the element
-// matrix (npe blocks of d x d) straight from coordinates into LDS.  Phase 2: one thread per output
-// value of the range (contiguous in HBM, so stores are coalesced) sums the LDS values listed by the
-// block-level contribution map, in ascending element order.
+// matrix (npe blocks of d x d) straight from coordinates into LDS.  Phase 2: one thread per block
+// entry of the range sums the LDS values listed by the block-level contribution map, in ascending
+// element order, and stores its d x d values (coalesced across threads).
 // ---------------------------------------------------------------------------------------------------
 constexpr int kFusedMaxNodes = 256;
 
@@ -192,62 +192,48 @@ __global__ __launch_bounds__(NT) void k_assemble_fused(fe_mesh m, fe_graph g,
     for (int k = 0; k < BS; ++k) S[k * stride + t] = blk[k];
   }
   __syncthreads();
-  // Phase 2: one thread per output value of the range, ascending-element sums.  Threads walk the
-  // range in the destination layout's order so that the stores are coalesced.
-  const int64_t nout = (int64_t)nbq * D * D;
-  for (int64_t t = threadIdx.x; t < nout; t += NT) {
-    int32_t bq;
-    int di, dj;
-    int64_t o;
-    if (LAYOUT == FE_LAYOUT_PLANES) {
-      const int k = (int)(t / nbq);
-      bq = B0 + (int32_t)(t - (int64_t)k * nbq);
-      di = k / D;
-      dj = k - di * D;
-      o = (int64_t)k * g.nnzb + bq;
-    } else {
-      o = (int64_t)B0 * D * D + t;
-      bq = -1;
-      di = dj = 0;
-    }
-    // node j of the output: largest j with s_bptr[j] <= (block of o)
-    const int32_t key = (LAYOUT == FE_LAYOUT_PLANES) ? bq : (int32_t)(o / (D * D));
+  // Phase 2: one thread per block entry (node j, neighbour q) of the range sums its d x d values
+  // over the contributing incidences in ascending element order.  Consecutive threads own
+  // consecutive block entries, so plane and BSR stores are coalesced.
+  for (int32_t t = threadIdx.x; t < nbq; t += NT) {
+    const int32_t bq = B0 + t;
     int lo = 0, hi = nn - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if (s_bptr[mid] <= key)
+      if (s_bptr[mid] <= bq)
         lo = mid;
       else
         hi = mid - 1;
     }
     const int j = lo;
-    if (LAYOUT != FE_LAYOUT_PLANES) {
-      const int32_t deg = s_bptr[j + 1] - s_bptr[j];
-      const int32_t l = (int32_t)(o - (int64_t)s_bptr[j] * D * D);
-      int q;
-      if (LAYOUT == FE_LAYOUT_CSR) {
-        di = l / (deg * D);
-        const int rem = l - di * deg * D;
-        q = rem / D;
-        dj = rem - q * D;
-      } else {
-        q = l / (D * D);
-        const int r2 = l - q * D * D;
-        di = r2 / D;
-        dj = r2 - di * D;
-      }
-      bq = s_bptr[j] + q;
-    }
-    const int32_t c0 = staged_bq ? s_bsp[bq - B0] : g.bseg_ptr[bq];
-    const int32_t c1 = staged_bq ? s_bsp[bq - B0 + 1] : g.bseg_ptr[bq + 1];
+    const int32_t c0 = staged_bq ? s_bsp[t] : g.bseg_ptr[bq];
+    const int32_t c1 = staged_bq ? s_bsp[t + 1] : g.bseg_ptr[bq + 1];
     const int32_t base = s_iptr[j] - I0;
-    double sum = 0.0;
+    double sum[D * D];
+#pragma unroll
+    for (int k = 0; k < D * D; ++k) sum[k] = 0.0;
     for (int32_t c = c0; c < c1; ++c) {
       const int v = staged_seg ? s_bseg[c - c_lo] : g.bseg[c];
       const int kl = v >> 2, b = v & 3;
-      sum += S[((b * D + di) * D + dj) * stride + base + kl];
+      const double* src = S + (b * D * D) * stride + base + kl;
+#pragma unroll
+      for (int k = 0; k < D * D; ++k) sum[k] += src[k * stride];
     }
-    vals[o] = sum;
+    if (LAYOUT == FE_LAYOUT_PLANES) {
+#pragma unroll
+      for (int k = 0; k < D * D; ++k) vals[(int64_t)k * g.nnzb + bq] = sum[k];
+    } else if (LAYOUT == FE_LAYOUT_BSR) {
+#pragma unroll
+      for (int k = 0; k < D * D; ++k) vals[(int64_t)bq * D * D + k] = sum[k];
+    } else {
+      const int32_t deg = s_bptr[j + 1] - s_bptr[j];
+      const int32_t q = bq - s_bptr[j];
+      const int64_t row0 = (int64_t)s_bptr[j] * D * D + (int64_t)q * D;
+#pragma unroll
+      for (int di = 0; di < D; ++di)
+#pragma unroll
+        for (int dj = 0; dj < D; ++dj) vals[row0 + (int64_t)di * deg * D + dj] = sum[di * D + dj];
+    }
   }
 }
 

@@ -41,7 +41,7 @@ template <int D, int LAYOUT>
 __global__ void k_gnn_logtr(int64_t nb, int64_t nnzb, const int32_t* __restrict__ bptr,
                             const int32_t* __restrict__ bcol, const double* __restrict__ vals,
                             const uint8_t* __restrict__ fixed, float* __restrict__ feat,
-                            float* __restrict__ logtr) {
+                            double* __restrict__ logtr) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= nb) return;
   int64_t qd = -1;
@@ -59,7 +59,7 @@ __global__ void k_gnn_logtr(int64_t nb, int64_t nnzb, const int32_t* __restrict_
     for (int b = 0; b < D; ++b)
       if (a != b) off += blk[a * D + b] * blk[a * D + b];
   const double safe = tr > 0 ? tr : 1.0;
-  logtr[i] = (float)log(safe);
+  logtr[i] = log(safe);
   float* f = feat + i * GF;
   int nfix = 0;
 #pragma unroll
@@ -71,25 +71,28 @@ __global__ void k_gnn_logtr(int64_t nb, int64_t nnzb, const int32_t* __restrict_
   f[6] = (float)(bptr[i + 1] - bptr[i]) / 16.f;
 }
 
+// Neighbourhood contrast: own log-trace minus the neighbours' mean, and the neighbours' standard
+// deviation (accumulated in double on values centred at the node's own log-trace).
 __global__ void k_gnn_contrast(int64_t nb, const int32_t* __restrict__ bptr,
-                               const int32_t* __restrict__ bcol, const float* __restrict__ logtr,
+                               const int32_t* __restrict__ bcol, const double* __restrict__ logtr,
                                float* __restrict__ feat) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= nb) return;
-  float s = 0.f, s2 = 0.f;
+  const double li = logtr[i];
+  double s = 0.0, s2 = 0.0;
   int n = 0;
   for (int32_t q = bptr[i]; q < bptr[i + 1]; ++q) {
     const int32_t j = bcol[q];
     if (j == i) continue;
-    const float v = logtr[j];
+    const double v = logtr[j] - li;
     s += v;
     s2 += v * v;
     ++n;
   }
-  const float mean = n ? s / n : logtr[i];
-  const float var = n ? fmaxf(s2 / n - mean * mean, 0.f) : 0.f;
-  feat[i * GF + 0] = logtr[i] - mean;
-  feat[i * GF + 7] = sqrtf(var);
+  const double m = n ? s / n : 0.0;
+  const double var = n ? fmax(s2 / n - m * m, 0.0) : 0.0;
+  feat[i * GF + 0] = (float)(n ? -m : 0.0);
+  feat[i * GF + 7] = (float)sqrt(var);
 }
 
 // ---- forward ------------------------------------------------------------------------------------
@@ -401,7 +404,7 @@ __global__ void k_gnn_adam(int n, float* __restrict__ P, const float* __restrict
 inline dim3 g1(int64_t n, int b = 256) { return dim3((unsigned)ceil_div(n, b)); }
 
 template <int D>
-int features_d(fe_handle* h, const fe_bsr* A, const uint8_t* fixed, float* feat, float* scratch) {
+int features_d(fe_handle* h, const fe_bsr* A, const uint8_t* fixed, float* feat, double* scratch) {
   if (A->layout == FE_LAYOUT_PLANES)
     hipLaunchKernelGGL((k_gnn_logtr<D, FE_LAYOUT_PLANES>), g1(A->n_brows), dim3(256), 0, h->stream,
                        A->n_brows, A->nnzb, A->bptr, A->bcol, (const double*)A->vals, fixed, feat, scratch);
@@ -422,7 +425,7 @@ extern "C" int fe_gnn_num_params(int dof) {
 }
 
 extern "C" int fe_gnn_features(fe_handle* h, const fe_bsr* A, const uint8_t* fixed, float* feat,
-                               float* scratch) {
+                               double* scratch) {
   FE_ENTER(h);
   FE_REQUIRE(A && fixed && feat && scratch, "fe_gnn_features: null argument");
   FE_REQUIRE(A->precision == 0, "fe_gnn_features: FP64 matrix expected");

@@ -154,6 +154,8 @@ constexpr int lanes_for() {
   return D == 3 ? 16 : 8;
 }
 
+inline int lanes_for_dof(int d) { return d == 3 ? 16 : 8; }
+
 // --------------------------------------------------------------------------------------------------
 // value conversion between precisions and layouts
 // --------------------------------------------------------------------------------------------------
@@ -324,23 +326,24 @@ __global__ __launch_bounds__(kVecNodes * D) void k_cg_vec(int64_t nb, double* __
   }
 }
 
+// Full grid: one node row per group of G lanes; each workgroup writes its partial dots (no fences,
+// no atomics); k_cg_reduce sums them in a fixed order and computes alpha, beta.
 template <typename T, int D, int G>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv(int64_t nb, int64_t nnzb,
                                                     const int32_t* __restrict__ bptr,
                                                     const int32_t* __restrict__ bcol,
                                                     const T* __restrict__ vals,
                                                     const T* __restrict__ u, const T* __restrict__ r,
-                                                    T* __restrict__ w, PcgState* st,
-                                                    double* partials) {
+                                                    T* __restrict__ w, const PcgState* st,
+                                                    double* __restrict__ partials) {
   __shared__ int s_skip;
   if (threadIdx.x == 0) s_skip = st->done || st->need_ru;
   __syncthreads();
   if (s_skip) return;
-  int64_t lo, hi;
-  block_range(nb, lo, hi);
-  const int grp = threadIdx.x / G, lane = threadIdx.x % G;
-  double gam = 0.0, del = 0.0, rr = 0.0;
-  for (int64_t i = lo + grp; i < hi; i += kBlock / G) {
+  const int64_t i = (int64_t)blockIdx.x * (kBlock / G) + threadIdx.x / G;
+  const int lane = threadIdx.x % G;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (i < nb) {
     T acc[D];
     bsr_row<T, D, G, FE_LAYOUT_PLANES>(bptr, bcol, vals, nnzb, u, i, lane, acc);
     if (lane < D) {
@@ -348,38 +351,76 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(int64_t nb, int64_t nnzb,
       const T wk = pick<T, D>(acc, lane);
       w[k] = wk;
       const double uk = u[k], rk = r[k];
-      gam += rk * uk;
-      del += (double)wk * uk;
-      rr += rk * rk;
+      v[0] = rk * uk;
+      v[1] = (double)wk * uk;
+      v[2] = rk * rk;
     }
   }
-  double v[3] = {gam, del, rr};
-  if (grid_sum<3>(v, partials, &st->counter[0]) && threadIdx.x == 0) {
-    const double g = v[0], dl = v[1];
-    st->iter += 1;
-    st->rr = v[2];
-    if (!isfinite(g) || !isfinite(dl) || !isfinite(v[2]) || g < 0.0) {
-      st->done = 3;
-      return;
-    }
-    st->gamma_prev = st->gamma;
-    st->gamma = g;
-    // convergence claims, the iteration limit and (mixed precision) the reliable-update threshold
-    // hand over to the true-residual check at the end of the chunk
-    if (v[2] <= st->tol2bb || st->iter >= st->max_iter ||
-        (st->mixed && v[2] < st->ru_delta2 * st->rr_ref)) {
-      st->need_ru = 1;
-      return;
-    }
-    const double beta = g / st->gamma_prev;
-    const double den = dl - beta * g / st->alpha;
-    if (!(den > 0.0)) {
-      st->done = 3;
-      return;
-    }
-    st->beta = beta;
-    st->alpha = g / den;
+  block_sum<3>(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) partials[(int64_t)k * gridDim.x + blockIdx.x] = v[k];
   }
+}
+
+constexpr int kRedThreads = 1024;
+
+// Fixed-order sum of the SpMV partials, then the Chronopoulos-Gear scalars for the next iteration.
+__global__ __launch_bounds__(kRedThreads) void k_cg_reduce(int nparts,
+                                                           const double* __restrict__ partials,
+                                                           PcgState* st) {
+  __shared__ double red[3][kRedThreads / 64];
+  __shared__ int s_skip;
+  if (threadIdx.x == 0) s_skip = st->done || st->need_ru;
+  __syncthreads();
+  if (s_skip) return;
+  double v[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += kRedThreads) s += partials[(int64_t)k * nparts + b];
+    v[k] = wave_sum(s);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) red[k][wid] = v[k];
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double s = 0.0;
+    for (int q = 0; q < kRedThreads / 64; ++q) s += red[k][q];
+    v[k] = s;
+  }
+  const double g = v[0], dl = v[1];
+  st->iter += 1;
+  st->rr = v[2];
+  if (!isfinite(g) || !isfinite(dl) || !isfinite(v[2]) || g < 0.0) {
+    st->done = 3;
+    return;
+  }
+  st->gamma_prev = st->gamma;
+  st->gamma = g;
+  // convergence claims, the iteration limit and (mixed precision) the reliable-update threshold
+  // hand over to the true-residual check at the end of the chunk
+  if (v[2] <= st->tol2bb || st->iter >= st->max_iter ||
+      (st->mixed && v[2] < st->ru_delta2 * st->rr_ref)) {
+    st->need_ru = 1;
+    return;
+  }
+  const double beta = g / st->gamma_prev;
+  const double den = dl - beta * g / st->alpha;
+  if (!(den > 0.0)) {
+    // the recurrences lost positivity (rounding, most often in FP32): restart from the true
+    // residual; a genuine breakdown shows up there as p.Ap <= 0
+    st->need_ru = 1;
+    st->first = 1;
+    st->restarts += 1;
+    return;
+  }
+  st->beta = beta;
+  st->alpha = g / den;
 }
 
 // Reliable update, part 1: r64 = b - A64 x, partial ||r64||^2.
@@ -608,6 +649,7 @@ struct Work {
   PcgState* st;
   PcgStatus* status;
   double* partials;
+  double* spmv_partials;
   void* r;
   void* u;
   void* w;
@@ -631,7 +673,9 @@ int pcg_grid(const fe_handle* h, int64_t nb) {
   return (int)g;
 }
 
-Work layout_work(char* base, int64_t n, int elem, int grid) {
+Work layout_work(char* base, int64_t nb, int dof, int elem, int grid) {
+  const int64_t n = nb * dof;
+  const int64_t spmv_blocks = ceil_div(nb, kBlock / lanes_for_dof(dof));
   Work w{};
   int64_t off = 0;
   auto take = [&](int64_t bytes) {
@@ -642,6 +686,7 @@ Work layout_work(char* base, int64_t n, int elem, int grid) {
   w.st = (PcgState*)take(sizeof(PcgState));
   w.status = (PcgStatus*)take(sizeof(PcgStatus));
   w.partials = (double*)take((int64_t)5 * grid * sizeof(double));
+  w.spmv_partials = (double*)take(3 * spmv_blocks * (int64_t)sizeof(double));
   w.r = take(n * elem);
   w.u = take(n * elem);
   w.w = take(n * elem);
@@ -674,6 +719,7 @@ int run_pcg(fe_handle* h, const fe_bsr* A64, const fe_bsr* Ait, const T* M, cons
   T* p = (T*)w.p;
   T* sv = (T*)w.s;
   const dim3 gvec((unsigned)ceil_div(nb, kVecNodes)), bvec(kVecNodes * D);
+  const dim3 gspmv((unsigned)ceil_div(nb, kBlock / G));
   auto launch_ru = [&]() -> int {
     hipLaunchKernelGGL((k_ru_residual<D, G>), dim3(grid), dim3(kBlock), 0, st, nb, nnzb, bptr, bcol,
                        v64, b, x, w.r64, w.st, w.partials);
@@ -699,9 +745,11 @@ int run_pcg(fe_handle* h, const fe_bsr* A64, const fe_bsr* Ait, const T* M, cons
     for (int it = 0; it < chunk; ++it) {
       hipLaunchKernelGGL((k_cg_vec<T, D, PC>), gvec, bvec, 0, st, nb, x, p, sv, r, u, wv, M, w.st);
       if (prof && it == 0) FE_CHECK_HIP(hipEventRecord(pev[slot][0], st));
-      hipLaunchKernelGGL((k_cg_spmv<T, D, G>), dim3(grid), dim3(kBlock), 0, st, nb, nnzb, bptr, bcol,
-                         vit, u, r, wv, w.st, w.partials);
+      hipLaunchKernelGGL((k_cg_spmv<T, D, G>), gspmv, dim3(kBlock), 0, st, nb, nnzb, bptr, bcol, vit, u,
+                         r, wv, w.st, w.spmv_partials);
       if (prof && it == 0) FE_CHECK_HIP(hipEventRecord(pev[slot][1], st));
+      hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kRedThreads), 0, st, (int)gspmv.x, w.spmv_partials,
+                         w.st);
     }
     int rc2 = launch_ru();
     if (rc2) return rc2;
@@ -935,7 +983,7 @@ extern "C" int fe_pcg_workspace(fe_handle* h, const fe_bsr* A, const fe_pcg_opts
   FE_REQUIRE(A && opts && bytes, "fe_pcg_workspace: null argument");
   const int grid = pcg_grid(h, A->n_brows);
   const int elem = opts->mixed ? (int)sizeof(float) : (int)sizeof(double);
-  *bytes = layout_work(nullptr, A->n_brows * A->dof, elem, grid).bytes;
+  *bytes = layout_work(nullptr, A->n_brows, A->dof, elem, grid).bytes;
   return FE_OK;
 }
 
@@ -965,7 +1013,7 @@ extern "C" int fe_pcg(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const 
   const int grid = pcg_grid(h, A64->n_brows);
   const int elem = o->mixed ? (int)sizeof(float) : (int)sizeof(double);
   const int64_t n = A64->n_brows * A64->dof;
-  Work w = layout_work((char*)work, n, elem, grid);
+  Work w = layout_work((char*)work, A64->n_brows, A64->dof, elem, grid);
   FE_REQUIRE(work_bytes >= w.bytes, "fe_pcg: workspace too small (%lld < %lld)",
              (long long)work_bytes, (long long)w.bytes);
   if (o->mixed) return dispatch_dof<float>(h, A64, A32, M32, b, x, w, grid, o, s);

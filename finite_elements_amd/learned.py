@@ -102,7 +102,7 @@ class LearnedPreconditioner:
         ctx = KD.graph.ctx
         N = KD.graph.n_nodes
         feat = torch.empty(N * N_FEATURES, dtype=torch.float32, device=ctx.torch_device)
-        scratch = torch.empty(N, dtype=torch.float32, device=ctx.torch_device)
+        scratch = torch.empty(N, dtype=torch.float64, device=ctx.torch_device)
         s = KD.struct()
         _hip.call('fe_gnn_features', ctx.h, ctypes.byref(s), _vp(fixed), _vp(feat), _vp(scratch))
         return feat

@@ -255,7 +255,7 @@ int fe_pcg(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M64, 
    fe_gnn_loss_grad accumulates, for one residual/error pair (r, e) with A e = r and z = M r, the loss
    L = 1 - (r.z)^2 / ((z.Az)(r.e)) into *loss_acc and dL/do into dlo. */
 int fe_gnn_num_params(int dof); /* returns the count (>= 0) or FE_ERR_ARG */
-int fe_gnn_features(fe_handle* h, const fe_bsr* A, const uint8_t* fixed, float* feat, float* scratch);
+int fe_gnn_features(fe_handle* h, const fe_bsr* A, const uint8_t* fixed, float* feat, double* scratch);
 int fe_gnn_forward(fe_handle* h, int64_t n_nodes, int dof, const int32_t* bptr, const int32_t* bcol,
                    const float* params, const float* feat, float* acts, float* aggs,
                    const double* binv, float* o, double* M);

@@ -57,9 +57,8 @@ def features(bptr, bcol, diag_blocks, fixed, d):
         nb = [j for j in bcol[bptr[i]:bptr[i + 1]] if j != i]
         if nb:
             v = logtr[nb]
-            mean = v.mean()
-            f[i, 0] = logtr[i] - mean
-            f[i, 7] = np.sqrt(max((v * v).mean() - mean * mean, 0.0))
+            f[i, 0] = logtr[i] - v.mean()
+            f[i, 7] = v.std()
     return f
 
 
