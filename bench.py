@@ -58,12 +58,12 @@ def fused_assembly_bytes(sysm):
 
 
 def pcg_spmv_bytes(sysm, value_bytes):
-    """Algorithmic bytes of one k_pcg_spmv launch: BSR values + block columns + row pointers, the
-    gathered z and p_old vectors once each, p_new and q written once."""
+    """Algorithmic bytes of one k_cg_spmv launch (w = A u and the dots r.u, w.u, r.r): the plane-major
+    block values, block columns and row pointers, the gathered u once, r read and w written once."""
     p = sysm.pattern
     d = sysm.dof
     n = sysm.n
-    return p.nnzb * (d * d * value_bytes + 4) + 4 * (sysm.n_nodes + 1) + 4 * n * value_bytes
+    return p.nnzb * (d * d * value_bytes + 4) + 4 * (sysm.n_nodes + 1) + 3 * n * value_bytes
 
 
 def time_assembly(sysm, reps=20):
@@ -81,12 +81,31 @@ def time_assembly(sysm, reps=20):
 
 
 def one_solve(sysm, pr, method, fixed, g, f_t, profile=False):
-    """Assemble, eliminate, set up the preconditioner and solve; returns (u, info)."""
+    """Assemble, eliminate, set up the preconditioner (network inference included for the learned
+    method) and solve; returns (u, info)."""
     K = sysm.stiffness_bsr()
     KD, b = sysm.prepare(K, fixed, g, f_t)
-    solver = sysm.solver(KD, precond=method['precond'], precision=method['precision'])
+    M = None
+    if method.get('model') is not None:
+        M = method['model'].preconditioner(sysm, KD, fixed)
+    solver = sysm.solver(KD, precond=method['precond'], precision=method['precision'], M64=M)
     u, info = solver.solve(b, tol=1e-8, max_iter=40000, profile=profile)
     return u, info
+
+
+def time_method(sysm, pr, method, fixed, g, f_t, steps, warmup, profile=False):
+    import torch
+    for _ in range(warmup):
+        one_solve(sysm, pr, method, fixed, g, f_t)
+    torch.cuda.synchronize()
+    times, infos = [], []
+    for _ in range(steps):
+        ts = time.perf_counter()
+        _, info = one_solve(sysm, pr, method, fixed, g, f_t, profile=profile)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - ts)
+        infos.append(info)
+    return times, infos
 
 
 def cpu_baseline(budget_s=30.0):
@@ -123,6 +142,8 @@ def main():
     ap.add_argument('--workload', default='cube64')
     ap.add_argument('--plain-steps', type=int, default=2)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-learned', action='store_true')
+    ap.add_argument('--weights', default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         raise SystemExit('bench.py measures one GPU: this build has no multi-GPU path (the sharding '
@@ -151,20 +172,30 @@ def main():
            'compulsory_gbps': asm_compulsory / asm_ms / 1e6}
     log(f'assembly {asm_ms:.3f} ms, {asm["gbps"]:.0f} GB/s algorithmic')
 
-    headline = {'precond': 'block_jacobi', 'precision': 'mixed'}
-    plain = {'precond': 'jacobi', 'precision': 'fp64'}
+    model = None
+    wpath = args.weights or os.path.join(ROOT, 'finite_elements_amd', 'weights',
+                                         'learned_block_jacobi_tet.npz')
+    if sysm.kind == 'tet_elastic' and os.path.exists(wpath) and not args.no_learned:
+        from finite_elements_amd.learned import LearnedPreconditioner
+        model = LearnedPreconditioner.load(wpath)
+        log(f'learned preconditioner weights: {os.path.relpath(wpath, ROOT)}')
+    block = {'name': 'block_jacobi_mixed', 'precond': 'block_jacobi', 'precision': 'mixed'}
+    learned_m = {'name': 'learned_mixed', 'precond': 'block_jacobi', 'precision': 'mixed', 'model': model}
+    plain = {'name': 'jacobi_fp64', 'precond': 'jacobi', 'precision': 'fp64'}
+    headline = learned_m if model is not None else block
 
     # ---- plain FP64 Jacobi-PCG (the in-GPU baseline) ----
-    one_solve(sysm, pr, plain, fixed, g, f_t)
-    torch.cuda.synchronize()
-    plain_times, plain_info = [], None
-    for _ in range(max(1, args.plain_steps)):
-        ts = time.perf_counter()
-        _, plain_info = one_solve(sysm, pr, plain, fixed, g, f_t)
-        torch.cuda.synchronize()
-        plain_times.append(time.perf_counter() - ts)
+    plain_times, plain_infos = time_method(sysm, pr, plain, fixed, g, f_t, max(1, args.plain_steps), 1)
     plain_s = float(np.median(plain_times))
+    plain_info = plain_infos[-1]
     log(f'plain FP64 Jacobi: {plain_s * 1e3:.1f} ms, {plain_info["iterations"]} it, relres {plain_info["relres"]:.2e}')
+    ablation = None
+    if model is not None:
+        bt, bi = time_method(sysm, pr, block, fixed, g, f_t, max(1, args.plain_steps), 1)
+        ablation = {'method': 'block-Jacobi mixed PCG without the network',
+                    'solves_per_s': 1.0 / float(np.median(bt)), 'ms': float(np.median(bt)) * 1e3,
+                    'iterations': bi[-1]['iterations'], 'relres_true_fp64': bi[-1]['relres']}
+        log(f'block-Jacobi mixed (no network): {ablation["ms"]:.1f} ms, {ablation["iterations"]} it')
 
     # ---- headline method: warmup then K timed steps ----
     for _ in range(args.warmup):
@@ -180,11 +211,12 @@ def main():
     ms_per_step = elapsed / args.steps * 1e3
     value = args.steps / elapsed
     info = infos[-1]
-    log(f'headline: {ms_per_step:.1f} ms/solve, {info["iterations"]} it, relres {info["relres"]:.2e}')
+    log(f'headline {headline["name"]}: {ms_per_step:.1f} ms/solve, {info["iterations"]} it, '
+        f'relres {info["relres"]:.2e}')
 
     spmv_ms = float(np.mean([i['spmv_ms'] for i in infos if i['spmv_samples'] > 0]))
     spmv_bytes = pcg_spmv_bytes(sysm, 4)
-    roofline = {'kernel': 'k_pcg_spmv<float,3,16>' if sysm.dof == 3 else f'k_pcg_spmv<float,{sysm.dof},8>',
+    roofline = {'kernel': 'k_cg_spmv<float,3,16>' if sysm.dof == 3 else f'k_cg_spmv<float,{sysm.dof},8>',
                 'bound': 'hbm', 'achieved': spmv_bytes / spmv_ms / 1e6, 'peak': HBM_PEAK_GBS,
                 'unit': 'GB/s', 'frac': spmv_bytes / spmv_ms / 1e6 / HBM_PEAK_GBS, 'traffic': None,
                 'bytes_per_launch': spmv_bytes, 'avg_ms': spmv_ms,
@@ -202,10 +234,14 @@ def main():
         'vs_baseline': None, 'dtype': 'f32+f64', 'data': 'synthetic',
         'config': {'workload': desc, 'dofs': int(sysm.n), 'nnz': int(sysm.pattern.nnz),
                    'elements': int(sysm.n_elem), 'tol': 1e-8,
-                   'method': 'fused assembly + FP32 block-Jacobi PCG with FP64 reliable updates',
+                   'method': ('fused assembly + learned block-Jacobi FP32 PCG with FP64 reliable updates'
+                              if model is not None else
+                              'fused assembly + FP32 block-Jacobi PCG with FP64 reliable updates'),
                    'step': 'assembly + Dirichlet elimination + preconditioner setup + PCG'},
-        'headline': {'iterations': info['iterations'], 'relres_true_fp64': info['relres'],
-                     'reliable_updates': info['reliable_updates']},
+        'headline': {'method': headline['name'], 'iterations': info['iterations'],
+                     'relres_true_fp64': info['relres'], 'reliable_updates': info['reliable_updates'],
+                     'restarts': info['restarts']},
+        'ablation_no_network': ablation,
         'plain_fp64_jacobi': {'solves_per_s': 1.0 / plain_s, 'ms': plain_s * 1e3,
                               'iterations': plain_info['iterations'], 'relres_true_fp64': plain_info['relres']},
         'speedup_vs_plain': value * plain_s,

@@ -71,6 +71,16 @@ class FePcgStats(ctypes.Structure):
                 ('spmv_samples', _i32), ('pad2', _i32)]
 
 
+class FeLobpcgOpts(ctypes.Structure):
+    _fields_ = [('n_variants', _i32), ('block', _i32), ('n_wanted', _i32), ('largest', _i32),
+                ('max_iter', _i32), ('check_every', _i32), ('tol', _dbl), ('k_stride', _i64),
+                ('m_stride', _i64), ('t_stride', _i64)]
+
+
+class FeLobpcgStats(ctypes.Structure):
+    _fields_ = [('max_iterations', _i32), ('unconverged', _i32), ('failed', _i32), ('status', _i32)]
+
+
 _P = ctypes.POINTER
 # name -> (restype, argtypes); every function returns an int status unless noted.
 SIGNATURES = {
@@ -102,6 +112,10 @@ SIGNATURES = {
     'fe_pcg_workspace': (_int, [_vp, _P(FeBsr), _P(FePcgOpts), _P(_i64)]),
     'fe_pcg': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp, _vp, _vp, _vp, _vp, _i64, _P(FePcgOpts),
                       _P(FePcgStats)]),
+    'fe_lobpcg_workspace': (_int, [_vp, _P(FeBsr), _P(FeLobpcgOpts), _P(_i64)]),
+    'fe_lobpcg': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64,
+                         _P(FeLobpcgOpts), _P(FeLobpcgStats)]),
+    'fe_sym_eig': (_int, [_vp, _int, _int, _vp, _vp, _vp]),
     'fe_gnn_num_params': (_int, [_int]),
     'fe_gnn_features': (_int, [_vp, _P(FeBsr), _vp, _vp, _vp]),
     'fe_gnn_forward': (_int, [_vp, _i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

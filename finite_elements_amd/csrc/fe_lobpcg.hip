@@ -1,0 +1,922 @@
+// Batched LOBPCG for K x = lambda M x on node-block matrices (FP64), and a batched Jacobi symmetric
+// eigensolver for the small Rayleigh-Ritz problems.
+//
+// Variants share one sparsity pattern (bptr/bcol); each has its own K values (and optionally its own
+// M values and block-Jacobi preconditioner).  Per variant and iteration (Knyazev's LOBPCG):
+//   K1 k_lob_resid   R = K X - M X diag(lambda), W = T R, column norms of R and M X (partials)
+//   K2 k_lob_spmm    K W and M W in one pass over the shared pattern
+//   K3 k_lob_gram    partial Gram matrices S^T K S, S^T M S of the basis S = [X W P]
+//   K4 k_lob_gsum    fixed-order sum of the partials
+//   K5 k_lob_rr      one workgroup: convergence test, then Rayleigh-Ritz with a rank-revealing
+//                    M-orthonormalisation of S (eigen-decomposition of the scaled S^T M S, directions
+//                    below a relative threshold dropped), both small eigenproblems by parallel cyclic
+//                    Jacobi in LDS
+//   K6 k_lob_update  X' = S C, P' = [W P] C_WP, with the K- and M-images recombined the same way
+//   K7 k_lob_flip    swap the double-buffered blocks, advance the iteration
+// Block vectors are stored row-major [n][m] (the m columns of one DOF row contiguous), so every
+// kernel reads whole rows with coalesced loads.  The host polls a status word one chunk behind.
+#include <algorithm>
+
+#include "fe_internal.h"
+
+namespace {
+
+constexpr int kMaxBlock = 32;
+constexpr int kMaxBasis = 3 * kMaxBlock;
+constexpr int kGramRows = 16;
+constexpr int kThreads = 256;
+constexpr int kRRThreads = 1024;
+constexpr int kNumBlocks = 15;  // X0 X1 KX0 KX1 MX0 MX1 P0 P1 KP0 KP1 MP0 MP1 W KW MW
+enum { B_X = 0, B_KX = 2, B_MX = 4, B_P = 6, B_KP = 8, B_MP = 10, B_W = 12, B_KW = 13, B_MW = 14 };
+
+struct LobState {
+  int32_t it, done, cur, init;
+  int32_t ns, kept, nconv, pad;
+  double lambda[kMaxBlock];
+  double resid[kMaxBlock];
+};
+
+struct LobStatus {
+  int32_t active, max_it, failed, pad;
+};
+
+struct LobParams {
+  int64_t nb, n, nnzb, nm;  // node rows, DOF rows, blocks, n*m
+  int32_t m, n_wanted, largest, max_iter;
+  double tol, drop;
+  int32_t nchunk_g, nchunk_r, ntg, nvar;
+};
+
+__device__ inline double* lob_block(double* base, const LobParams& p, int v, int idx) {
+  return base + ((int64_t)v * kNumBlocks + idx) * p.nm;
+}
+
+// --------------------------------------------------------------------------------------------------
+// K1: residual block, preconditioned residual, column norms
+// --------------------------------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_lob_resid(LobParams p, const double* __restrict__ T,
+                                                        int64_t t_stride,
+                                                        const uint8_t* __restrict__ fixed,
+                                                        const LobState* st, double* blk,
+                                                        double* __restrict__ rpart) {
+  const int v = blockIdx.y;
+  __shared__ int s_skip, s_cur;
+  __shared__ double s_lam[kMaxBlock];
+  __shared__ double s_rr[kThreads], s_bb[kThreads];
+  if (threadIdx.x == 0) {
+    s_skip = st[v].done;
+    s_cur = st[v].cur;
+  }
+  if (threadIdx.x < p.m) s_lam[threadIdx.x] = st[v].lambda[threadIdx.x];
+  __syncthreads();
+  if (s_skip) return;
+  const int m = p.m;
+  const int npg = kThreads / m;
+  const int g = threadIdx.x / m, j = threadIdx.x % m;
+  const double* KX = lob_block(blk, p, v, B_KX + s_cur);
+  const double* MX = lob_block(blk, p, v, B_MX + s_cur);
+  double* W = lob_block(blk, p, v, B_W);
+  const double* Tv = T ? T + (int64_t)v * t_stride : nullptr;
+  const int64_t per = ceil_div(p.nb, (int64_t)gridDim.x);
+  const int64_t lo = min(p.nb, (int64_t)blockIdx.x * per), hi = min(p.nb, lo + per);
+  double rr = 0.0, bb = 0.0;
+  if (g < npg) {
+    const double lam = s_lam[j];
+    for (int64_t i = lo + g; i < hi; i += npg) {
+      double R[D];
+#pragma unroll
+      for (int e = 0; e < D; ++e) {
+        const int64_t at = (i * D + e) * m + j;
+        const double mx = MX[at];
+        R[e] = KX[at] - lam * mx;
+        rr += R[e] * R[e];
+        bb += mx * mx;
+      }
+#pragma unroll
+      for (int e = 0; e < D; ++e) {
+        double w = R[e];
+        if (Tv) {
+          w = 0.0;
+#pragma unroll
+          for (int f = 0; f < D; ++f) w += Tv[(i * D + e) * D + f] * R[f];
+        }
+        if (fixed && fixed[i * D + e]) w = 0.0;
+        W[(i * D + e) * m + j] = w;
+      }
+    }
+  }
+  s_rr[threadIdx.x] = rr;
+  s_bb[threadIdx.x] = bb;
+  __syncthreads();
+  if (threadIdx.x < m) {
+    double a = 0.0, b = 0.0;
+    for (int q = 0; q < npg; ++q) {
+      a += s_rr[q * m + threadIdx.x];
+      b += s_bb[q * m + threadIdx.x];
+    }
+    double* out = rpart + ((int64_t)v * gridDim.x + blockIdx.x) * 2 * m;
+    out[threadIdx.x] = a;
+    out[m + threadIdx.x] = b;
+  }
+}
+
+// --------------------------------------------------------------------------------------------------
+// K2: K Y and M Y for the block Y = X (start) or W, one thread per (node row, column)
+// --------------------------------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_lob_spmm(LobParams p, const int32_t* __restrict__ bptr,
+                                                       const int32_t* __restrict__ bcol,
+                                                       const double* __restrict__ Kv, int64_t k_stride,
+                                                       const double* __restrict__ Mv, int64_t m_stride,
+                                                       const LobState* st, double* blk, int start) {
+  const int v = blockIdx.y;
+  __shared__ int s_skip, s_cur;
+  if (threadIdx.x == 0) {
+    s_skip = st[v].done;
+    s_cur = st[v].cur;
+  }
+  __syncthreads();
+  if (s_skip) return;
+  const int m = p.m;
+  const int npg = kThreads / m;
+  const int g = threadIdx.x / m, j = threadIdx.x % m;
+  const int64_t i = (int64_t)blockIdx.x * npg + g;
+  if (g >= npg || i >= p.nb) return;
+  const double* Y = lob_block(blk, p, v, start ? B_X + s_cur : B_W);
+  double* KY = lob_block(blk, p, v, start ? B_KX + s_cur : B_KW);
+  double* MY = lob_block(blk, p, v, start ? B_MX + s_cur : B_MW);
+  const double* K = Kv + (int64_t)v * k_stride;
+  const double* M = Mv + (int64_t)v * m_stride;
+  double ak[D], am[D];
+#pragma unroll
+  for (int e = 0; e < D; ++e) ak[e] = am[e] = 0.0;
+  const int32_t q0 = bptr[i], q1 = bptr[i + 1];
+  for (int32_t q = q0; q < q1; ++q) {
+    const int64_t c = bcol[q];
+    double y[D];
+#pragma unroll
+    for (int f = 0; f < D; ++f) y[f] = Y[(c * D + f) * m + j];
+#pragma unroll
+    for (int e = 0; e < D; ++e)
+#pragma unroll
+      for (int f = 0; f < D; ++f) {
+        const int64_t at = (int64_t)(e * D + f) * p.nnzb + q;
+        ak[e] += K[at] * y[f];
+        am[e] += M[at] * y[f];
+      }
+  }
+#pragma unroll
+  for (int e = 0; e < D; ++e) {
+    KY[(i * D + e) * m + j] = ak[e];
+    MY[(i * D + e) * m + j] = am[e];
+  }
+}
+
+// --------------------------------------------------------------------------------------------------
+// K3: partial Gram matrices over a range of DOF rows.  Each thread owns one 4x4 tile (ta <= tb) of
+// both S^T K S and S^T M S; rows are staged through LDS kGramRows at a time.
+// --------------------------------------------------------------------------------------------------
+__device__ inline const double* basis_col_src(const double* X, const double* W, const double* P,
+                                              int c, int m, int& cc) {
+  if (c < m) {
+    cc = c;
+    return X;
+  }
+  if (c < 2 * m) {
+    cc = c - m;
+    return W;
+  }
+  cc = c - 2 * m;
+  return P;
+}
+
+__global__ __launch_bounds__(kThreads) void k_lob_gram(LobParams p, const LobState* st, double* blk,
+                                                       double* __restrict__ gpart) {
+  const int v = blockIdx.y;
+  const int chunk = blockIdx.x / p.ntg, tg = blockIdx.x % p.ntg;
+  __shared__ double sS[kGramRows][kMaxBasis], sK[kGramRows][kMaxBasis], sM[kGramRows][kMaxBasis];
+  __shared__ int s_skip, s_cur, s_ns;
+  if (threadIdx.x == 0) {
+    s_skip = st[v].done;
+    s_cur = st[v].cur;
+    s_ns = st[v].ns;
+  }
+  __syncthreads();
+  if (s_skip) return;
+  const int m = p.m, ns = s_ns, cur = s_cur;
+  const int nt = ns / 4;
+  const int ntiles = nt * (nt + 1) / 2;
+  const int tile = tg * kThreads + threadIdx.x;
+  int ta = -1, tb = -1;
+  if (tile < ntiles) {
+    int t = tile, a = 0;
+    while (t >= nt - a) {
+      t -= nt - a;
+      ++a;
+    }
+    ta = a;
+    tb = a + t;
+  }
+  const double* X = lob_block(blk, p, v, B_X + cur);
+  const double* KX = lob_block(blk, p, v, B_KX + cur);
+  const double* MX = lob_block(blk, p, v, B_MX + cur);
+  const double* W = lob_block(blk, p, v, B_W);
+  const double* KW = lob_block(blk, p, v, B_KW);
+  const double* MW = lob_block(blk, p, v, B_MW);
+  const double* P = lob_block(blk, p, v, B_P + cur);
+  const double* KP = lob_block(blk, p, v, B_KP + cur);
+  const double* MP = lob_block(blk, p, v, B_MP + cur);
+  const int64_t per = ceil_div(p.n, (int64_t)p.nchunk_g);
+  const int64_t lo = min(p.n, (int64_t)chunk * per), hi = min(p.n, lo + per);
+  double ga[4][4], gb[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) ga[a][b] = gb[a][b] = 0.0;
+  for (int64_t r0 = lo; r0 < hi; r0 += kGramRows) {
+    for (int idx = threadIdx.x; idx < kGramRows * ns; idx += kThreads) {
+      const int r = idx / ns, c = idx % ns;
+      const int64_t row = r0 + r;
+      int cc;
+      double s = 0.0, k = 0.0, mm = 0.0;
+      if (row < hi) {
+        const double* src = basis_col_src(X, W, P, c, m, cc);
+        const double* srck = basis_col_src(KX, KW, KP, c, m, cc);
+        const double* srcm = basis_col_src(MX, MW, MP, c, m, cc);
+        s = src[row * m + cc];
+        k = srck[row * m + cc];
+        mm = srcm[row * m + cc];
+      }
+      sS[r][c] = s;
+      sK[r][c] = k;
+      sM[r][c] = mm;
+    }
+    __syncthreads();
+    if (ta >= 0) {
+#pragma unroll 4
+      for (int r = 0; r < kGramRows; ++r) {
+        double sa[4], kb[4], mb[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          sa[a] = sS[r][4 * ta + a];
+          kb[a] = sK[r][4 * tb + a];
+          mb[a] = sM[r][4 * tb + a];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            ga[a][b] += sa[a] * kb[b];
+            gb[a][b] += sa[a] * mb[b];
+          }
+      }
+    }
+    __syncthreads();
+  }
+  if (ta < 0) return;
+  const int NS = 3 * m;
+  double* out = gpart + ((int64_t)v * p.nchunk_g + chunk) * 2 * NS * NS;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      out[(4 * ta + a) * NS + 4 * tb + b] = ga[a][b];
+      out[NS * NS + (4 * ta + a) * NS + 4 * tb + b] = gb[a][b];
+    }
+}
+
+// K4: fixed-order sum of the Gram partials into G[v][2][NS][NS] (both triangles).
+__global__ __launch_bounds__(kThreads) void k_lob_gsum(LobParams p, const LobState* st,
+                                                       const double* __restrict__ gpart,
+                                                       double* __restrict__ G) {
+  const int v = blockIdx.y;
+  if (st[v].done) return;
+  const int NS = 3 * p.m, ns = st[v].ns;
+  const int e = blockIdx.x * kThreads + threadIdx.x;
+  if (e >= 2 * NS * NS) return;
+  const int mat = e / (NS * NS), a = (e % (NS * NS)) / NS, b = e % NS;
+  if (a >= ns || b >= ns) return;
+  // (a, b) with a > b is only computed inside diagonal tiles; read the upper entry instead
+  const int a4 = a / 4, b4 = b / 4;
+  const int ra = (a4 > b4) ? b : a, rb = (a4 > b4) ? a : b;
+  double s = 0.0;
+  const double* src = gpart + (int64_t)v * p.nchunk_g * 2 * NS * NS + mat * NS * NS + ra * NS + rb;
+  for (int c = 0; c < p.nchunk_g; ++c) s += src[(int64_t)c * 2 * NS * NS];
+  G[((int64_t)v * 2 + mat) * NS * NS + a * NS + b] = s;
+}
+
+// --------------------------------------------------------------------------------------------------
+// Parallel cyclic Jacobi on an r x r symmetric matrix in LDS (row-major, leading dimension r).
+// On return the diagonal of A holds the eigenvalues and the columns of V the eigenvectors.
+// Round-robin pairing: every step rotates r/2 disjoint planes at once (rows, then columns).
+// --------------------------------------------------------------------------------------------------
+__device__ void jacobi_eig(double* A, double* V, int r, int max_sweeps) {
+  __shared__ double s_c[kMaxBasis / 2 + 1], s_s[kMaxBasis / 2 + 1];
+  __shared__ int s_p[kMaxBasis / 2 + 1], s_q[kMaxBasis / 2 + 1];
+  __shared__ double s_off[kRRThreads / 64], s_dia[kRRThreads / 64];
+  __shared__ int s_stop;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  for (int idx = tid; idx < r * r; idx += nth) V[idx] = (idx / r == idx % r) ? 1.0 : 0.0;
+  const int re = r + (r & 1), npair = re / 2;
+  __syncthreads();
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    double off = 0.0, dia = 0.0;
+    for (int idx = tid; idx < r * r; idx += nth) {
+      const double a = A[idx];
+      if (idx / r == idx % r)
+        dia += a * a;
+      else
+        off += a * a;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      off += __shfl_xor(off, o, 64);
+      dia += __shfl_xor(dia, o, 64);
+    }
+    if ((tid & 63) == 0) {
+      s_off[tid >> 6] = off;
+      s_dia[tid >> 6] = dia;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double o2 = 0.0, d2 = 0.0;
+      for (int w = 0; w < (nth + 63) / 64; ++w) {
+        o2 += s_off[w];
+        d2 += s_dia[w];
+      }
+      s_stop = !(o2 > 1e-30 * d2);
+    }
+    __syncthreads();
+    if (s_stop) break;
+    for (int step = 0; step < re - 1; ++step) {
+      if (tid < npair) {
+        auto pos = [&](int i) { return i == 0 ? 0 : ((i - 1 + step) % (re - 1)) + 1; };
+        int pp = pos(tid), qq = pos(re - 1 - tid);
+        if (pp > qq) {
+          const int t = pp;
+          pp = qq;
+          qq = t;
+        }
+        double c = 1.0, s = 0.0;
+        if (qq < r) {
+          const double apq = A[pp * r + qq];
+          if (apq != 0.0) {
+            const double app = A[pp * r + pp], aqq = A[qq * r + qq];
+            const double tau = (aqq - app) / (2.0 * apq);
+            double t;
+            if (fabs(tau) > 1e150)
+              t = 0.5 / tau;
+            else
+              t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+            c = 1.0 / sqrt(1.0 + t * t);
+            s = t * c;
+          }
+        }
+        s_p[tid] = pp;
+        s_q[tid] = qq;
+        s_c[tid] = c;
+        s_s[tid] = s;
+      }
+      __syncthreads();
+      for (int idx = tid; idx < npair * r; idx += nth) {
+        const int k = idx / r, j = idx % r;
+        const int qq = s_q[k];
+        const double s = s_s[k];
+        if (qq >= r || s == 0.0) continue;
+        const int pp = s_p[k];
+        const double c = s_c[k];
+        const double ap = A[pp * r + j], aq = A[qq * r + j];
+        A[pp * r + j] = c * ap - s * aq;
+        A[qq * r + j] = s * ap + c * aq;
+      }
+      __syncthreads();
+      for (int idx = tid; idx < npair * r; idx += nth) {
+        const int k = idx / r, i = idx % r;
+        const int qq = s_q[k];
+        const double s = s_s[k];
+        if (qq >= r || s == 0.0) continue;
+        const int pp = s_p[k];
+        const double c = s_c[k];
+        const double ap = A[i * r + pp], aq = A[i * r + qq];
+        A[i * r + pp] = c * ap - s * aq;
+        A[i * r + qq] = s * ap + c * aq;
+        const double vp = V[i * r + pp], vq = V[i * r + qq];
+        V[i * r + pp] = c * vp - s * vq;
+        V[i * r + qq] = s * vp + c * vq;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+constexpr int kJacobiSweeps = 30;
+
+// Batched symmetric eigen-decomposition: one workgroup per matrix.
+__global__ __launch_bounds__(kRRThreads) void k_sym_eig(int r, const double* __restrict__ A,
+                                                        double* __restrict__ w,
+                                                        double* __restrict__ V) {
+  extern __shared__ double lds[];
+  double* sA = lds;
+  double* sV = lds + r * r;
+  const int64_t mat = blockIdx.x;
+  for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) {
+    const int i = idx / r, j = idx % r;
+    sA[idx] = 0.5 * (A[mat * r * r + i * r + j] + A[mat * r * r + j * r + i]);
+  }
+  __syncthreads();
+  jacobi_eig(sA, sV, r, kJacobiSweeps);
+  for (int idx = threadIdx.x; idx < r * r; idx += blockDim.x) V[mat * r * r + idx] = sV[idx];
+  for (int i = threadIdx.x; i < r; i += blockDim.x) w[mat * r + i] = sA[i * r + i];
+}
+
+// --------------------------------------------------------------------------------------------------
+// K5: convergence test and Rayleigh-Ritz, one workgroup per variant.
+// --------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kRRThreads) void k_lob_rr(LobParams p, LobState* st,
+                                                       const double* __restrict__ rpart,
+                                                       const double* __restrict__ G,
+                                                       double* __restrict__ Zg,
+                                                       double* __restrict__ Tg,
+                                                       double* __restrict__ Cg) {
+  extern __shared__ double lds[];
+  const int v = blockIdx.x;
+  LobState* s = st + v;
+  __shared__ int s_skip, s_ns, s_init, s_keep;
+  __shared__ int s_kept[kMaxBasis];
+  __shared__ int s_sel[kMaxBlock];
+  __shared__ double s_scale[kMaxBasis];
+  __shared__ double s_mu[kMaxBasis];
+  const int m = p.m, NS = 3 * m, tid = threadIdx.x;
+  if (tid == 0) {
+    s_skip = s->done;
+    s_ns = s->ns;
+    s_init = s->init;
+  }
+  __syncthreads();
+  if (s_skip) return;
+  const int ns = s_ns;
+  // ---- convergence of the current block (its residuals were formed by k_lob_resid) ----
+  if (!s_init) {
+    if (tid < m) {
+      double a = 0.0, b = 0.0;
+      const double* src = rpart + (int64_t)v * p.nchunk_r * 2 * m;
+      for (int c = 0; c < p.nchunk_r; ++c) {
+        a += src[(int64_t)c * 2 * m + tid];
+        b += src[(int64_t)c * 2 * m + m + tid];
+      }
+      const double lam = fabs(s->lambda[tid]);
+      const double den = (lam > 0.0 ? lam : 1.0) * sqrt(b);
+      s->resid[tid] = den > 0.0 ? sqrt(a) / den : sqrt(a);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int nc = 0;
+      for (int j = 0; j < p.n_wanted; ++j) nc += s->resid[j] <= p.tol;
+      s->nconv = nc;
+      if (nc == p.n_wanted)
+        s->done = 1;
+      else if (s->it >= p.max_iter)
+        s->done = 2;
+      s_skip = s->done;
+    }
+    __syncthreads();
+    if (s_skip) return;
+  }
+  double* sA = lds;
+  double* sV = lds + NS * NS;  // both used with leading dimension = the current size
+  const double* GK = G + (int64_t)v * 2 * NS * NS;
+  const double* GM = GK + NS * NS;
+  double* Z = Zg + (int64_t)v * NS * NS;
+  double* T1 = Tg + (int64_t)v * NS * NS;
+  // ---- rank-revealing M-orthonormalisation of the basis: eig of D GM D, D = diag(GM)^-1/2 ----
+  if (tid < ns) {
+    const double d = GM[tid * NS + tid];
+    s_scale[tid] = d > 0.0 ? 1.0 / sqrt(d) : 0.0;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < ns * ns; idx += blockDim.x) {
+    const int a = idx / ns, b = idx % ns;
+    sA[idx] = 0.5 * (GM[a * NS + b] + GM[b * NS + a]) * s_scale[a] * s_scale[b];
+  }
+  __syncthreads();
+  jacobi_eig(sA, sV, ns, kJacobiSweeps);
+  if (tid == 0) {
+    double mx = 0.0;
+    for (int k = 0; k < ns; ++k) mx = fmax(mx, sA[k * ns + k]);
+    int kept = 0;
+    for (int k = 0; k < ns; ++k) {
+      const double mu = sA[k * ns + k];
+      if (mu > p.drop * mx) {
+        s_kept[kept] = k;
+        s_mu[kept] = mu;
+        ++kept;
+      }
+    }
+    s_keep = kept;
+    s->kept = kept;
+  }
+  __syncthreads();
+  const int rk = s_keep;
+  if (rk < m) {
+    if (tid == 0) s->done = 3;
+    return;
+  }
+  // Z = D Q_kept diag(mu)^-1/2 (ns x rk), leading dimension NS
+  for (int idx = tid; idx < ns * rk; idx += blockDim.x) {
+    const int a = idx / rk, k = idx % rk;
+    Z[a * NS + k] = s_scale[a] * sV[a * ns + s_kept[k]] / sqrt(s_mu[k]);
+  }
+  __syncthreads();
+  // T1 = GK Z (ns x rk), then H = Z^T T1 (rk x rk) into sA
+  for (int idx = tid; idx < ns * rk; idx += blockDim.x) {
+    const int a = idx / rk, k = idx % rk;
+    double t = 0.0;
+    for (int b = 0; b < ns; ++b) t += 0.5 * (GK[a * NS + b] + GK[b * NS + a]) * Z[b * NS + k];
+    T1[a * NS + k] = t;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < rk * rk; idx += blockDim.x) {
+    const int k = idx / rk, l = idx % rk;
+    double h = 0.0;
+    for (int a = 0; a < ns; ++a) h += Z[a * NS + k] * T1[a * NS + l];
+    sA[idx] = h;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < rk * rk; idx += blockDim.x) {
+    const int k = idx / rk, l = idx % rk;
+    if (k < l) {
+      const double h = 0.5 * (sA[k * rk + l] + sA[l * rk + k]);
+      sA[k * rk + l] = h;
+      sA[l * rk + k] = h;
+    }
+  }
+  __syncthreads();
+  jacobi_eig(sA, sV, rk, kJacobiSweeps);
+  // select the m smallest (largest) Ritz values in order
+  if (tid == 0) {
+    bool used[kMaxBasis];
+    for (int k = 0; k < rk; ++k) used[k] = false;
+    for (int j = 0; j < m; ++j) {
+      int best = -1;
+      for (int k = 0; k < rk; ++k) {
+        if (used[k]) continue;
+        const double t = sA[k * rk + k];
+        if (best < 0 || (p.largest ? t > sA[best * rk + best] : t < sA[best * rk + best])) best = k;
+      }
+      used[best] = true;
+      s_sel[j] = best;
+      s->lambda[j] = sA[best * rk + best];
+    }
+  }
+  __syncthreads();
+  // C = Z Y_sel (ns x m)
+  double* C = Cg + (int64_t)v * NS * m;
+  for (int idx = tid; idx < ns * m; idx += blockDim.x) {
+    const int a = idx / m, j = idx % m;
+    double c = 0.0;
+    for (int k = 0; k < rk; ++k) c += Z[a * NS + k] * sV[k * rk + s_sel[j]];
+    C[idx] = c;
+  }
+}
+
+// --------------------------------------------------------------------------------------------------
+// K6: X' = S C, P' = [W P] C_WP for the basis and its K-, M-images; one thread per (row, column).
+// --------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_lob_update(LobParams p, const LobState* st, double* blk,
+                                                         const double* __restrict__ Cg) {
+  const int v = blockIdx.y;
+  __shared__ double sC[kMaxBasis * kMaxBlock];
+  __shared__ int s_skip, s_cur, s_ns;
+  if (threadIdx.x == 0) {
+    s_skip = st[v].done;
+    s_cur = st[v].cur;
+    s_ns = st[v].ns;
+  }
+  __syncthreads();
+  if (s_skip) return;
+  const int m = p.m, ns = s_ns, cur = s_cur;
+  const double* C = Cg + (int64_t)v * 3 * m * m;
+  for (int idx = threadIdx.x; idx < ns * m; idx += kThreads) sC[idx] = C[idx];
+  __syncthreads();
+  const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= p.nm) return;
+  const int64_t row = t / m;
+  const int j = (int)(t % m);
+  for (int which = 0; which < 3; ++which) {
+    const double* X = lob_block(blk, p, v, B_X + 2 * which + cur);
+    const double* P = lob_block(blk, p, v, B_P + 2 * which + cur);
+    const double* W = lob_block(blk, p, v, B_W + which);
+    double xa = 0.0, pa = 0.0;
+    for (int a = 0; a < m; ++a) xa += X[row * m + a] * sC[a * m + j];
+    if (ns > m)
+      for (int a = 0; a < m; ++a) pa += W[row * m + a] * sC[(m + a) * m + j];
+    if (ns > 2 * m)
+      for (int a = 0; a < m; ++a) pa += P[row * m + a] * sC[(2 * m + a) * m + j];
+    lob_block(blk, p, v, B_X + 2 * which + 1 - cur)[t] = xa + pa;
+    lob_block(blk, p, v, B_P + 2 * which + 1 - cur)[t] = pa;
+  }
+}
+
+// K7: the new blocks become current; the basis gains W (and P after the first iteration).
+__global__ void k_lob_flip(LobParams p, LobState* st) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= p.nvar || st[v].done) return;
+  LobState* s = st + v;
+  s->cur ^= 1;
+  if (s->init) {
+    s->init = 0;
+    s->ns = 2 * p.m;
+  } else {
+    s->it += 1;
+    s->ns = 3 * p.m;
+  }
+}
+
+__global__ void k_lob_status(LobParams p, const LobState* st, LobStatus* out) {
+  __shared__ int s_act[kThreads], s_max[kThreads], s_fail[kThreads];
+  int act = 0, mx = 0, fail = 0;
+  for (int v = threadIdx.x; v < p.nvar; v += blockDim.x) {
+    act += st[v].done == 0;
+    mx = max(mx, st[v].it);
+    fail += st[v].done == 3;
+  }
+  s_act[threadIdx.x] = act;
+  s_max[threadIdx.x] = mx;
+  s_fail[threadIdx.x] = fail;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int t = 1; t < (int)blockDim.x; ++t) {
+      act += s_act[t];
+      mx = max(mx, s_max[t]);
+      fail += s_fail[t];
+    }
+    out->active = act;
+    out->max_it = mx;
+    out->failed = fail;
+  }
+}
+
+__global__ void k_lob_init(LobParams p, LobState* st) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= p.nvar) return;
+  LobState* s = st + v;
+  s->it = 0;
+  s->done = 0;
+  s->cur = 0;
+  s->init = 1;
+  s->ns = p.m;
+  s->kept = 0;
+  s->nconv = 0;
+  for (int j = 0; j < kMaxBlock; ++j) {
+    s->lambda[j] = 0.0;
+    s->resid[j] = 0.0;
+  }
+}
+
+// Copy the current X of every variant to the caller's block and export lambda / residuals.
+__global__ void k_lob_export(LobParams p, const LobState* st, double* blk, double* X, double* lam,
+                             double* res, int32_t* iters, int32_t* status) {
+  const int v = blockIdx.y;
+  const int cur = st[v].cur;
+  const double* src = lob_block(blk, p, v, B_X + cur);
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < p.nm;
+       t += (int64_t)gridDim.x * blockDim.x)
+    X[(int64_t)v * p.nm + t] = src[t];
+  if (blockIdx.x == 0 && threadIdx.x < p.m) {
+    lam[v * p.m + threadIdx.x] = st[v].lambda[threadIdx.x];
+    res[v * p.m + threadIdx.x] = st[v].resid[threadIdx.x];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    iters[v] = st[v].it;
+    status[v] = st[v].done;
+  }
+}
+
+// --------------------------------------------------------------------------------------------------
+// host side
+// --------------------------------------------------------------------------------------------------
+constexpr int64_t kAlign = 256;
+inline int64_t align_up(int64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+struct LobWork {
+  LobState* st;
+  LobStatus* status;
+  double* blk;
+  double* gpart;
+  double* rpart;
+  double* G;
+  double* C;
+  double* Z;
+  double* T1;
+  int64_t bytes;
+};
+
+LobParams make_params(const fe_bsr* K, const fe_lobpcg_opts* o) {
+  LobParams p{};
+  p.nb = K->n_brows;
+  p.n = K->n_brows * K->dof;
+  p.nnzb = K->nnzb;
+  p.m = o->block;
+  p.nm = p.n * p.m;
+  p.n_wanted = o->n_wanted;
+  p.largest = o->largest;
+  p.max_iter = o->max_iter;
+  p.tol = o->tol;
+  p.drop = 1e-11;
+  p.nvar = o->n_variants;
+  // Gram chunks: enough workgroups per variant to fill the GPU with one variant, fewer per variant
+  // when many variants run together
+  int64_t chunks = 512 / std::max(1, o->n_variants);
+  chunks = std::max<int64_t>(8, std::min<int64_t>(chunks, ceil_div(p.n, 64)));
+  p.nchunk_g = (int)std::max<int64_t>(1, chunks);
+  p.nchunk_r = (int)std::max<int64_t>(1, std::min<int64_t>(256, ceil_div(p.nb, 64)));
+  const int nt = 3 * p.m / 4;
+  p.ntg = (int)ceil_div(nt * (nt + 1) / 2, kThreads);
+  return p;
+}
+
+LobWork layout_lob(char* base, const LobParams& p) {
+  LobWork w{};
+  int64_t off = 0;
+  auto take = [&](int64_t bytes) {
+    char* q = base ? base + off : nullptr;
+    off += align_up(bytes);
+    return q;
+  };
+  const int64_t NS = 3 * p.m, nv = p.nvar;
+  w.st = (LobState*)take(nv * (int64_t)sizeof(LobState));
+  w.status = (LobStatus*)take(sizeof(LobStatus));
+  w.blk = (double*)take(nv * kNumBlocks * p.nm * (int64_t)sizeof(double));
+  w.gpart = (double*)take(nv * p.nchunk_g * 2 * NS * NS * (int64_t)sizeof(double));
+  w.rpart = (double*)take(nv * p.nchunk_r * 2 * p.m * (int64_t)sizeof(double));
+  w.G = (double*)take(nv * 2 * NS * NS * (int64_t)sizeof(double));
+  w.C = (double*)take(nv * NS * p.m * (int64_t)sizeof(double));
+  w.Z = (double*)take(nv * NS * NS * (int64_t)sizeof(double));
+  w.T1 = (double*)take(nv * NS * NS * (int64_t)sizeof(double));
+  w.bytes = off;
+  return w;
+}
+
+int check_opts(const fe_bsr* K, const fe_lobpcg_opts* o) {
+  FE_REQUIRE(K && o, "fe_lobpcg: null argument");
+  FE_REQUIRE(K->dof >= 1 && K->dof <= 3, "fe_lobpcg: dof must be 1..3");
+  FE_REQUIRE(K->precision == 0 && K->layout == FE_LAYOUT_PLANES,
+             "fe_lobpcg: matrices must be FP64 in FE_LAYOUT_PLANES");
+  FE_REQUIRE(o->block >= 4 && o->block <= kMaxBlock && o->block % 4 == 0,
+             "fe_lobpcg: block must be a multiple of 4 in [4, %d]", kMaxBlock);
+  FE_REQUIRE(o->n_wanted >= 1 && o->n_wanted <= o->block, "fe_lobpcg: need 1 <= n_wanted <= block");
+  FE_REQUIRE(o->n_variants >= 1 && o->n_variants <= 65535, "fe_lobpcg: n_variants out of range");
+  FE_REQUIRE(o->max_iter >= 1 && o->tol > 0, "fe_lobpcg: max_iter and tol must be positive");
+  FE_REQUIRE(K->n_brows * K->dof >= 3 * (int64_t)o->block,
+             "fe_lobpcg: the system must have at least 3 * block DOFs");
+  return FE_OK;
+}
+
+template <int D>
+int run_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
+               const uint8_t* fixed, double* X, double* lam, double* res, int32_t* iters,
+               int32_t* vstatus, const LobWork& w, const LobParams& p, const fe_lobpcg_opts* o,
+               fe_lobpcg_stats* stats) {
+  hipStream_t st = h->stream;
+  const int nv = p.nvar;
+  const dim3 gvar((unsigned)ceil_div(nv, 64));
+  hipLaunchKernelGGL(k_lob_init, gvar, dim3(64), 0, st, p, w.st);
+  FE_CHECK_LAUNCH();
+  // caller's initial block into X0 of every variant
+  for (int v = 0; v < nv; ++v)
+    FE_CHECK_HIP(hipMemcpyAsync(w.blk + (int64_t)v * kNumBlocks * p.nm, X + (int64_t)v * p.nm,
+                                p.nm * sizeof(double), hipMemcpyDeviceToDevice, st));
+  // P blocks start at zero so that unused basis columns read finite values
+  for (int v = 0; v < nv; ++v)
+    for (int b = B_P; b < B_W; ++b)
+      FE_CHECK_HIP(hipMemsetAsync(w.blk + ((int64_t)v * kNumBlocks + b) * p.nm, 0,
+                                  p.nm * sizeof(double), st));
+  const int npg = kThreads / p.m;
+  const dim3 gspmm((unsigned)ceil_div(p.nb, npg), nv);
+  const dim3 gres((unsigned)p.nchunk_r, nv);
+  const dim3 ggram((unsigned)(p.nchunk_g * p.ntg), nv);
+  const int NS = 3 * p.m;
+  const dim3 ggsum((unsigned)ceil_div(2 * NS * NS, kThreads), nv);
+  const dim3 gupd((unsigned)ceil_div(p.nm, kThreads), nv);
+  const size_t rr_lds = 2 * (size_t)NS * NS * sizeof(double);
+  FE_CHECK_HIP(hipFuncSetAttribute((const void*)k_lob_rr, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)rr_lds));
+  const double* Kv = (const double*)K->vals;
+  const double* Mv = (const double*)M->vals;
+  auto rr_update = [&]() -> int {
+    hipLaunchKernelGGL(k_lob_gram, ggram, dim3(kThreads), 0, st, p, w.st, w.blk, w.gpart);
+    hipLaunchKernelGGL(k_lob_gsum, ggsum, dim3(kThreads), 0, st, p, w.st, w.gpart, w.G);
+    hipLaunchKernelGGL(k_lob_rr, dim3(nv), dim3(kRRThreads), rr_lds, st, p, w.st, w.rpart, w.G, w.Z,
+                       w.T1, w.C);
+    hipLaunchKernelGGL(k_lob_update, gupd, dim3(kThreads), 0, st, p, w.st, w.blk, w.C);
+    hipLaunchKernelGGL(k_lob_flip, gvar, dim3(64), 0, st, p, w.st);
+    FE_CHECK_LAUNCH();
+    return FE_OK;
+  };
+  // start: K X, M X, Rayleigh-Ritz on X alone
+  hipLaunchKernelGGL((k_lob_spmm<D>), gspmm, dim3(kThreads), 0, st, p, K->bptr, K->bcol, Kv,
+                     o->k_stride, Mv, o->m_stride, w.st, w.blk, 1);
+  FE_CHECK_LAUNCH();
+  int rc = rr_update();
+  if (rc) return rc;
+  const int chunk = o->check_every > 0 ? o->check_every : 8;
+  LobStatus* pinned = (LobStatus*)((char*)h->pinned + 1024);
+  hipEvent_t ev[2];
+  FE_CHECK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  FE_CHECK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  auto enqueue = [&](int slot) -> int {
+    for (int it = 0; it < chunk; ++it) {
+      hipLaunchKernelGGL((k_lob_resid<D>), gres, dim3(kThreads), 0, st, p, T, o->t_stride, fixed,
+                         w.st, w.blk, w.rpart);
+      hipLaunchKernelGGL((k_lob_spmm<D>), gspmm, dim3(kThreads), 0, st, p, K->bptr, K->bcol, Kv,
+                         o->k_stride, Mv, o->m_stride, w.st, w.blk, 0);
+      int rc2 = rr_update();
+      if (rc2) return rc2;
+    }
+    hipLaunchKernelGGL(k_lob_status, dim3(1), dim3(kThreads), 0, st, p, w.st, w.status);
+    FE_CHECK_HIP(hipMemcpyAsync(&pinned[slot], w.status, sizeof(LobStatus), hipMemcpyDeviceToHost, st));
+    FE_CHECK_HIP(hipEventRecord(ev[slot], st));
+    return FE_OK;
+  };
+  // one chunk always queued ahead of the polled one; the last residual check needs max_iter + 1
+  // residual passes, so the bound below covers every variant reaching max_iter
+  const int64_t max_chunks = (int64_t)o->max_iter / chunk + 4;
+  LobStatus last{};
+  rc = enqueue(0);
+  if (rc == FE_OK) rc = enqueue(1);
+  for (int64_t c = 0; rc == FE_OK && c < max_chunks; ++c) {
+    const int slot = (int)(c & 1);
+    FE_CHECK_HIP(hipEventSynchronize(ev[slot]));
+    last = pinned[slot];
+    if (last.active == 0) break;
+    rc = enqueue(slot);
+  }
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_lob_export, dim3((unsigned)std::min<int64_t>(1024, ceil_div(p.nm, kThreads)), nv),
+                     dim3(kThreads), 0, st, p, w.st, w.blk, X, lam, res, iters, vstatus);
+  hipLaunchKernelGGL(k_lob_status, dim3(1), dim3(kThreads), 0, st, p, w.st, w.status);
+  FE_CHECK_HIP(hipMemcpyAsync(&pinned[0], w.status, sizeof(LobStatus), hipMemcpyDeviceToHost, st));
+  FE_CHECK_HIP(hipStreamSynchronize(st));
+  last = pinned[0];
+  stats->max_iterations = last.max_it;
+  stats->unconverged = last.active;
+  stats->failed = last.failed;
+  stats->status = (last.active == 0 && last.failed == 0) ? FE_OK
+                  : last.failed                           ? FE_ERR_BREAKDOWN
+                                                          : FE_ERR_NOT_CONVERGED;
+  return FE_OK;
+}
+
+}  // namespace
+
+extern "C" int fe_lobpcg_workspace(fe_handle* h, const fe_bsr* K, const fe_lobpcg_opts* o,
+                                   int64_t* bytes) {
+  FE_ENTER(h);
+  int rc = check_opts(K, o);
+  if (rc) return rc;
+  FE_REQUIRE(bytes, "fe_lobpcg_workspace: null argument");
+  *bytes = layout_lob(nullptr, make_params(K, o)).bytes;
+  return FE_OK;
+}
+
+extern "C" int fe_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
+                         const uint8_t* fixed, double* X, double* lambda, double* resid,
+                         int32_t* iters, int32_t* vstatus, void* work, int64_t work_bytes,
+                         const fe_lobpcg_opts* o, fe_lobpcg_stats* stats) {
+  FE_ENTER(h);
+  int rc = check_opts(K, o);
+  if (rc) return rc;
+  FE_REQUIRE(M && X && lambda && resid && iters && vstatus && work && stats,
+             "fe_lobpcg: null argument");
+  FE_REQUIRE(M->precision == 0 && M->layout == FE_LAYOUT_PLANES && M->dof == K->dof &&
+                 M->n_brows == K->n_brows && M->nnzb == K->nnzb,
+             "fe_lobpcg: M must be FP64 FE_LAYOUT_PLANES on the pattern of K");
+  FE_REQUIRE(o->k_stride >= 0 && o->m_stride >= 0 && o->t_stride >= 0,
+             "fe_lobpcg: strides must be non-negative");
+  *stats = fe_lobpcg_stats{};
+  const LobParams p = make_params(K, o);
+  const LobWork w = layout_lob((char*)work, p);
+  FE_REQUIRE(work_bytes >= w.bytes, "fe_lobpcg: workspace too small (%lld < %lld)",
+             (long long)work_bytes, (long long)w.bytes);
+  switch (K->dof) {
+    case 1: return run_lobpcg<1>(h, K, M, T, fixed, X, lambda, resid, iters, vstatus, w, p, o, stats);
+    case 2: return run_lobpcg<2>(h, K, M, T, fixed, X, lambda, resid, iters, vstatus, w, p, o, stats);
+    default: return run_lobpcg<3>(h, K, M, T, fixed, X, lambda, resid, iters, vstatus, w, p, o, stats);
+  }
+}
+
+extern "C" int fe_sym_eig(fe_handle* h, int n_mat, int r, const double* A, double* w, double* V) {
+  FE_ENTER(h);
+  FE_REQUIRE(A && w && V, "fe_sym_eig: null argument");
+  FE_REQUIRE(n_mat >= 0 && r >= 1 && r <= kMaxBasis, "fe_sym_eig: need 1 <= r <= %d", kMaxBasis);
+  if (n_mat == 0) return FE_OK;
+  const size_t lds = 2 * (size_t)r * r * sizeof(double);
+  FE_CHECK_HIP(hipFuncSetAttribute((const void*)k_sym_eig, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+  hipLaunchKernelGGL(k_sym_eig, dim3((unsigned)n_mat), dim3(kRRThreads), lds, h->stream, r, A, w, V);
+  FE_CHECK_LAUNCH();
+  return FE_OK;
+}

@@ -1,12 +1,13 @@
 // Sparse matrix-vector products, preconditioner setup and the device-driven PCG.
 //
-// PCG layout: two kernels per iteration (Chronopoulos-Gear recurrences, see below).  Every global
-// reduction runs on a fixed grid whose per-workgroup partials the last workgroup to finish sums in a
-// fixed order (run-to-run bitwise reproducible).
+// PCG layout: three kernels per iteration (Chronopoulos-Gear recurrences, see below): a streaming
+// vector update, the SpMV with the iteration's three dot products as per-workgroup partials, and a
+// one-workgroup kernel that sums the partials in a fixed order (run-to-run bitwise reproducible).
 // Every chunk of iterations ends with a conditional reliable update (true FP64 residual r = b - A x):
 // it runs when the recursive residual reached the tolerance (so convergence is always judged on the
 // true FP64 residual), when the iteration limit is hit, or (mixed precision) when the residual fell
-// by ru_delta since the last update.  The host only polls a status word between chunks.
+// by ru_delta since the last update.  Its reductions use the last-workgroup-sums pattern (fixed
+// order as well).  The host only polls a status word between chunks.
 #include "fe_internal.h"
 
 namespace {
@@ -326,8 +327,15 @@ __global__ __launch_bounds__(kVecNodes * D) void k_cg_vec(int64_t nb, double* __
   }
 }
 
-// Full grid: one node row per group of G lanes; each workgroup writes its partial dots (no fences,
-// no atomics); k_cg_reduce sums them in a fixed order and computes alpha, beta.
+// Each workgroup owns kSpmvPasses * (kBlock / G) consecutive node rows (one row per group of G lanes
+// and pass) and writes its partial dots (no fences, no atomics); k_cg_reduce sums them in a fixed
+// order and computes alpha, beta.  The row count per workgroup keeps the partials few (nb / 128 for
+// D = 3) so the single-workgroup reduction stays short.
+constexpr int kSpmvPasses = 8;
+
+template <int G>
+constexpr int64_t cg_rows_per_block() { return (int64_t)kSpmvPasses * (kBlock / G); }
+
 template <typename T, int D, int G>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv(int64_t nb, int64_t nnzb,
                                                     const int32_t* __restrict__ bptr,
@@ -340,10 +348,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(int64_t nb, int64_t nnzb,
   if (threadIdx.x == 0) s_skip = st->done || st->need_ru;
   __syncthreads();
   if (s_skip) return;
-  const int64_t i = (int64_t)blockIdx.x * (kBlock / G) + threadIdx.x / G;
   const int lane = threadIdx.x % G;
+  const int64_t row0 = (int64_t)blockIdx.x * cg_rows_per_block<G>() + threadIdx.x / G;
   double v[3] = {0.0, 0.0, 0.0};
-  if (i < nb) {
+  for (int pass = 0; pass < kSpmvPasses; ++pass) {
+    const int64_t i = row0 + (int64_t)pass * (kBlock / G);
+    if (i >= nb) break;
     T acc[D];
     bsr_row<T, D, G, FE_LAYOUT_PLANES>(bptr, bcol, vals, nnzb, u, i, lane, acc);
     if (lane < D) {
@@ -351,9 +361,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(int64_t nb, int64_t nnzb,
       const T wk = pick<T, D>(acc, lane);
       w[k] = wk;
       const double uk = u[k], rk = r[k];
-      v[0] = rk * uk;
-      v[1] = (double)wk * uk;
-      v[2] = rk * rk;
+      v[0] += rk * uk;
+      v[1] += (double)wk * uk;
+      v[2] += rk * rk;
     }
   }
   block_sum<3>(v);
@@ -366,33 +376,30 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(int64_t nb, int64_t nnzb,
 constexpr int kRedThreads = 1024;
 
 // Fixed-order sum of the SpMV partials, then the Chronopoulos-Gear scalars for the next iteration.
+// The partial loads are issued before the state word is examined (they are plain reads), the three
+// sums share each loop trip so their loads are in flight together, and wave 0 finishes the sum.
 __global__ __launch_bounds__(kRedThreads) void k_cg_reduce(int nparts,
                                                            const double* __restrict__ partials,
                                                            PcgState* st) {
   __shared__ double red[3][kRedThreads / 64];
-  __shared__ int s_skip;
-  if (threadIdx.x == 0) s_skip = st->done || st->need_ru;
-  __syncthreads();
-  if (s_skip) return;
-  double v[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nparts; b += kRedThreads) s += partials[(int64_t)k * nparts + b];
-    v[k] = wave_sum(s);
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += kRedThreads) {
+    a0 += partials[b];
+    a1 += partials[nparts + b];
+    a2 += partials[2 * nparts + b];
   }
+  const int skip = st->done || st->need_ru;
+  if (skip) return;
+  double v[3] = {wave_sum(a0), wave_sum(a1), wave_sum(a2)};
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0)
 #pragma unroll
     for (int k = 0; k < 3; ++k) red[k][wid] = v[k];
   __syncthreads();
-  if (threadIdx.x != 0) return;
+  if (wid != 0) return;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    double s = 0.0;
-    for (int q = 0; q < kRedThreads / 64; ++q) s += red[k][q];
-    v[k] = s;
-  }
+  for (int k = 0; k < 3; ++k) v[k] = wave_sum(lane < kRedThreads / 64 ? red[k][lane] : 0.0);
+  if (lane != 0) return;
   const double g = v[0], dl = v[1];
   st->iter += 1;
   st->rr = v[2];
@@ -675,7 +682,7 @@ int pcg_grid(const fe_handle* h, int64_t nb) {
 
 Work layout_work(char* base, int64_t nb, int dof, int elem, int grid) {
   const int64_t n = nb * dof;
-  const int64_t spmv_blocks = ceil_div(nb, kBlock / lanes_for_dof(dof));
+  const int64_t spmv_blocks = ceil_div(nb, (int64_t)kSpmvPasses * (kBlock / lanes_for_dof(dof)));
   Work w{};
   int64_t off = 0;
   auto take = [&](int64_t bytes) {
@@ -719,7 +726,7 @@ int run_pcg(fe_handle* h, const fe_bsr* A64, const fe_bsr* Ait, const T* M, cons
   T* p = (T*)w.p;
   T* sv = (T*)w.s;
   const dim3 gvec((unsigned)ceil_div(nb, kVecNodes)), bvec(kVecNodes * D);
-  const dim3 gspmv((unsigned)ceil_div(nb, kBlock / G));
+  const dim3 gspmv((unsigned)ceil_div(nb, cg_rows_per_block<G>()));
   auto launch_ru = [&]() -> int {
     hipLaunchKernelGGL((k_ru_residual<D, G>), dim3(grid), dim3(kBlock), 0, st, nb, nnzb, bptr, bcol,
                        v64, b, x, w.r64, w.st, w.partials);

@@ -1,0 +1,180 @@
+"""Modal analysis on the GPU: batched LOBPCG (``fe_lobpcg``) for K phi = lambda M phi.
+
+Replaces ``FiniteElementAnalysis.modal_analysis`` (finite_elements/analysis.py:741-796):
+
+* ``order='largest'`` -- the reference calls ``eigsh(K, M, which='LM', k)`` on the unconstrained K and
+  M; here LOBPCG iterates for the largest eigenvalues of the same unconstrained pencil.
+* ``order='smallest'`` -- the reference inverts the dense (unconstrained, hence singular) K and calls
+  ``eigs``; here LOBPCG with the block-Jacobi preconditioner of K iterates for the smallest eigenvalues
+  of the pencil restricted to the free DOFs (the analysis's Dirichlet conditions remove the
+  constrained DOFs), which is the well-posed form of that request.
+
+Both return ``(eigvals, eigvecs.T)`` like the reference, eigenvalues ascending.  Many variants of one
+mesh (same pattern, different K values) are solved in one batched call by ``lobpcg``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence, Union
+
+import numpy as np
+import torch
+
+from . import _hip, device
+from .device import BsrMatrix
+
+
+def _vp(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _planes64(A: BsrMatrix) -> BsrMatrix:
+    if A.layout != _hip.FE_LAYOUT_PLANES or A.precision != 0:
+        A = A.convert(_hip.FE_LAYOUT_PLANES, torch.float64)
+    return A
+
+
+def lobpcg(K: Union[BsrMatrix, Sequence[BsrMatrix]], M: Union[BsrMatrix, Sequence[BsrMatrix]],
+           n_wanted: int, block: Optional[int] = None, fixed: Optional[torch.Tensor] = None,
+           precond: Union[str, None, torch.Tensor, Sequence[torch.Tensor]] = 'block_jacobi',
+           largest: bool = False, tol: float = 1e-6, max_iter: int = 2000, X0=None, seed: int = 0,
+           check_every: int = 8):
+    """Lowest (or largest) eigenpairs of one or several variants K_v phi = lambda M_v phi.
+
+    K, M: one matrix or a list of variants on one pattern (FP64; converted to the plane layout).  A
+    single M is shared by all variants.  fixed: uint8 DOF mask of constrained DOFs (their rows and
+    columns of K and M must already be decoupled, e.g. by ``device.apply_dirichlet``); the iteration
+    stays in the free subspace.  precond: 'block_jacobi' (blocks of each K_v), None, or explicit
+    FP64 blocks [n_brows][d][d] (one tensor or one per variant).  Returns (eigvals [nvar][n_wanted]
+    numpy, eigvecs [nvar][n][n_wanted] torch FP64, info).
+    """
+    Ks = [K] if isinstance(K, BsrMatrix) else list(K)
+    Ms = [M] if isinstance(M, BsrMatrix) else list(M)
+    nvar = len(Ks)
+    if nvar == 0:
+        raise ValueError('lobpcg: no matrices')
+    if len(Ms) not in (1, nvar):
+        raise ValueError('lobpcg: give one M or one per variant')
+    g = Ks[0].graph
+    ctx = g.ctx
+    d = Ks[0].dof
+    n = g.n_nodes * d
+    if block is None:
+        block = min(32, max(4, ((n_wanted + max(4, n_wanted // 4) + 3) // 4) * 4))
+    if not 1 <= n_wanted <= block:
+        raise ValueError('lobpcg: need 1 <= n_wanted <= block')
+    Ks = [_planes64(A) for A in Ks]
+    Ms = [_planes64(A) for A in Ms]
+    for A in Ks + Ms:
+        if A.graph is not g or A.dof != d:
+            raise ValueError('lobpcg: all matrices must share one pattern')
+    nv_vals = Ks[0].vals.numel()
+    Kall = Ks[0].vals if nvar == 1 else torch.stack([A.vals for A in Ks])
+    Mall = Ms[0].vals if len(Ms) == 1 else torch.stack([A.vals for A in Ms])
+    k_stride = nv_vals if nvar > 1 else 0
+    m_stride = nv_vals if len(Ms) > 1 else 0
+    # preconditioner blocks
+    T = None
+    t_stride = 0
+    if isinstance(precond, str):
+        if precond != 'block_jacobi':
+            raise ValueError("precond must be 'block_jacobi', None or explicit blocks")
+        Ts = [device.precond_setup(A, 'block_jacobi') for A in Ks]
+        T = Ts[0] if nvar == 1 else torch.stack(Ts)
+        t_stride = Ts[0].numel() if nvar > 1 else 0
+    elif isinstance(precond, torch.Tensor):
+        T = precond.to(torch.float64).contiguous()
+        t_stride = 0
+    elif precond is not None:
+        Ts = [t.to(torch.float64).contiguous() for t in precond]
+        T = torch.stack(Ts)
+        t_stride = Ts[0].numel()
+    # initial block
+    if X0 is None:
+        rng = np.random.default_rng(seed)
+        X0 = rng.standard_normal((nvar, n, block))
+    X0 = np.asarray(X0, dtype=np.float64).reshape(nvar, n, block)
+    X = torch.as_tensor(X0, device=ctx.torch_device).contiguous().clone()
+    if fixed is not None:
+        X[:, fixed.bool(), :] = 0.0
+    lam = torch.zeros(nvar * block, dtype=torch.float64, device=ctx.torch_device)
+    res = torch.zeros_like(lam)
+    iters = torch.zeros(nvar, dtype=torch.int32, device=ctx.torch_device)
+    vstat = torch.zeros(nvar, dtype=torch.int32, device=ctx.torch_device)
+    opts = _hip.FeLobpcgOpts(nvar, block, n_wanted, int(bool(largest)), int(max_iter), int(check_every),
+                             float(tol), k_stride, m_stride, t_stride)
+    ks = _hip.FeBsr(g.n_nodes, g.nnzb, d, 0, _hip.FE_LAYOUT_PLANES, 0, _vp(g.bptr), _vp(g.bcol),
+                    _vp(Kall))
+    ms = _hip.FeBsr(g.n_nodes, g.nnzb, d, 0, _hip.FE_LAYOUT_PLANES, 0, _vp(g.bptr), _vp(g.bcol),
+                    _vp(Mall))
+    nbytes = ctypes.c_int64()
+    _hip.call('fe_lobpcg_workspace', ctx.h, ctypes.byref(ks), ctypes.byref(opts), ctypes.byref(nbytes))
+    work = torch.empty(nbytes.value, dtype=torch.uint8, device=ctx.torch_device)
+    stats = _hip.FeLobpcgStats()
+    _hip.call('fe_lobpcg', ctx.h, ctypes.byref(ks), ctypes.byref(ms), _vp(T), _vp(fixed), _vp(X),
+              _vp(lam), _vp(res), _vp(iters), _vp(vstat), _vp(work), work.numel(), ctypes.byref(opts),
+              ctypes.byref(stats))
+    lam_h = lam.cpu().numpy().reshape(nvar, block)
+    res_h = res.cpu().numpy().reshape(nvar, block)
+    info = {'iterations': iters.cpu().numpy(), 'variant_status': vstat.cpu().numpy(),
+            'max_iterations': stats.max_iterations, 'unconverged': stats.unconverged,
+            'failed': stats.failed, 'status': stats.status, 'resid': res_h[:, :n_wanted],
+            'block': block}
+    order = np.argsort(-lam_h[:, :n_wanted], axis=1) if largest else None
+    vals = lam_h[:, :n_wanted]
+    vecs = X[:, :, :n_wanted]
+    if largest:
+        # ascending like eigsh
+        idx = torch.as_tensor(order[:, ::-1].copy(), device=ctx.torch_device)
+        vals = np.take_along_axis(vals, order[:, ::-1], axis=1)
+        vecs = torch.gather(vecs, 2, idx[:, None, :].expand(-1, n, -1))
+    return vals, vecs, info
+
+
+def sym_eig(A: torch.Tensor):
+    """Batched symmetric eigen-decomposition of [n_mat][r][r] FP64 matrices on the GPU (Jacobi).
+
+    Returns (w [n_mat][r], V [n_mat][r][r]) with eigenvalues ascending and eigenvectors in columns."""
+    from .device import Context
+    if A.dim() == 2:
+        A = A[None]
+    ctx = Context.get(A.device.index or 0)
+    A = A.to(torch.float64).contiguous()
+    nmat, r, _ = A.shape
+    w = torch.empty(nmat, r, dtype=torch.float64, device=A.device)
+    V = torch.empty_like(A)
+    _hip.call('fe_sym_eig', ctx.h, nmat, r, _vp(A), _vp(w), _vp(V))
+    order = torch.argsort(w, dim=1)
+    w = torch.gather(w, 1, order)
+    V = torch.gather(V, 2, order[:, None, :].expand(-1, r, -1))
+    return w, V
+
+
+def modal_analysis(analysis, order: str, k: int, method: str = 'lobpcg_gpu', tol: float = 1e-7,
+                   max_iter: int = 5000, block: Optional[int] = None, seed: int = 0):
+    """(eigvals, eigvecs.T) of the analysis's K, M (finite_elements/analysis.py:741-796)."""
+    if order not in ('largest', 'smallest'):
+        raise ValueError("Order parameter should be either 'largest' or 'smallest'")
+    if method != 'lobpcg_gpu':
+        raise ValueError("finite_elements_amd computes modes on the GPU only (method='lobpcg_gpu')")
+    sysm = analysis.system()
+    analysis._plane_mode()
+    K = sysm.stiffness_bsr()
+    Mm = sysm.mass_bsr()
+    if order == 'largest':
+        vals, vecs, info = lobpcg(K, Mm, k, block=block, precond=None, largest=True, tol=tol,
+                                  max_iter=max_iter, seed=seed)
+    else:
+        pos, _ = analysis.boundary_condition_dofs()
+        fixed, _ = sysm.dirichlet(pos)
+        KD = device.apply_dirichlet(K, fixed)
+        MD = device.apply_dirichlet(Mm, fixed)
+        vals, vecs, info = lobpcg(KD, MD, k, block=block, fixed=fixed, precond='block_jacobi',
+                                  tol=tol, max_iter=max_iter, seed=seed)
+    analysis.last_modal_info = info
+    if info['status'] != 0:
+        import warnings
+        warnings.warn(f'LOBPCG did not converge: status {info["status"]}, '
+                      f'max residual {np.max(info["resid"]):.2e}')
+    return vals[0], vecs[0].cpu().numpy().T

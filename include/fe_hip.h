@@ -246,6 +246,46 @@ int fe_pcg(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M64, 
            const double* b, double* x, void* work, int64_t work_bytes, const fe_pcg_opts* opts,
            fe_pcg_stats* stats);
 
+/* ---- eigen-solve ---------------------------------------------------------------------------------
+   Batched LOBPCG for K x = lambda M x (replaces the eigsh / dense-inverse eigs calls of
+   finite_elements/analysis.py:741-796).  Variants share the pattern of K; variant v uses the values
+   at K->vals + v * k_stride (M likewise; stride 0 = shared) and the block-Jacobi blocks
+   T + v * t_stride ([n_brows][dof][dof], FP64; T = NULL: no preconditioner, the choice for
+   largest = 1).  Matrices FP64 in FE_LAYOUT_PLANES.  X: [n_variants][n][block] row-major, the initial
+   block on entry (zero on fixed DOFs), the Ritz vectors on return (M-orthonormal, ascending eigenvalue
+   for largest = 0).  lambda/resid: [n_variants][block]; resid = ||K x - l M x|| / (|l| ||M x||).
+   A variant stops when its n_wanted first pairs reach tol. fixed (n bytes, may be NULL) zeroes the
+   preconditioned residual on constrained DOFs, which keeps the iteration in the free subspace when K
+   and M carry identity / zero rows there. */
+typedef struct {
+  int32_t n_variants;
+  int32_t block;        /* columns of the iterated block: multiple of 4, 4..32 */
+  int32_t n_wanted;     /* <= block */
+  int32_t largest;      /* 0: smallest eigenvalues, 1: largest */
+  int32_t max_iter;
+  int32_t check_every;  /* iterations per launched chunk between host polls */
+  double tol;
+  int64_t k_stride;     /* values between consecutive variants of K (0: one K for all) */
+  int64_t m_stride;
+  int64_t t_stride;     /* doubles between consecutive variants of T */
+} fe_lobpcg_opts;
+
+typedef struct {
+  int32_t max_iterations;
+  int32_t unconverged;  /* variants that reached max_iter */
+  int32_t failed;       /* variants whose basis lost rank below the block size */
+  int32_t status;       /* FE_OK, FE_ERR_NOT_CONVERGED or FE_ERR_BREAKDOWN */
+} fe_lobpcg_stats;
+
+int fe_lobpcg_workspace(fe_handle* h, const fe_bsr* K, const fe_lobpcg_opts* opts, int64_t* bytes);
+int fe_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T, const uint8_t* fixed,
+              double* X, double* lambda, double* resid, int32_t* iters, int32_t* variant_status,
+              void* work, int64_t work_bytes, const fe_lobpcg_opts* opts, fe_lobpcg_stats* stats);
+/* Batched symmetric eigen-decomposition (parallel cyclic Jacobi, one workgroup per matrix, FP64):
+   A [n_mat][r][r] (symmetrised on read), w [n_mat][r] eigenvalues (unordered), V [n_mat][r][r]
+   eigenvectors in columns; r <= 96.  The Rayleigh-Ritz kernel of fe_lobpcg uses the same routine. */
+int fe_sym_eig(fe_handle* h, int n_mat, int r, const double* A, double* w, double* V);
+
 /* ---- learned preconditioner ----------------------------------------------------------------------
    No counterpart in the reference (its solve is a direct spsolve): a per-node graph network over the
    node graph of K whose output scales each block-Jacobi block, M_i = diag(s_i) K_ii^-1 diag(s_i),
