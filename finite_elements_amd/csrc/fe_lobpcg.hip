@@ -37,7 +37,7 @@ struct LobState {
 };
 
 struct LobStatus {
-  int32_t active, max_it, failed, pad;
+  int32_t active, max_it, failed, unconverged;
 };
 
 struct LobParams {
@@ -553,20 +553,18 @@ __global__ __launch_bounds__(kRRThreads) void k_lob_rr(LobParams p, LobState* st
   }
   __syncthreads();
   jacobi_eig(sA, sV, rk, kJacobiSweeps);
-  // select the m smallest (largest) Ritz values in order
-  if (tid == 0) {
-    bool used[kMaxBasis];
-    for (int k = 0; k < rk; ++k) used[k] = false;
-    for (int j = 0; j < m; ++j) {
-      int best = -1;
-      for (int k = 0; k < rk; ++k) {
-        if (used[k]) continue;
-        const double t = sA[k * rk + k];
-        if (best < 0 || (p.largest ? t > sA[best * rk + best] : t < sA[best * rk + best])) best = k;
-      }
-      used[best] = true;
-      s_sel[j] = best;
-      s->lambda[j] = sA[best * rk + best];
+  // select the m smallest (largest) Ritz values in order: thread k ranks value k (ties by index)
+  if (tid < rk) {
+    const double t = sA[tid * rk + tid];
+    int rank = 0;
+    for (int l = 0; l < rk; ++l) {
+      const double u = sA[l * rk + l];
+      const bool before = p.largest ? (u > t) : (u < t);
+      rank += before || (u == t && l < tid);
+    }
+    if (rank < m) {
+      s_sel[rank] = tid;
+      s->lambda[rank] = t;
     }
   }
   __syncthreads();
@@ -634,26 +632,30 @@ __global__ void k_lob_flip(LobParams p, LobState* st) {
 }
 
 __global__ void k_lob_status(LobParams p, const LobState* st, LobStatus* out) {
-  __shared__ int s_act[kThreads], s_max[kThreads], s_fail[kThreads];
-  int act = 0, mx = 0, fail = 0;
+  __shared__ int s_act[kThreads], s_max[kThreads], s_fail[kThreads], s_unc[kThreads];
+  int act = 0, mx = 0, fail = 0, unc = 0;
   for (int v = threadIdx.x; v < p.nvar; v += blockDim.x) {
     act += st[v].done == 0;
     mx = max(mx, st[v].it);
     fail += st[v].done == 3;
+    unc += st[v].done == 2;
   }
   s_act[threadIdx.x] = act;
   s_max[threadIdx.x] = mx;
   s_fail[threadIdx.x] = fail;
+  s_unc[threadIdx.x] = unc;
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int t = 1; t < (int)blockDim.x; ++t) {
       act += s_act[t];
       mx = max(mx, s_max[t]);
       fail += s_fail[t];
+      unc += s_unc[t];
     }
     out->active = act;
     out->max_it = mx;
     out->failed = fail;
+    out->unconverged = unc;
   }
 }
 
@@ -862,11 +864,11 @@ int run_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
   FE_CHECK_HIP(hipStreamSynchronize(st));
   last = pinned[0];
   stats->max_iterations = last.max_it;
-  stats->unconverged = last.active;
+  stats->unconverged = last.unconverged + last.active;
   stats->failed = last.failed;
-  stats->status = (last.active == 0 && last.failed == 0) ? FE_OK
-                  : last.failed                           ? FE_ERR_BREAKDOWN
-                                                          : FE_ERR_NOT_CONVERGED;
+  stats->status = last.failed                       ? FE_ERR_BREAKDOWN
+                  : (last.unconverged + last.active) ? FE_ERR_NOT_CONVERGED
+                                                     : FE_OK;
   return FE_OK;
 }
 
