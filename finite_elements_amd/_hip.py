@@ -62,13 +62,18 @@ class FeBsr(ctypes.Structure):
 
 class FePcgOpts(ctypes.Structure):
     _fields_ = [('tol', _dbl), ('max_iter', _i32), ('precond', _i32), ('mixed', _i32),
-                ('check_every', _i32), ('ru_delta', _dbl), ('profile', _i32), ('pad', _i32)]
+                ('check_every', _i32), ('ru_delta', _dbl), ('profile', _i32), ('graph', _i32)]
 
 
 class FePcgStats(ctypes.Structure):
     _fields_ = [('iterations', _i32), ('reliable_updates', _i32), ('status', _i32), ('restarts', _i32),
                 ('relres', _dbl), ('relres_rec', _dbl), ('bnorm', _dbl), ('spmv_ms', _dbl),
                 ('spmv_samples', _i32), ('pad2', _i32)]
+
+
+class FePcgSegStats(ctypes.Structure):
+    _fields_ = [('iterations', _i32), ('reliable_updates', _i32), ('status', _i32), ('restarts', _i32),
+                ('relres', _dbl), ('relres_rec', _dbl), ('bnorm', _dbl)]
 
 
 class FeLobpcgOpts(ctypes.Structure):
@@ -112,6 +117,9 @@ SIGNATURES = {
     'fe_pcg_workspace': (_int, [_vp, _P(FeBsr), _P(FePcgOpts), _P(_i64)]),
     'fe_pcg': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp, _vp, _vp, _vp, _vp, _i64, _P(FePcgOpts),
                       _P(FePcgStats)]),
+    'fe_pcg_batched_workspace': (_int, [_vp, _P(FeBsr), _i32, _vp, _P(FePcgOpts), _P(_i64)]),
+    'fe_pcg_batched': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp, _vp, _i32, _vp, _vp, _vp, _vp, _i64,
+                              _P(FePcgOpts), _P(FePcgStats), _vp]),
     'fe_lobpcg_workspace': (_int, [_vp, _P(FeBsr), _P(FeLobpcgOpts), _P(_i64)]),
     'fe_lobpcg': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64,
                          _P(FeLobpcgOpts), _P(FeLobpcgStats)]),

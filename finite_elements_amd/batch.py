@@ -1,0 +1,97 @@
+"""Many independent meshes (design variants, e.g. uneven plates) assembled and solved together.
+
+The meshes are concatenated into one block-diagonal system (disjoint node ranges, one segment per
+mesh).  One fused-assembly launch builds every stiffness matrix (row-block schedule picked by the
+autotuner of ``FESystem.tune_assembly``), and one batched PCG (``fe_pcg_batched``) solves all of them:
+each segment iterates with its own scalars, stops on its own true FP64 residual, and the chunks of
+iterations can be replayed from a captured HIP graph.  Each mesh's result is what the reference would
+compute for that mesh alone with ``FiniteElementAnalysis.solve`` (finite_elements/analysis.py:798-830),
+on the eliminated system.
+"""
+
+from __future__ import annotations
+
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from .meshgen import Problem
+from .system import FESystem
+
+
+def concatenate(problems: Sequence[Problem]):
+    """(coords, conn, materials, fixed_dofs, fixed_values, f, node_offsets) of the combined system."""
+    if not problems:
+        raise ValueError('no problems to batch')
+    kind = problems[0].kind
+    plane_mode = problems[0].plane_mode
+    for pr in problems:
+        if pr.kind != kind or pr.plane_mode != plane_mode:
+            raise ValueError('a batch needs one element kind and one plane mode')
+    d = problems[0].dof
+    nodes = np.array([pr.coords.shape[0] for pr in problems], dtype=np.int64)
+    offs = np.zeros(len(problems) + 1, dtype=np.int64)
+    offs[1:] = np.cumsum(nodes)
+    coords = np.concatenate([pr.coords for pr in problems])
+    conn = np.concatenate([pr.conn.astype(np.int64) + o for pr, o in zip(problems, offs[:-1])])
+    keys = problems[0].materials.keys()
+    mats = {k: np.concatenate([np.asarray(pr.materials[k], dtype=np.float64) for pr in problems])
+            for k in keys}
+    fixed = np.concatenate([np.asarray(pr.fixed_dofs, dtype=np.int64) + d * o
+                            for pr, o in zip(problems, offs[:-1])])
+    vals = np.concatenate([np.zeros(len(pr.fixed_dofs)) if pr.fixed_values is None
+                           else np.asarray(pr.fixed_values, dtype=np.float64) for pr in problems])
+    f = np.concatenate([np.zeros(pr.coords.shape[0] * d) if pr.f is None else np.asarray(pr.f)
+                        for pr in problems])
+    return kind, plane_mode, coords, conn.astype(np.int32), mats, fixed, vals, f, offs
+
+
+class MeshBatch:
+    """A batch of independent meshes of one element kind on one GPU."""
+
+    def __init__(self, problems: Sequence[Problem], device_index: int = 0, tune: bool = True):
+        (kind, plane_mode, coords, conn, mats, fixed, vals, f, offs) = concatenate(problems)
+        self.problems = list(problems)
+        self.node_offsets = offs
+        self.sysm = FESystem(kind, coords, conn, mats, plane_mode=plane_mode, device_index=device_index)
+        self.dof = self.sysm.dof
+        self.fixed_dofs = fixed
+        self.fixed_values = vals
+        self.f = f
+        self.tune_times = None
+        if tune:
+            best, self.tune_times = self.sysm.tune_assembly()
+            self.schedule = best
+        self._fixed = None
+        self._solver = None
+
+    @property
+    def n_meshes(self) -> int:
+        return len(self.problems)
+
+    def assemble(self):
+        """Stiffness of every mesh (one fused launch) with the Dirichlet conditions eliminated."""
+        s = self.sysm
+        if self._fixed is None:
+            self._fixed = s.dirichlet(self.fixed_dofs, self.fixed_values)
+            self._f = s.ctx.tensor(self.f)
+        fixed, g = self._fixed
+        K = s.stiffness_bsr()
+        KD, b = s.prepare(K, fixed, g, self._f)
+        return K, KD, b
+
+    def solve(self, tol: float = 1e-8, precision: str = 'mixed', precond: str = 'block_jacobi',
+              graph: bool = True, max_iter: Optional[int] = None, check_every: int = 16):
+        """Assemble and solve every mesh; returns (u on the device, info with per-mesh arrays)."""
+        K, KD, b = self.assemble()
+        solver = self.sysm.solver(KD, precond, precision, segments=self.node_offsets, graph=graph,
+                                  check_every=check_every)
+        u, info = solver.solve(b, tol=tol, max_iter=max_iter)
+        return u, info
+
+    def split(self, u: torch.Tensor) -> List[np.ndarray]:
+        """Per-mesh solution vectors (host) from the batched solution."""
+        host = u.detach().cpu().numpy()
+        d = self.dof
+        return [host[d * a:d * b] for a, b in zip(self.node_offsets[:-1], self.node_offsets[1:])]

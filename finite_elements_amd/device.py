@@ -390,12 +390,22 @@ def precond_setup(A: BsrMatrix, kind: str = 'block_jacobi') -> torch.Tensor:
     return out
 
 
+def _to_f32(ctx: Context, src: torch.Tensor) -> torch.Tensor:
+    dst = torch.empty(src.numel(), dtype=torch.float32, device=src.device)
+    cast_f64_to_f32(ctx, src, dst)
+    return dst
+
+
 class PcgSolver:
-    """PCG on a Dirichlet-applied SPD matrix, with its workspace kept between solves."""
+    """PCG on a Dirichlet-applied SPD matrix, with its workspace kept between solves.
+
+    ``segments``: node-row offsets (n_seg + 1 entries) of independent blocks of a block-diagonal
+    system (e.g. many plates assembled together); each block then iterates and stops on its own
+    (``fe_pcg_batched``).  ``graph``: replay each chunk of iterations from a captured HIP graph."""
 
     def __init__(self, A64: BsrMatrix, precond: str = 'jacobi', mixed: bool = False,
                  M64: Optional[torch.Tensor] = None, A32: Optional[BsrMatrix] = None,
-                 check_every: int = 16, ru_delta: float = 0.1):
+                 check_every: int = 16, ru_delta: float = 0.1, segments=None, graph: bool = False):
         self.ctx = A64.graph.ctx
         if A64.layout != _hip.FE_LAYOUT_PLANES or A64.precision != 0:
             A64 = A64.convert(_hip.FE_LAYOUT_PLANES, torch.float64)
@@ -406,27 +416,36 @@ class PcgSolver:
             if M64 is None else M64
         if self.mixed:
             self.A32 = A64.convert(_hip.FE_LAYOUT_PLANES, torch.float32) if A32 is None else A32
-            self.M32 = self.M64.to(torch.float32)
+            self.M32 = _to_f32(self.ctx, self.M64)
         else:
             self.A32 = None
             self.M32 = None
         self.check_every = check_every
         self.ru_delta = ru_delta
+        self.graph = bool(graph)
+        if segments is not None:
+            seg = np.ascontiguousarray(np.asarray(segments, dtype=np.int32))
+            if seg.ndim != 1 or seg.size < 2 or seg[0] != 0 or seg[-1] != A64.graph.n_nodes:
+                raise ValueError('segments must be node offsets from 0 to n_nodes')
+            self.segments = seg
+        else:
+            self.segments = None
         self._work = None
 
     def set_preconditioner(self, M64: torch.Tensor):
         self.M64 = M64
         if self.mixed:
-            self.M32 = M64.to(torch.float32)
+            self.M32 = _to_f32(self.ctx, M64)
 
     def _opts(self, tol, max_iter, profile=False):
         pc = _hip.FE_PC_JACOBI if self.precond == 'jacobi' else _hip.FE_PC_BLOCK_JACOBI
         return _hip.FePcgOpts(float(tol), int(max_iter), pc, int(self.mixed), int(self.check_every),
-                              float(self.ru_delta), int(profile), 0)
+                              float(self.ru_delta), int(profile), int(self.graph))
 
     def solve(self, b: torch.Tensor, x: Optional[torch.Tensor] = None, tol: float = 1e-8,
               max_iter: Optional[int] = None, profile: bool = False):
-        """Solve from initial guess x (zeros if None); returns (x, info dict)."""
+        """Solve from initial guess x (zeros if None); returns (x, info dict).  With segments, info
+        also holds per-segment arrays (iterations, status, relres, ...)."""
         ctx = self.ctx
         if max_iter is None:
             max_iter = max(2000, min(10 * self.A64.n, 50000))
@@ -436,15 +455,35 @@ class PcgSolver:
         a64 = self.A64.struct()
         a32 = self.A32.struct() if self.A32 is not None else None
         nbytes = ctypes.c_int64()
-        _hip.call('fe_pcg_workspace', ctx.h, ctypes.byref(a64), ctypes.byref(opts), ctypes.byref(nbytes))
+        if self.segments is None:
+            _hip.call('fe_pcg_workspace', ctx.h, ctypes.byref(a64), ctypes.byref(opts),
+                      ctypes.byref(nbytes))
+        else:
+            nseg = self.segments.size - 1
+            segp = self.segments.ctypes.data_as(ctypes.c_void_p)
+            _hip.call('fe_pcg_batched_workspace', ctx.h, ctypes.byref(a64), nseg, segp,
+                      ctypes.byref(opts), ctypes.byref(nbytes))
         if self._work is None or self._work.numel() < nbytes.value:
             self._work = torch.empty(nbytes.value, dtype=torch.uint8, device=ctx.torch_device)
         stats = _hip.FePcgStats()
-        _hip.call('fe_pcg', ctx.h, ctypes.byref(a64), ctypes.byref(a32) if a32 is not None else None,
-                  _vp(self.M64), _vp(self.M32), _vp(b), _vp(x), _vp(self._work), self._work.numel(),
-                  ctypes.byref(opts), ctypes.byref(stats))
+        a32p = ctypes.byref(a32) if a32 is not None else None
+        seg_info = None
+        if self.segments is None:
+            _hip.call('fe_pcg', ctx.h, ctypes.byref(a64), a32p, _vp(self.M64), _vp(self.M32), _vp(b),
+                      _vp(x), _vp(self._work), self._work.numel(), ctypes.byref(opts),
+                      ctypes.byref(stats))
+        else:
+            seg_stats = (_hip.FePcgSegStats * nseg)()
+            _hip.call('fe_pcg_batched', ctx.h, ctypes.byref(a64), a32p, _vp(self.M64), _vp(self.M32),
+                      nseg, segp, _vp(b), _vp(x), _vp(self._work), self._work.numel(),
+                      ctypes.byref(opts), ctypes.byref(stats), ctypes.cast(seg_stats, ctypes.c_void_p))
+            seg_info = {f: np.array([getattr(q, f) for q in seg_stats])
+                        for f in ('iterations', 'reliable_updates', 'status', 'restarts', 'relres',
+                                  'relres_rec', 'bnorm')}
         info = {'iterations': stats.iterations, 'reliable_updates': stats.reliable_updates,
                 'restarts': stats.restarts,
                 'status': stats.status, 'relres': stats.relres, 'relres_rec': stats.relres_rec,
                 'bnorm': stats.bnorm, 'spmv_ms': stats.spmv_ms, 'spmv_samples': stats.spmv_samples}
+        if seg_info is not None:
+            info['segments'] = seg_info
         return x, info

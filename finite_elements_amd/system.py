@@ -135,14 +135,14 @@ class FESystem:
 
     def solver(self, KD: BsrMatrix, precond: str = 'jacobi', precision: str = 'fp64',
                M64: Optional[torch.Tensor] = None, check_every: int = 16,
-               ru_delta: float = 0.1) -> PcgSolver:
+               ru_delta: float = 0.1, segments=None, graph: bool = False) -> PcgSolver:
         if precision not in ('fp64', 'mixed'):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         if precond not in ('jacobi', 'block_jacobi'):
             raise ValueError("precond must be 'jacobi' or 'block_jacobi' (or a learned block "
                              "preconditioner passed as M64)")
         return PcgSolver(KD, precond=precond, mixed=(precision == 'mixed'), M64=M64,
-                         check_every=check_every, ru_delta=ru_delta)
+                         check_every=check_every, ru_delta=ru_delta, segments=segments, graph=graph)
 
     def solve(self, f, fixed_dofs, values=None, tol: float = 1e-8, precond: str = 'jacobi',
               precision: str = 'fp64', max_iter: Optional[int] = None):

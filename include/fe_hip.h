@@ -125,7 +125,7 @@ typedef struct {
   int32_t check_every;   /* iterations per launched chunk between host polls */
   double ru_delta;       /* mixed: reliable update when ||r|| fell by this factor since the last one */
   int32_t profile;       /* 1: time the first SpMV kernel of every chunk with HIP events */
-  int32_t pad;
+  int32_t graph;         /* 1: capture one chunk of iterations in a HIP graph and replay it */
 } fe_pcg_opts;
 
 typedef struct {
@@ -245,6 +245,28 @@ int fe_pcg_workspace(fe_handle* h, const fe_bsr* A, const fe_pcg_opts* opts, int
 int fe_pcg(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M64, const void* M32,
            const double* b, double* x, void* work, int64_t work_bytes, const fe_pcg_opts* opts,
            fe_pcg_stats* stats);
+
+/* Batched PCG over independent segments of one block-diagonal system (replaces one spsolve per design
+   variant, finite_elements/analysis.py:819-822): segment g owns node rows [seg_nodes[g],
+   seg_nodes[g+1]) (host array, n_seg + 1 entries, 0 .. n_brows) and no block couples two segments.
+   Each segment iterates with its own scalars and stops on its own true FP64 residual; seg_stats
+   (host, n_seg entries, may be NULL) receives per-segment results, stats the worst status, the
+   largest residuals and iteration count. */
+typedef struct {
+  int32_t iterations;
+  int32_t reliable_updates;
+  int32_t status;
+  int32_t restarts;
+  double relres;         /* true FP64 ||b - A x|| / ||b|| of the segment */
+  double relres_rec;
+  double bnorm;
+} fe_pcg_seg_stats;
+int fe_pcg_batched_workspace(fe_handle* h, const fe_bsr* A, int32_t n_seg, const int32_t* seg_nodes,
+                             const fe_pcg_opts* opts, int64_t* bytes);
+int fe_pcg_batched(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M64,
+                   const void* M32, int32_t n_seg, const int32_t* seg_nodes, const double* b,
+                   double* x, void* work, int64_t work_bytes, const fe_pcg_opts* opts,
+                   fe_pcg_stats* stats, fe_pcg_seg_stats* seg_stats);
 
 /* ---- eigen-solve ---------------------------------------------------------------------------------
    Batched LOBPCG for K x = lambda M x (replaces the eigsh / dense-inverse eigs calls of
