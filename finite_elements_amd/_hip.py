@@ -68,12 +68,12 @@ class FePcgOpts(ctypes.Structure):
 class FePcgStats(ctypes.Structure):
     _fields_ = [('iterations', _i32), ('reliable_updates', _i32), ('status', _i32), ('restarts', _i32),
                 ('relres', _dbl), ('relres_rec', _dbl), ('bnorm', _dbl), ('spmv_ms', _dbl),
-                ('spmv_samples', _i32), ('pad2', _i32)]
+                ('spmv_samples', _i32), ('fallbacks', _i32)]
 
 
 class FePcgSegStats(ctypes.Structure):
     _fields_ = [('iterations', _i32), ('reliable_updates', _i32), ('status', _i32), ('restarts', _i32),
-                ('relres', _dbl), ('relres_rec', _dbl), ('bnorm', _dbl)]
+                ('fallback', _i32), ('pad', _i32), ('relres', _dbl), ('relres_rec', _dbl), ('bnorm', _dbl)]
 
 
 class FeLobpcgOpts(ctypes.Structure):

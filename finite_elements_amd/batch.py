@@ -81,9 +81,13 @@ class MeshBatch:
         KD, b = s.prepare(K, fixed, g, self._f)
         return K, KD, b
 
-    def solve(self, tol: float = 1e-8, precision: str = 'mixed', precond: str = 'block_jacobi',
+    def solve(self, tol: float = 1e-8, precision: str = 'fp64', precond: str = 'block_jacobi',
               graph: bool = True, max_iter: Optional[int] = None, check_every: int = 16):
-        """Assemble and solve every mesh; returns (u on the device, info with per-mesh arrays)."""
+        """Assemble and solve every mesh; returns (u on the device, info with per-mesh arrays).
+
+        precision 'mixed' iterates in FP32 inside FP64 reliable updates and finishes every mesh whose
+        FP32 iteration fails (long thin plates: condition number times the FP32 unit roundoff near
+        one) with FP64 PCG; 'fp64' iterates in FP64 throughout."""
         K, KD, b = self.assemble()
         solver = self.sysm.solver(KD, precond, precision, segments=self.node_offsets, graph=graph,
                                   check_every=check_every)

@@ -53,6 +53,8 @@ struct Tile {
 };
 
 constexpr int kStallLimit = 10;
+constexpr int kMaxRestarts32 = 3;   // FP32: more restarts than this and the segment goes to FP64
+constexpr int kMaxRestarts64 = 50;
 constexpr int kVecNodes = 64;  // nodes per pass of the streaming kernels (64 * D threads)
 constexpr int kSpmvPasses = 8;
 constexpr int kRed = 256;      // threads of the per-segment reduction kernels
@@ -285,6 +287,7 @@ __global__ __launch_bounds__(kRed) void k_ru_decide(const int32_t* __restrict__ 
       s->first = 1;
       s->restarts += 1;
     }
+    if (s->restarts > (s->mixed ? kMaxRestarts32 : kMaxRestarts64)) s->done = 2;
   }
   if (s->done) {
     s->rr = v[0];
@@ -507,6 +510,7 @@ __global__ void k_pcg_export(int nseg, const PcgState* st, fe_pcg_seg_stats* out
   o.iterations = s.iter;
   o.reliable_updates = s.n_ru;
   o.restarts = s.restarts;
+  o.fallback = 0;
   o.status = s.done == 1 ? FE_OK : s.done == 3 ? FE_ERR_BREAKDOWN : FE_ERR_NOT_CONVERGED;
   o.bnorm = sqrt(s.bb);
   o.relres = s.bb > 0 ? sqrt(s.rr_true / s.bb) : 0.0;
@@ -789,7 +793,7 @@ int pcg_common(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M
                int nseg, const int32_t* seg_nodes, const double* b, double* x, void* work,
                int64_t work_bytes, const fe_pcg_opts* o, fe_pcg_stats* s, fe_pcg_seg_stats* seg_out,
                const char* who) {
-  const int elem = o->mixed ? (int)sizeof(float) : (int)sizeof(double);
+  const int elem = (int)sizeof(double);  // FP64-sized vectors: the FP64 fallback reuses them
   const int rows = tile_rows_dof(A64->dof);
   const int64_t ntiles = count_tiles(nseg, seg_nodes, rows);
   FE_REQUIRE(ntiles < (int64_t)1 << 31, "%s: too many tiles", who);
@@ -825,6 +829,39 @@ int pcg_common(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M
                            hipMemcpyDeviceToHost));
   } else {
     for (auto& q : seg) q = fe_pcg_seg_stats{};
+  }
+  // plain FP64 PCG finishes every segment whose FP32 iteration failed, from the x it reached;
+  // segments that already converged stop at the first true-residual check
+  int failed = 0;
+  for (const auto& q : seg) failed += q.status != FE_OK;
+  if (o->mixed && M64 && failed && ntiles > 0) {
+    // a segment that ended worse than the zero vector restarts from zero
+    const int D = A64->dof;
+    for (int g = 0; g < nseg; ++g)
+      if (seg[g].status != FE_OK && !(seg[g].relres < 1.0) && seg_nodes[g + 1] > seg_nodes[g])
+        FE_CHECK_HIP(hipMemsetAsync(x + (int64_t)D * seg_nodes[g], 0,
+                                    (size_t)D * (seg_nodes[g + 1] - seg_nodes[g]) * sizeof(double),
+                                    h->stream));
+    fe_pcg_opts o64 = *o;
+    o64.mixed = 0;
+    o64.profile = 0;
+    Problem pb64{A64, A64, M64, b, x, nseg, ntiles};
+    fe_pcg_stats s64{};
+    rc = dispatch_dof<double>(h, pb64, w, &o64, &s64);
+    if (rc) return rc;
+    std::vector<fe_pcg_seg_stats> seg64((size_t)nseg);
+    FE_CHECK_HIP(hipMemcpy(seg64.data(), w.seg_stats, nseg * sizeof(fe_pcg_seg_stats),
+                           hipMemcpyDeviceToHost));
+    for (int g = 0; g < nseg; ++g) {
+      if (seg[g].status == FE_OK) continue;
+      fe_pcg_seg_stats q = seg64[g];
+      q.iterations += seg[g].iterations;
+      q.reliable_updates += seg[g].reliable_updates;
+      q.restarts += seg[g].restarts;
+      q.fallback = 1;
+      seg[g] = q;
+    }
+    s->fallbacks = failed;
   }
   // aggregate: worst status, largest residuals, most iterations
   s->status = FE_OK;
@@ -864,7 +901,7 @@ extern "C" int fe_pcg_batched_workspace(fe_handle* h, const fe_bsr* A, int32_t n
   FE_REQUIRE(A->dof >= 1 && A->dof <= 3, "fe_pcg_batched_workspace: dof must be 1..3");
   int rc = check_segments(A, n_seg, seg_nodes, "fe_pcg_batched_workspace");
   if (rc) return rc;
-  const int elem = opts->mixed ? (int)sizeof(float) : (int)sizeof(double);
+  const int elem = (int)sizeof(double);
   const int64_t ntiles = count_tiles(n_seg, seg_nodes, tile_rows_dof(A->dof));
   *bytes = layout_work(nullptr, A->n_brows, A->dof, elem, n_seg, ntiles).bytes;
   return FE_OK;

@@ -478,12 +478,13 @@ class PcgSolver:
                       nseg, segp, _vp(b), _vp(x), _vp(self._work), self._work.numel(),
                       ctypes.byref(opts), ctypes.byref(stats), ctypes.cast(seg_stats, ctypes.c_void_p))
             seg_info = {f: np.array([getattr(q, f) for q in seg_stats])
-                        for f in ('iterations', 'reliable_updates', 'status', 'restarts', 'relres',
-                                  'relres_rec', 'bnorm')}
+                        for f in ('iterations', 'reliable_updates', 'status', 'restarts', 'fallback',
+                                  'relres', 'relres_rec', 'bnorm')}
         info = {'iterations': stats.iterations, 'reliable_updates': stats.reliable_updates,
                 'restarts': stats.restarts,
                 'status': stats.status, 'relres': stats.relres, 'relres_rec': stats.relres_rec,
-                'bnorm': stats.bnorm, 'spmv_ms': stats.spmv_ms, 'spmv_samples': stats.spmv_samples}
+                'bnorm': stats.bnorm, 'spmv_ms': stats.spmv_ms, 'spmv_samples': stats.spmv_samples,
+                'fallbacks': stats.fallbacks}
         if seg_info is not None:
             info['segments'] = seg_info
         return x, info

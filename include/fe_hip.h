@@ -138,7 +138,7 @@ typedef struct {
   double bnorm;
   double spmv_ms;        /* profile: mean duration of the sampled SpMV kernels */
   int32_t spmv_samples;  /* profile: number of samples */
-  int32_t pad2;
+  int32_t fallbacks;     /* mixed: segments re-solved in FP64 after the FP32 iteration failed */
 } fe_pcg_stats;
 
 /* ---- context ---------------------------------------------------------------------------------- */
@@ -240,7 +240,9 @@ int fe_precond_setup(fe_handle* h, const fe_bsr* A, int kind, double* out);
    A64/M64 are FP64; with opts->mixed the FP32 copies A32/M32 run the iteration and FP64 reliable
    updates recompute the true residual.  Matrices must be in FE_LAYOUT_PLANES.  x is the initial guess
    on entry.  When the true residual disagrees with the recursive one the iteration restarts from the
-   true residual; ten reliable updates without progress end the solve as not converged. */
+   true residual; ten reliable updates without progress (or, in FP32, more than three restarts) end
+   the iteration as not converged.  Mixed precision with M64 given: a system whose FP32 iteration
+   failed is finished by FP64 PCG from the reached x (plain PCG is the fallback). */
 int fe_pcg_workspace(fe_handle* h, const fe_bsr* A, const fe_pcg_opts* opts, int64_t* bytes);
 int fe_pcg(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M64, const void* M32,
            const double* b, double* x, void* work, int64_t work_bytes, const fe_pcg_opts* opts,
@@ -257,6 +259,8 @@ typedef struct {
   int32_t reliable_updates;
   int32_t status;
   int32_t restarts;
+  int32_t fallback;      /* 1: the FP32 iteration failed and the segment was finished in FP64 */
+  int32_t pad;
   double relres;         /* true FP64 ||b - A x|| / ||b|| of the segment */
   double relres_rec;
   double bnorm;
