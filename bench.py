@@ -24,6 +24,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = ('solves/s to FP64 rel-res 1e-8, learned vs plain PCG; assembly GB/s vs HBM peak')
 HBM_PEAK_GBS = 8000.0
+MAX_ITER = 40000
+COPY_BYTES = 512 << 20
+TRAFFIC_JSON = os.path.join(ROOT, 'profiles', 'cube64_traffic.json')
 
 
 def log(msg):
@@ -89,7 +92,7 @@ def one_solve(sysm, pr, method, fixed, g, f_t, profile=False):
     if method.get('model') is not None:
         M = method['model'].preconditioner(sysm, KD, fixed)
     solver = sysm.solver(KD, precond=method['precond'], precision=method['precision'], M64=M)
-    u, info = solver.solve(b, tol=1e-8, max_iter=40000, profile=profile)
+    u, info = solver.solve(b, tol=1e-8, max_iter=MAX_ITER, profile=profile)
     return u, info
 
 
@@ -134,6 +137,85 @@ def cpu_baseline(budget_s=30.0):
             'object_setup_s': t1 - t0, 'finite': bool(np.all(np.isfinite(x)))}
 
 
+def copy_calibration(reps=3):
+    """A plain device-to-device copy of COPY_BYTES: the byte counters of this kernel calibrate the
+    FETCH_SIZE / WRITE_SIZE readings of the path's kernels."""
+    import torch
+    a = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device='cuda').fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+
+
+def _pmc_rows(folder):
+    import csv
+    import glob
+    rows = []
+    for path in glob.glob(os.path.join(folder, '**', '*counter_collection.csv'), recursive=True):
+        with open(path) as fh:
+            rows.extend(csv.DictReader(fh))
+    return rows
+
+
+def _per_kernel(rows, counter):
+    """{kernel name: [value per dispatch]} for one counter (summed over the dispatch's entries)."""
+    per = {}
+    for r in rows:
+        if r.get('Counter_Name') != counter:
+            continue
+        key = (r['Kernel_Name'], r.get('Dispatch_Id', r.get('Correlation_Id')))
+        per[key] = per.get(key, 0.0) + float(r['Counter_Value'])
+    out = {}
+    for (name, _), v in per.items():
+        out.setdefault(name, []).append(v)
+    return out
+
+
+def summarize_pmc(fetch_dir, write_dir):
+    """Average FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the path's kernels, calibrated by the
+    plain copy kernel of COPY_BYTES; writes profiles/cube64_traffic.json and prints a summary."""
+    fetch = _per_kernel(_pmc_rows(fetch_dir), 'FETCH_SIZE')
+    write = _per_kernel(_pmc_rows(write_dir), 'WRITE_SIZE')
+
+    def find(names, pred):
+        return [n for n in names if pred(n)]
+    copy_names = find(fetch, lambda n: 'copy' in n.lower() and 'anonymous namespace' not in n)
+    if not copy_names:
+        raise SystemExit('no copy kernel among the profiled dispatches')
+    # the calibration copy is the dispatch with the largest byte count among copy kernels
+    cname = max(copy_names, key=lambda n: max(fetch[n]))
+    cf = max(fetch[cname]) * 1024.0 / COPY_BYTES
+    cw = max(write.get(cname, [COPY_BYTES / 1024.0])) * 1024.0 / COPY_BYTES
+    out = {'calibration': {'kernel': cname, 'bytes': COPY_BYTES, 'fetch_factor': cf, 'write_factor': cw},
+           'kernels': {}}
+    for name in sorted(fetch):
+        if not any(k in name for k in ('k_cg_spmv', 'k_cg_vec', 'k_cg_reduce', 'k_assemble_fused',
+                                        'k_ru_', 'k_spmv_bsr')):
+            continue
+        # dispatches that did the work (launches of finished solves return at once)
+        fa = np.array(fetch[name])
+        wa = np.array(write.get(name, [0.0]))
+        f = float(fa[fa >= 0.5 * fa.max()].mean()) * 1024.0 / cf
+        w = float(wa[wa >= 0.5 * wa.max()].mean()) * 1024.0 / cw if wa.max() > 0 else 0.0
+        short = name.replace('void ', '').replace('(anonymous namespace)::', '').split('(')[0]
+        out['kernels'][short] = {'dispatches': int((fa >= 0.5 * fa.max()).sum()), 'fetch_bytes': f,
+                                 'write_bytes': w, 'traffic_bytes': f + w}
+    with open(TRAFFIC_JSON, 'w') as fh:
+        json.dump(out, fh, indent=1, sort_keys=True)
+    print(json.dumps(out, indent=1, sort_keys=True))
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary, or None."""
+    try:
+        with open(TRAFFIC_JSON) as fh:
+            data = json.load(fh)
+        return float(data['kernels'][kernel]['traffic_bytes'])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -144,7 +226,19 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-learned', action='store_true')
     ap.add_argument('--weights', default=None)
+    ap.add_argument('--no-tune', action='store_true', help='keep the default assembly schedule')
+    ap.add_argument('--max-iter', type=int, default=40000,
+                    help='PCG iteration cap (lowered only for short counter-collection runs)')
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='first run a plain 512 MiB device copy (reference for HBM byte counters)')
+    ap.add_argument('--summarize-pmc', nargs=2, metavar=('FETCH_DIR', 'WRITE_DIR'),
+                    help='summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) and exit')
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc)
+        return
+    global MAX_ITER
+    MAX_ITER = args.max_iter
     if args.gpus != 1:
         raise SystemExit('bench.py measures one GPU: this build has no multi-GPU path (the sharding '
                          'design is described in DESIGN.md); run with --gpus 1')
@@ -159,8 +253,13 @@ def main():
     f_t = sysm.ctx.tensor(pr.f)
     log(f'setup {time.perf_counter() - t0:.1f} s: {sysm.n} DOF, nnz {sysm.pattern.nnz}')
 
+    if args.copy_calibration:
+        copy_calibration()
     # ---- assembly: autotune the row-block schedule, then time the best ----
-    best, tune_times = sysm.tune_assembly()
+    if args.no_tune:
+        best, tune_times = (sysm.graph.threads, sysm.graph.max_inc), {}
+    else:
+        best, tune_times = sysm.tune_assembly()
     log(f'assembly autotune: best (threads, max_inc) = {best}; ' +
         ', '.join(f'{k}:{v:.3f}ms' for k, v in sorted(tune_times.items())))
     asm_ms = time_assembly(sysm)
@@ -185,13 +284,16 @@ def main():
     headline = learned_m if model is not None else block
 
     # ---- plain FP64 Jacobi-PCG (the in-GPU baseline) ----
-    plain_times, plain_infos = time_method(sysm, pr, plain, fixed, g, f_t, max(1, args.plain_steps), 1)
-    plain_s = float(np.median(plain_times))
-    plain_info = plain_infos[-1]
-    log(f'plain FP64 Jacobi: {plain_s * 1e3:.1f} ms, {plain_info["iterations"]} it, relres {plain_info["relres"]:.2e}')
+    plain_s, plain_info = None, None
+    if args.plain_steps > 0:
+        plain_times, plain_infos = time_method(sysm, pr, plain, fixed, g, f_t, args.plain_steps, 1)
+        plain_s = float(np.median(plain_times))
+        plain_info = plain_infos[-1]
+        log(f'plain FP64 Jacobi: {plain_s * 1e3:.1f} ms, {plain_info["iterations"]} it, '
+            f'relres {plain_info["relres"]:.2e}')
     ablation = None
-    if model is not None:
-        bt, bi = time_method(sysm, pr, block, fixed, g, f_t, max(1, args.plain_steps), 1)
+    if model is not None and args.plain_steps > 0:
+        bt, bi = time_method(sysm, pr, block, fixed, g, f_t, args.plain_steps, 1)
         ablation = {'method': 'block-Jacobi mixed PCG without the network',
                     'solves_per_s': 1.0 / float(np.median(bt)), 'ms': float(np.median(bt)) * 1e3,
                     'iterations': bi[-1]['iterations'], 'relres_true_fp64': bi[-1]['relres']}
@@ -203,8 +305,9 @@ def main():
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     infos = []
-    for _ in range(args.steps):
-        _, info = one_solve(sysm, pr, headline, fixed, g, f_t, profile=True)
+    for k in range(args.steps):
+        # the last step also times the iteration's SpMV kernel (20 launches after convergence)
+        _, info = one_solve(sysm, pr, headline, fixed, g, f_t, profile=(k == args.steps - 1))
         infos.append(info)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
@@ -218,7 +321,9 @@ def main():
     spmv_bytes = pcg_spmv_bytes(sysm, 4)
     roofline = {'kernel': 'k_cg_spmv<float,3,16>' if sysm.dof == 3 else f'k_cg_spmv<float,{sysm.dof},8>',
                 'bound': 'hbm', 'achieved': spmv_bytes / spmv_ms / 1e6, 'peak': HBM_PEAK_GBS,
-                'unit': 'GB/s', 'frac': spmv_bytes / spmv_ms / 1e6 / HBM_PEAK_GBS, 'traffic': None,
+                'unit': 'GB/s', 'frac': spmv_bytes / spmv_ms / 1e6 / HBM_PEAK_GBS,
+                'traffic': measured_traffic('k_cg_spmv<float, 3, 16>' if sysm.dof == 3
+                                            else f'k_cg_spmv<float, {sysm.dof}, 8>'),
                 'bytes_per_launch': spmv_bytes, 'avg_ms': spmv_ms,
                 'samples': int(sum(i['spmv_samples'] for i in infos))}
 
@@ -242,9 +347,10 @@ def main():
                      'relres_true_fp64': info['relres'], 'reliable_updates': info['reliable_updates'],
                      'restarts': info['restarts']},
         'ablation_no_network': ablation,
-        'plain_fp64_jacobi': {'solves_per_s': 1.0 / plain_s, 'ms': plain_s * 1e3,
-                              'iterations': plain_info['iterations'], 'relres_true_fp64': plain_info['relres']},
-        'speedup_vs_plain': value * plain_s,
+        'plain_fp64_jacobi': None if plain_s is None else
+        {'solves_per_s': 1.0 / plain_s, 'ms': plain_s * 1e3, 'iterations': plain_info['iterations'],
+         'relres_true_fp64': plain_info['relres']},
+        'speedup_vs_plain': None if plain_s is None else value * plain_s,
         'assembly': asm,
         'roofline': roofline,
         'cpu_baseline': cpu,

@@ -124,7 +124,8 @@ typedef struct {
   int32_t mixed;         /* 0: FP64 CG; 1: FP32 SpMV/preconditioner with FP64 reliable updates */
   int32_t check_every;   /* iterations per launched chunk between host polls */
   double ru_delta;       /* mixed: reliable update when ||r|| fell by this factor since the last one */
-  int32_t profile;       /* 1: time the first SpMV kernel of every chunk with HIP events */
+  int32_t profile;       /* 1: after the solve, time 20 back-to-back launches of the iteration's
+                            SpMV kernel on the final state with HIP events (stats->spmv_ms) */
   int32_t graph;         /* 1: capture one chunk of iterations in a HIP graph and replay it */
 } fe_pcg_opts;
 
