@@ -91,7 +91,8 @@ def one_solve(sysm, pr, method, fixed, g, f_t, profile=False):
     M = None
     if method.get('model') is not None:
         M = method['model'].preconditioner(sysm, KD, fixed)
-    solver = sysm.solver(KD, precond=method['precond'], precision=method['precision'], M64=M)
+    solver = sysm.solver(KD, precond=method['precond'], precision=method['precision'], M64=M,
+                         coarse=method.get('coarse'))
     u, info = solver.solve(b, tol=1e-8, max_iter=MAX_ITER, profile=profile)
     return u, info
 
@@ -278,10 +279,14 @@ def main():
         from finite_elements_amd.learned import LearnedPreconditioner
         model = LearnedPreconditioner.load(wpath)
         log(f'learned preconditioner weights: {os.path.relpath(wpath, ROOT)}')
+    coarse = sysm.coarse_space(pr.fixed_dofs) if sysm.dof > 0 else None
+    log(f'coarse space: {coarse.host.n_agg} aggregates, {coarse.n_coarse} coarse DOF')
     block = {'name': 'block_jacobi_mixed', 'precond': 'block_jacobi', 'precision': 'mixed'}
-    learned_m = {'name': 'learned_mixed', 'precond': 'block_jacobi', 'precision': 'mixed', 'model': model}
+    two = {'name': 'two_level_mixed', 'precond': 'two_level', 'precision': 'mixed', 'coarse': coarse}
+    learned_m = {'name': 'learned_two_level_mixed', 'precond': 'two_level', 'precision': 'mixed',
+                 'model': model, 'coarse': coarse}
     plain = {'name': 'jacobi_fp64', 'precond': 'jacobi', 'precision': 'fp64'}
-    headline = learned_m if model is not None else block
+    headline = learned_m if model is not None else two
 
     # ---- plain FP64 Jacobi-PCG (the in-GPU baseline) ----
     plain_s, plain_info = None, None
@@ -291,13 +296,14 @@ def main():
         plain_info = plain_infos[-1]
         log(f'plain FP64 Jacobi: {plain_s * 1e3:.1f} ms, {plain_info["iterations"]} it, '
             f'relres {plain_info["relres"]:.2e}')
-    ablation = None
-    if model is not None and args.plain_steps > 0:
-        bt, bi = time_method(sysm, pr, block, fixed, g, f_t, args.plain_steps, 1)
-        ablation = {'method': 'block-Jacobi mixed PCG without the network',
-                    'solves_per_s': 1.0 / float(np.median(bt)), 'ms': float(np.median(bt)) * 1e3,
-                    'iterations': bi[-1]['iterations'], 'relres_true_fp64': bi[-1]['relres']}
-        log(f'block-Jacobi mixed (no network): {ablation["ms"]:.1f} ms, {ablation["iterations"]} it')
+    ablations = {}
+    if args.plain_steps > 0:
+        for m in ([two] if model is not None else []) + [block]:
+            bt, bi = time_method(sysm, pr, m, fixed, g, f_t, args.plain_steps, 1)
+            ablations[m['name']] = {'solves_per_s': 1.0 / float(np.median(bt)),
+                                    'ms': float(np.median(bt)) * 1e3, 'iterations': bi[-1]['iterations'],
+                                    'relres_true_fp64': bi[-1]['relres']}
+            log(f'{m["name"]}: {ablations[m["name"]]["ms"]:.1f} ms, {bi[-1]["iterations"]} it')
 
     # ---- headline method: warmup then K timed steps ----
     for _ in range(args.warmup):
@@ -339,14 +345,16 @@ def main():
         'vs_baseline': None, 'dtype': 'f32+f64', 'data': 'synthetic',
         'config': {'workload': desc, 'dofs': int(sysm.n), 'nnz': int(sysm.pattern.nnz),
                    'elements': int(sysm.n_elem), 'tol': 1e-8,
-                   'method': ('fused assembly + learned block-Jacobi FP32 PCG with FP64 reliable updates'
-                              if model is not None else
-                              'fused assembly + FP32 block-Jacobi PCG with FP64 reliable updates'),
+                   'method': ('fused assembly + two-level (learned block Jacobi + rigid-body coarse '
+                              'correction) FP32 PCG with FP64 reliable updates' if model is not None else
+                              'fused assembly + two-level (block Jacobi + rigid-body coarse correction) '
+                              'FP32 PCG with FP64 reliable updates'),
+                   'coarse_dofs': int(coarse.n_coarse),
                    'step': 'assembly + Dirichlet elimination + preconditioner setup + PCG'},
         'headline': {'method': headline['name'], 'iterations': info['iterations'],
                      'relres_true_fp64': info['relres'], 'reliable_updates': info['reliable_updates'],
-                     'restarts': info['restarts']},
-        'ablation_no_network': ablation,
+                     'restarts': info['restarts'], 'fallbacks': info.get('fallbacks', 0)},
+        'ablations': ablations,
         'plain_fp64_jacobi': None if plain_s is None else
         {'solves_per_s': 1.0 / plain_s, 'ms': plain_s * 1e3, 'iterations': plain_info['iterations'],
          'relres_true_fp64': plain_info['relres']},

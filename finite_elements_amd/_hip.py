@@ -76,6 +76,12 @@ class FePcgSegStats(ctypes.Structure):
                 ('fallback', _i32), ('pad', _i32), ('relres', _dbl), ('relres_rec', _dbl), ('bnorm', _dbl)]
 
 
+class FeCoarse(ctypes.Structure):
+    _fields_ = [('n_nodes', _i64), ('n_agg', _i64), ('n_pairs', _i64), ('dof', _i32), ('nc', _i32),
+                ('node_agg', _vp), ('agg_ptr', _vp), ('agg_nodes', _vp), ('pair_ptr', _vp),
+                ('pair_ij', _vp), ('pair_q', _vp), ('pair_row', _vp), ('P', _vp), ('P32', _vp)]
+
+
 class FeLobpcgOpts(ctypes.Structure):
     _fields_ = [('n_variants', _i32), ('block', _i32), ('n_wanted', _i32), ('largest', _i32),
                 ('max_iter', _i32), ('check_every', _i32), ('tol', _dbl), ('k_stride', _i64),
@@ -120,6 +126,12 @@ SIGNATURES = {
     'fe_pcg_batched_workspace': (_int, [_vp, _P(FeBsr), _i32, _vp, _P(FePcgOpts), _P(_i64)]),
     'fe_pcg_batched': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp, _vp, _i32, _vp, _vp, _vp, _vp, _i64,
                               _P(FePcgOpts), _P(FePcgStats), _vp]),
+    'fe_coarse_ld': (_i64, [_i64]),
+    'fe_coarse_operator': (_int, [_vp, _P(FeBsr), _P(FeCoarse), _int, _vp, _vp, _vp]),
+    'fe_coarse_apply': (_int, [_vp, _P(FeCoarse), _int, _vp, _vp, _vp, _vp]),
+    'fe_pcg_coarse_workspace': (_int, [_vp, _P(FeBsr), _P(FeCoarse), _P(FePcgOpts), _P(_i64)]),
+    'fe_pcg_coarse': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp, _vp, _P(FeCoarse), _vp, _vp, _vp, _vp,
+                             _vp, _i64, _P(FePcgOpts), _P(FePcgStats)]),
     'fe_lobpcg_workspace': (_int, [_vp, _P(FeBsr), _P(FeLobpcgOpts), _P(_i64)]),
     'fe_lobpcg': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64,
                          _P(FeLobpcgOpts), _P(FeLobpcgStats)]),

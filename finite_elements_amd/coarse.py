@@ -1,0 +1,140 @@
+"""Coarse space of the two-level preconditioner: aggregated rigid-body modes.
+
+The nodes are grouped into aggregates of about ``target`` nodes by spatial bins. The bins follow the
+mesh's own coordinates, so any mesh works; aggregates need not be connected. On every aggregate the
+rigid-body modes are evaluated at its nodes:
+
+* 3D elasticity: 3 translations and 3 rotations about the aggregate centroid;
+* 2D elasticity: 2 translations and 1 rotation;
+* scalar problems: the constant.
+
+Rows of constrained DOFs are zeroed, and each aggregate's modes are orthonormalised (QR, rank
+revealed). The result is the tentative prolongator P, with one d x nc block per node. The coarse
+operator A_c = P^T A P is assembled on the GPU per solve (``fe_coarse_operator``) from the block
+pairs grouped here by coarse pair. The preconditioner is M = M_bj + P A_c^-1 P^T, block Jacobi plus
+the coarse correction: SPD, fixed, and cheap to apply (``fe_pcg_coarse``).
+
+Everything here depends on the mesh and the constrained DOFs only, not on the matrix values, and is
+built once per mesh on the host.
+"""
+
+from __future__ import annotations
+
+from typing import Optional
+
+import numpy as np
+
+
+def rigid_modes(coords: np.ndarray, dof: int) -> np.ndarray:
+    """(n_nodes, dof, nc) rigid-body modes at the given (centred) coordinates."""
+    n = coords.shape[0]
+    if dof == 1:
+        return np.ones((n, 1, 1))
+    if dof == 2:
+        Z = np.zeros((n, 2, 3))
+        Z[:, 0, 0] = 1.0
+        Z[:, 1, 1] = 1.0
+        Z[:, 0, 2] = -coords[:, 1]
+        Z[:, 1, 2] = coords[:, 0]
+        return Z
+    Z = np.zeros((n, 3, 6))
+    Z[:, 0, 0] = Z[:, 1, 1] = Z[:, 2, 2] = 1.0
+    x, y, z = coords[:, 0], coords[:, 1], coords[:, 2]
+    # rotations about x, y, z: omega x r
+    Z[:, 1, 3], Z[:, 2, 3] = -z, y
+    Z[:, 0, 4], Z[:, 2, 4] = z, -x
+    Z[:, 0, 5], Z[:, 1, 5] = -y, x
+    return Z
+
+
+def spatial_aggregates(coords: np.ndarray, target: int) -> np.ndarray:
+    """Aggregate id per node: spatial bins holding about ``target`` nodes each."""
+    n, dim = coords.shape
+    lo = coords.min(axis=0)
+    ext = coords.max(axis=0) - lo
+    active = ext > 0
+    k = max(1, int(active.sum()))
+    # bins per active dimension so that n / prod(bins) ~ target, bins proportional to the extent
+    vol = float(np.prod(ext[active])) if active.any() else 1.0
+    width = (vol * target / max(n, 1)) ** (1.0 / k)
+    bins = np.ones(dim, dtype=np.int64)
+    for a in range(dim):
+        if active[a]:
+            bins[a] = max(1, int(round(ext[a] / width)))
+    idx = np.zeros((n, dim), dtype=np.int64)
+    for a in range(dim):
+        if active[a]:
+            idx[:, a] = np.minimum((coords[:, a] - lo[a]) / ext[a] * bins[a], bins[a] - 1).astype(np.int64)
+    flat = np.zeros(n, dtype=np.int64)
+    for a in reversed(range(dim)):
+        flat = flat * bins[a] + idx[:, a]
+    # renumber the non-empty bins 0..n_agg-1
+    _, agg = np.unique(flat, return_inverse=True)
+    return agg.astype(np.int64)
+
+
+class CoarseSpace:
+    """Host description of P: aggregates, the per-node prolongator blocks and the coarse pair map."""
+
+    def __init__(self, coords: np.ndarray, bptr: np.ndarray, bcol: np.ndarray, dof: int,
+                 fixed_dofs: Optional[np.ndarray] = None, target: Optional[int] = None,
+                 node_offsets: Optional[np.ndarray] = None):
+        coords = np.asarray(coords, dtype=np.float64)
+        n = coords.shape[0]
+        self.dof = dof
+        self.nc = {1: 1, 2: 3, 3: 6}[dof]
+        if target is None:
+            target = 729 if coords.shape[1] == 3 else 81
+        self.target = target
+        # aggregates never cross the given node ranges (independent meshes of a batch)
+        offs = np.array([0, n]) if node_offsets is None else np.asarray(node_offsets, dtype=np.int64)
+        agg = np.empty(n, dtype=np.int64)
+        base = 0
+        for a, b in zip(offs[:-1], offs[1:]):
+            if b > a:
+                loc = spatial_aggregates(coords[a:b], target)
+                agg[a:b] = loc + base
+                base += int(loc.max()) + 1
+        self.n_agg = base
+        self.node_agg = agg.astype(np.int32)
+        order = np.argsort(agg, kind='stable')
+        self.agg_nodes = order.astype(np.int32)
+        self.agg_ptr = np.zeros(self.n_agg + 1, dtype=np.int32)
+        self.agg_ptr[1:] = np.cumsum(np.bincount(agg, minlength=self.n_agg))
+        fixed = np.zeros(n * dof, dtype=bool)
+        if fixed_dofs is not None and len(fixed_dofs):
+            fixed[np.asarray(fixed_dofs, dtype=np.int64)] = True
+        fixed = fixed.reshape(n, dof)
+        # prolongator blocks, orthonormal per aggregate
+        P = np.zeros((n, dof, self.nc))
+        self.rank = np.zeros(self.n_agg, dtype=np.int32)
+        for g in range(self.n_agg):
+            nodes = order[self.agg_ptr[g]:self.agg_ptr[g + 1]]
+            x = coords[nodes] - coords[nodes].mean(axis=0)
+            scale = max(np.abs(x).max(), 1e-300)
+            Z = rigid_modes(x / scale, dof)
+            Z[fixed[nodes]] = 0.0
+            M = Z.reshape(-1, self.nc)
+            Q, R = np.linalg.qr(M)
+            dgn = np.abs(np.diag(R))
+            keep = dgn > 1e-10 * max(dgn.max(), 1e-300)
+            Q[:, ~keep] = 0.0
+            Q = Q.reshape(len(nodes), dof, self.nc)
+            Q[fixed[nodes]] = 0.0  # exactly zero on constrained DOFs (QR leaves rounding there)
+            self.rank[g] = int(keep.sum())
+            P[nodes] = Q
+        self.P = P
+        self.n_coarse = self.n_agg * self.nc
+        # coarse pair map: every block (i, j) of the pattern contributes to pair (agg i, agg j)
+        bptr = np.asarray(bptr, dtype=np.int64)
+        bcol = np.asarray(bcol, dtype=np.int64)
+        rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(bptr))
+        key = agg[rows] * self.n_agg + agg[bcol]
+        perm = np.argsort(key, kind='stable')
+        keys = key[perm]
+        uniq, start = np.unique(keys, return_index=True)
+        self.n_pairs = uniq.size
+        self.pair_ptr = np.append(start, keys.size).astype(np.int32)
+        self.pair_ij = np.stack([uniq // self.n_agg, uniq % self.n_agg], axis=1).astype(np.int32)
+        self.pair_q = perm.astype(np.int32)
+        self.pair_row = rows[perm].astype(np.int32)

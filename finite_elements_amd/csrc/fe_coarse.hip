@@ -1,0 +1,364 @@
+// Two-level preconditioner, coarse part: the coarse operator A_c = P^T A P of aggregated rigid-body
+// modes, its inverse, and the coarse correction u += P A_c^-1 P^T r.
+//
+//   k_coarse_ac       one wavefront per coarse block pair (I, J): the sum over the fine blocks (i, j)
+//                     with agg(i) = I, agg(j) = J of Z_i^T A_ij Z_j, in a fixed order (no atomics)
+//   k_gj_*            in-place blocked Gauss-Jordan inversion of the dense SPD A_c (FP64, 64 x 64
+//                     blocks, no pivoting: every pivot block is a Schur complement of an SPD matrix)
+//   k_coarse_restrict one workgroup per aggregate: r_c = P^T r
+//   k_coarse_gemv     one wavefront per coarse row: z = A_c^-1 r_c
+//   k_coarse_prolong  one thread per DOF: u += P z
+// The dense inverse has a leading dimension padded to a multiple of 64 (identity in the padding).
+#include <algorithm>
+
+#include "fe_sparse.h"
+
+namespace {
+
+using namespace fe_sparse;
+
+constexpr int kNB = 64;  // Gauss-Jordan block size
+
+// ---------------------------------------------------------------------------------------------------
+// coarse operator
+// ---------------------------------------------------------------------------------------------------
+template <int D, int NC>
+__global__ __launch_bounds__(64) void k_coarse_ac(fe_coarse C, int64_t nnzb, const int32_t* __restrict__ bcol,
+                                                  const double* __restrict__ vals, int64_t ld,
+                                                  double* __restrict__ Ac) {
+  const int64_t pr = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (lane >= NC * NC) return;
+  const int a = lane / NC, b = lane % NC;
+  const int32_t I = C.pair_ij[2 * pr], J = C.pair_ij[2 * pr + 1];
+  double acc = 0.0;
+  for (int32_t t = C.pair_ptr[pr]; t < C.pair_ptr[pr + 1]; ++t) {
+    const int64_t q = C.pair_q[t];
+    const int64_t i = C.pair_row[t], j = bcol[q];
+    double zj[D];
+#pragma unroll
+    for (int f = 0; f < D; ++f) zj[f] = C.P[(j * D + f) * NC + b];
+#pragma unroll
+    for (int e = 0; e < D; ++e) {
+      double s = 0.0;
+#pragma unroll
+      for (int f = 0; f < D; ++f) s += vals[(int64_t)(e * D + f) * nnzb + q] * zj[f];
+      acc += C.P[(i * D + e) * NC + a] * s;
+    }
+  }
+  // a coarse column that is zero (rank-deficient aggregate) gets a decoupled unit diagonal
+  if (I == J && a == b && acc == 0.0) acc = 1.0;
+  Ac[((int64_t)I * NC + a) * ld + (int64_t)J * NC + b] = acc;
+}
+
+__global__ void k_identity_pad(int64_t n, int64_t ld, double* __restrict__ A) {
+  const int64_t k = n + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < ld) A[k * ld + k] = 1.0;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// blocked Gauss-Jordan inversion (in place, FP64)
+// ---------------------------------------------------------------------------------------------------
+// Invert the 64 x 64 block at (k, k) into LDS sS (in-place Gauss-Jordan, 256 threads).
+__device__ void gj_block_inverse(const double* __restrict__ A, int64_t ld, int k, double (*sS)[kNB + 1]) {
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < kNB * kNB; idx += blockDim.x) {
+    const int r = idx / kNB, c = idx % kNB;
+    sS[r][c] = A[(int64_t)(k * kNB + r) * ld + k * kNB + c];
+  }
+  __syncthreads();
+  __shared__ double s_piv;
+  __shared__ double s_col[kNB];
+  for (int p = 0; p < kNB; ++p) {
+    if (tid == 0) s_piv = 1.0 / sS[p][p];
+    if (tid < kNB) s_col[tid] = sS[tid][p];
+    __syncthreads();
+    const double piv = s_piv;
+    // pivot row: scaled, with the inverse's entry in the pivot position
+    if (tid < kNB) sS[p][tid] = (tid == p ? 1.0 : sS[p][tid]) * piv;
+    __syncthreads();
+    for (int idx = tid; idx < kNB * kNB; idx += blockDim.x) {
+      const int r = idx / kNB, c = idx % kNB;
+      if (r == p) continue;
+      const double f = s_col[r];
+      const double base = (c == p) ? 0.0 : sS[r][c];
+      sS[r][c] = base - f * sS[p][c];
+    }
+    __syncthreads();
+  }
+}
+
+// C (64 x 64, global) = alpha * X * Y (+ C if accumulate), X and Y 64 x 64 tiles in LDS.
+__device__ void tile_gemm_store(double (*sX)[kNB + 1], double (*sY)[kNB + 1], double* __restrict__ Cg,
+                                int64_t ld, double alpha, bool accumulate) {
+  // 256 threads, 4 x 4 outputs each
+  const int tr = (threadIdx.x / 16) * 4, tc = (threadIdx.x % 16) * 4;
+  double acc[4][4] = {};
+  for (int kk = 0; kk < kNB; ++kk) {
+    double x[4], y[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      x[a] = sX[tr + a][kk];
+      y[a] = sY[kk][tc + a];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] += x[a] * y[b];
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      double* c = Cg + (int64_t)(tr + a) * ld + tc + b;
+      *c = (accumulate ? *c : 0.0) + alpha * acc[a][b];
+    }
+}
+
+__device__ void load_tile(const double* __restrict__ A, int64_t ld, int bi, int bj,
+                          double (*s)[kNB + 1]) {
+  for (int idx = threadIdx.x; idx < kNB * kNB; idx += blockDim.x) {
+    const int r = idx / kNB, c = idx % kNB;
+    s[r][c] = A[(int64_t)(bi * kNB + r) * ld + bj * kNB + c];
+  }
+}
+
+// step k, part 1 (one workgroup per block column j): A_kj <- inv(A_kk) A_kj for j != k; workgroup 0
+// also stores inv(A_kk) in S.
+__global__ __launch_bounds__(256) void k_gj_row(int64_t ld, int k, double* __restrict__ A,
+                                                double* __restrict__ S) {
+  __shared__ double sP[kNB][kNB + 1], sT[kNB][kNB + 1];
+  const int j = blockIdx.x;
+  if (j == k && blockIdx.x != 0) return;
+  gj_block_inverse(A, ld, k, sP);
+  if (blockIdx.x == 0)
+    for (int idx = threadIdx.x; idx < kNB * kNB; idx += blockDim.x) S[idx] = sP[idx / kNB][idx % kNB];
+  if (j == k) return;
+  load_tile(A, ld, k, j, sT);
+  __syncthreads();
+  tile_gemm_store(sP, sT, A + (int64_t)(k * kNB) * ld + j * kNB, ld, 1.0, false);
+}
+
+// step k, part 2 (workgroup per block (i, j), i, j != k): A_ij -= A_ik A_kj.
+__global__ __launch_bounds__(256) void k_gj_update(int64_t ld, int k, double* __restrict__ A) {
+  __shared__ double sX[kNB][kNB + 1], sY[kNB][kNB + 1];
+  const int nb = (int)(ld / kNB);
+  const int i = blockIdx.x / nb, j = blockIdx.x % nb;
+  if (i == k || j == k) return;
+  load_tile(A, ld, i, k, sX);
+  load_tile(A, ld, k, j, sY);
+  __syncthreads();
+  tile_gemm_store(sX, sY, A + (int64_t)(i * kNB) * ld + j * kNB, ld, -1.0, true);
+}
+
+// step k, part 3 (workgroup per block row i): A_ik <- -A_ik S for i != k; A_kk <- S.
+__global__ __launch_bounds__(256) void k_gj_col(int64_t ld, int k, double* __restrict__ A,
+                                                const double* __restrict__ S) {
+  __shared__ double sX[kNB][kNB + 1], sY[kNB][kNB + 1];
+  const int i = blockIdx.x;
+  if (i == k) {
+    for (int idx = threadIdx.x; idx < kNB * kNB; idx += blockDim.x)
+      A[(int64_t)(k * kNB + idx / kNB) * ld + k * kNB + idx % kNB] = S[idx];
+    return;
+  }
+  load_tile(A, ld, i, k, sX);
+  for (int idx = threadIdx.x; idx < kNB * kNB; idx += blockDim.x) sY[idx / kNB][idx % kNB] = S[idx];
+  __syncthreads();
+  tile_gemm_store(sX, sY, A + (int64_t)(i * kNB) * ld + k * kNB, ld, -1.0, false);
+}
+
+// Symmetrise the inverse (average with its transpose) and make the FP32 copy.
+__global__ void k_sym_cast(int64_t n, int64_t ld, double* __restrict__ A, float* __restrict__ A32) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n * n) return;
+  const int64_t r = t / n, c = t % n;
+  if (r > c) return;
+  const double v = 0.5 * (A[r * ld + c] + A[c * ld + r]);
+  A[r * ld + c] = v;
+  A[c * ld + r] = v;
+  if (A32) {
+    A32[r * ld + c] = (float)v;
+    A32[c * ld + r] = (float)v;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------
+// coarse correction
+// ---------------------------------------------------------------------------------------------------
+// mode 0: run while the segment iterates; mode 1: run during its reliable update; mode 2: always.
+__device__ inline bool coarse_active(const int32_t* state, int mode) {
+  if (mode == 2) return true;
+  const int done = state[0], need_ru = state[1];
+  return mode == 0 ? (!done && !need_ru) : (!done && need_ru);
+}
+
+template <typename T, int D, int NC>
+__global__ __launch_bounds__(256) void k_coarse_restrict(fe_coarse C, const T* __restrict__ P,
+                                                         const T* __restrict__ r, T* __restrict__ rc,
+                                                         const int32_t* state, int mode) {
+  __shared__ double red[NC][4];
+  __shared__ int s_skip;
+  if (threadIdx.x == 0) s_skip = state && !coarse_active(state, mode);
+  __syncthreads();
+  if (s_skip) return;
+  const int g = blockIdx.x;
+  double acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = 0.0;
+  for (int32_t t = C.agg_ptr[g] + threadIdx.x; t < C.agg_ptr[g + 1]; t += blockDim.x) {
+    const int64_t i = C.agg_nodes[t];
+#pragma unroll
+    for (int e = 0; e < D; ++e) {
+      const double re = (double)r[i * D + e];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[c] += (double)P[(i * D + e) * NC + c] * re;
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    acc[c] = wave_sum(acc[c]);
+    if (lane == 0) red[c][wid] = acc[c];
+  }
+  __syncthreads();
+  if (threadIdx.x < NC) {
+    const int c = threadIdx.x;
+    rc[(int64_t)g * NC + c] = (T)((red[c][0] + red[c][1]) + (red[c][2] + red[c][3]));
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_coarse_gemv(int64_t n, int64_t ld, const T* __restrict__ Ainv,
+                                                     const T* __restrict__ x, T* __restrict__ y,
+                                                     const int32_t* state, int mode) {
+  __shared__ int s_skip;
+  if (threadIdx.x == 0) s_skip = state && !coarse_active(state, mode);
+  __syncthreads();
+  if (s_skip) return;
+  const int64_t row = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  double acc = 0.0;
+  const T* a = Ainv + row * ld;
+  for (int64_t c = lane; c < n; c += 64) acc += (double)a[c] * (double)x[c];
+  acc = wave_sum(acc);
+  if (lane == 0) y[row] = (T)acc;
+}
+
+template <typename T, int D, int NC>
+__global__ __launch_bounds__(256) void k_coarse_prolong(fe_coarse C, const T* __restrict__ P,
+                                                        const T* __restrict__ z, T* __restrict__ u,
+                                                        const int32_t* state, int mode) {
+  __shared__ int s_skip;
+  if (threadIdx.x == 0) s_skip = state && !coarse_active(state, mode);
+  __syncthreads();
+  if (s_skip) return;
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= C.n_nodes * D) return;
+  const int64_t i = k / D;
+  const T* zg = z + (int64_t)C.node_agg[i] * NC;
+  const T* pk = P + k * NC;
+  T s = T(0);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) s += pk[c] * zg[c];
+  u[k] += s;
+}
+
+int check_coarse(const fe_coarse* C, const char* who) {
+  FE_REQUIRE(C, "%s: null coarse space", who);
+  FE_REQUIRE(C->dof >= 1 && C->dof <= 3, "%s: dof must be 1..3", who);
+  FE_REQUIRE(C->nc == (C->dof == 1 ? 1 : C->dof == 2 ? 3 : 6),
+             "%s: nc must be 1, 3 or 6 rigid-body modes for dof 1, 2 or 3", who);
+  FE_REQUIRE(C->n_agg >= 1 && C->n_pairs >= 1 && C->n_nodes >= 1, "%s: empty coarse space", who);
+  FE_REQUIRE(C->node_agg && C->agg_ptr && C->agg_nodes && C->pair_ptr && C->pair_ij && C->pair_q &&
+                 C->pair_row && C->P,
+             "%s: null coarse array", who);
+  FE_REQUIRE(C->n_agg * C->nc <= 16384, "%s: coarse dimension %lld above 16384", who,
+             (long long)(C->n_agg * C->nc));
+  return FE_OK;
+}
+
+}  // namespace
+
+// Launch the three coarse-correction kernels on `stream` (used by the PCG and by fe_coarse_apply).
+// state: the segment's {done, need_ru} words (or null); mode as in coarse_active.
+int fe_coarse_correction(hipStream_t stream, const fe_coarse* C, int fp32, const void* Ainv,
+                         const void* r, void* u, void* rc, void* zc, const int32_t* state,
+                         int mode) {
+  const int64_t nct = C->n_agg * C->nc, ld = fe_coarse_ld(nct);
+  const dim3 gr((unsigned)C->n_agg), gg((unsigned)ceil_div(nct, 4)),
+      gp((unsigned)ceil_div(C->n_nodes * C->dof, 256));
+#define FE_CC(T, D, NC)                                                                            \
+  {                                                                                                \
+    const T* P = fp32 ? (const T*)C->P32 : (const T*)C->P;                                         \
+    hipLaunchKernelGGL((k_coarse_restrict<T, D, NC>), gr, dim3(256), 0, stream, *C, P, (const T*)r, \
+                       (T*)rc, state, mode);                                                       \
+    hipLaunchKernelGGL((k_coarse_gemv<T>), gg, dim3(256), 0, stream, nct, ld, (const T*)Ainv,       \
+                       (const T*)rc, (T*)zc, state, mode);                                         \
+    hipLaunchKernelGGL((k_coarse_prolong<T, D, NC>), gp, dim3(256), 0, stream, *C, P, (const T*)zc, \
+                       (T*)u, state, mode);                                                        \
+  }
+  if (fp32) {
+    if (C->dof == 1) FE_CC(float, 1, 1) else if (C->dof == 2) FE_CC(float, 2, 3) else FE_CC(float, 3, 6)
+  } else {
+    if (C->dof == 1) FE_CC(double, 1, 1) else if (C->dof == 2) FE_CC(double, 2, 3) else FE_CC(double, 3, 6)
+  }
+#undef FE_CC
+  FE_CHECK_LAUNCH();
+  return FE_OK;
+}
+
+extern "C" int64_t fe_coarse_ld(int64_t n_coarse) { return (n_coarse + kNB - 1) / kNB * kNB; }
+
+extern "C" int fe_coarse_operator(fe_handle* h, const fe_bsr* A, const fe_coarse* C, int invert,
+                                  double* Ac, float* Ac32, double* scratch) {
+  FE_ENTER(h);
+  int rc = check_bsr(A, "fe_coarse_operator");
+  if (rc) return rc;
+  rc = check_coarse(C, "fe_coarse_operator");
+  if (rc) return rc;
+  FE_REQUIRE(A->precision == 0 && A->layout == FE_LAYOUT_PLANES,
+             "fe_coarse_operator: A must be FP64 in FE_LAYOUT_PLANES");
+  FE_REQUIRE(A->dof == C->dof && A->n_brows == C->n_nodes, "fe_coarse_operator: A and C disagree");
+  FE_REQUIRE(Ac && (!invert || scratch), "fe_coarse_operator: null output");
+  const int64_t nct = C->n_agg * C->nc, ld = fe_coarse_ld(nct);
+  hipStream_t st = h->stream;
+  FE_CHECK_HIP(hipMemsetAsync(Ac, 0, (size_t)ld * ld * sizeof(double), st));
+  const double* v = (const double*)A->vals;
+#define FE_AC(D, NC)                                                                               \
+  hipLaunchKernelGGL((k_coarse_ac<D, NC>), dim3((unsigned)C->n_pairs), dim3(64), 0, st, *C, A->nnzb, \
+                     A->bcol, v, ld, Ac);
+  if (C->dof == 1) FE_AC(1, 1) else if (C->dof == 2) FE_AC(2, 3) else FE_AC(3, 6)
+#undef FE_AC
+  if (ld > nct)
+    hipLaunchKernelGGL(k_identity_pad, dim3((unsigned)ceil_div(ld - nct, 64)), dim3(64), 0, st, nct,
+                       ld, Ac);
+  FE_CHECK_LAUNCH();
+  if (!invert) return FE_OK;
+  const int nb = (int)(ld / kNB);
+  for (int k = 0; k < nb; ++k) {
+    hipLaunchKernelGGL(k_gj_row, dim3((unsigned)nb), dim3(256), 0, st, ld, k, Ac, scratch);
+    hipLaunchKernelGGL(k_gj_update, dim3((unsigned)(nb * nb)), dim3(256), 0, st, ld, k, Ac);
+    hipLaunchKernelGGL(k_gj_col, dim3((unsigned)nb), dim3(256), 0, st, ld, k, Ac, scratch);
+  }
+  hipLaunchKernelGGL(k_sym_cast, dim3((unsigned)ceil_div(nct * nct, 256)), dim3(256), 0, st, nct, ld,
+                     Ac, Ac32);
+  FE_CHECK_LAUNCH();
+  return FE_OK;
+}
+
+extern "C" int fe_coarse_apply(fe_handle* h, const fe_coarse* C, int precision, const void* Ainv,
+                               const void* r, void* u, void* work) {
+  FE_ENTER(h);
+  int rc = check_coarse(C, "fe_coarse_apply");
+  if (rc) return rc;
+  FE_REQUIRE(precision == 0 || precision == 1, "fe_coarse_apply: precision must be 0 or 1");
+  FE_REQUIRE(precision == 0 || C->P32, "fe_coarse_apply: FP32 needs P32");
+  FE_REQUIRE(Ainv && r && u && work, "fe_coarse_apply: null argument");
+  const int64_t nct = C->n_agg * C->nc;
+  const size_t el = precision ? sizeof(float) : sizeof(double);
+  char* w = (char*)work;
+  void* rcv = w;
+  void* zcv = w + ((nct * el + 255) / 256) * 256;
+  return fe_coarse_correction(h->stream, C, precision, Ainv, r, u, rcv, zcv, nullptr, 2);
+}

@@ -548,6 +548,8 @@ struct Work {
   void* p;
   void* s;
   double* r64;
+  void* rc;  // coarse restriction / solution (two-level only)
+  void* zc;
   int64_t bytes;
 };
 
@@ -558,7 +560,8 @@ int64_t count_tiles(int nseg, const int32_t* seg_nodes, int rows) {
   return t == 0 ? 0 : (t + 7) / 8 * 8;  // padded with empty tiles to a multiple of the 8 XCDs
 }
 
-Work layout_work(char* base, int64_t nb, int dof, int elem, int nseg, int64_t ntiles) {
+Work layout_work(char* base, int64_t nb, int dof, int elem, int nseg, int64_t ntiles,
+                 int64_t n_coarse = 0) {
   const int64_t n = nb * dof;
   Work w{};
   int64_t off = 0;
@@ -579,6 +582,8 @@ Work layout_work(char* base, int64_t nb, int dof, int elem, int nseg, int64_t nt
   w.p = take(n * elem);
   w.s = take(n * elem);
   w.r64 = (double*)take(n * (int64_t)sizeof(double));
+  w.rc = n_coarse ? take(n_coarse * (int64_t)sizeof(double)) : nullptr;
+  w.zc = n_coarse ? take(n_coarse * (int64_t)sizeof(double)) : nullptr;
   w.bytes = off;
   return w;
 }
@@ -591,6 +596,8 @@ struct Problem {
   double* x;
   int nseg;
   int64_t ntiles;
+  const fe_coarse* C;  // two-level: coarse space and the inverse coarse operator (iteration precision)
+  const void* Ainv;
 };
 
 template <typename T, int D, int PC>
@@ -618,16 +625,25 @@ int run_pcg(fe_handle* h, const Problem& pb, const Work& w, const fe_pcg_opts* o
   T* p = (T*)w.p;
   T* sv = (T*)w.s;
   const dim3 bvec(kVecNodes * D);
-  auto launch_ru = [&](hipStream_t q) {
+  auto launch_ru = [&](hipStream_t q) -> int {
     hipLaunchKernelGGL((k_ru_residual<D, G>), gt, dim3(kBlock), 0, q, w.tiles, nt, nnzb, bptr, bcol,
                        v64, pb.b, pb.x, w.r64, w.st, w.partials);
     hipLaunchKernelGGL(k_ru_decide, gseg, dim3(kRed), 0, q, w.seg_tiles, nt, w.partials, w.st);
     hipLaunchKernelGGL((k_ru_prep<T, D, PC>), gt, bvec, 0, q, w.tiles, w.r64, r, u, p, sv, M, w.st);
+    if (pb.C) {
+      int rc3 = fe_coarse_correction(q, pb.C, (int)(sizeof(T) == 4), pb.Ainv, r, u, w.rc, w.zc,
+                                     &w.st[0].done, 1);
+      if (rc3) return rc3;
+    }
     hipLaunchKernelGGL((k_ru_spmv<T, D, G>), gt, dim3(kBlock), 0, q, w.tiles, nt, nnzb, bptr, bcol,
                        vit, u, p, r, wv, sv, w.st, w.partials);
     hipLaunchKernelGGL(k_ru_finish, gseg, dim3(kRed), 0, q, w.seg_tiles, nt, w.partials, w.st);
+    return FE_OK;
   };
-  launch_ru(st);
+  {
+    int rc0 = launch_ru(st);
+    if (rc0) return rc0;
+  }
   FE_CHECK_LAUNCH();
   const int chunk = o->check_every > 0 ? o->check_every : 16;
   PcgStatus* pinned = (PcgStatus*)h->pinned;
@@ -650,11 +666,19 @@ int run_pcg(fe_handle* h, const Problem& pb, const Work& w, const fe_pcg_opts* o
   auto chunk_body = [&](int slot) -> int {
     for (int it = 0; it < chunk; ++it) {
       hipLaunchKernelGGL((k_cg_vec<T, D, PC>), gt, bvec, 0, cs, w.tiles, pb.x, p, sv, r, u, wv, M, w.st);
+      if (pb.C) {
+        int rc3 = fe_coarse_correction(cs, pb.C, (int)(sizeof(T) == 4), pb.Ainv, r, u, w.rc, w.zc,
+                                       &w.st[0].done, 0);
+        if (rc3) return rc3;
+      }
       hipLaunchKernelGGL((k_cg_spmv<T, D, G>), gt, dim3(kBlock), 0, cs, w.tiles, nt, nnzb, bptr, bcol,
                          vit, u, r, wv, w.st, w.partials, 0);
       hipLaunchKernelGGL(k_cg_reduce, gseg, dim3(kRed), 0, cs, w.seg_tiles, nt, w.partials, w.st);
     }
-    launch_ru(cs);
+    {
+      int rc4 = launch_ru(cs);
+      if (rc4) return rc4;
+    }
     hipLaunchKernelGGL(k_pcg_status, dim3(1), dim3(1024), 0, cs, nseg, w.st, w.status);
     FE_CHECK_HIP(hipMemcpyAsync(&pinned[slot], w.status, sizeof(PcgStatus), hipMemcpyDeviceToHost, cs));
     return FE_OK;
@@ -799,12 +823,14 @@ int check_segments(const fe_bsr* A, int nseg, const int32_t* seg_nodes, const ch
 int pcg_common(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M64, const void* M32,
                int nseg, const int32_t* seg_nodes, const double* b, double* x, void* work,
                int64_t work_bytes, const fe_pcg_opts* o, fe_pcg_stats* s, fe_pcg_seg_stats* seg_out,
-               const char* who) {
+               const char* who, const fe_coarse* C = nullptr, const void* Ainv64 = nullptr,
+               const void* Ainv32 = nullptr) {
   const int elem = (int)sizeof(double);  // FP64-sized vectors: the FP64 fallback reuses them
   const int rows = tile_rows_dof(A64->dof);
   const int64_t ntiles = count_tiles(nseg, seg_nodes, rows);
   FE_REQUIRE(ntiles < (int64_t)1 << 31, "%s: too many tiles", who);
-  Work w = layout_work((char*)work, A64->n_brows, A64->dof, elem, nseg, ntiles);
+  Work w = layout_work((char*)work, A64->n_brows, A64->dof, elem, nseg, ntiles,
+                       C ? C->n_agg * C->nc : 0);
   FE_REQUIRE(work_bytes >= w.bytes, "%s: workspace too small (%lld < %lld)", who,
              (long long)work_bytes, (long long)w.bytes);
   // tile list and per-segment tile offsets
@@ -823,7 +849,8 @@ int pcg_common(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M
                               hipMemcpyHostToDevice, h->stream));
   FE_CHECK_HIP(hipMemcpyAsync(w.seg_tiles, seg_tiles.data(), seg_tiles.size() * sizeof(int32_t),
                               hipMemcpyHostToDevice, h->stream));
-  Problem pb{A64, o->mixed ? A32 : A64, o->mixed ? M32 : M64, b, x, nseg, ntiles};
+  Problem pb{A64, o->mixed ? A32 : A64, o->mixed ? M32 : M64, b, x, nseg, ntiles, C,
+             o->mixed ? Ainv32 : Ainv64};
   *s = fe_pcg_stats{};
   int rc = ntiles == 0 ? FE_OK
            : o->mixed  ? dispatch_dof<float>(h, pb, w, o, s)
@@ -843,7 +870,7 @@ int pcg_common(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M
   // segments that already converged stop at the first true-residual check
   int failed = 0;
   for (const auto& q : seg) failed += q.status != FE_OK;
-  if (o->mixed && M64 && failed && ntiles > 0) {
+  if (o->mixed && M64 && failed && ntiles > 0 && (!C || Ainv64)) {
     // a segment that ended worse than the zero vector restarts from zero
     const int D = A64->dof;
     for (int g = 0; g < nseg; ++g)
@@ -854,7 +881,7 @@ int pcg_common(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M
     fe_pcg_opts o64 = *o;
     o64.mixed = 0;
     o64.profile = 0;
-    Problem pb64{A64, A64, M64, b, x, nseg, ntiles};
+    Problem pb64{A64, A64, M64, b, x, nseg, ntiles, C, Ainv64};
     fe_pcg_stats s64{};
     rc = dispatch_dof<double>(h, pb64, w, &o64, &s64);
     if (rc) return rc;
@@ -942,4 +969,35 @@ extern "C" int fe_pcg_batched(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32
   if (rc) return rc;
   return pcg_common(h, A64, A32, M64, M32, n_seg, seg_nodes, b, x, work, work_bytes, o, s, seg_stats,
                     "fe_pcg_batched");
+}
+
+extern "C" int fe_pcg_coarse_workspace(fe_handle* h, const fe_bsr* A, const fe_coarse* C,
+                                       const fe_pcg_opts* opts, int64_t* bytes) {
+  FE_ENTER(h);
+  FE_REQUIRE(A && C && opts && bytes, "fe_pcg_coarse_workspace: null argument");
+  FE_REQUIRE(A->dof >= 1 && A->dof <= 3, "fe_pcg_coarse_workspace: dof must be 1..3");
+  const int32_t seg[2] = {0, (int32_t)A->n_brows};
+  const int64_t ntiles = count_tiles(1, seg, tile_rows_dof(A->dof));
+  *bytes = layout_work(nullptr, A->n_brows, A->dof, (int)sizeof(double), 1, ntiles, C->n_agg * C->nc)
+               .bytes;
+  return FE_OK;
+}
+
+extern "C" int fe_pcg_coarse(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M64,
+                             const void* M32, const fe_coarse* C, const void* Ainv64,
+                             const void* Ainv32, const double* b, double* x, void* work,
+                             int64_t work_bytes, const fe_pcg_opts* o, fe_pcg_stats* s) {
+  FE_ENTER(h);
+  FE_REQUIRE(b && x && work && o && s && C, "fe_pcg_coarse: null argument");
+  int rc = check_pcg_args(A64, A32, M64, M32, o, "fe_pcg_coarse");
+  if (rc) return rc;
+  FE_REQUIRE(C->dof == A64->dof && C->n_nodes == A64->n_brows, "fe_pcg_coarse: coarse space and A disagree");
+  FE_REQUIRE(o->mixed ? (Ainv32 && C->P32) : (Ainv64 != nullptr),
+             "fe_pcg_coarse: missing coarse inverse (FP32 inverse and P32 for mixed precision)");
+  FE_REQUIRE(A64->n_brows < ((int64_t)1 << 31), "fe_pcg_coarse: too many node rows");
+  *s = fe_pcg_stats{};
+  if (A64->n_brows == 0) return FE_OK;
+  const int32_t seg[2] = {0, (int32_t)A64->n_brows};
+  return pcg_common(h, A64, A32, M64, M32, 1, seg, b, x, work, work_bytes, o, s, nullptr,
+                    "fe_pcg_coarse", C, Ainv64, Ainv32);
 }

@@ -390,6 +390,54 @@ def precond_setup(A: BsrMatrix, kind: str = 'block_jacobi') -> torch.Tensor:
     return out
 
 
+class DeviceCoarse:
+    """Device copy of a coarse space (``coarse.CoarseSpace``) and its coarse operators."""
+
+    def __init__(self, ctx: Context, cs):
+        self.ctx = ctx
+        self.host = cs
+        t = ctx.tensor
+        self.node_agg = t(cs.node_agg)
+        self.agg_ptr = t(cs.agg_ptr)
+        self.agg_nodes = t(cs.agg_nodes)
+        self.pair_ptr = t(cs.pair_ptr)
+        self.pair_ij = t(np.ascontiguousarray(cs.pair_ij))
+        self.pair_q = t(cs.pair_q)
+        self.pair_row = t(cs.pair_row)
+        self.P = t(np.ascontiguousarray(cs.P.reshape(-1)))
+        self.P32 = _to_f32(ctx, self.P)
+        self.n_coarse = cs.n_coarse
+        self.ld = int(_hip.load().fe_coarse_ld(cs.n_coarse))
+
+    def struct(self) -> _hip.FeCoarse:
+        cs = self.host
+        return _hip.FeCoarse(cs.P.shape[0], cs.n_agg, cs.n_pairs, cs.dof, cs.nc, _vp(self.node_agg),
+                             _vp(self.agg_ptr), _vp(self.agg_nodes), _vp(self.pair_ptr), _vp(self.pair_ij),
+                             _vp(self.pair_q), _vp(self.pair_row), _vp(self.P), _vp(self.P32))
+
+    def operator(self, A64: BsrMatrix, invert: bool = True):
+        """(A_c or A_c^-1 [ld][ld] FP64, FP32 copy of the inverse or None) for the FP64 plane matrix A64."""
+        ctx = self.ctx
+        Ac = torch.empty(self.ld * self.ld, dtype=torch.float64, device=ctx.torch_device)
+        A32 = torch.empty(self.ld * self.ld, dtype=torch.float32, device=ctx.torch_device) if invert else None
+        scratch = torch.empty(64 * 64, dtype=torch.float64, device=ctx.torch_device)
+        a = A64.struct()
+        c = self.struct()
+        _hip.call('fe_coarse_operator', ctx.h, ctypes.byref(a), ctypes.byref(c), int(invert), _vp(Ac),
+                  _vp(A32), _vp(scratch))
+        return Ac, A32
+
+    def apply(self, Ainv: torch.Tensor, r: torch.Tensor, u: torch.Tensor):
+        """u += P Ainv P^T r (FP64 or FP32 by the dtype of r)."""
+        prec = 0 if r.dtype == torch.float64 else 1
+        el = 8 if prec == 0 else 4
+        work = torch.empty(2 * ((self.n_coarse * el + 255) // 256) * 256, dtype=torch.uint8,
+                           device=r.device)
+        c = self.struct()
+        _hip.call('fe_coarse_apply', self.ctx.h, ctypes.byref(c), prec, _vp(Ainv), _vp(r), _vp(u), _vp(work))
+        return u
+
+
 def _to_f32(ctx: Context, src: torch.Tensor) -> torch.Tensor:
     dst = torch.empty(src.numel(), dtype=torch.float32, device=src.device)
     cast_f64_to_f32(ctx, src, dst)
@@ -405,7 +453,8 @@ class PcgSolver:
 
     def __init__(self, A64: BsrMatrix, precond: str = 'jacobi', mixed: bool = False,
                  M64: Optional[torch.Tensor] = None, A32: Optional[BsrMatrix] = None,
-                 check_every: int = 16, ru_delta: float = 0.1, segments=None, graph: bool = False):
+                 check_every: int = 16, ru_delta: float = 0.1, segments=None, graph: bool = False,
+                 coarse: Optional['DeviceCoarse'] = None):
         self.ctx = A64.graph.ctx
         if A64.layout != _hip.FE_LAYOUT_PLANES or A64.precision != 0:
             A64 = A64.convert(_hip.FE_LAYOUT_PLANES, torch.float64)
@@ -423,6 +472,13 @@ class PcgSolver:
         self.check_every = check_every
         self.ru_delta = ru_delta
         self.graph = bool(graph)
+        self.coarse = coarse
+        self.Ainv64 = self.Ainv32 = None
+        if coarse is not None:
+            if segments is not None:
+                raise ValueError('the two-level preconditioner is for one system (no segments)')
+            # coarse operator and its inverse: part of the preconditioner set-up of every solver
+            self.Ainv64, self.Ainv32 = coarse.operator(A64, invert=True)
         if segments is not None:
             seg = np.ascontiguousarray(np.asarray(segments, dtype=np.int32))
             if seg.ndim != 1 or seg.size < 2 or seg[0] != 0 or seg[-1] != A64.graph.n_nodes:
@@ -455,7 +511,11 @@ class PcgSolver:
         a64 = self.A64.struct()
         a32 = self.A32.struct() if self.A32 is not None else None
         nbytes = ctypes.c_int64()
-        if self.segments is None:
+        cstruct = self.coarse.struct() if self.coarse is not None else None
+        if cstruct is not None:
+            _hip.call('fe_pcg_coarse_workspace', ctx.h, ctypes.byref(a64), ctypes.byref(cstruct),
+                      ctypes.byref(opts), ctypes.byref(nbytes))
+        elif self.segments is None:
             _hip.call('fe_pcg_workspace', ctx.h, ctypes.byref(a64), ctypes.byref(opts),
                       ctypes.byref(nbytes))
         else:
@@ -468,7 +528,11 @@ class PcgSolver:
         stats = _hip.FePcgStats()
         a32p = ctypes.byref(a32) if a32 is not None else None
         seg_info = None
-        if self.segments is None:
+        if cstruct is not None:
+            _hip.call('fe_pcg_coarse', ctx.h, ctypes.byref(a64), a32p, _vp(self.M64), _vp(self.M32),
+                      ctypes.byref(cstruct), _vp(self.Ainv64), _vp(self.Ainv32), _vp(b), _vp(x),
+                      _vp(self._work), self._work.numel(), ctypes.byref(opts), ctypes.byref(stats))
+        elif self.segments is None:
             _hip.call('fe_pcg', ctx.h, ctypes.byref(a64), a32p, _vp(self.M64), _vp(self.M32), _vp(b),
                       _vp(x), _vp(self._work), self._work.numel(), ctypes.byref(opts),
                       ctypes.byref(stats))

@@ -133,16 +133,38 @@ class FESystem:
         KD = device.apply_dirichlet(K, fixed)
         return KD, b
 
+    def coarse_space(self, fixed_dofs, target: Optional[int] = None):
+        """Device coarse space of aggregated rigid-body modes for these constrained DOFs (cached)."""
+        from .coarse import CoarseSpace
+        from .device import DeviceCoarse
+        fixed_dofs = np.asarray(fixed_dofs, dtype=np.int64)
+        key = (fixed_dofs.tobytes(), target)
+        cache = getattr(self, '_coarse_cache', None)
+        if cache is not None and cache[0] == key:
+            return cache[1]
+        cs = CoarseSpace(self._coords, self.pattern.bptr, self.pattern.bcol, self.dof, fixed_dofs, target)
+        dc = DeviceCoarse(self.ctx, cs)
+        self._coarse_cache = (key, dc)
+        return dc
+
     def solver(self, KD: BsrMatrix, precond: str = 'jacobi', precision: str = 'fp64',
                M64: Optional[torch.Tensor] = None, check_every: int = 16,
-               ru_delta: float = 0.1, segments=None, graph: bool = False) -> PcgSolver:
+               ru_delta: float = 0.1, segments=None, graph: bool = False,
+               coarse=None) -> PcgSolver:
+        """PCG for the Dirichlet-applied KD.  precond 'jacobi', 'block_jacobi' or 'two_level'
+        (block Jacobi -- or the learned blocks M64 -- plus the coarse correction of ``coarse``, a
+        ``DeviceCoarse`` from ``coarse_space``)."""
         if precision not in ('fp64', 'mixed'):
             raise ValueError("precision must be 'fp64' or 'mixed'")
-        if precond not in ('jacobi', 'block_jacobi'):
-            raise ValueError("precond must be 'jacobi' or 'block_jacobi' (or a learned block "
-                             "preconditioner passed as M64)")
-        return PcgSolver(KD, precond=precond, mixed=(precision == 'mixed'), M64=M64,
-                         check_every=check_every, ru_delta=ru_delta, segments=segments, graph=graph)
+        if precond not in ('jacobi', 'block_jacobi', 'two_level'):
+            raise ValueError("precond must be 'jacobi', 'block_jacobi' or 'two_level' (or a learned "
+                             "block preconditioner passed as M64)")
+        if precond == 'two_level' and coarse is None:
+            raise ValueError("precond='two_level' needs coarse=FESystem.coarse_space(fixed_dofs)")
+        return PcgSolver(KD, precond='block_jacobi' if precond == 'two_level' else precond,
+                         mixed=(precision == 'mixed'), M64=M64, check_every=check_every,
+                         ru_delta=ru_delta, segments=segments, graph=graph,
+                         coarse=coarse if precond == 'two_level' else None)
 
     def solve(self, f, fixed_dofs, values=None, tol: float = 1e-8, precond: str = 'jacobi',
               precision: str = 'fp64', max_iter: Optional[int] = None):

@@ -273,6 +273,47 @@ int fe_pcg_batched(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const voi
                    double* x, void* work, int64_t work_bytes, const fe_pcg_opts* opts,
                    fe_pcg_stats* stats, fe_pcg_seg_stats* seg_stats);
 
+/* ---- two-level preconditioner -------------------------------------------------------------------
+   No counterpart in the reference (its solve is a direct spsolve, analysis.py:819-822).  M = M_bj +
+   P A_c^-1 P^T: block Jacobi plus a coarse correction over aggregated rigid-body modes.  P has one
+   dof x nc block per node (nc = 1, 3, 6 for dof = 1, 2, 3), zero on constrained DOFs, orthonormal per
+   aggregate; the pair arrays group the blocks q of the pattern (row pair_row, column bcol[q]) by the
+   coarse block pair (agg(row), agg(col)) = pair_ij.  All host-built once per mesh (coarse.py). */
+typedef struct {
+  int64_t n_nodes;
+  int64_t n_agg;
+  int64_t n_pairs;
+  int32_t dof;
+  int32_t nc;
+  const int32_t* node_agg;   /* [n_nodes] aggregate of each node */
+  const int32_t* agg_ptr;    /* [n_agg + 1] ranges of agg_nodes */
+  const int32_t* agg_nodes;  /* nodes grouped by aggregate */
+  const int32_t* pair_ptr;   /* [n_pairs + 1] ranges of pair_q / pair_row */
+  const int32_t* pair_ij;    /* [n_pairs][2] coarse block pair (I, J) */
+  const int32_t* pair_q;     /* block indices grouped by pair (ascending within a pair) */
+  const int32_t* pair_row;   /* node row of each pair_q entry */
+  const double* P;           /* [n_nodes][dof][nc] */
+  const float* P32;          /* FP32 copy for mixed precision (may be NULL otherwise) */
+} fe_coarse;
+/* Leading dimension of the dense coarse matrices: n_agg * nc rounded up to a multiple of 64. */
+int64_t fe_coarse_ld(int64_t n_coarse);
+/* A_c = P^T A P (FP64 planes A) into Ac [ld][ld] (identity in the padding); with invert, Ac is
+   replaced by A_c^-1 (blocked Gauss-Jordan, FP64, symmetrised) and Ac32 (may be NULL) receives its
+   FP32 copy; scratch holds 64 * 64 doubles. */
+int fe_coarse_operator(fe_handle* h, const fe_bsr* A, const fe_coarse* C, int invert, double* Ac,
+                       float* Ac32, double* scratch);
+/* u += P Ainv P^T r in FP64 (precision 0) or FP32 (1); work: 2 * round_up(n_coarse * elem, 256) bytes. */
+int fe_coarse_apply(fe_handle* h, const fe_coarse* C, int precision, const void* Ainv, const void* r,
+                    void* u, void* work);
+/* PCG with the two-level preconditioner (one system): as fe_pcg, with the coarse correction added to
+   the block-Jacobi (or learned) M after every update of the residual.  Ainv64/Ainv32 from
+   fe_coarse_operator (Ainv32 for opts->mixed). */
+int fe_pcg_coarse_workspace(fe_handle* h, const fe_bsr* A, const fe_coarse* C, const fe_pcg_opts* opts,
+                            int64_t* bytes);
+int fe_pcg_coarse(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M64, const void* M32,
+                  const fe_coarse* C, const void* Ainv64, const void* Ainv32, const double* b, double* x,
+                  void* work, int64_t work_bytes, const fe_pcg_opts* opts, fe_pcg_stats* stats);
+
 /* ---- eigen-solve ---------------------------------------------------------------------------------
    Batched LOBPCG for K x = lambda M x (replaces the eigsh / dense-inverse eigs calls of
    finite_elements/analysis.py:741-796).  Variants share the pattern of K; variant v uses the values

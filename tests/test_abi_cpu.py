@@ -18,7 +18,7 @@ HEADER = os.path.join(ROOT, 'include', 'fe_hip.h')
 
 def declared_functions():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r'^\s*(?:int|const char\*)\s+(fe_\w+)\s*\(', text, re.M)))
+    return sorted(set(re.findall(r'^\s*(?:int|int64_t|const char\*)\s+(fe_\w+)\s*\(', text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -34,7 +34,9 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match_header():
     structs = {'fe_mesh': _hip.FeMesh, 'fe_graph': _hip.FeGraph, 'fe_bsr': _hip.FeBsr,
-               'fe_pcg_opts': _hip.FePcgOpts, 'fe_pcg_stats': _hip.FePcgStats}
+               'fe_pcg_opts': _hip.FePcgOpts, 'fe_pcg_stats': _hip.FePcgStats,
+               'fe_pcg_seg_stats': _hip.FePcgSegStats, 'fe_coarse': _hip.FeCoarse,
+               'fe_lobpcg_opts': _hip.FeLobpcgOpts, 'fe_lobpcg_stats': _hip.FeLobpcgStats}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "fe_hip.h"', 'int main(void) {']
     for cname, cls in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')

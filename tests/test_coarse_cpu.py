@@ -1,0 +1,72 @@
+"""Host side of the two-level preconditioner: aggregates, the orthonormal rigid-body prolongator and the
+coarse pair map (finite_elements_amd/coarse.py) against the oracle's assembled P^T A P."""
+
+import numpy as np
+import scipy.sparse as sp
+
+from finite_elements_amd import meshgen
+from finite_elements_amd.coarse import CoarseSpace, rigid_modes
+from oracle import coarse as oc, vectorized as vz
+
+
+def _cube(n=6):
+    pr = meshgen.cube_elasticity(n=n)
+    K = vz.assemble_csr(pr.kind, pr.coords, pr.conn, pr.materials).tocsr()
+    B = sp.csr_matrix((np.ones(K.nnz), K.indices, K.indptr), shape=K.shape)
+    # node graph from the scalar pattern
+    nn = pr.coords.shape[0]
+    G = sp.csr_matrix(K.shape)
+    rows = np.repeat(np.arange(K.shape[0]), np.diff(K.indptr)) // 3
+    cols = K.indices // 3
+    Gn = sp.csr_matrix((np.ones(rows.size), (rows, cols)), shape=(nn, nn))
+    Gn.sum_duplicates()
+    Gn.sort_indices()
+    return pr, K, Gn.indptr, Gn.indices
+
+
+def test_rigid_modes_are_zero_energy():
+    pr, K, bptr, bcol = _cube(4)
+    Z = rigid_modes(pr.coords - pr.coords.mean(axis=0), 3).reshape(-1, 6)
+    # an unconstrained stiffness maps every rigid-body mode to zero force
+    assert np.abs(K @ Z).max() <= 1e-6 * np.abs(K).max()
+
+
+def test_prolongator_orthonormal_and_constrained():
+    pr, K, bptr, bcol = _cube(6)
+    cs = CoarseSpace(pr.coords, bptr, bcol, 3, pr.fixed_dofs, target=27)
+    P = oc.prolongator(cs)
+    PtP = (P.T @ P).toarray()
+    keep = np.diag(PtP) > 0
+    np.testing.assert_allclose(PtP[np.ix_(keep, keep)], np.eye(keep.sum()), atol=1e-12)
+    assert np.abs(P[pr.fixed_dofs]).max() == 0.0
+    assert cs.n_agg == len(np.unique(cs.node_agg))
+    assert np.all(np.diff(cs.agg_ptr) > 0)
+    # every block of the pattern sits in exactly one coarse pair, pairs sorted, rows consistent
+    assert cs.pair_q.size == bcol.size and np.array_equal(np.sort(cs.pair_q), np.arange(bcol.size))
+    rows = np.repeat(np.arange(len(bptr) - 1), np.diff(bptr))
+    assert np.array_equal(cs.pair_row, rows[cs.pair_q])
+    for p in range(cs.n_pairs):
+        qs = cs.pair_q[cs.pair_ptr[p]:cs.pair_ptr[p + 1]]
+        assert np.all(cs.node_agg[rows[qs]] == cs.pair_ij[p, 0])
+        assert np.all(cs.node_agg[bcol[qs]] == cs.pair_ij[p, 1])
+
+
+def test_two_level_oracle_reduces_iterations():
+    """The oracle's two-level PCG needs far fewer iterations than block Jacobi (the point of it)."""
+    pr, K, bptr, bcol = _cube(8)
+    cs = CoarseSpace(pr.coords, bptr, bcol, 3, pr.fixed_dofs, target=27)
+    KD = vz.dirichlet_matrix(K, pr.fixed_dofs)
+    b = pr.f.copy()
+    b[pr.fixed_dofs] = 0.0
+    binv = oc.block_jacobi_inverse(KD, 3)
+    _, it_bj = oc.pcg(KD, b, lambda r: np.einsum('nij,nj->ni', binv, r.reshape(-1, 3)).reshape(-1))
+    Ac, P = oc.coarse_operator(cs, KD)
+    L = np.linalg.cholesky(Ac)
+
+    def m2(r):
+        u = np.einsum('nij,nj->ni', binv, r.reshape(-1, 3)).reshape(-1)
+        return u + P @ np.linalg.solve(L.T, np.linalg.solve(L, P.T @ r))
+    x, it_2 = oc.pcg(KD, b, m2)
+    ref = vz.reduced_solve(K, pr.f, pr.fixed_dofs)
+    assert np.linalg.norm(x - ref) <= 1e-6 * np.linalg.norm(ref)
+    assert it_2 < 0.6 * it_bj, (it_2, it_bj)

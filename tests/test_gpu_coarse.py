@@ -1,0 +1,105 @@
+"""Two-level preconditioner on the GPU: coarse operator, its inverse and the coarse correction against
+the oracle (oracle/coarse.py), and two-level PCG against direct solves."""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+from finite_elements_amd import _hip, device, meshgen
+from finite_elements_amd.system import FESystem
+from oracle import coarse as oc, vectorized as vz
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(m):
+    print(m, flush=True)
+
+
+def _host_matrix(KD):
+    d = KD.dof
+    vals = KD.convert(_hip.FE_LAYOUT_BSR).vals.cpu().numpy().reshape(-1, d, d)
+    g = KD.graph.pattern
+    n = g.n_nodes * d
+    return sp.bsr_matrix((vals, g.bcol, g.bptr), shape=(n, n)).tocsr()
+
+
+def _setup(pr, target):
+    s = FESystem(pr.kind, pr.coords, pr.conn, pr.materials, plane_mode=pr.plane_mode)
+    fixed, g = s.dirichlet(pr.fixed_dofs, pr.fixed_values)
+    KD, b = s.prepare(s.stiffness_bsr(), fixed, g, s.ctx.tensor(pr.f))
+    dc = s.coarse_space(pr.fixed_dofs, target=target)
+    return s, KD, b, dc
+
+
+@pytest.mark.parametrize('case', ['cube', 'cantilever', 'magnetic'])
+def test_coarse_operator_and_inverse(ctx, case):
+    _say(f'coarse operator {case}')
+    pr = {'cube': lambda: meshgen.cube_elasticity(n=6),
+          'cantilever': lambda: meshgen.cantilever_2d(),
+          'magnetic': lambda: meshgen.magnetostatic_2d(n=30)}[case]()
+    s, KD, b, dc = _setup(pr, 27 if case == 'cube' else 16)
+    A = _host_matrix(KD)
+    ref, P = oc.coarse_operator(dc.host, A)
+    n = ref.shape[0]
+    Ac, _ = dc.operator(KD, invert=False)
+    Ac = Ac.cpu().numpy().reshape(dc.ld, dc.ld)
+    assert np.abs(Ac[:n, :n] - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert np.array_equal(Ac[n:, n:], np.eye(dc.ld - n)) and not Ac[:n, n:].any()
+    inv64, inv32 = dc.operator(KD, invert=True)
+    inv64 = inv64.cpu().numpy().reshape(dc.ld, dc.ld)[:n, :n]
+    refinv = np.linalg.inv(ref)
+    err = np.abs(inv64 - refinv).max() / np.abs(refinv).max()
+    _say(f'  n_coarse {n}, inverse rel err {err:.2e}, cond {np.linalg.cond(ref):.2e}')
+    assert err <= 1e-12 * np.linalg.cond(ref)
+    np.testing.assert_allclose(inv32.cpu().numpy().reshape(dc.ld, dc.ld)[:n, :n], inv64.astype(np.float32),
+                               rtol=0, atol=0)
+    # the correction u += P A_c^-1 P^T r in both precisions
+    rng = np.random.default_rng(1)
+    r = rng.standard_normal(A.shape[0])
+    want = P @ np.linalg.solve(ref, P.T @ r)
+    u = ctx.tensor(np.zeros(A.shape[0]))
+    dc.apply(dc.operator(KD)[0], ctx.tensor(r), u)
+    assert np.abs(u.cpu().numpy() - want).max() <= 1e-10 * np.abs(want).max()
+    u32 = torch.zeros(A.shape[0], dtype=torch.float32, device=ctx.torch_device)
+    dc.apply(dc.operator(KD)[1], ctx.tensor(r).float(), u32)
+    assert np.abs(u32.cpu().numpy() - want).max() <= 2e-3 * np.abs(want).max()
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_two_level_pcg_matches_direct_and_oracle_iterations(ctx, precision):
+    _say(f'two-level pcg {precision}')
+    pr = meshgen.cube_elasticity(n=10)
+    s, KD, b, dc = _setup(pr, 64)
+    tol = 1e-10 if precision == 'fp64' else 1e-8
+    u, info = s.solver(KD, 'two_level', precision, coarse=dc).solve(b, tol=tol)
+    _, ib = s.solver(KD, 'block_jacobi', precision).solve(b, tol=tol)
+    _say(f'  two-level {info["iterations"]} it (relres {info["relres"]:.2e}), block Jacobi {ib["iterations"]} it')
+    assert info['status'] == 0 and info['relres'] <= tol
+    assert info['iterations'] < 0.5 * ib['iterations']
+    K = vz.assemble_csr(pr.kind, pr.coords, pr.conn, pr.materials)
+    ref = vz.reduced_solve(K, pr.f, pr.fixed_dofs)
+    assert np.linalg.norm(u.cpu().numpy() - ref) <= 1e-5 * np.linalg.norm(ref)
+    if precision == 'fp64':
+        # the oracle's textbook two-level PCG takes about as many iterations
+        A = _host_matrix(KD)
+        binv = oc.block_jacobi_inverse(A, 3)
+        bh = b.cpu().numpy()
+        _, it_o = oc.pcg(A, bh, lambda r: oc.apply_two_level(dc.host, A, binv, r), tol=tol)
+        _say(f'  oracle two-level PCG {it_o} it')
+        assert abs(info['iterations'] - it_o) <= 0.1 * it_o + 3
+
+
+def test_two_level_2d_and_determinism(ctx):
+    _say('two-level 2D')
+    pr = meshgen.cantilever_2d()
+    s, KD, b, dc = _setup(pr, 16)
+    sol = s.solver(KD, 'two_level', 'fp64', coarse=dc)
+    u1, i1 = sol.solve(b, tol=1e-10)
+    u2, i2 = sol.solve(b, tol=1e-10)
+    assert i1['status'] == 0 and i1['iterations'] == i2['iterations']
+    assert np.array_equal(u1.cpu().numpy(), u2.cpu().numpy())
+    K = vz.assemble_csr(pr.kind, pr.coords, pr.conn, pr.materials, plane_mode=pr.plane_mode)
+    ref = vz.reduced_solve(K, pr.f, pr.fixed_dofs)
+    assert np.linalg.norm(u1.cpu().numpy() - ref) <= 1e-6 * np.linalg.norm(ref)
