@@ -140,7 +140,7 @@ SIGNATURES = {
     'fe_gnn_features': (_int, [_vp, _P(FeBsr), _vp, _vp, _vp]),
     'fe_gnn_forward': (_int, [_vp, _i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     'fe_gnn_loss_workspace': (_int, [_i64, _P(_i64)]),
-    'fe_gnn_loss_grad': (_int, [_vp, _P(FeBsr), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'fe_gnn_loss_grad': (_int, [_vp, _P(FeBsr), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     'fe_gnn_backward_workspace': (_int, [_i64, _P(_i64)]),
     'fe_gnn_backward': (_int, [_vp, _i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     'fe_gnn_adam': (_int, [_vp, _int, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,

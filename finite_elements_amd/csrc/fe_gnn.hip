@@ -161,11 +161,13 @@ __global__ void k_gnn_head(int64_t nb, const float* __restrict__ h, const float*
 }
 
 // ---- loss pieces ----------------------------------------------------------------------------------
-// z = M r, and y = Binv (s * r) (kept for the gradient).
+// z = M r (+ zc, a correction independent of the network, e.g. the coarse part of a two-level
+// preconditioner), and y = Binv (s * r) (kept for the gradient).
 template <int D>
 __global__ void k_gnn_apply(int64_t nb, const double* __restrict__ M, const double* __restrict__ binv,
                             const float* __restrict__ o, const double* __restrict__ r,
-                            double* __restrict__ z, double* __restrict__ y) {
+                            const double* __restrict__ zc, double* __restrict__ z,
+                            double* __restrict__ y) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= nb) return;
   double s[D], u[D];
@@ -183,7 +185,7 @@ __global__ void k_gnn_apply(int64_t nb, const double* __restrict__ M, const doub
       zz += M[i * D * D + a * D + b] * r[i * D + b];
     }
     y[i * D + a] = zy;
-    z[i * D + a] = zz;
+    z[i * D + a] = zz + (zc ? zc[i * D + a] : 0.0);
   }
 }
 
@@ -468,8 +470,8 @@ extern "C" int fe_gnn_forward(fe_handle* h, int64_t n_nodes, int dof, const int3
 }
 
 extern "C" int fe_gnn_loss_grad(fe_handle* h, const fe_bsr* A, const double* binv, const double* M,
-                                const float* o, const double* r, const double* e, double* work,
-                                double* loss_acc, float* dlo) {
+                                const float* o, const double* r, const double* e, const double* zc,
+                                double* work, double* loss_acc, float* dlo) {
   FE_ENTER(h);
   FE_REQUIRE(A && binv && M && o && r && e && work && loss_acc && dlo, "fe_gnn_loss_grad: null argument");
   FE_REQUIRE(A->precision == 0 && A->layout == FE_LAYOUT_PLANES,
@@ -482,9 +484,9 @@ extern "C" int fe_gnn_loss_grad(fe_handle* h, const fe_bsr* A, const double* bin
   double* partials = work + 3 * n;        // 3 * kRedBlocks
   double* scal = partials + 3 * kRedBlocks;  // 4
   const int D = A->dof;
-  if (D == 1) hipLaunchKernelGGL(k_gnn_apply<1>, g1(nb), dim3(256), 0, h->stream, nb, M, binv, o, r, z, y);
-  else if (D == 2) hipLaunchKernelGGL(k_gnn_apply<2>, g1(nb), dim3(256), 0, h->stream, nb, M, binv, o, r, z, y);
-  else hipLaunchKernelGGL(k_gnn_apply<3>, g1(nb), dim3(256), 0, h->stream, nb, M, binv, o, r, z, y);
+  if (D == 1) hipLaunchKernelGGL(k_gnn_apply<1>, g1(nb), dim3(256), 0, h->stream, nb, M, binv, o, r, zc, z, y);
+  else if (D == 2) hipLaunchKernelGGL(k_gnn_apply<2>, g1(nb), dim3(256), 0, h->stream, nb, M, binv, o, r, zc, z, y);
+  else hipLaunchKernelGGL(k_gnn_apply<3>, g1(nb), dim3(256), 0, h->stream, nb, M, binv, o, r, zc, z, y);
   FE_CHECK_LAUNCH();
   int rc = fe_spmv(h, A, z, az);
   if (rc) return rc;

@@ -142,9 +142,10 @@ class LearnedPreconditioner:
         loss = torch.zeros(1, dtype=torch.float64, device=ctx.torch_device)
         dlo = torch.zeros(N * d, dtype=torch.float32, device=ctx.torch_device)
         s = sample.KD.struct()
-        for r, e in sample.pairs:
+        zcs = sample.zcs if sample.zcs is not None else [None] * len(sample.pairs)
+        for (r, e), zc in zip(sample.pairs, zcs):
             _hip.call('fe_gnn_loss_grad', ctx.h, ctypes.byref(s), _vp(sample.binv), _vp(M), _vp(o),
-                      _vp(r), _vp(e), _vp(work), _vp(loss), _vp(dlo))
+                      _vp(r), _vp(e), _vp(zc), _vp(work), _vp(loss), _vp(dlo))
         dlo.mul_(1.0 / len(sample.pairs))
         nb = ctypes.c_int64()
         _hip.call('fe_gnn_backward_workspace', N, ctypes.byref(nb))
@@ -182,20 +183,28 @@ class LearnedPreconditioner:
 
 
 class TrainingSystem:
-    """One training matrix with its residual/error pairs, resident on the device."""
+    """One training matrix with its residual/error pairs, resident on the device.  zcs: per pair, the
+    network-independent part of the preconditioned residual (the coarse correction of the two-level
+    preconditioner), or None for plain block Jacobi."""
 
-    def __init__(self, KD: BsrMatrix, fixed: torch.Tensor, pairs, binv: torch.Tensor, feat=None):
+    def __init__(self, KD: BsrMatrix, fixed: torch.Tensor, pairs, binv: torch.Tensor, feat=None,
+                 zcs=None):
         self.KD = KD
         self.fixed = fixed
         self.pairs = pairs
         self.binv = binv
         self.feat = feat
+        self.zcs = zcs
 
 
 def make_training_systems(n: int = 32, seeds: Sequence[int] = range(1000, 1010),
                           snapshots: Sequence[int] = (10, 40, 160), sigma: float = 1.0,
-                          tol: float = 1e-10, log=None, sysm=None) -> List[TrainingSystem]:
-    """Residual/error pairs from plain block-Jacobi PCG on lognormal-field cubes (one per seed)."""
+                          tol: float = 1e-10, log=None, sysm=None, two_level: bool = False,
+                          coarse_target: Optional[int] = None) -> List[TrainingSystem]:
+    """Residual/error pairs from plain PCG on lognormal-field cubes (one system per seed).
+
+    two_level: the pairs come from two-level PCG (block Jacobi + coarse correction) and every pair
+    carries its coarse correction, so the network learns the smoother of that preconditioner."""
     from .system import FESystem
     out = []
     for seed in seeds:
@@ -207,15 +216,18 @@ def make_training_systems(n: int = 32, seeds: Sequence[int] = range(1000, 1010),
         K = sysm.stiffness_bsr()
         fixed, g = sysm.dirichlet(pr.fixed_dofs)
         KD, b = sysm.prepare(K, fixed, g, sysm.ctx.tensor(pr.f))
-        solver = sysm.solver(KD, 'block_jacobi', 'fp64')
+        coarse = sysm.coarse_space(pr.fixed_dofs, target=coarse_target) if two_level else None
+        solver = sysm.solver(KD, 'two_level' if two_level else 'block_jacobi', 'fp64', coarse=coarse)
         xstar, info = solver.solve(b, tol=tol)
-        pairs = []
+        pairs, zcs = [], []
         for k in snapshots:
             xk, _ = solver.solve(b, tol=tol, max_iter=k)
             rk = b - device.spmv(solver.A64, xk)
             pairs.append((rk, xstar - xk))
+            if two_level:
+                zcs.append(coarse.apply(solver.Ainv64, rk, torch.zeros_like(rk)))
         binv = device.precond_setup(KD, 'block_jacobi')
-        ts = TrainingSystem(solver.A64, fixed, pairs, binv)
+        ts = TrainingSystem(solver.A64, fixed, pairs, binv, zcs=zcs if two_level else None)
         out.append(ts)
         if log:
             log(f'training system seed {seed}: {info["iterations"]} it to {info["relres"]:.1e}')
@@ -224,14 +236,16 @@ def make_training_systems(n: int = 32, seeds: Sequence[int] = range(1000, 1010),
 
 def solve_with_fallback(sysm, KD: BsrMatrix, b: torch.Tensor, fixed: torch.Tensor,
                         model: LearnedPreconditioner, precision: str = 'mixed', tol: float = 1e-8,
-                        max_iter: Optional[int] = None):
-    """Learned-preconditioned PCG; plain block-Jacobi PCG re-solves if it fails to converge."""
+                        max_iter: Optional[int] = None, coarse=None):
+    """Learned-preconditioned PCG (two-level when ``coarse`` is given); the same PCG without the
+    network re-solves if it fails to converge."""
+    pc = 'two_level' if coarse is not None else 'block_jacobi'
     M = model.preconditioner(sysm, KD, fixed)
-    solver = sysm.solver(KD, 'block_jacobi', precision, M64=M)
+    solver = sysm.solver(KD, pc, precision, M64=M, coarse=coarse)
     x, info = solver.solve(b, tol=tol, max_iter=max_iter)
     info['fallback'] = False
     if info['status'] != 0:
-        solver = sysm.solver(KD, 'block_jacobi', precision)
+        solver = sysm.solver(KD, pc, precision, coarse=coarse)
         x, info = solver.solve(b, tol=tol, max_iter=max_iter)
         info['fallback'] = True
     return x, info

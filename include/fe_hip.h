@@ -360,8 +360,10 @@ int fe_sym_eig(fe_handle* h, int n_mat, int r, const double* A, double* w, doubl
    s_i = exp(clamp(o_i, -4, 4)).  3 message-passing layers of width 32 on 8 scale-free node features.
    Parameters are one FP32 vector of fe_gnn_num_params(dof) entries.
    acts: [3][n_nodes][32] layer outputs; aggs: [n_nodes][8 + 32 + 32] neighbour means (for backward).
-   fe_gnn_loss_grad accumulates, for one residual/error pair (r, e) with A e = r and z = M r, the loss
-   L = 1 - (r.z)^2 / ((z.Az)(r.e)) into *loss_acc and dL/do into dlo. */
+   fe_gnn_loss_grad accumulates, for one residual/error pair (r, e) with A e = r and z = M r + zc, the
+   loss L = 1 - (r.z)^2 / ((z.Az)(r.e)) into *loss_acc and dL/do into dlo.  zc (may be NULL) is a part
+   of the preconditioned residual that does not depend on the network: the coarse correction
+   P A_c^-1 P^T r when the learned blocks are the smoother of the two-level preconditioner. */
 int fe_gnn_num_params(int dof); /* returns the count (>= 0) or FE_ERR_ARG */
 int fe_gnn_features(fe_handle* h, const fe_bsr* A, const uint8_t* fixed, float* feat, double* scratch);
 int fe_gnn_forward(fe_handle* h, int64_t n_nodes, int dof, const int32_t* bptr, const int32_t* bcol,
@@ -369,7 +371,7 @@ int fe_gnn_forward(fe_handle* h, int64_t n_nodes, int dof, const int32_t* bptr, 
                    const double* binv, float* o, double* M);
 int fe_gnn_loss_workspace(int64_t n_dofs, int64_t* doubles);
 int fe_gnn_loss_grad(fe_handle* h, const fe_bsr* A, const double* binv, const double* M,
-                     const float* o, const double* r, const double* e, double* work,
+                     const float* o, const double* r, const double* e, const double* zc, double* work,
                      double* loss_acc, float* dlo);
 int fe_gnn_backward_workspace(int64_t n_nodes, int64_t* floats);
 int fe_gnn_backward(fe_handle* h, int64_t n_nodes, int dof, const int32_t* bptr, const int32_t* bcol,

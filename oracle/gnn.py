@@ -84,19 +84,22 @@ def forward(params, feat, avg, d):
     return h @ Wo.T + bo
 
 
-def loss(params, feat, avg, binv, A, pairs, d):
-    """Mean loss over the (r, e) pairs; binv (N, d, d); A scipy (n, n)."""
+def loss(params, feat, avg, binv, A, pairs, d, zc=None):
+    """Mean loss over the (r, e) pairs; binv (N, d, d); A scipy (n, n).  zc: optional per-pair vectors
+    added to z = M r (a network-independent part of the preconditioner, e.g. a coarse correction)."""
     o = forward(params, feat, avg, d)
     s = torch.exp(torch.clamp(o, -4.0, 4.0))  # (N, d)
     B = torch.as_tensor(binv, dtype=torch.float64)
     Ac = A.tocoo()
     At = torch.sparse_coo_tensor(np.vstack([Ac.row, Ac.col]), Ac.data, A.shape, dtype=torch.float64)
     total = 0.0
-    for r, e in pairs:
+    for k, (r, e) in enumerate(pairs):
         r = torch.as_tensor(r, dtype=torch.float64)
         e = torch.as_tensor(e, dtype=torch.float64)
         rn = r.reshape(-1, d)
         z = (s * torch.einsum('nij,nj->ni', B, s * rn)).reshape(-1)
+        if zc is not None:
+            z = z + torch.as_tensor(zc[k], dtype=torch.float64)
         az = torch.sparse.mm(At, z[:, None])[:, 0]
         rz = torch.dot(r, z)
         total = total + 1.0 - rz * rz / (torch.dot(z, az) * torch.dot(r, e))
