@@ -200,11 +200,13 @@ class TrainingSystem:
 def make_training_systems(n: int = 32, seeds: Sequence[int] = range(1000, 1010),
                           snapshots: Sequence[int] = (10, 40, 160), sigma: float = 1.0,
                           tol: float = 1e-10, log=None, sysm=None, two_level: bool = False,
-                          coarse_target: Optional[int] = None) -> List[TrainingSystem]:
+                          coarse_target: Optional[int] = None, probes: int = 0) -> List[TrainingSystem]:
     """Residual/error pairs from plain PCG on lognormal-field cubes (one system per seed).
 
     two_level: the pairs come from two-level PCG (block Jacobi + coarse correction) and every pair
-    carries its coarse correction, so the network learns the smoother of that preconditioner."""
+    carries its coarse correction, so the network learns the smoother of that preconditioner.
+    probes: extra pairs per system with a random (rough) error e, r = K e, so the loss also sees
+    the top of the spectrum that the smooth PCG errors miss."""
     from .system import FESystem
     out = []
     for seed in seeds:
@@ -226,6 +228,15 @@ def make_training_systems(n: int = 32, seeds: Sequence[int] = range(1000, 1010),
             pairs.append((rk, xstar - xk))
             if two_level:
                 zcs.append(coarse.apply(solver.Ainv64, rk, torch.zeros_like(rk)))
+        if probes:
+            gen = torch.Generator(device='cpu').manual_seed(int(seed))
+            for _ in range(probes):
+                e = torch.randn(b.numel(), generator=gen, dtype=torch.float64).to(b.device)
+                e[fixed.bool()] = 0.0
+                r = device.spmv(solver.A64, e)
+                pairs.append((r, e))
+                if two_level:
+                    zcs.append(coarse.apply(solver.Ainv64, r, torch.zeros_like(r)))
         binv = device.precond_setup(KD, 'block_jacobi')
         ts = TrainingSystem(solver.A64, fixed, pairs, binv, zcs=zcs if two_level else None)
         out.append(ts)

@@ -28,6 +28,7 @@ def main(argv=None):
     ap.add_argument('--epochs', type=int, default=20)
     ap.add_argument('--lr', type=float, default=3e-3)
     ap.add_argument('--snapshots', default='3,10,30,90')
+    ap.add_argument('--probes', type=int, default=2, help='random-error pairs per training system')
     ap.add_argument('--block-jacobi-only', action='store_true',
                     help='train the network as plain block Jacobi (no coarse correction)')
     ap.add_argument('--eval-n', type=int, default=32)
@@ -45,7 +46,8 @@ def main(argv=None):
     snaps = tuple(int(s) for s in args.snapshots.split(','))
     two = not args.block_jacobi_only
     systems = learned.make_training_systems(n=args.n, seeds=range(1000, 1000 + args.systems),
-                                            snapshots=snaps, log=log, two_level=two)
+                                            snapshots=snaps, log=log, two_level=two,
+                                            probes=args.probes)
     log(f'{len(systems)} training systems in {time.perf_counter() - t0:.1f} s')
     model = learned.LearnedPreconditioner(3, seed=0)
     t1 = time.perf_counter()
