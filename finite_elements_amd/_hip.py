@@ -57,7 +57,7 @@ class FeGraph(ctypes.Structure):
 
 class FeBsr(ctypes.Structure):
     _fields_ = [('n_brows', _i64), ('nnzb', _i64), ('dof', _i32), ('precision', _i32),
-                ('layout', _i32), ('pad', _i32), ('bptr', _vp), ('bcol', _vp), ('vals', _vp)]
+                ('layout', _i32), ('row_align', _i32), ('bptr', _vp), ('bcol', _vp), ('vals', _vp)]
 
 
 class FePcgOpts(ctypes.Structure):
@@ -119,6 +119,7 @@ SIGNATURES = {
     'fe_spmv_csr': (_int, [_vp, _i64, _vp, _vp, _vp, _vp, _vp]),
     'fe_cast_f64_to_f32': (_int, [_vp, _i64, _vp, _vp]),
     'fe_bsr_convert': (_int, [_vp, _P(FeBsr), _P(FeBsr)]),
+    'fe_bsr_repack': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp]),
     'fe_precond_setup': (_int, [_vp, _P(FeBsr), _int, _vp]),
     'fe_pcg_workspace': (_int, [_vp, _P(FeBsr), _P(FePcgOpts), _P(_i64)]),
     'fe_pcg': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp, _vp, _vp, _vp, _vp, _i64, _P(FePcgOpts),

@@ -405,6 +405,7 @@ extern "C" int fe_apply_dirichlet_bsr(fe_handle* h, const fe_bsr* A, const uint8
   FE_ENTER(h);
   FE_REQUIRE(A && fixed && vals, "fe_apply_dirichlet_bsr: null argument");
   FE_REQUIRE(A->precision == 0, "fe_apply_dirichlet_bsr: FP64 matrix expected");
+  FE_REQUIRE(A->row_align == 0, "fe_apply_dirichlet_bsr: the matrix's own pattern expected (row_align 0)");
   FE_REQUIRE(A->dof >= 1 && A->dof <= 3, "fe_apply_dirichlet_bsr: dof must be 1..3");
   FE_REQUIRE(A->layout == FE_LAYOUT_BSR || A->layout == FE_LAYOUT_PLANES,
              "fe_apply_dirichlet_bsr: layout must be FE_LAYOUT_BSR or FE_LAYOUT_PLANES");

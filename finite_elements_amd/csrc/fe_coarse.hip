@@ -1,7 +1,7 @@
 // Two-level preconditioner, coarse part: the coarse operator A_c = P^T A P of aggregated rigid-body
 // modes, its inverse, and the coarse correction u += P A_c^-1 P^T r.
 //
-//   k_coarse_ac       one wavefront per coarse block pair (I, J): the sum over the fine blocks (i, j)
+//   k_coarse_ac       one workgroup per coarse block pair (I, J): the sum over the fine blocks (i, j)
 //                     with agg(i) = I, agg(j) = J of Z_i^T A_ij Z_j, in a fixed order (no atomics)
 //   k_gj_*            in-place blocked Gauss-Jordan inversion of the dense SPD A_c (FP64, 64 x 64
 //                     blocks, no pivoting: every pivot block is a Schur complement of an SPD matrix)
@@ -22,33 +22,71 @@ constexpr int kNB = 64;  // Gauss-Jordan block size
 // ---------------------------------------------------------------------------------------------------
 // coarse operator
 // ---------------------------------------------------------------------------------------------------
+constexpr int kAcThreads = 128;
+
+// One workgroup per coarse pair (I, J); its threads stride over the pair's fine blocks (ascending
+// order), each accumulating Z_i^T A_ij Z_j, then a fixed-order workgroup reduction.
 template <int D, int NC>
-__global__ __launch_bounds__(64) void k_coarse_ac(fe_coarse C, int64_t nnzb, const int32_t* __restrict__ bcol,
-                                                  const double* __restrict__ vals, int64_t ld,
-                                                  double* __restrict__ Ac) {
+__global__ __launch_bounds__(kAcThreads) void k_coarse_ac(fe_coarse C, int64_t nnzb,
+                                                          const int32_t* __restrict__ bcol,
+                                                          const double* __restrict__ vals, int64_t ld,
+                                                          double* __restrict__ Ac) {
+  __shared__ double red[NC * NC][kAcThreads / 64];
   const int64_t pr = blockIdx.x;
-  const int lane = threadIdx.x;
-  if (lane >= NC * NC) return;
-  const int a = lane / NC, b = lane % NC;
   const int32_t I = C.pair_ij[2 * pr], J = C.pair_ij[2 * pr + 1];
-  double acc = 0.0;
-  for (int32_t t = C.pair_ptr[pr]; t < C.pair_ptr[pr + 1]; ++t) {
+  double acc[NC * NC];
+#pragma unroll
+  for (int k = 0; k < NC * NC; ++k) acc[k] = 0.0;
+  for (int32_t t = C.pair_ptr[pr] + threadIdx.x; t < C.pair_ptr[pr + 1]; t += kAcThreads) {
     const int64_t q = C.pair_q[t];
     const int64_t i = C.pair_row[t], j = bcol[q];
-    double zj[D];
+    double a[D * D];
 #pragma unroll
-    for (int f = 0; f < D; ++f) zj[f] = C.P[(j * D + f) * NC + b];
+    for (int k = 0; k < D * D; ++k) a[k] = vals[(int64_t)k * nnzb + q];
+    // T = A_ij Z_j (D x NC)
+    double tz[D][NC];
+#pragma unroll
+    for (int e = 0; e < D; ++e)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) tz[e][c] = 0.0;
+#pragma unroll
+    for (int f = 0; f < D; ++f) {
+      double zj[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) zj[c] = C.P[(j * D + f) * NC + c];
+#pragma unroll
+      for (int e = 0; e < D; ++e)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) tz[e][c] += a[e * D + f] * zj[c];
+    }
+    // acc += Z_i^T T
 #pragma unroll
     for (int e = 0; e < D; ++e) {
-      double s = 0.0;
+      double zi[NC];
 #pragma unroll
-      for (int f = 0; f < D; ++f) s += vals[(int64_t)(e * D + f) * nnzb + q] * zj[f];
-      acc += C.P[(i * D + e) * NC + a] * s;
+      for (int c = 0; c < NC; ++c) zi[c] = C.P[(i * D + e) * NC + c];
+#pragma unroll
+      for (int x = 0; x < NC; ++x)
+#pragma unroll
+        for (int y = 0; y < NC; ++y) acc[x * NC + y] += zi[x] * tz[e][y];
     }
   }
-  // a coarse column that is zero (rank-deficient aggregate) gets a decoupled unit diagonal
-  if (I == J && a == b && acc == 0.0) acc = 1.0;
-  Ac[((int64_t)I * NC + a) * ld + (int64_t)J * NC + b] = acc;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NC * NC; ++k) {
+    const double v = wave_sum(acc[k]);
+    if (lane == 0) red[k][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NC * NC) {
+    const int k = threadIdx.x, x = k / NC, y = k % NC;
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < kAcThreads / 64; ++w) v += red[k][w];
+    // a coarse column that is zero (rank-deficient aggregate) gets a decoupled unit diagonal
+    if (I == J && x == y && v == 0.0) v = 1.0;
+    Ac[((int64_t)I * NC + x) * ld + (int64_t)J * NC + y] = v;
+  }
 }
 
 __global__ void k_identity_pad(int64_t n, int64_t ld, double* __restrict__ A) {
@@ -59,35 +97,6 @@ __global__ void k_identity_pad(int64_t n, int64_t ld, double* __restrict__ A) {
 // ---------------------------------------------------------------------------------------------------
 // blocked Gauss-Jordan inversion (in place, FP64)
 // ---------------------------------------------------------------------------------------------------
-// Invert the 64 x 64 block at (k, k) into LDS sS (in-place Gauss-Jordan, 256 threads).
-__device__ void gj_block_inverse(const double* __restrict__ A, int64_t ld, int k, double (*sS)[kNB + 1]) {
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < kNB * kNB; idx += blockDim.x) {
-    const int r = idx / kNB, c = idx % kNB;
-    sS[r][c] = A[(int64_t)(k * kNB + r) * ld + k * kNB + c];
-  }
-  __syncthreads();
-  __shared__ double s_piv;
-  __shared__ double s_col[kNB];
-  for (int p = 0; p < kNB; ++p) {
-    if (tid == 0) s_piv = 1.0 / sS[p][p];
-    if (tid < kNB) s_col[tid] = sS[tid][p];
-    __syncthreads();
-    const double piv = s_piv;
-    // pivot row: scaled, with the inverse's entry in the pivot position
-    if (tid < kNB) sS[p][tid] = (tid == p ? 1.0 : sS[p][tid]) * piv;
-    __syncthreads();
-    for (int idx = tid; idx < kNB * kNB; idx += blockDim.x) {
-      const int r = idx / kNB, c = idx % kNB;
-      if (r == p) continue;
-      const double f = s_col[r];
-      const double base = (c == p) ? 0.0 : sS[r][c];
-      sS[r][c] = base - f * sS[p][c];
-    }
-    __syncthreads();
-  }
-}
-
 // C (64 x 64, global) = alpha * X * Y (+ C if accumulate), X and Y 64 x 64 tiles in LDS.
 __device__ void tile_gemm_store(double (*sX)[kNB + 1], double (*sY)[kNB + 1], double* __restrict__ Cg,
                                 int64_t ld, double alpha, bool accumulate) {
@@ -123,17 +132,45 @@ __device__ void load_tile(const double* __restrict__ A, int64_t ld, int bi, int 
   }
 }
 
-// step k, part 1 (one workgroup per block column j): A_kj <- inv(A_kk) A_kj for j != k; workgroup 0
-// also stores inv(A_kk) in S.
+// step k, part 0 (one workgroup of 1024 threads): S = inv(A_kk) by in-place Gauss-Jordan.  Thread
+// (g, c) keeps rows 4g..4g+3 of column c in registers; each elimination step broadcasts the pivot row
+// and the pivot column through LDS buffers that alternate between steps, so one barrier per step
+// separates their writes from the reads.
+constexpr int kInvThreads = 1024;
+
+__global__ __launch_bounds__(kInvThreads) void k_gj_inv(int64_t ld, int k, const double* __restrict__ A,
+                                                        double* __restrict__ S) {
+  __shared__ double sRow[2][kNB], sCol[2][kNB];
+  const int c = threadIdx.x % kNB, rg = threadIdx.x / kNB;
+  double a[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) a[q] = A[(int64_t)(k * kNB + rg * 4 + q) * ld + k * kNB + c];
+  for (int p = 0; p < kNB; ++p) {
+    const int buf = p & 1, qp = p & 3;
+    if (rg == (p >> 2)) sRow[buf][c] = qp == 0 ? a[0] : qp == 1 ? a[1] : qp == 2 ? a[2] : a[3];
+    if (c == p)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sCol[buf][rg * 4 + q] = a[q];
+    __syncthreads();
+    const double piv = 1.0 / sRow[buf][p];
+    const double prow = (c == p ? 1.0 : sRow[buf][c]) * piv;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = rg * 4 + q;
+      a[q] = (r == p) ? prow : (c == p ? 0.0 : a[q]) - sCol[buf][r] * prow;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) S[(rg * 4 + q) * kNB + c] = a[q];
+}
+
+// step k, part 1 (one workgroup per block column j != k): A_kj <- S A_kj.
 __global__ __launch_bounds__(256) void k_gj_row(int64_t ld, int k, double* __restrict__ A,
-                                                double* __restrict__ S) {
+                                                const double* __restrict__ S) {
   __shared__ double sP[kNB][kNB + 1], sT[kNB][kNB + 1];
   const int j = blockIdx.x;
-  if (j == k && blockIdx.x != 0) return;
-  gj_block_inverse(A, ld, k, sP);
-  if (blockIdx.x == 0)
-    for (int idx = threadIdx.x; idx < kNB * kNB; idx += blockDim.x) S[idx] = sP[idx / kNB][idx % kNB];
   if (j == k) return;
+  for (int idx = threadIdx.x; idx < kNB * kNB; idx += blockDim.x) sP[idx / kNB][idx % kNB] = S[idx];
   load_tile(A, ld, k, j, sT);
   __syncthreads();
   tile_gemm_store(sP, sT, A + (int64_t)(k * kNB) * ld + j * kNB, ld, 1.0, false);
@@ -192,11 +229,14 @@ __device__ inline bool coarse_active(const int32_t* state, int mode) {
   return mode == 0 ? (!done && !need_ru) : (!done && need_ru);
 }
 
+constexpr int kRestrictThreads = 1024;
+
 template <typename T, int D, int NC>
-__global__ __launch_bounds__(256) void k_coarse_restrict(fe_coarse C, const T* __restrict__ P,
-                                                         const T* __restrict__ r, T* __restrict__ rc,
-                                                         const int32_t* state, int mode) {
-  __shared__ double red[NC][4];
+__global__ __launch_bounds__(kRestrictThreads) void k_coarse_restrict(fe_coarse C, const T* __restrict__ P,
+                                                                      const T* __restrict__ r,
+                                                                      T* __restrict__ rc,
+                                                                      const int32_t* state, int mode) {
+  __shared__ double red[NC][kRestrictThreads / 64];
   __shared__ int s_skip;
   if (threadIdx.x == 0) s_skip = state && !coarse_active(state, mode);
   __syncthreads();
@@ -223,14 +263,29 @@ __global__ __launch_bounds__(256) void k_coarse_restrict(fe_coarse C, const T* _
   __syncthreads();
   if (threadIdx.x < NC) {
     const int c = threadIdx.x;
-    rc[(int64_t)g * NC + c] = (T)((red[c][0] + red[c][1]) + (red[c][2] + red[c][3]));
+    double v = 0.0;
+    for (int q = 0; q < kRestrictThreads / 64; ++q) v += red[c][q];
+    rc[(int64_t)g * NC + c] = (T)v;
   }
 }
 
 template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using type = float4;
+};
+template <>
+struct Vec4<double> {
+  using type = double4;
+};
+
+// One wavefront per row; 4-wide vector loads of the row (rows start 64-aligned: ld % 64 == 0).
+template <typename T>
 __global__ __launch_bounds__(256) void k_coarse_gemv(int64_t n, int64_t ld, const T* __restrict__ Ainv,
                                                      const T* __restrict__ x, T* __restrict__ y,
                                                      const int32_t* state, int mode) {
+  using V = typename Vec4<T>::type;
   __shared__ int s_skip;
   if (threadIdx.x == 0) s_skip = state && !coarse_active(state, mode);
   __syncthreads();
@@ -238,9 +293,15 @@ __global__ __launch_bounds__(256) void k_coarse_gemv(int64_t n, int64_t ld, cons
   const int64_t row = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
-  double acc = 0.0;
   const T* a = Ainv + row * ld;
-  for (int64_t c = lane; c < n; c += 64) acc += (double)a[c] * (double)x[c];
+  const int64_t n4 = n & ~(int64_t)3;
+  double acc = 0.0;
+  for (int64_t c = 4 * lane; c < n4; c += 256) {
+    const V av = *reinterpret_cast<const V*>(a + c);
+    acc += (double)av.x * (double)x[c] + (double)av.y * (double)x[c + 1] + (double)av.z * (double)x[c + 2] +
+           (double)av.w * (double)x[c + 3];
+  }
+  for (int64_t c = n4 + lane; c < n; c += 64) acc += (double)a[c] * (double)x[c];
   acc = wave_sum(acc);
   if (lane == 0) y[row] = (T)acc;
 }
@@ -291,8 +352,9 @@ int fe_coarse_correction(hipStream_t stream, const fe_coarse* C, int fp32, const
 #define FE_CC(T, D, NC)                                                                            \
   {                                                                                                \
     const T* P = fp32 ? (const T*)C->P32 : (const T*)C->P;                                         \
-    hipLaunchKernelGGL((k_coarse_restrict<T, D, NC>), gr, dim3(256), 0, stream, *C, P, (const T*)r, \
-                       (T*)rc, state, mode);                                                       \
+    hipLaunchKernelGGL((k_coarse_restrict<T, D, NC>), gr, dim3(kRestrictThreads), 0, stream, *C, P,  \
+                       (const T*)r,                                                                 \
+                       (T*)rc, state, mode);                                                        \
     hipLaunchKernelGGL((k_coarse_gemv<T>), gg, dim3(256), 0, stream, nct, ld, (const T*)Ainv,       \
                        (const T*)rc, (T*)zc, state, mode);                                         \
     hipLaunchKernelGGL((k_coarse_prolong<T, D, NC>), gp, dim3(256), 0, stream, *C, P, (const T*)zc, \
@@ -320,13 +382,15 @@ extern "C" int fe_coarse_operator(fe_handle* h, const fe_bsr* A, const fe_coarse
   FE_REQUIRE(A->precision == 0 && A->layout == FE_LAYOUT_PLANES,
              "fe_coarse_operator: A must be FP64 in FE_LAYOUT_PLANES");
   FE_REQUIRE(A->dof == C->dof && A->n_brows == C->n_nodes, "fe_coarse_operator: A and C disagree");
+  FE_REQUIRE(A->row_align == 0, "fe_coarse_operator: the pattern of the coarse pair map expected");
   FE_REQUIRE(Ac && (!invert || scratch), "fe_coarse_operator: null output");
   const int64_t nct = C->n_agg * C->nc, ld = fe_coarse_ld(nct);
   hipStream_t st = h->stream;
   FE_CHECK_HIP(hipMemsetAsync(Ac, 0, (size_t)ld * ld * sizeof(double), st));
   const double* v = (const double*)A->vals;
 #define FE_AC(D, NC)                                                                               \
-  hipLaunchKernelGGL((k_coarse_ac<D, NC>), dim3((unsigned)C->n_pairs), dim3(64), 0, st, *C, A->nnzb, \
+  hipLaunchKernelGGL((k_coarse_ac<D, NC>), dim3((unsigned)C->n_pairs), dim3(kAcThreads), 0, st, *C, \
+                     A->nnzb,                                                                        \
                      A->bcol, v, ld, Ac);
   if (C->dof == 1) FE_AC(1, 1) else if (C->dof == 2) FE_AC(2, 3) else FE_AC(3, 6)
 #undef FE_AC
@@ -337,6 +401,7 @@ extern "C" int fe_coarse_operator(fe_handle* h, const fe_bsr* A, const fe_coarse
   if (!invert) return FE_OK;
   const int nb = (int)(ld / kNB);
   for (int k = 0; k < nb; ++k) {
+    hipLaunchKernelGGL(k_gj_inv, dim3(1), dim3(kInvThreads), 0, st, ld, k, Ac, scratch);
     hipLaunchKernelGGL(k_gj_row, dim3((unsigned)nb), dim3(256), 0, st, ld, k, Ac, scratch);
     hipLaunchKernelGGL(k_gj_update, dim3((unsigned)(nb * nb)), dim3(256), 0, st, ld, k, Ac);
     hipLaunchKernelGGL(k_gj_col, dim3((unsigned)nb), dim3(256), 0, st, ld, k, Ac, scratch);

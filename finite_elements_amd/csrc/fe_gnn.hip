@@ -431,6 +431,7 @@ extern "C" int fe_gnn_features(fe_handle* h, const fe_bsr* A, const uint8_t* fix
   FE_ENTER(h);
   FE_REQUIRE(A && fixed && feat && scratch, "fe_gnn_features: null argument");
   FE_REQUIRE(A->precision == 0, "fe_gnn_features: FP64 matrix expected");
+  FE_REQUIRE(A->row_align == 0, "fe_gnn_features: the matrix's own pattern expected (row_align 0)");
   if (A->n_brows == 0) return FE_OK;
   switch (A->dof) {
     case 1: return features_d<1>(h, A, fixed, feat, scratch);
@@ -474,6 +475,7 @@ extern "C" int fe_gnn_loss_grad(fe_handle* h, const fe_bsr* A, const double* bin
                                 double* work, double* loss_acc, float* dlo) {
   FE_ENTER(h);
   FE_REQUIRE(A && binv && M && o && r && e && work && loss_acc && dlo, "fe_gnn_loss_grad: null argument");
+  FE_REQUIRE(A->row_align == 0, "fe_gnn_loss_grad: the matrix's own pattern expected (row_align 0)");
   FE_REQUIRE(A->precision == 0 && A->layout == FE_LAYOUT_PLANES,
              "fe_gnn_loss_grad: FP64 FE_LAYOUT_PLANES matrix expected");
   const int64_t nb = A->n_brows, n = nb * A->dof;

@@ -763,6 +763,7 @@ LobWork layout_lob(char* base, const LobParams& p) {
 int check_opts(const fe_bsr* K, const fe_lobpcg_opts* o) {
   FE_REQUIRE(K && o, "fe_lobpcg: null argument");
   FE_REQUIRE(K->dof >= 1 && K->dof <= 3, "fe_lobpcg: dof must be 1..3");
+  FE_REQUIRE(K->row_align == 0, "fe_lobpcg: the matrix's own pattern expected (row_align 0)");
   FE_REQUIRE(K->precision == 0 && K->layout == FE_LAYOUT_PLANES,
              "fe_lobpcg: matrices must be FP64 in FE_LAYOUT_PLANES");
   FE_REQUIRE(o->block >= 4 && o->block <= kMaxBlock && o->block % 4 == 0,
@@ -894,7 +895,7 @@ extern "C" int fe_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const d
   FE_REQUIRE(M && X && lambda && resid && iters && vstatus && work && stats,
              "fe_lobpcg: null argument");
   FE_REQUIRE(M->precision == 0 && M->layout == FE_LAYOUT_PLANES && M->dof == K->dof &&
-                 M->n_brows == K->n_brows && M->nnzb == K->nnzb,
+                 M->n_brows == K->n_brows && M->nnzb == K->nnzb && M->row_align == 0,
              "fe_lobpcg: M must be FP64 FE_LAYOUT_PLANES on the pattern of K");
   FE_REQUIRE(o->k_stride >= 0 && o->m_stride >= 0 && o->t_stride >= 0,
              "fe_lobpcg: strides must be non-negative");

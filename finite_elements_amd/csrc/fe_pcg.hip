@@ -157,7 +157,8 @@ __global__ __launch_bounds__(kVecNodes * D) void k_cg_vec(const Tile* __restrict
   }
 }
 
-template <typename T, int D, int G>
+// VEC: the matrix is on a padded pattern (row_align 4) and G lanes walk a row with 16-byte loads.
+template <typename T, int D, int G, bool VEC>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv(const Tile* __restrict__ tiles, int ntiles,
                                                     int64_t nnzb, const int32_t* __restrict__ bptr,
                                                     const int32_t* __restrict__ bcol,
@@ -175,7 +176,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(const Tile* __restrict__ til
   double v[3] = {0.0, 0.0, 0.0};
   for (int64_t i = tl.row0 + threadIdx.x / G; i < tl.row1; i += kBlock / G) {
     T acc[D];
-    bsr_row<T, D, G, FE_LAYOUT_PLANES>(bptr, bcol, vals, nnzb, u, i, lane, acc);
+    if constexpr (VEC)
+      bsr_row_vec<T, D, G>(bptr, bcol, vals, nnzb, u, i, lane, acc);
+    else
+      bsr_row<T, D, G, FE_LAYOUT_PLANES>(bptr, bcol, vals, nnzb, u, i, lane, acc);
     if (lane < D) {
       const int64_t k = i * D + lane;
       const T wk = pick<T, D>(acc, lane);
@@ -605,6 +609,13 @@ int run_pcg(fe_handle* h, const Problem& pb, const Work& w, const fe_pcg_opts* o
   constexpr int G = lanes_for<D>();
   const fe_bsr* A64 = pb.A64;
   const int64_t nnzb = A64->nnzb, n = A64->n_brows * D;
+  // the iteration matrix may be on a padded pattern of its own
+  const fe_bsr* Ait = pb.Ait;
+  const int64_t nnzb_it = Ait->nnzb;
+  const int32_t* bptr_it = Ait->bptr;
+  const int32_t* bcol_it = Ait->bcol;
+  const bool vec = row_aligned(Ait);
+  constexpr int GV = G / Vec16<T>::W;
   const int nt = (int)pb.ntiles, nseg = pb.nseg;
   hipStream_t st = h->stream;
   const T* M = (const T*)pb.M;
@@ -618,12 +629,20 @@ int run_pcg(fe_handle* h, const Problem& pb, const Work& w, const fe_pcg_opts* o
   const int32_t* bptr = A64->bptr;
   const int32_t* bcol = A64->bcol;
   const double* v64 = (const double*)A64->vals;
-  const T* vit = (const T*)pb.Ait->vals;
+  const T* vit = (const T*)Ait->vals;
   T* r = (T*)w.r;
   T* u = (T*)w.u;
   T* wv = (T*)w.w;
   T* p = (T*)w.p;
   T* sv = (T*)w.s;
+  auto launch_spmv = [&](hipStream_t q, int force) {
+    if (vec)
+      hipLaunchKernelGGL((k_cg_spmv<T, D, GV, true>), gt, dim3(kBlock), 0, q, w.tiles, nt, nnzb_it,
+                         bptr_it, bcol_it, vit, u, r, wv, w.st, w.partials, force);
+    else
+      hipLaunchKernelGGL((k_cg_spmv<T, D, G, false>), gt, dim3(kBlock), 0, q, w.tiles, nt, nnzb_it,
+                         bptr_it, bcol_it, vit, u, r, wv, w.st, w.partials, force);
+  };
   const dim3 bvec(kVecNodes * D);
   auto launch_ru = [&](hipStream_t q) -> int {
     hipLaunchKernelGGL((k_ru_residual<D, G>), gt, dim3(kBlock), 0, q, w.tiles, nt, nnzb, bptr, bcol,
@@ -635,8 +654,8 @@ int run_pcg(fe_handle* h, const Problem& pb, const Work& w, const fe_pcg_opts* o
                                      &w.st[0].done, 1);
       if (rc3) return rc3;
     }
-    hipLaunchKernelGGL((k_ru_spmv<T, D, G>), gt, dim3(kBlock), 0, q, w.tiles, nt, nnzb, bptr, bcol,
-                       vit, u, p, r, wv, sv, w.st, w.partials);
+    hipLaunchKernelGGL((k_ru_spmv<T, D, G>), gt, dim3(kBlock), 0, q, w.tiles, nt, nnzb_it, bptr_it,
+                       bcol_it, vit, u, p, r, wv, sv, w.st, w.partials);
     hipLaunchKernelGGL(k_ru_finish, gseg, dim3(kRed), 0, q, w.seg_tiles, nt, w.partials, w.st);
     return FE_OK;
   };
@@ -671,8 +690,7 @@ int run_pcg(fe_handle* h, const Problem& pb, const Work& w, const fe_pcg_opts* o
                                        &w.st[0].done, 0);
         if (rc3) return rc3;
       }
-      hipLaunchKernelGGL((k_cg_spmv<T, D, G>), gt, dim3(kBlock), 0, cs, w.tiles, nt, nnzb, bptr, bcol,
-                         vit, u, r, wv, w.st, w.partials, 0);
+      launch_spmv(cs, 0);
       hipLaunchKernelGGL(k_cg_reduce, gseg, dim3(kRed), 0, cs, w.seg_tiles, nt, w.partials, w.st);
     }
     {
@@ -752,8 +770,7 @@ int run_pcg(fe_handle* h, const Problem& pb, const Work& w, const fe_pcg_opts* o
     FE_CHECK_HIP(hipEventCreate(&p1));
     FE_CHECK_HIP(hipEventRecord(p0, st));
     for (int k = 0; k < kProfLaunches; ++k)
-      hipLaunchKernelGGL((k_cg_spmv<T, D, G>), gt, dim3(kBlock), 0, st, w.tiles, nt, nnzb, bptr, bcol,
-                         vit, u, r, wv, w.st, w.partials, 1);
+      launch_spmv(st, 1);
     FE_CHECK_HIP(hipEventRecord(p1, st));
     FE_CHECK_HIP(hipEventSynchronize(p1));
     float ms = 0.f;
@@ -803,10 +820,18 @@ int check_pcg_args(const fe_bsr* A64, const fe_bsr* A32, const void* M64, const 
     if (rc) return rc;
     FE_REQUIRE(M32, "%s: mixed precision needs the FP32 preconditioner", who);
     FE_REQUIRE(A32->precision == 1 && A32->layout == FE_LAYOUT_PLANES && A32->dof == A64->dof &&
-                   A32->n_brows == A64->n_brows && A32->nnzb == A64->nnzb,
+                   A32->n_brows == A64->n_brows && (A32->nnzb == A64->nnzb || A32->row_align == 4),
                "%s: A32 must be the FP32 FE_LAYOUT_PLANES copy of A64", who);
   } else {
     FE_REQUIRE(M64, "%s: missing preconditioner", who);
+    if (A32) {
+      rc = check_bsr(A32, who);
+      if (rc) return rc;
+      FE_REQUIRE(A32->precision == 0 && A32->layout == FE_LAYOUT_PLANES && A32->dof == A64->dof &&
+                     A32->n_brows == A64->n_brows && (A32->nnzb == A64->nnzb || A32->row_align == 4),
+                 "%s: without mixed precision A32 must be NULL or an FP64 FE_LAYOUT_PLANES copy of A64",
+                 who);
+    }
   }
   return FE_OK;
 }
@@ -849,7 +874,7 @@ int pcg_common(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M
                               hipMemcpyHostToDevice, h->stream));
   FE_CHECK_HIP(hipMemcpyAsync(w.seg_tiles, seg_tiles.data(), seg_tiles.size() * sizeof(int32_t),
                               hipMemcpyHostToDevice, h->stream));
-  Problem pb{A64, o->mixed ? A32 : A64, o->mixed ? M32 : M64, b, x, nseg, ntiles, C,
+  Problem pb{A64, A32 ? A32 : A64, o->mixed ? M32 : M64, b, x, nseg, ntiles, C,
              o->mixed ? Ainv32 : Ainv64};
   *s = fe_pcg_stats{};
   int rc = ntiles == 0 ? FE_OK

@@ -21,6 +21,22 @@ __global__ __launch_bounds__(kBlock) void k_spmv_bsr(int64_t nb, int64_t nnzb,
   if (lane < D) y[i * D + lane] = pick<T, D>(acc, lane);
 }
 
+// The same product on a padded pattern (16-byte loads of W blocks, G = lanes_for<D>() / W lanes).
+template <typename T, int D, int G>
+__global__ __launch_bounds__(kBlock) void k_spmv_vec(int64_t nb, int64_t nnzb,
+                                                     const int32_t* __restrict__ bptr,
+                                                     const int32_t* __restrict__ bcol,
+                                                     const T* __restrict__ vals,
+                                                     const T* __restrict__ x, T* __restrict__ y) {
+  const int64_t gid = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  const int64_t i = gid / G;
+  const int lane = (int)(gid % G);
+  if (i >= nb) return;
+  T acc[D];
+  bsr_row_vec<T, D, G>(bptr, bcol, vals, nnzb, x, i, lane, acc);
+  if (lane < D) y[i * D + lane] = pick<T, D>(acc, lane);
+}
+
 template <int G>
 __global__ __launch_bounds__(kBlock) void k_spmv_csr(int64_t n, const int32_t* __restrict__ rowptr,
                                                      const int32_t* __restrict__ colind,
@@ -56,6 +72,25 @@ __global__ void k_bsr_convert(int64_t nnzb, const S* __restrict__ src, T* __rest
     k = (int)(t - q * D * D);
   }
   dst[vix<D, LT>(q, k, nnzb)] = (T)src[vix<D, LS>(q, k, nnzb)];
+}
+
+// dst block q = src block map[q] (or zero where map[q] < 0), in destination order.
+template <typename S, typename T, int D, int LS, int LT>
+__global__ void k_bsr_repack(int64_t nnzb_dst, int64_t nnzb_src, const int32_t* __restrict__ map,
+                             const S* __restrict__ src, T* __restrict__ dst) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nnzb_dst * D * D) return;
+  int64_t q;
+  int k;
+  if (LT == FE_LAYOUT_PLANES) {
+    k = (int)(t / nnzb_dst);
+    q = t - (int64_t)k * nnzb_dst;
+  } else {
+    q = t / (D * D);
+    k = (int)(t - q * D * D);
+  }
+  const int32_t qs = map[q];
+  dst[vix<D, LT>(q, k, nnzb_dst)] = qs >= 0 ? (T)src[vix<D, LS>(qs, k, nnzb_src)] : T(0);
 }
 
 // --------------------------------------------------------------------------------------------------
@@ -115,6 +150,13 @@ __global__ void k_precond_setup(int64_t nb, int64_t nnzb, const int32_t* __restr
 
 template <typename T, int D, int LAYOUT>
 void launch_spmv(fe_handle* h, const fe_bsr* A, const void* x, void* y) {
+  if (row_aligned(A)) {
+    constexpr int GV = lanes_for<D>() / Vec16<T>::W;
+    hipLaunchKernelGGL((k_spmv_vec<T, D, GV>), dim3((unsigned)ceil_div(A->n_brows * GV, kBlock)),
+                       dim3(kBlock), 0, h->stream, A->n_brows, A->nnzb, A->bptr, A->bcol,
+                       (const T*)A->vals, (const T*)x, (T*)y);
+    return;
+  }
   constexpr int G = lanes_for<D>();
   const int64_t threads = A->n_brows * G;
   hipLaunchKernelGGL((k_spmv_bsr<T, D, G, LAYOUT>), dim3((unsigned)ceil_div(threads, kBlock)),
@@ -151,6 +193,32 @@ void launch_convert(fe_handle* h, const fe_bsr* A, const fe_bsr* B) {
       hipLaunchKernelGGL((k_bsr_convert<S, T, D, FE_LAYOUT_BSR, FE_LAYOUT_BSR>), grid, block, 0,
                          h->stream, A->nnzb, src, dst);
   }
+}
+
+template <typename S, typename T, int D>
+void launch_repack(fe_handle* h, const fe_bsr* A, const fe_bsr* B, const int32_t* map) {
+  const int64_t n = B->nnzb * D * D;
+  dim3 grid((unsigned)ceil_div(n, 256)), block(256);
+  const S* src = (const S*)A->vals;
+  T* dst = (T*)B->vals;
+#define FE_REPACK(LS, LT)                                                                           \
+  hipLaunchKernelGGL((k_bsr_repack<S, T, D, LS, LT>), grid, block, 0, h->stream, B->nnzb, A->nnzb, map, \
+                     src, dst)
+  if (A->layout == FE_LAYOUT_PLANES) {
+    if (B->layout == FE_LAYOUT_PLANES) FE_REPACK(FE_LAYOUT_PLANES, FE_LAYOUT_PLANES);
+    else FE_REPACK(FE_LAYOUT_PLANES, FE_LAYOUT_BSR);
+  } else {
+    if (B->layout == FE_LAYOUT_PLANES) FE_REPACK(FE_LAYOUT_BSR, FE_LAYOUT_PLANES);
+    else FE_REPACK(FE_LAYOUT_BSR, FE_LAYOUT_BSR);
+  }
+#undef FE_REPACK
+}
+
+template <typename S, typename T>
+void launch_repack_d(fe_handle* h, const fe_bsr* A, const fe_bsr* B, const int32_t* map) {
+  if (A->dof == 1) launch_repack<S, T, 1>(h, A, B, map);
+  else if (A->dof == 2) launch_repack<S, T, 2>(h, A, B, map);
+  else launch_repack<S, T, 3>(h, A, B, map);
 }
 
 template <typename S, typename T>
@@ -213,12 +281,33 @@ extern "C" int fe_bsr_convert(fe_handle* h, const fe_bsr* src, const fe_bsr* dst
   return FE_OK;
 }
 
+extern "C" int fe_bsr_repack(fe_handle* h, const fe_bsr* src, const fe_bsr* dst,
+                             const int32_t* src_block) {
+  FE_ENTER(h);
+  int rc = check_bsr(src, "fe_bsr_repack(src)");
+  if (rc) return rc;
+  rc = check_bsr(dst, "fe_bsr_repack(dst)");
+  if (rc) return rc;
+  FE_REQUIRE(src->dof == dst->dof && src->n_brows == dst->n_brows,
+             "fe_bsr_repack: src and dst must have the same rows and dof");
+  FE_REQUIRE(src->vals != dst->vals, "fe_bsr_repack: must be out-of-place");
+  if (dst->nnzb == 0) return FE_OK;
+  FE_REQUIRE(src_block, "fe_bsr_repack: null block map");
+  if (src->precision == 0 && dst->precision == 0) launch_repack_d<double, double>(h, src, dst, src_block);
+  else if (src->precision == 0) launch_repack_d<double, float>(h, src, dst, src_block);
+  else if (dst->precision == 0) launch_repack_d<float, double>(h, src, dst, src_block);
+  else launch_repack_d<float, float>(h, src, dst, src_block);
+  FE_CHECK_LAUNCH();
+  return FE_OK;
+}
+
 extern "C" int fe_precond_setup(fe_handle* h, const fe_bsr* A, int kind, double* out) {
   FE_ENTER(h);
   int rc = check_bsr(A, "fe_precond_setup");
   if (rc) return rc;
   FE_REQUIRE(out, "fe_precond_setup: null output");
   FE_REQUIRE(A->precision == 0, "fe_precond_setup: FP64 matrix expected");
+  FE_REQUIRE(A->row_align == 0, "fe_precond_setup: the matrix's own pattern expected (row_align 0)");
   FE_REQUIRE(kind == FE_PC_JACOBI || kind == FE_PC_BLOCK_JACOBI, "fe_precond_setup: bad kind");
   if (A->n_brows == 0) return FE_OK;
   dim3 grid((unsigned)ceil_div(A->n_brows, 256)), block(256);

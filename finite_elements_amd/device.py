@@ -227,6 +227,13 @@ class DeviceGraph:
         self.bseg_ptr = ctx.tensor(pat.bseg_ptr)
         self.bseg = ctx.tensor(pat.bseg.view(np.int16))
         self.set_schedule(pat.blk, pat.max_inc)
+        self._padded = None
+
+    def padded(self) -> 'PaddedGraph':
+        """The padded pattern of the same rows (row_align = 4), built once."""
+        if self._padded is None:
+            self._padded = PaddedGraph(self)
+        return self._padded
 
     def set_schedule(self, blk: np.ndarray, max_inc: int, threads: int = 256):
         self.blk = self.ctx.tensor(np.asarray(blk, dtype=np.int32))
@@ -239,6 +246,47 @@ class DeviceGraph:
                             self.threads,
                             _vp(self.bptr), _vp(self.bcol), _vp(self.inc_ptr), _vp(self.inc),
                             _vp(self.bseg_ptr), _vp(self.bseg), _vp(self.blk))
+
+
+def pad_pattern(bptr: np.ndarray, bcol: np.ndarray, align: int = 4):
+    """(bptr, bcol, src_block) of the padded pattern: every block row extended to a multiple of
+    ``align`` blocks by blocks whose column is the row itself; src_block[q] is the original block of
+    padded block q, or -1 for padding."""
+    bptr = np.asarray(bptr, dtype=np.int64)
+    n = bptr.size - 1
+    nnzb = int(bptr[-1])
+    deg = np.diff(bptr)
+    pdeg = (deg + align - 1) // align * align
+    pptr = np.zeros(n + 1, dtype=np.int64)
+    pptr[1:] = np.cumsum(pdeg)
+    if pptr[-1] >= 2 ** 31:
+        raise ValueError('padded pattern exceeds 2^31 blocks')
+    pos = np.repeat(pptr[:-1] - bptr[:-1], deg) + np.arange(nnzb, dtype=np.int64)
+    pcol = np.repeat(np.arange(n, dtype=np.int32), pdeg)
+    pcol[pos] = np.asarray(bcol, dtype=np.int32)
+    src = np.full(int(pptr[-1]), -1, dtype=np.int32)
+    src[pos] = np.arange(nnzb, dtype=np.int32)
+    return pptr.astype(np.int32), pcol, src
+
+
+class PaddedGraph:
+    """Padded pattern of a DeviceGraph's rows: each block row is extended to a multiple of 4 blocks
+    with zero blocks whose column is the row itself, so that the solvers' SpMV reads 4 (FP32) or 2
+    (FP64) blocks of a value plane with one 16-byte load.  ``src_block`` maps every padded block to
+    its block in the original pattern (-1 for padding)."""
+
+    row_align = 4
+
+    def __init__(self, graph: DeviceGraph):
+        a = self.row_align
+        self.ctx = graph.ctx
+        self.base = graph
+        self.n_nodes = graph.n_nodes
+        pptr, bcol, src = pad_pattern(graph.pattern.bptr, graph.pattern.bcol, a)
+        self.nnzb = int(pptr[-1])
+        self.bptr = self.ctx.tensor(pptr)
+        self.bcol = self.ctx.tensor(bcol)
+        self.src_block = self.ctx.tensor(src)
 
 
 @dataclass
@@ -259,8 +307,19 @@ class BsrMatrix:
         return self.graph.n_nodes * self.dof
 
     def struct(self) -> _hip.FeBsr:
-        return _hip.FeBsr(self.graph.n_nodes, self.graph.nnzb, self.dof, self.precision, self.layout, 0,
-                          _vp(self.graph.bptr), _vp(self.graph.bcol), _vp(self.vals))
+        return _hip.FeBsr(self.graph.n_nodes, self.graph.nnzb, self.dof, self.precision, self.layout,
+                          getattr(self.graph, 'row_align', 0), _vp(self.graph.bptr),
+                          _vp(self.graph.bcol), _vp(self.vals))
+
+    def padded(self, dtype=torch.float64) -> 'BsrMatrix':
+        """Iteration copy on the padded pattern (FE_LAYOUT_PLANES, row_align 4) in ``dtype``."""
+        pg = self.graph.padded()
+        out = BsrMatrix(pg, torch.empty(pg.nnzb * self.dof * self.dof, dtype=dtype,
+                                        device=self.vals.device), self.dof, _hip.FE_LAYOUT_PLANES)
+        a, b = self.struct(), out.struct()
+        _hip.call('fe_bsr_repack', self.graph.ctx.h, ctypes.byref(a), ctypes.byref(b),
+                  _vp(pg.src_block))
+        return out
 
     def convert(self, layout: int, dtype=torch.float64) -> 'BsrMatrix':
         out = BsrMatrix(self.graph, torch.empty(self.vals.numel(), dtype=dtype, device=self.vals.device),
@@ -463,11 +522,13 @@ class PcgSolver:
         self.mixed = bool(mixed)
         self.M64 = precond_setup(A64, 'jacobi' if precond == 'jacobi' else 'block_jacobi') \
             if M64 is None else M64
+        # the iteration matrix: a copy on the padded pattern (16-byte loads in the SpMV), FP32 with
+        # mixed precision; A64 keeps the true residual
         if self.mixed:
-            self.A32 = A64.convert(_hip.FE_LAYOUT_PLANES, torch.float32) if A32 is None else A32
+            self.A32 = A64.padded(torch.float32) if A32 is None else A32
             self.M32 = _to_f32(self.ctx, self.M64)
         else:
-            self.A32 = None
+            self.A32 = A64.padded(torch.float64) if A32 is None else A32
             self.M32 = None
         self.check_every = check_every
         self.ru_delta = ru_delta

@@ -104,14 +104,17 @@ typedef struct {
 
 /* Block sparse matrix (node-block rows) on the device; precision 0 = FP64, 1 = FP32; layout
    FE_LAYOUT_BSR or FE_LAYOUT_PLANES (the solver's layout: every value load is coalesced across the
-   lanes that walk a block row). */
+   lanes that walk a block row).  row_align 0, or 4 for a padded pattern (FE_LAYOUT_PLANES only):
+   every block row starts at a multiple of 4 blocks and holds a multiple of 4 blocks, the padding
+   blocks having value 0 and the row itself as column, with bcol and vals 16-byte aligned.  The SpMV
+   of the solvers then reads 4 (FP32) or 2 (FP64) blocks of a plane with one 16-byte load. */
 typedef struct {
   int64_t n_brows;
   int64_t nnzb;
   int32_t dof;
   int32_t precision;
   int32_t layout;
-  int32_t pad;
+  int32_t row_align;
   const int32_t* bptr;
   const int32_t* bcol;
   const void* vals;
@@ -232,6 +235,11 @@ int fe_spmv_csr(fe_handle* h, int64_t n_rows, const int32_t* rowptr, const int32
 int fe_cast_f64_to_f32(fe_handle* h, int64_t n, const double* src, float* dst);
 /* Copy the values of src into dst->vals converting precision and layout (same pattern). */
 int fe_bsr_convert(fe_handle* h, const fe_bsr* src, const fe_bsr* dst);
+/* Gather blocks into another pattern of the same rows (e.g. the padded pattern of row_align = 4):
+   dst block q = src block src_block[q] converted to dst's precision and layout, or zero where
+   src_block[q] < 0.  src_block: device array of dst->nnzb entries.  No counterpart in the reference:
+   it builds the solver's iteration copy of the matrix. */
+int fe_bsr_repack(fe_handle* h, const fe_bsr* src, const fe_bsr* dst, const int32_t* src_block);
 
 /* ---- solver ------------------------------------------------------------------------------------- */
 /* Preconditioner setup from the FP64 (Dirichlet-applied) matrix: FE_PC_JACOBI writes n values,
@@ -239,7 +247,9 @@ int fe_bsr_convert(fe_handle* h, const fe_bsr* src, const fe_bsr* dst);
 int fe_precond_setup(fe_handle* h, const fe_bsr* A, int kind, double* out);
 /* PCG on the Dirichlet-applied SPD system (replaces spsolve at finite_elements/analysis.py:819-822).
    A64/M64 are FP64; with opts->mixed the FP32 copies A32/M32 run the iteration and FP64 reliable
-   updates recompute the true residual.  Matrices must be in FE_LAYOUT_PLANES.  x is the initial guess
+   updates recompute the true residual.  Without opts->mixed, A32 is NULL (A64 iterates) or an FP64
+   iteration copy of A64.  An iteration copy may be on the padded pattern of the same rows
+   (row_align = 4, from fe_bsr_repack).  Matrices must be in FE_LAYOUT_PLANES.  x is the initial guess
    on entry.  When the true residual disagrees with the recursive one the iteration restarts from the
    true residual; ten reliable updates without progress (or, in FP32, more than three restarts) end
    the iteration as not converged.  Mixed precision with M64 given: a system whose FP32 iteration

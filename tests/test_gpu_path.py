@@ -10,6 +10,7 @@ against a direct FP64 solve reported and bounded.
 import numpy as np
 import pytest
 import scipy.sparse as sp
+import torch
 
 from finite_elements_amd import _hip, device, meshgen
 from finite_elements_amd.system import FESystem
@@ -138,6 +139,11 @@ def test_spmv(ctx):
         assert np.array_equal(yb, y)
         y32 = device.spmv(A.to_f32(), ctx.tensor(x.astype(np.float32))).cpu().numpy()
         assert np.linalg.norm(y32 - want) <= 1e-5 * np.linalg.norm(want) + 1e-6 * np.abs(K).max()
+        # the solvers' iteration copies on the padded pattern (16-byte vector loads)
+        yp = device.spmv(A.padded(torch.float64), ctx.tensor(x)).cpu().numpy()
+        assert np.max(np.abs(yp - want)) <= 1e-12 * np.abs(K).max() * np.abs(x).sum() / 10
+        yp32 = device.spmv(A.padded(torch.float32), ctx.tensor(x.astype(np.float32))).cpu().numpy()
+        assert np.linalg.norm(yp32 - want) <= 1e-5 * np.linalg.norm(want) + 1e-6 * np.abs(K).max()
         rp = ctx.tensor(s.pattern.rowptr)
         ci = ctx.tensor(s.pattern.colind)
         yc = device.spmv_csr(ctx, rp, ci, s.stiffness(_hip.FE_LAYOUT_CSR), ctx.tensor(x)).cpu().numpy()
