@@ -84,7 +84,7 @@ class CoarseSpace:
         self.dof = dof
         self.nc = {1: 1, 2: 3, 3: 6}[dof]
         if target is None:
-            target = 729 if coords.shape[1] == 3 else 81
+            target = 1000 if coords.shape[1] == 3 else 81
         self.target = target
         # aggregates never cross the given node ranges (independent meshes of a batch)
         offs = np.array([0, n]) if node_offsets is None else np.asarray(node_offsets, dtype=np.int64)

@@ -43,10 +43,12 @@ __global__ __launch_bounds__(kGatherBlock) void k_assemble_gather(
 
 // ---------------------------------------------------------------------------------------------------
 // Fused assembly.  A workgroup owns the node range [blk[w], blk[w+1]).  Phase 1: one thread per
-// node-element incidence (node i = local vertex a of element e) computes row block a of the element
-// matrix (npe blocks of d x d) straight from coordinates into LDS.  Phase 2: one thread per block
-// entry of the range sums the LDS values listed by the block-level contribution map, in ascending
-// element order, and stores its d x d values (coalesced across threads).
+// distinct element incident to the range (welem, ascending) evaluates the element's geometry record
+// (shape-function gradients, measure, material coefficients) straight from coordinates into LDS, so
+// an element shared by several nodes of the range is evaluated once.  Phase 2: one thread per block
+// entry (node i, neighbour j) of the range sums block (a, b) of every element holding both nodes,
+// evaluated from the records, in ascending element order, and stores its d x d values (coalesced
+// across threads).  No element matrix reaches HBM and no atomics are used.
 // ---------------------------------------------------------------------------------------------------
 constexpr int kFusedMaxNodes = 256;
 
@@ -65,10 +67,18 @@ struct KindTraits<FE_KIND_TRI_MAGNETIC> {
   static constexpr int NPE = 3, D = 1;
 };
 
-// Row block a of the element matrix of element e; out[(b*D + di)*D + dj] for b < NPE.
+// Doubles in the geometry record of one element (material coefficients pre-scaled by the measure):
+//   tet stiffness  g[4][3], vol*diag, vol*off, vol*shear        tet / triangle mass  f
+//   tri stiffness  gx[3], gy[3], tA*diag, tA*off, tA*shear      magnetic  gx[3], gy[3], area/mu
 template <int KIND, int WHAT>
-__device__ inline void element_row_block(const fe_mesh& m, int64_t e, int a, double* out) {
-  constexpr int NPE = KindTraits<KIND>::NPE, D = KindTraits<KIND>::D;
+constexpr int record_size() {
+  return WHAT == FE_MASS ? 1 : KIND == FE_KIND_TET_ELASTIC ? 15 : KIND == FE_KIND_TRI_ELASTIC ? 9 : 7;
+}
+
+// Record of element e, field k stored at S[k].
+template <int KIND, int WHAT>
+__device__ inline void element_record(const fe_mesh& m, int64_t e, double* S) {
+  constexpr int NPE = KindTraits<KIND>::NPE;
   const int64_t ne = m.n_elem;
   double px[NPE], py[NPE], pz[NPE];
 #pragma unroll
@@ -82,119 +92,198 @@ __device__ inline void element_row_block(const fe_mesh& m, int64_t e, int a, dou
     TetGeom g;
     tet_geom(px, py, pz, g);
     if (WHAT == FE_MASS) {
-      const double f = m.rho[e] * g.vol / 20;
-#pragma unroll
-      for (int b = 0; b < NPE; ++b)
-#pragma unroll
-        for (int di = 0; di < D; ++di)
-#pragma unroll
-          for (int dj = 0; dj < D; ++dj)
-            out[(b * D + di) * D + dj] = (di == dj) ? ((a == b) ? 2 * f : f) : 0.0;
+      S[0] = m.rho[e] * g.vol / 20;
     } else {
       double diag, off, shear;
       tet_coeffs(m.mat0[e], m.mat1[e], diag, off, shear);
-      double ga[3];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) ga[k] = (a == 0) ? g.g[0][k] : (a == 1) ? g.g[1][k]
-                                        : (a == 2) ? g.g[2][k] : g.g[3][k];
+      for (int v = 0; v < 4; ++v)
 #pragma unroll
-      for (int b = 0; b < NPE; ++b)
-#pragma unroll
-        for (int di = 0; di < D; ++di)
-#pragma unroll
-          for (int dj = 0; dj < D; ++dj)
-            out[(b * D + di) * D + dj] = g.vol * bdb_entry<3>(ga, g.g[b], di, dj, diag, off, shear);
+        for (int k = 0; k < 3; ++k) S[v * 3 + k] = g.g[v][k];
+      S[12] = g.vol * diag;
+      S[13] = g.vol * off;
+      S[14] = g.vol * shear;
     }
   } else if constexpr (KIND == FE_KIND_TRI_ELASTIC) {
     TriGeom g;
     tri_elastic_geom(px, py, g);
     if (WHAT == FE_MASS) {
-      const double f = m.rho[e] * g.area * m.mat2[e] / 12;
-#pragma unroll
-      for (int b = 0; b < NPE; ++b)
-#pragma unroll
-        for (int di = 0; di < D; ++di)
-#pragma unroll
-          for (int dj = 0; dj < D; ++dj)
-            out[(b * D + di) * D + dj] = (di == dj) ? ((a == b) ? 2 * f : f) : 0.0;
+      S[0] = m.rho[e] * g.area * m.mat2[e] / 12;
     } else {
       double diag, off, shear;
-      // plane mode is passed in mat slot-independent field of the mesh
       tri_elastic_coeffs(m.mat0[e], m.mat1[e], m.plane_mode, diag, off, shear);
       const double scale = m.mat2[e] * g.area;
-      const double ga[2] = {(a == 0) ? g.gx[0] : (a == 1) ? g.gx[1] : g.gx[2],
-                            (a == 0) ? g.gy[0] : (a == 1) ? g.gy[1] : g.gy[2]};
 #pragma unroll
-      for (int b = 0; b < NPE; ++b) {
-        const double gb[2] = {g.gx[b], g.gy[b]};
-#pragma unroll
-        for (int di = 0; di < D; ++di)
-#pragma unroll
-          for (int dj = 0; dj < D; ++dj)
-            out[(b * D + di) * D + dj] = scale * bdb_entry<2>(ga, gb, di, dj, diag, off, shear);
+      for (int v = 0; v < 3; ++v) {
+        S[v] = g.gx[v];
+        S[3 + v] = g.gy[v];
       }
+      S[6] = scale * diag;
+      S[7] = scale * off;
+      S[8] = scale * shear;
     }
   } else {
     TriGeom g;
     tri_magnetic_geom(px, py, g);
-    const double inv_mu = 1 / m.mat0[e];
-    const double gax = (a == 0) ? g.gx[0] : (a == 1) ? g.gx[1] : g.gx[2];
-    const double gay = (a == 0) ? g.gy[0] : (a == 1) ? g.gy[1] : g.gy[2];
 #pragma unroll
-    for (int b = 0; b < NPE; ++b) out[b] = inv_mu * (gax * g.gx[b] + gay * g.gy[b]) * g.area;
+    for (int v = 0; v < 3; ++v) {
+      S[v] = g.gx[v];
+      S[3 + v] = g.gy[v];
+    }
+    S[6] = g.area / m.mat0[e];
   }
 }
 
-// Contribution-map entries staged in LDS per workgroup (larger ranges read HBM directly).
-constexpr int kStageBq = 512;
-constexpr int kStageSeg = 1024;
+// Gradient of local vertex a from a record (gx[.], gy[.] planes for triangles).
+template <int KIND, int D>
+__device__ inline void record_grad(const double* R, int a, double (&ga)[D]) {
+  if constexpr (KIND == FE_KIND_TET_ELASTIC) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ga[k] = R[a * 3 + k];
+  } else {
+    ga[0] = R[a];
+    ga[1] = R[3 + a];
+  }
+}
+
+// blk = block (a, b) of the element matrix whose record starts at R (used for a != b).  With P = g_a g_b^T:
+//   elasticity (B^T D B)_ab scaled:  entry (d, d) = Dv P_dd + Mv (tr P - P_dd),
+//                                    entry (i, j) = Lv P_ij + Mv P_ji          (i != j)
+//   magnetic  (A/mu) (b_a b_b + c_a c_b);  masses  f I.
+template <int KIND, int WHAT, int D>
+__device__ inline void block_ab(const double* R, int a, int b, double (&blk)[D * D]) {
+  if constexpr (WHAT == FE_MASS) {
+    const double f = (a == b) ? 2 * R[0] : R[0];
+#pragma unroll
+    for (int k = 0; k < D * D; ++k) blk[k] = (k % (D + 1) == 0) ? f : 0.0;
+  } else if constexpr (KIND == FE_KIND_TRI_MAGNETIC) {
+    blk[0] = R[6] * (R[a] * R[b] + R[3 + a] * R[3 + b]);
+  } else {
+    constexpr int C = KIND == FE_KIND_TET_ELASTIC ? 12 : 6;  // first coefficient field
+    constexpr int GD = KIND == FE_KIND_TET_ELASTIC ? 3 : 2;
+    double ga[GD], gb[GD];
+    record_grad<KIND, GD>(R, a, ga);
+    record_grad<KIND, GD>(R, b, gb);
+    const double Dv = R[C], Lv = R[C + 1], Mv = R[C + 2];
+    double P[GD][GD];
+#pragma unroll
+    for (int i = 0; i < GD; ++i)
+#pragma unroll
+      for (int j = 0; j < GD; ++j) P[i][j] = ga[i] * gb[j];
+    double tr = P[0][0];
+#pragma unroll
+    for (int i = 1; i < GD; ++i) tr += P[i][i];
+#pragma unroll
+    for (int i = 0; i < GD; ++i)
+#pragma unroll
+      for (int j = 0; j < GD; ++j)
+        blk[i * GD + j] = (i == j) ? Dv * P[i][i] + Mv * (tr - P[i][i]) : Lv * P[i][j] + Mv * P[j][i];
+  }
+}
+
+// The diagonal block (a, a) of the same element matrix (P = g_a g_a^T is symmetric).
+template <int KIND, int WHAT, int D>
+__device__ inline void block_aa(const double* R, int a, double (&blk)[D * D]) {
+  if constexpr (WHAT == FE_MASS) {
+    const double f2 = 2 * R[0];
+#pragma unroll
+    for (int k = 0; k < D * D; ++k) blk[k] = (k % (D + 1) == 0) ? f2 : 0.0;
+  } else if constexpr (KIND == FE_KIND_TRI_MAGNETIC) {
+    const double gx = R[a], gy = R[3 + a];
+    blk[0] = R[6] * (gx * gx + gy * gy);
+  } else {
+    constexpr int C = KIND == FE_KIND_TET_ELASTIC ? 12 : 6;
+    constexpr int GD = KIND == FE_KIND_TET_ELASTIC ? 3 : 2;
+    double ga[GD];
+    record_grad<KIND, GD>(R, a, ga);
+    const double Dv = R[C], LM = R[C + 1] + R[C + 2],
+                 Mv = R[C + 2];
+    double sq[GD];
+    double tr = 0.0;
+#pragma unroll
+    for (int i = 0; i < GD; ++i) {
+      sq[i] = ga[i] * ga[i];
+      tr += sq[i];
+    }
+#pragma unroll
+    for (int i = 0; i < GD; ++i)
+#pragma unroll
+      for (int j = 0; j < GD; ++j)
+        blk[i * GD + j] = (i == j) ? Dv * sq[i] + Mv * (tr - sq[i]) : LM * (ga[i] * ga[j]);
+  }
+}
+
+// Dynamic LDS of one workgroup: the element records, the block contribution pointers and entries of
+// the range (at most npe entries per incidence, and every block has one) and the packed incidence
+// codes.  Everything phase 2 reads is in LDS.
+template <int KIND, int WHAT>
+inline size_t fused_lds_bytes(int max_elem, int max_inc) {
+  constexpr int NPE = KindTraits<KIND>::NPE;
+  return (size_t)record_size<KIND, WHAT>() * max_elem * sizeof(double) +
+         ((size_t)NPE * max_inc + 1) * sizeof(int32_t) + (size_t)NPE * max_inc * sizeof(uint16_t) +
+         (size_t)max_inc * sizeof(uint16_t) + 16;
+}
+
+// Store the d x d block sum of block entry bq (row node j of the range) in the requested layout.
+template <int D, int LAYOUT>
+__device__ inline void store_block(double* __restrict__ vals, const fe_graph& g, const int32_t* s_bptr,
+                                   int j, int32_t bq, int k, double v) {
+  if (LAYOUT == FE_LAYOUT_PLANES) {
+    vals[(int64_t)k * g.nnzb + bq] = v;
+  } else if (LAYOUT == FE_LAYOUT_BSR) {
+    vals[(int64_t)bq * D * D + k] = v;
+  } else {
+    const int32_t deg = s_bptr[j + 1] - s_bptr[j];
+    const int64_t row0 = (int64_t)s_bptr[j] * D * D + (int64_t)(bq - s_bptr[j]) * D;
+    vals[row0 + (int64_t)(k / D) * deg * D + k % D] = v;
+  }
+}
 
 template <int KIND, int WHAT, int LAYOUT, int NT>
 __global__ __launch_bounds__(NT) void k_assemble_fused(fe_mesh m, fe_graph g,
                                                        double* __restrict__ vals) {
-  constexpr int NPE = KindTraits<KIND>::NPE, D = KindTraits<KIND>::D;
-  constexpr int BS = NPE * D * D;
-  extern __shared__ double S[];  // [BS][stride]
+  constexpr int D = KindTraits<KIND>::D;
+  constexpr int RS = record_size<KIND, WHAT>();
+  constexpr int DL = 4;  // lanes per diagonal block
+  constexpr int NPE = KindTraits<KIND>::NPE;
+  extern __shared__ double S[];  // [max_elem][RS] records (RS odd: few bank conflicts), then the maps
   __shared__ int32_t s_bptr[kFusedMaxNodes + 1];
   __shared__ int32_t s_iptr[kFusedMaxNodes + 1];
-  __shared__ int32_t s_bsp[kStageBq + 1];
-  __shared__ uint16_t s_bseg[kStageSeg];
+  __shared__ int32_t s_diag[kFusedMaxNodes];
+  const int stride = g.max_elem;
+  int32_t* s_bsp = reinterpret_cast<int32_t*>(S + (size_t)RS * stride);  // NPE * max_inc + 1
+  uint16_t* s_bseg = reinterpret_cast<uint16_t*>(s_bsp + NPE * g.max_inc + 1);  // NPE * max_inc
+  uint16_t* s_sa = s_bseg + NPE * g.max_inc;                                     // max_inc
   const int64_t w = xcd_remap(blockIdx.x, gridDim.x);
   const int32_t n0 = g.blk[w], n1 = g.blk[w + 1];
   const int32_t nn = n1 - n0;
-  for (int t = threadIdx.x; t <= nn; t += NT) {
-    s_bptr[t] = g.bptr[n0 + t];
-    s_iptr[t] = g.inc_ptr[n0 + t];
-  }
-  __syncthreads();
-  const int32_t I0 = s_iptr[0];
-  const int32_t ninc = s_iptr[nn] - I0;
-  const int32_t B0 = s_bptr[0], B1 = s_bptr[nn];
+  const int32_t E0 = g.welem_ptr[w];
+  const int32_t nel = g.welem_ptr[w + 1] - E0;
+  const int32_t I0 = g.inc_ptr[n0], ninc = g.inc_ptr[n1] - I0;
+  const int32_t B0 = g.bptr[n0], B1 = g.bptr[n1];
   const int32_t nbq = B1 - B0;
-  const int stride = g.max_inc;
-  // Stage the block-level contribution map of the range (independent of phase 1's loads).
-  const bool staged_bq = nbq + 1 <= kStageBq + 1;
-  if (staged_bq)
-    for (int t = threadIdx.x; t <= nbq; t += NT) s_bsp[t] = g.bseg_ptr[B0 + t];
   const int32_t c_lo = g.bseg_ptr[B0];
   const int32_t nseg = g.bseg_ptr[B1] - c_lo;
-  const bool staged_seg = nseg <= kStageSeg;
-  if (staged_seg)
-    for (int t = threadIdx.x; t < nseg; t += NT) s_bseg[t] = g.bseg[c_lo + t];
-  // Phase 1: element-matrix row blocks of every incidence of the range.
-  for (int t = threadIdx.x; t < ninc; t += NT) {
-    const int32_t packed = g.inc[I0 + t];
-    const int64_t e = packed / NPE;
-    const int a = packed - (int)e * NPE;
-    double blk[BS];
-    element_row_block<KIND, WHAT>(m, e, a, blk);
-#pragma unroll
-    for (int k = 0; k < BS; ++k) S[k * stride + t] = blk[k];
+  // a range beyond the schedule's stated maxima would overrun the LDS: such a workgroup writes nothing
+  if (nn > kFusedMaxNodes || nel > stride || ninc > g.max_inc || nbq > NPE * g.max_inc ||
+      nseg > NPE * g.max_inc)
+    return;
+  // Stage row pointers, incidence codes and the block-level contribution map of the range.
+  for (int t = threadIdx.x; t <= nn; t += NT) {
+    s_bptr[t] = g.bptr[n0 + t];
+    s_iptr[t] = g.inc_ptr[n0 + t] - I0;
+    if (t < nn) s_diag[t] = -1;  // a node without elements has no diagonal entry
   }
+  for (int t = threadIdx.x; t < ninc; t += NT) s_sa[t] = g.inc_sa[I0 + t];
+  for (int t = threadIdx.x; t <= nbq; t += NT) s_bsp[t] = g.bseg_ptr[B0 + t] - c_lo;
+  for (int t = threadIdx.x; t < nseg; t += NT) s_bseg[t] = g.bseg[c_lo + t];
+  // Phase 1: the geometry record of every distinct element of the range.
+  for (int t = threadIdx.x; t < nel; t += NT) element_record<KIND, WHAT>(m, g.welem[E0 + t], S + t * RS);
   __syncthreads();
-  // Phase 2: one thread per block entry (node j, neighbour q) of the range sums its d x d values
-  // over the contributing incidences in ascending element order.  Consecutive threads own
-  // consecutive block entries, so plane and BSR stores are coalesced.
+  // Phase 2a: one thread per off-diagonal block entry (node j, neighbour q) of the range, summing
+  // block (a, b) of the elements holding both nodes in ascending element order.  Consecutive threads
+  // own consecutive block entries, so the stores are coalesced.  A diagonal entry (its contributions
+  // have a == b) is only located here: a node's diagonal sums over all its incident elements.
   for (int32_t t = threadIdx.x; t < nbq; t += NT) {
     const int32_t bq = B0 + t;
     int lo = 0, hi = nn - 1;
@@ -206,34 +295,53 @@ __global__ __launch_bounds__(NT) void k_assemble_fused(fe_mesh m, fe_graph g,
         hi = mid - 1;
     }
     const int j = lo;
-    const int32_t c0 = staged_bq ? s_bsp[t] : g.bseg_ptr[bq];
-    const int32_t c1 = staged_bq ? s_bsp[t + 1] : g.bseg_ptr[bq + 1];
-    const int32_t base = s_iptr[j] - I0;
+    const int32_t c0 = s_bsp[t], c1 = s_bsp[t + 1];
+    const int32_t base = s_iptr[j];
+    {
+      const int v = s_bseg[c0];
+      if ((s_sa[base + (v >> 2)] & 3) == (v & 3)) {
+        s_diag[j] = bq;
+        continue;
+      }
+    }
     double sum[D * D];
 #pragma unroll
     for (int k = 0; k < D * D; ++k) sum[k] = 0.0;
     for (int32_t c = c0; c < c1; ++c) {
-      const int v = staged_seg ? s_bseg[c - c_lo] : g.bseg[c];
-      const int kl = v >> 2, b = v & 3;
-      const double* src = S + (b * D * D) * stride + base + kl;
+      const int v = s_bseg[c];
+      const int sa = s_sa[base + (v >> 2)];
+      double blk[D * D];
+      block_ab<KIND, WHAT, D>(S + (sa >> 2) * RS, sa & 3, v & 3, blk);
 #pragma unroll
-      for (int k = 0; k < D * D; ++k) sum[k] += src[k * stride];
+      for (int k = 0; k < D * D; ++k) sum[k] += blk[k];
     }
-    if (LAYOUT == FE_LAYOUT_PLANES) {
 #pragma unroll
-      for (int k = 0; k < D * D; ++k) vals[(int64_t)k * g.nnzb + bq] = sum[k];
-    } else if (LAYOUT == FE_LAYOUT_BSR) {
+    for (int k = 0; k < D * D; ++k) store_block<D, LAYOUT>(vals, g, s_bptr, j, bq, k, sum[k]);
+  }
+  __syncthreads();
+  // Phase 2b: DL lanes per diagonal block; lane l sums the node's incidences l, l + DL, ... (ascending
+  // element order), then the lanes combine by a fixed butterfly.
+  for (int t = threadIdx.x; t < nn * DL; t += NT) {
+    const int j = t / DL, l = t % DL;
+    double sum[D * D];
 #pragma unroll
-      for (int k = 0; k < D * D; ++k) vals[(int64_t)bq * D * D + k] = sum[k];
-    } else {
-      const int32_t deg = s_bptr[j + 1] - s_bptr[j];
-      const int32_t q = bq - s_bptr[j];
-      const int64_t row0 = (int64_t)s_bptr[j] * D * D + (int64_t)q * D;
+    for (int k = 0; k < D * D; ++k) sum[k] = 0.0;
+    for (int32_t k = s_iptr[j] + l; k < s_iptr[j + 1]; k += DL) {
+      const int sa = s_sa[k];
+      double blk[D * D];
+      block_aa<KIND, WHAT, D>(S + (sa >> 2) * RS, sa & 3, blk);
 #pragma unroll
-      for (int di = 0; di < D; ++di)
-#pragma unroll
-        for (int dj = 0; dj < D; ++dj) vals[row0 + (int64_t)di * deg * D + dj] = sum[di * D + dj];
+      for (int q = 0; q < D * D; ++q) sum[q] += blk[q];
     }
+#pragma unroll
+    for (int off = DL / 2; off > 0; off >>= 1)
+#pragma unroll
+      for (int k = 0; k < D * D; ++k) sum[k] += __shfl_xor(sum[k], off, DL);
+    const int32_t bq = s_diag[j];
+    if (bq >= 0)
+#pragma unroll
+      for (int k = 0; k < D * D; ++k)
+        if (k % DL == l) store_block<D, LAYOUT>(vals, g, s_bptr, j, bq, k, sum[k]);
   }
 }
 
@@ -255,8 +363,12 @@ int launch_fused_nt(fe_handle* h, const fe_mesh* m, const fe_graph* g, double* v
     FE_LAUNCH_NT(128);
   else if (nt == 256)
     FE_LAUNCH_NT(256);
+  else if (nt == 512)
+    FE_LAUNCH_NT(512);
+  else if (nt == 1024)
+    FE_LAUNCH_NT(1024);
   else
-    return fe_set_error(FE_ERR_ARG, "fe_assemble_fused: threads must be 64, 128 or 256");
+    return fe_set_error(FE_ERR_ARG, "fe_assemble_fused: threads must be 64, 128, 256, 512 or 1024");
 #undef FE_LAUNCH_NT
   FE_CHECK_LAUNCH();
   return FE_OK;
@@ -265,12 +377,12 @@ int launch_fused_nt(fe_handle* h, const fe_mesh* m, const fe_graph* g, double* v
 template <int KIND>
 int launch_fused(fe_handle* h, const fe_mesh* m, const fe_graph* g, int what, int layout,
                  double* vals) {
-  constexpr int NPE = KindTraits<KIND>::NPE, D = KindTraits<KIND>::D;
-  const size_t lds = (size_t)NPE * D * D * g->max_inc * sizeof(double);
-  if (lds > 140 * 1024)
-    return fe_set_error(FE_ERR_ARG, "fe_assemble_fused: schedule needs %zu bytes of LDS", lds);
   if (what == FE_MASS && KIND == FE_KIND_TRI_MAGNETIC)
     return fe_set_error(FE_ERR_UNSUPPORTED, "fe_assemble_fused: magnetic elements have no mass matrix");
+  const size_t lds = what == FE_MASS ? fused_lds_bytes<KIND, FE_MASS>(g->max_elem, g->max_inc)
+                                     : fused_lds_bytes<KIND, FE_STIFFNESS>(g->max_elem, g->max_inc);
+  if (lds > 144 * 1024)
+    return fe_set_error(FE_ERR_ARG, "fe_assemble_fused: schedule needs %zu bytes of LDS", lds);
   if constexpr (KIND != FE_KIND_TRI_MAGNETIC) {
     if (what == FE_MASS) {
       if (layout == FE_LAYOUT_CSR) return launch_fused_nt<KIND, FE_MASS, FE_LAYOUT_CSR>(h, m, g, vals, lds);
@@ -355,11 +467,12 @@ extern "C" int fe_assemble_fused(fe_handle* h, const fe_mesh* m, const fe_graph*
   FE_REQUIRE(what == FE_STIFFNESS || what == FE_MASS, "fe_assemble_fused: bad 'what'");
   FE_REQUIRE(layout == FE_LAYOUT_CSR || layout == FE_LAYOUT_BSR || layout == FE_LAYOUT_PLANES,
              "fe_assemble_fused: bad layout");
-  FE_REQUIRE(g->n_blk >= 0 && g->max_inc >= 0, "fe_assemble_fused: bad schedule");
+  FE_REQUIRE(g->n_blk >= 0 && g->max_inc >= 0 && g->max_elem >= 0 && g->max_elem <= 16383,
+             "fe_assemble_fused: bad schedule");
   if (g->n_blk == 0 || g->nnzb == 0) return FE_OK;
-  FE_REQUIRE(m->conn && m->x && m->y && g->bptr && g->inc_ptr && g->inc && g->bseg_ptr && g->bseg &&
-                 g->blk,
-             "fe_assemble_fused: null mesh or graph array");
+  FE_REQUIRE(m->conn && m->x && m->y && g->bptr && g->inc_ptr && g->bseg_ptr && g->bseg && g->blk &&
+                 g->welem_ptr && g->welem && g->inc_sa,
+             "fe_assemble_fused: null mesh or graph array (element schedule from fe_schedule_elements)");
   FE_REQUIRE(m->n_nodes == g->n_nodes, "fe_assemble_fused: mesh and graph node counts differ");
   switch (m->kind) {
     case FE_KIND_TRI_ELASTIC:

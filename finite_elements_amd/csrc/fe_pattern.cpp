@@ -169,6 +169,9 @@ int build(fe_pattern& p, const int32_t* conn, int with_gather) {
 
 }  // namespace
 
+// element slots of one workgroup: 14 bits of the packed (slot << 2 | local vertex) incidence code
+constexpr size_t kMaxElemSlots = 16383;
+
 extern "C" int fe_pattern_schedule(fe_pattern* p, int max_inc_per_block) {
   if (!p) return fe_set_error(FE_ERR_ARG, "fe_pattern_schedule: null pattern");
   if (max_inc_per_block < 1 || max_inc_per_block > 4096)
@@ -197,6 +200,44 @@ extern "C" int fe_pattern_schedule(fe_pattern* p, int max_inc_per_block) {
     max_inc = std::max(max_inc, cur_inc);
   }
   p->max_inc = max_inc;
+  return FE_OK;
+}
+
+extern "C" int fe_schedule_elements(const int32_t* inc_ptr, const int32_t* inc, int npe,
+                                    int64_t n_nodes, const int32_t* blk, int64_t n_blk,
+                                    int32_t* welem_ptr, int32_t* welem, uint16_t* inc_sa,
+                                    int32_t* max_elem) {
+  if (!inc_ptr || !blk || !welem_ptr || !max_elem || (n_nodes > 0 && (!inc || !welem || !inc_sa)))
+    return fe_set_error(FE_ERR_ARG, "fe_schedule_elements: null argument");
+  if (npe < 2 || npe > 4) return fe_set_error(FE_ERR_ARG, "fe_schedule_elements: npe must be 2..4");
+  if (n_blk < 0 || blk[0] != 0 || blk[n_blk] != n_nodes)
+    return fe_set_error(FE_ERR_ARG, "fe_schedule_elements: schedule must cover nodes 0..n_nodes");
+  std::vector<int32_t> ids;
+  int64_t out = 0;
+  int32_t mx = 0;
+  welem_ptr[0] = 0;
+  for (int64_t w = 0; w < n_blk; ++w) {
+    if (blk[w + 1] < blk[w])
+      return fe_set_error(FE_ERR_ARG, "fe_schedule_elements: schedule offsets must not decrease");
+    const int32_t i0 = inc_ptr[blk[w]], i1 = inc_ptr[blk[w + 1]];
+    ids.resize((size_t)(i1 - i0));
+    for (int32_t t = i0; t < i1; ++t) ids[t - i0] = inc[t] / npe;
+    std::sort(ids.begin(), ids.end());
+    ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+    if (ids.size() > kMaxElemSlots)
+      return fe_set_error(FE_ERR_ARG, "fe_schedule_elements: %zu elements in one workgroup > %d",
+                          ids.size(), (int)kMaxElemSlots);
+    std::copy(ids.begin(), ids.end(), welem + out);
+    for (int32_t t = i0; t < i1; ++t) {
+      const int32_t e = inc[t] / npe, a = inc[t] - e * npe;
+      const int32_t slot = (int32_t)(std::lower_bound(ids.begin(), ids.end(), e) - ids.begin());
+      inc_sa[t] = (uint16_t)((slot << 2) | a);
+    }
+    out += (int64_t)ids.size();
+    welem_ptr[w + 1] = (int32_t)out;
+    mx = std::max(mx, (int32_t)ids.size());
+  }
+  *max_elem = mx;
   return FE_OK;
 }
 

@@ -211,6 +211,24 @@ class DeviceMesh:
                            _vp(self.mat2), _vp(self.rho))
 
 
+def element_schedule(pat: Pattern, blk: np.ndarray):
+    """(welem_ptr, welem, inc_sa, max_elem) of the fused assembly for the node ranges ``blk``
+    (fe_schedule_elements, host code): the ascending distinct elements of every workgroup and, per
+    incidence, (slot << 2) | local vertex."""
+    blk = np.ascontiguousarray(np.asarray(blk, dtype=np.int32))
+    n_inc = int(pat.inc.size)
+    wptr = np.empty(blk.size, dtype=np.int32)
+    welem = np.empty(max(n_inc, 1), dtype=np.int32)
+    inc_sa = np.empty(max(n_inc, 1), dtype=np.uint16)
+    max_elem = ctypes.c_int32()
+    _hip.call('fe_schedule_elements', pat.inc_ptr.ctypes.data_as(ctypes.c_void_p),
+              pat.inc.ctypes.data_as(ctypes.c_void_p), pat.npe, pat.n_nodes,
+              blk.ctypes.data_as(ctypes.c_void_p), blk.size - 1,
+              wptr.ctypes.data_as(ctypes.c_void_p), welem.ctypes.data_as(ctypes.c_void_p),
+              inc_sa.ctypes.data_as(ctypes.c_void_p), ctypes.byref(max_elem))
+    return wptr, welem[:int(wptr[-1])], inc_sa[:n_inc], int(max_elem.value)
+
+
 class DeviceGraph:
     """Pattern arrays on the device (for the fused assembly, SpMV and the solver)."""
 
@@ -236,16 +254,25 @@ class DeviceGraph:
         return self._padded
 
     def set_schedule(self, blk: np.ndarray, max_inc: int, threads: int = 256):
-        self.blk = self.ctx.tensor(np.asarray(blk, dtype=np.int32))
-        self.n_blk = len(blk) - 1
+        """Row-block schedule of the fused assembly and its element schedule (the distinct elements
+        of every workgroup and each incidence's slot among them)."""
+        blk = np.ascontiguousarray(np.asarray(blk, dtype=np.int32))
+        wptr, welem, inc_sa, max_elem = element_schedule(self.pattern, blk)
+        self.blk = self.ctx.tensor(blk)
+        self.n_blk = blk.size - 1
         self.max_inc = int(max_inc)
         self.threads = int(threads)
+        self.max_elem = max_elem
+        self.welem_ptr = self.ctx.tensor(wptr)
+        self.welem = self.ctx.tensor(welem if welem.size else np.zeros(1, np.int32))
+        self.inc_sa = self.ctx.tensor((inc_sa if inc_sa.size else np.zeros(1, np.uint16)).view(np.int16))
 
     def struct(self) -> _hip.FeGraph:
         return _hip.FeGraph(self.n_nodes, self.nnzb, int(self.inc.numel()), self.n_blk, self.max_inc,
                             self.threads,
                             _vp(self.bptr), _vp(self.bcol), _vp(self.inc_ptr), _vp(self.inc),
-                            _vp(self.bseg_ptr), _vp(self.bseg), _vp(self.blk))
+                            _vp(self.bseg_ptr), _vp(self.bseg), _vp(self.blk), self.max_elem, 0,
+                            _vp(self.welem_ptr), _vp(self.welem), _vp(self.inc_sa))
 
 
 def pad_pattern(bptr: np.ndarray, bcol: np.ndarray, align: int = 4):

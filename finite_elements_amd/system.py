@@ -57,7 +57,7 @@ class FESystem:
     # ---------------------------------------------------------------------------------------------
     def set_schedule(self, max_inc: int, threads: int = 256):
         """Re-tile the fused assembly: row-block schedule with at most ``max_inc`` node-element
-        incidences per workgroup, ``threads`` lanes per workgroup (64, 128 or 256)."""
+        incidences per workgroup, ``threads`` lanes per workgroup (64, 128, 256, 512 or 1024)."""
         blk, biggest = Pattern.schedule_for(self.pattern.inc_ptr, max_inc)
         self.graph.set_schedule(blk, biggest, threads)
 
@@ -67,9 +67,9 @@ class FESystem:
         if candidates is None:
             base = int(np.diff(self.pattern.inc_ptr).max()) if self.n_nodes else 1
             cands = []
-            for threads in (64, 128, 256):
-                for mi in (32, 48, 64, 96, 128, 192, 256):
-                    if mi >= base and mi <= 4 * threads:
+            for threads in (128, 256, 512, 1024):
+                for mi in (48, 64, 96, 128, 192, 256, 384, 512, 768, 1024, 1536):
+                    if mi >= base and mi <= 8 * threads:
                         cands.append((threads, mi))
             candidates = cands
         out = self.stiffness(layout)

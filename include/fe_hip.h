@@ -85,14 +85,17 @@ typedef struct {
  *   bseg_ptr/bseg : for block entry (i, q) the contributing incidences of node i, ascending element,
  *                   bseg = (k_local << 2) | b  with k_local the incidence's rank within node i's list
  *                   and b the local vertex of the column node
- *   blk : node ranges per workgroup of the fused assembly (row-block schedule)                      */
+ *   blk : node ranges per workgroup of the fused assembly (row-block schedule)
+ *   welem_ptr/welem : per workgroup, the ascending distinct elements incident to its node range
+ *   inc_sa : per incidence, (slot << 2) | a with slot the element's position in its workgroup's
+ *            welem list and a its local vertex (fe_schedule_elements builds these three)          */
 typedef struct {
   int64_t n_nodes;
   int64_t nnzb;
   int64_t n_inc;
   int64_t n_blk;
   int32_t max_inc;  /* largest incidence count of one workgroup's node range */
-  int32_t threads;  /* workgroup size of the fused assembly: 64, 128 or 256 (0 = 256) */
+  int32_t threads;  /* workgroup size of the fused assembly: 64, 128, 256, 512 or 1024 (0 = 256) */
   const int32_t* bptr;
   const int32_t* bcol;
   const int32_t* inc_ptr;
@@ -100,6 +103,11 @@ typedef struct {
   const int32_t* bseg_ptr;
   const uint16_t* bseg;
   const int32_t* blk;
+  int32_t max_elem; /* largest welem count of one workgroup */
+  int32_t pad;
+  const int32_t* welem_ptr;
+  const int32_t* welem;
+  const uint16_t* inc_sa;
 } fe_graph;
 
 /* Block sparse matrix (node-block rows) on the device; precision 0 = FP64, 1 = FP32; layout
@@ -202,6 +210,14 @@ int fe_pattern_schedule(fe_pattern* p, int max_inc_per_block);
 int fe_pattern_info(const fe_pattern* p, int64_t* info);
 int fe_pattern_copy(const fe_pattern* p, int which, void* host_dst);
 int fe_pattern_destroy(fe_pattern* p);
+/* Element schedule of the fused assembly for the node ranges blk (n_blk + 1 entries): per workgroup
+   the ascending distinct elements incident to its range (welem_ptr: n_blk + 1 entries, welem:
+   capacity n_inc) and per incidence its packed (slot << 2 | a) code (inc_sa: n_inc entries);
+   max_elem receives the largest element count of one workgroup.  Host arrays; no counterpart in the
+   reference (its assembly is a host loop). */
+int fe_schedule_elements(const int32_t* inc_ptr, const int32_t* inc, int npe, int64_t n_nodes,
+                         const int32_t* blk, int64_t n_blk, int32_t* welem_ptr, int32_t* welem,
+                         uint16_t* inc_sa, int32_t* max_elem);
 
 /* ---- assembly ------------------------------------------------------------------------------------ */
 /* Two-phase assembly from element matrices: vals[p] = sum over the sorted contribution segment of p.

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from finite_elements_amd import meshgen
-from finite_elements_amd.device import Pattern
+from finite_elements_amd.device import Pattern, element_schedule
 from finite_elements_amd.gmsh import read_msh
 from oracle import vectorized as vec
 from tests.conftest import GOLDEN, rowscaled_err
@@ -97,6 +97,41 @@ def test_pattern_bit_exact_and_maps(name, kind, coords, conn):
         csr_vals[b0 * d * d:b1 * d * d] = chunk.transpose(1, 0, 2).reshape(-1)
     got.data = csr_vals
     assert rowscaled_err(K, got) < 1e-13
+
+
+@pytest.mark.parametrize('name,kind,coords,conn', _cases(), ids=[c[0] for c in _cases()])
+@pytest.mark.parametrize('max_inc', [64, 256])
+def test_element_schedule(name, kind, coords, conn, max_inc):
+    """CPU emulation of the fused assembly's element schedule: every block contribution resolves,
+    through its incidence's packed slot code, to the element and local vertex the incidence names,
+    and each workgroup's element list is the ascending set of its incidences' elements."""
+    d = DOF[kind]
+    N = coords.shape[0]
+    npe = conn.shape[1]
+    pat = Pattern(conn, d, N)
+    cap = max(max_inc, int(np.diff(pat.inc_ptr).max()))
+    blk, biggest = Pattern.schedule_for(pat.inc_ptr, cap)
+    wptr, welem, inc_sa, max_elem = element_schedule(pat, blk)
+    assert wptr[0] == 0 and wptr[-1] == welem.size
+    assert max_elem == np.diff(wptr).max()
+    for w in range(blk.size - 1):
+        i0, i1 = pat.inc_ptr[blk[w]], pat.inc_ptr[blk[w + 1]]
+        elems = pat.inc[i0:i1] // npe
+        mine = welem[wptr[w]:wptr[w + 1]]
+        assert np.array_equal(mine, np.unique(elems))
+        slot = inc_sa[i0:i1].astype(np.int64) >> 2
+        a = inc_sa[i0:i1].astype(np.int64) & 3
+        assert np.array_equal(mine[slot], elems)
+        assert np.array_equal(a, pat.inc[i0:i1] % npe)
+    # phase 2 emulation: block (i, j) <- element (slot, a, b) via bseg and inc_sa
+    owner = np.repeat(np.arange(pat.nnzb), np.diff(pat.bseg_ptr))
+    i = np.repeat(np.arange(N), np.diff(pat.bptr))[owner]
+    t = pat.inc_ptr[i] + (pat.bseg.astype(np.int64) >> 2)
+    w_of_node = np.repeat(np.arange(blk.size - 1), np.diff(blk))
+    e = welem[wptr[w_of_node[i]] + (inc_sa[t].astype(np.int64) >> 2)]
+    a = inc_sa[t].astype(np.int64) & 3
+    b = pat.bseg.astype(np.int64) & 3
+    assert np.all(conn[e, a] == i) and np.all(conn[e, b] == pat.bcol[owner])
 
 
 def test_schedule_covers_nodes():
