@@ -117,12 +117,11 @@ def time_method(sysm, pr, method, fixed, g, f_t, steps, warmup, profile=False):
     return times, infos
 
 
-def cpu_baseline(budget_s=30.0):
-    """The reference's algorithm (oracle restatement) on a bounded sample: 14^3 cube."""
+def _cpu_solve_once(n):
+    """One solve of the reference's algorithm (oracle restatement) on an n^3 cube; returns timings."""
     from finite_elements_amd import meshgen
     from oracle import reference as ref
     import scipy.sparse.linalg
-    n = int(os.environ.get('FE_CPU_SAMPLE_N', '14'))
     pr = meshgen.cube_elasticity(n=n, seed=4)
     t0 = time.perf_counter()
     prob = ref.problem_from_arrays(pr.kind, pr.coords, pr.conn, pr.materials, plane_strain=True,
@@ -133,14 +132,35 @@ def cpu_baseline(budget_s=30.0):
     t2 = time.perf_counter()
     x = scipy.sparse.linalg.spsolve(K, F, permc_spec='NATURAL', use_umfpack=True)
     t3 = time.perf_counter()
-    solve_s = t3 - t1
+    return {'n': n, 'dofs': int(pr.n), 'elements': int(pr.conn.shape[0]), 'assemble_s': t2 - t1,
+            'solve_s': t3 - t2, 'total_s': t3 - t1, 'object_setup_s': t1 - t0,
+            'finite': bool(np.all(np.isfinite(x)))}
+
+
+def cpu_baseline():
+    """The reference's algorithm (oracle restatement: per-element Python loop, scipy COO->CSR,
+    natural-order SuperLU on the Lagrange system) on a ladder of small cubes, about 10-20 s of CPU
+    work; the value is the largest rung (14^3), and a power-law fit of the ladder is extrapolated to
+    64^3 (labelled as such: natural-order LU of the 64^3 system does not fit in memory)."""
+    sizes = [int(v) for v in os.environ.get('FE_CPU_LADDER', '8,10,12,14').split(',')]
+    ladder = [_cpu_solve_once(n) for n in sizes]
+    top = ladder[-1]
+    # assembly and LU solve extrapolated apart (their exponents differ)
+    d = np.log([r['dofs'] for r in ladder])
+    fits = {k: np.polyfit(d, np.log([r[k] for r in ladder]), 1) for k in ('assemble_s', 'solve_s')}
+    extrap = float(sum(np.exp(b + a * np.log(823875)) for a, b in fits.values()))
     cores = len(os.sched_getaffinity(0))
-    return {'value': 1.0 / solve_s, 'unit': 'solves/s', 'cores': 1, 'kind': 'port',
-            'sample': (f'{n}^3 cube ({pr.n} DOF, {pr.conn.shape[0]} tets): oracle per-element Python '
-                       f'loop + scipy COO->CSR ({t2 - t1:.2f} s) + spsolve NATURAL on the Lagrange '
-                       f'system ({t3 - t2:.2f} s); single-threaded Python/SuperLU; {cores} host cores '
-                       'visible; the 64^3 case is infeasible for natural-order LU'),
-            'object_setup_s': t1 - t0, 'finite': bool(np.all(np.isfinite(x)))}
+    return {'value': 1.0 / top['total_s'], 'unit': 'solves/s', 'cores': 1, 'kind': 'port',
+            'sample': (f"{top['n']}^3 cube ({top['dofs']} DOF, {top['elements']} tets): oracle "
+                       f"per-element Python loop + scipy COO->CSR ({top['assemble_s']:.2f} s) + spsolve "
+                       f"NATURAL on the Lagrange system ({top['solve_s']:.2f} s); single-threaded "
+                       f'Python/SuperLU; {cores} host cores visible'),
+            'ladder': [{k: r[k] for k in ('n', 'dofs', 'total_s', 'assemble_s', 'solve_s')}
+                       for r in ladder],
+            'fit_exponents': {k: float(v[0]) for k, v in fits.items()},
+            'extrapolated_64cube_s': extrap,
+            'extrapolation_note': 'power-law fit of the ladder, not a measurement',
+            'finite': all(r['finite'] for r in ladder)}
 
 
 def copy_calibration(reps=3):
@@ -225,7 +245,9 @@ def measured_traffic(kernel):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--steps', type=int, default=10)
+    ap.add_argument('--cold-steps', type=int, default=5,
+                    help='extra solves after the timed ones, each after a cache flush (reported apart)')
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--workload', default='cube64')
     ap.add_argument('--plain-steps', type=int, default=2)
@@ -315,13 +337,29 @@ def main():
         one_solve(sysm, pr, headline, fixed, g, f_t)
     torch.cuda.synchronize()
     t_start = time.perf_counter()
-    infos = []
+    infos, step_s = [], []
     for k in range(args.steps):
+        ts = time.perf_counter()
         # the last step also times the iteration's SpMV kernel (20 launches after convergence)
         _, info = one_solve(sysm, pr, headline, fixed, g, f_t, profile=(k == args.steps - 1))
+        torch.cuda.synchronize()
+        step_s.append(time.perf_counter() - ts)
         infos.append(info)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
+    # cold variant (outside the timed region): L2 and the 256 MB Infinity Cache flushed before each
+    # solve by writing a 1 GiB buffer
+    cold_s = []
+    if args.cold_steps > 0:
+        flush = torch.empty(1 << 28, dtype=torch.float32, device='cuda')
+        for _ in range(args.cold_steps):
+            flush.fill_(1.0)
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            one_solve(sysm, pr, headline, fixed, g, f_t)
+            torch.cuda.synchronize()
+            cold_s.append(time.perf_counter() - ts)
+        del flush
     ms_per_step = elapsed / args.steps * 1e3
     value = args.steps / elapsed
     info = infos[-1]
@@ -358,6 +396,8 @@ def main():
                    'coarse_dofs': int(coarse.n_coarse),
                    'step': 'assembly + Dirichlet elimination + preconditioner setup + PCG'},
         'headline': {'method': headline['name'], 'iterations': info['iterations'],
+                     'median_ms_warm': float(np.median(step_s)) * 1e3,
+                     'median_ms_cold': float(np.median(cold_s)) * 1e3 if cold_s else None,
                      'relres_true_fp64': info['relres'], 'reliable_updates': info['reliable_updates'],
                      'restarts': info['restarts'], 'fallbacks': info.get('fallbacks', 0)},
         'ablations': ablations,

@@ -53,7 +53,7 @@ class FeGraph(ctypes.Structure):
     _fields_ = [('n_nodes', _i64), ('nnzb', _i64), ('n_inc', _i64), ('n_blk', _i64),
                 ('max_inc', _i32), ('threads', _i32), ('bptr', _vp), ('bcol', _vp), ('inc_ptr', _vp),
                 ('inc', _vp), ('bseg_ptr', _vp), ('bseg', _vp), ('blk', _vp), ('max_elem', _i32),
-                ('pad', _i32), ('welem_ptr', _vp), ('welem', _vp), ('inc_sa', _vp)]
+                ('pad', _i32), ('welem_ptr', _vp), ('welem', _vp), ('inc_sa', _vp), ('bdiag', _vp)]
 
 
 class FeBsr(ctypes.Structure):
@@ -120,6 +120,7 @@ SIGNATURES = {
     'fe_spmv_csr': (_int, [_vp, _i64, _vp, _vp, _vp, _vp, _vp]),
     'fe_cast_f64_to_f32': (_int, [_vp, _i64, _vp, _vp]),
     'fe_bsr_convert': (_int, [_vp, _P(FeBsr), _P(FeBsr)]),
+    'fe_element_fields': (_int, [_vp, _P(FeMesh), _vp, _vp, _vp, _vp]),
     'fe_schedule_elements': (_int, [_vp, _vp, _int, _i64, _vp, _i64, _vp, _vp, _vp, _P(_i32)]),
     'fe_bsr_repack': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp]),
     'fe_precond_setup': (_int, [_vp, _P(FeBsr), _int, _vp]),

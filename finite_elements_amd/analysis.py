@@ -411,7 +411,8 @@ class FiniteElementAnalysis(FiniteElements):
         ``result_vector`` = n DOF values followed by one Lagrange multiplier per collapsed condition,
         in the reference's condition order.  method: 'gpu' (PCG on the MI355X).  precision: 'fp64' or
         'mixed' (FP32 SpMV/preconditioner inside FP64 reliable updates).  precond: 'jacobi',
-        'block_jacobi' or 'learned' (needs ``learned``, a trained ``LearnedPreconditioner``).
+        'block_jacobi', 'two_level' (block Jacobi plus a coarse correction over aggregated rigid-body
+        modes) or 'learned' (needs ``learned``, a trained ``LearnedPreconditioner``).
         Convergence is judged on the true FP64 residual of the eliminated system.
         """
         if method != 'gpu':
@@ -433,7 +434,8 @@ class FiniteElementAnalysis(FiniteElements):
                 raise ValueError("precond='learned' needs a trained LearnedPreconditioner")
             M64 = learned.preconditioner(sysm, KD, fixed)
             pc = 'block_jacobi'
-        solver = sysm.solver(KD, pc, precision, M64=M64)
+        coarse = sysm.coarse_space(pos) if pc == 'two_level' else None
+        solver = sysm.solver(KD, pc, precision, M64=M64, coarse=coarse)
         u, info = solver.solve(b, tol=tol, max_iter=max_iter)
         from . import device
         ku = device.spmv(K, u)

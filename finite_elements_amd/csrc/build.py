@@ -15,7 +15,7 @@ LIB_DIR = os.path.join(os.path.dirname(HERE), 'lib')
 LIB = os.path.join(LIB_DIR, 'libfe_hip.so')
 OBJ_DIR = os.path.join(ROOT, '_build')
 
-SOURCES = ['fe_api.hip', 'fe_element.hip', 'fe_assembly.hip', 'fe_solve.hip', 'fe_pcg.hip', 'fe_coarse.hip', 'fe_gnn.hip', 'fe_lobpcg.hip',
+SOURCES = ['fe_api.hip', 'fe_element.hip', 'fe_assembly.hip', 'fe_solve.hip', 'fe_pcg.hip', 'fe_coarse.hip', 'fe_gnn.hip', 'fe_lobpcg.hip', 'fe_post.hip',
            'fe_pattern.cpp']
 
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')

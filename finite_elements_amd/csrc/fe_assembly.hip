@@ -67,9 +67,13 @@ struct KindTraits<FE_KIND_TRI_MAGNETIC> {
   static constexpr int NPE = 3, D = 1;
 };
 
-// Doubles in the geometry record of one element (material coefficients pre-scaled by the measure):
-//   tet stiffness  g[4][3], vol*diag, vol*off, vol*shear        tet / triangle mass  f
-//   tri stiffness  gx[3], gy[3], tA*diag, tA*off, tA*shear      magnetic  gx[3], gy[3], area/mu
+// Doubles in the geometry record of one element (material coefficients pre-scaled by the measure;
+// an odd record size keeps the lanes' LDS accesses spread over the banks):
+//   tet stiffness  g[4][3], vol*off, vol*shear, 0         tet / triangle mass  f
+//   tri stiffness  gx[3], gy[3], tA*off, tA*shear, 0      magnetic  gx[3], gy[3], area/mu
+// The isotropic D matrices have diag = off + 2 shear (elements.py:418-453, 781-794), so with
+// lambda = off and mu = shear block (a, b) of B^T D B is  lambda g_a g_b^T + mu g_b g_a^T +
+// mu (g_a . g_b) I.
 template <int KIND, int WHAT>
 constexpr int record_size() {
   return WHAT == FE_MASS ? 1 : KIND == FE_KIND_TET_ELASTIC ? 15 : KIND == FE_KIND_TRI_ELASTIC ? 9 : 7;
@@ -100,9 +104,9 @@ __device__ inline void element_record(const fe_mesh& m, int64_t e, double* S) {
       for (int v = 0; v < 4; ++v)
 #pragma unroll
         for (int k = 0; k < 3; ++k) S[v * 3 + k] = g.g[v][k];
-      S[12] = g.vol * diag;
-      S[13] = g.vol * off;
-      S[14] = g.vol * shear;
+      S[12] = g.vol * off;
+      S[13] = g.vol * shear;
+      S[14] = 0.0;
     }
   } else if constexpr (KIND == FE_KIND_TRI_ELASTIC) {
     TriGeom g;
@@ -118,9 +122,9 @@ __device__ inline void element_record(const fe_mesh& m, int64_t e, double* S) {
         S[v] = g.gx[v];
         S[3 + v] = g.gy[v];
       }
-      S[6] = scale * diag;
-      S[7] = scale * off;
-      S[8] = scale * shear;
+      S[6] = scale * off;
+      S[7] = scale * shear;
+      S[8] = 0.0;
     }
   } else {
     TriGeom g;
@@ -146,70 +150,75 @@ __device__ inline void record_grad(const double* R, int a, double (&ga)[D]) {
   }
 }
 
-// blk = block (a, b) of the element matrix whose record starts at R (used for a != b).  With P = g_a g_b^T:
-//   elasticity (B^T D B)_ab scaled:  entry (d, d) = Dv P_dd + Mv (tr P - P_dd),
-//                                    entry (i, j) = Lv P_ij + Mv P_ji          (i != j)
-//   magnetic  (A/mu) (b_a b_b + c_a c_b);  masses  f I.
+// sum += block (a, b), a != b, of the element matrix whose record starts at R:
+//   elasticity  sum_ij += (lambda g_a,i) g_b,j + (mu g_b,i) g_a,j + [i == j] mu (g_a . g_b)
+//   magnetic    (A/mu) (b_a b_b + c_a c_b);  masses  f I.
 template <int KIND, int WHAT, int D>
-__device__ inline void block_ab(const double* R, int a, int b, double (&blk)[D * D]) {
+__device__ inline void acc_ab(const double* R, int a, int b, double (&sum)[D * D]) {
   if constexpr (WHAT == FE_MASS) {
     const double f = (a == b) ? 2 * R[0] : R[0];
 #pragma unroll
-    for (int k = 0; k < D * D; ++k) blk[k] = (k % (D + 1) == 0) ? f : 0.0;
+    for (int d = 0; d < D; ++d) sum[d * D + d] += f;
   } else if constexpr (KIND == FE_KIND_TRI_MAGNETIC) {
-    blk[0] = R[6] * (R[a] * R[b] + R[3 + a] * R[3 + b]);
+    sum[0] += R[6] * (R[a] * R[b] + R[3 + a] * R[3 + b]);
   } else {
-    constexpr int C = KIND == FE_KIND_TET_ELASTIC ? 12 : 6;  // first coefficient field
-    constexpr int GD = KIND == FE_KIND_TET_ELASTIC ? 3 : 2;
-    double ga[GD], gb[GD];
-    record_grad<KIND, GD>(R, a, ga);
-    record_grad<KIND, GD>(R, b, gb);
-    const double Dv = R[C], Lv = R[C + 1], Mv = R[C + 2];
-    double P[GD][GD];
+    constexpr int C = KIND == FE_KIND_TET_ELASTIC ? 12 : 6;  // lambda, mu
+    double ga[D], gb[D];
+    record_grad<KIND, D>(R, a, ga);
+    record_grad<KIND, D>(R, b, gb);
+    const double lam = R[C], mu = R[C + 1];
+    double la[D], mb[D];
+    double dot = ga[0] * gb[0];
 #pragma unroll
-    for (int i = 0; i < GD; ++i)
+    for (int i = 1; i < D; ++i) dot += ga[i] * gb[i];
 #pragma unroll
-      for (int j = 0; j < GD; ++j) P[i][j] = ga[i] * gb[j];
-    double tr = P[0][0];
+    for (int i = 0; i < D; ++i) {
+      la[i] = lam * ga[i];
+      mb[i] = mu * gb[i];
+    }
+    const double mdot = mu * dot;
 #pragma unroll
-    for (int i = 1; i < GD; ++i) tr += P[i][i];
+    for (int i = 0; i < D; ++i)
 #pragma unroll
-    for (int i = 0; i < GD; ++i)
-#pragma unroll
-      for (int j = 0; j < GD; ++j)
-        blk[i * GD + j] = (i == j) ? Dv * P[i][i] + Mv * (tr - P[i][i]) : Lv * P[i][j] + Mv * P[j][i];
+      for (int j = 0; j < D; ++j) {
+        double v = la[i] * gb[j] + mb[i] * ga[j];
+        if (i == j) v += mdot;
+        sum[i * D + j] += v;
+      }
   }
 }
 
-// The diagonal block (a, a) of the same element matrix (P = g_a g_a^T is symmetric).
+// sum += the diagonal block (a, a): (lambda + mu) g_a g_a^T + mu |g_a|^2 I (symmetric).
 template <int KIND, int WHAT, int D>
-__device__ inline void block_aa(const double* R, int a, double (&blk)[D * D]) {
+__device__ inline void acc_aa(const double* R, int a, double (&sum)[D * D]) {
   if constexpr (WHAT == FE_MASS) {
     const double f2 = 2 * R[0];
 #pragma unroll
-    for (int k = 0; k < D * D; ++k) blk[k] = (k % (D + 1) == 0) ? f2 : 0.0;
+    for (int d = 0; d < D; ++d) sum[d * D + d] += f2;
   } else if constexpr (KIND == FE_KIND_TRI_MAGNETIC) {
     const double gx = R[a], gy = R[3 + a];
-    blk[0] = R[6] * (gx * gx + gy * gy);
+    sum[0] += R[6] * (gx * gx + gy * gy);
   } else {
     constexpr int C = KIND == FE_KIND_TET_ELASTIC ? 12 : 6;
-    constexpr int GD = KIND == FE_KIND_TET_ELASTIC ? 3 : 2;
-    double ga[GD];
-    record_grad<KIND, GD>(R, a, ga);
-    const double Dv = R[C], LM = R[C + 1] + R[C + 2],
-                 Mv = R[C + 2];
-    double sq[GD];
-    double tr = 0.0;
+    double ga[D];
+    record_grad<KIND, D>(R, a, ga);
+    const double lm = R[C] + R[C + 1], mu = R[C + 1];
+    double sq = ga[0] * ga[0];
 #pragma unroll
-    for (int i = 0; i < GD; ++i) {
-      sq[i] = ga[i] * ga[i];
-      tr += sq[i];
-    }
+    for (int i = 1; i < D; ++i) sq += ga[i] * ga[i];
+    const double msq = mu * sq;
+    double lg[D];
 #pragma unroll
-    for (int i = 0; i < GD; ++i)
+    for (int i = 0; i < D; ++i) lg[i] = lm * ga[i];
 #pragma unroll
-      for (int j = 0; j < GD; ++j)
-        blk[i * GD + j] = (i == j) ? Dv * sq[i] + Mv * (tr - sq[i]) : LM * (ga[i] * ga[j]);
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = i; j < D; ++j) {
+        double v = lg[i] * ga[j];
+        if (i == j) v += msq;
+        sum[i * D + j] += v;
+        if (j != i) sum[j * D + i] += v;
+      }
   }
 }
 
@@ -272,7 +281,7 @@ __global__ __launch_bounds__(NT) void k_assemble_fused(fe_mesh m, fe_graph g,
   for (int t = threadIdx.x; t <= nn; t += NT) {
     s_bptr[t] = g.bptr[n0 + t];
     s_iptr[t] = g.inc_ptr[n0 + t] - I0;
-    if (t < nn) s_diag[t] = -1;  // a node without elements has no diagonal entry
+    if (t < nn) s_diag[t] = g.bdiag[n0 + t];
   }
   for (int t = threadIdx.x; t < ninc; t += NT) s_sa[t] = g.inc_sa[I0 + t];
   for (int t = threadIdx.x; t <= nbq; t += NT) s_bsp[t] = g.bseg_ptr[B0 + t] - c_lo;
@@ -280,68 +289,61 @@ __global__ __launch_bounds__(NT) void k_assemble_fused(fe_mesh m, fe_graph g,
   // Phase 1: the geometry record of every distinct element of the range.
   for (int t = threadIdx.x; t < nel; t += NT) element_record<KIND, WHAT>(m, g.welem[E0 + t], S + t * RS);
   __syncthreads();
-  // Phase 2a: one thread per off-diagonal block entry (node j, neighbour q) of the range, summing
-  // block (a, b) of the elements holding both nodes in ascending element order.  Consecutive threads
-  // own consecutive block entries, so the stores are coalesced.  A diagonal entry (its contributions
-  // have a == b) is only located here: a node's diagonal sums over all its incident elements.
-  for (int32_t t = threadIdx.x; t < nbq; t += NT) {
-    const int32_t bq = B0 + t;
-    int lo = 0, hi = nn - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (s_bptr[mid] <= bq)
-        lo = mid;
-      else
-        hi = mid - 1;
-    }
-    const int j = lo;
-    const int32_t c0 = s_bsp[t], c1 = s_bsp[t + 1];
-    const int32_t base = s_iptr[j];
-    {
-      const int v = s_bseg[c0];
-      if ((s_sa[base + (v >> 2)] & 3) == (v & 3)) {
-        s_diag[j] = bq;
-        continue;
+  // Phase 2, one loop over two kinds of work item (whole wavefronts take one kind):
+  //  * items [0, nbq): one thread per off-diagonal block entry (node j, neighbour q), summing block
+  //    (a, b) of the elements holding both nodes in ascending element order; consecutive threads own
+  //    consecutive block entries, so the stores are coalesced;
+  //  * items from nbq rounded up to 64: DL lanes per diagonal block; lane l sums the node's
+  //    incidences l, l + DL, ... (ascending element order), then the lanes combine by a fixed
+  //    butterfly.
+  const int32_t nbq64 = (nbq + 63) & ~63;
+  const int32_t items = nbq64 + nn * DL;
+  for (int32_t t = threadIdx.x; t < items; t += NT) {
+    if (t < nbq64) {
+      if (t >= nbq) continue;
+      const int32_t bq = B0 + t;
+      int lo = 0, hi = nn - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_bptr[mid] <= bq)
+          lo = mid;
+        else
+          hi = mid - 1;
       }
+      const int j = lo;
+      if (bq == s_diag[j]) continue;
+      const int32_t c0 = s_bsp[t], c1 = s_bsp[t + 1];
+      const int32_t base = s_iptr[j];
+      double sum[D * D];
+#pragma unroll
+      for (int k = 0; k < D * D; ++k) sum[k] = 0.0;
+      for (int32_t c = c0; c < c1; ++c) {
+        const int v = s_bseg[c];
+        const int sa = s_sa[base + (v >> 2)];
+        acc_ab<KIND, WHAT, D>(S + (sa >> 2) * RS, sa & 3, v & 3, sum);
+      }
+#pragma unroll
+      for (int k = 0; k < D * D; ++k) store_block<D, LAYOUT>(vals, g, s_bptr, j, bq, k, sum[k]);
+    } else {
+      const int u = t - nbq64;
+      const int j = u / DL, l = u % DL;
+      double sum[D * D];
+#pragma unroll
+      for (int k = 0; k < D * D; ++k) sum[k] = 0.0;
+      for (int32_t k = s_iptr[j] + l; k < s_iptr[j + 1]; k += DL) {
+        const int sa = s_sa[k];
+        acc_aa<KIND, WHAT, D>(S + (sa >> 2) * RS, sa & 3, sum);
+      }
+#pragma unroll
+      for (int off = DL / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < D * D; ++k) sum[k] += __shfl_xor(sum[k], off, DL);
+      const int32_t bq = s_diag[j];
+      if (bq >= 0)
+#pragma unroll
+        for (int k = 0; k < D * D; ++k)
+          if (k % DL == l) store_block<D, LAYOUT>(vals, g, s_bptr, j, bq, k, sum[k]);
     }
-    double sum[D * D];
-#pragma unroll
-    for (int k = 0; k < D * D; ++k) sum[k] = 0.0;
-    for (int32_t c = c0; c < c1; ++c) {
-      const int v = s_bseg[c];
-      const int sa = s_sa[base + (v >> 2)];
-      double blk[D * D];
-      block_ab<KIND, WHAT, D>(S + (sa >> 2) * RS, sa & 3, v & 3, blk);
-#pragma unroll
-      for (int k = 0; k < D * D; ++k) sum[k] += blk[k];
-    }
-#pragma unroll
-    for (int k = 0; k < D * D; ++k) store_block<D, LAYOUT>(vals, g, s_bptr, j, bq, k, sum[k]);
-  }
-  __syncthreads();
-  // Phase 2b: DL lanes per diagonal block; lane l sums the node's incidences l, l + DL, ... (ascending
-  // element order), then the lanes combine by a fixed butterfly.
-  for (int t = threadIdx.x; t < nn * DL; t += NT) {
-    const int j = t / DL, l = t % DL;
-    double sum[D * D];
-#pragma unroll
-    for (int k = 0; k < D * D; ++k) sum[k] = 0.0;
-    for (int32_t k = s_iptr[j] + l; k < s_iptr[j + 1]; k += DL) {
-      const int sa = s_sa[k];
-      double blk[D * D];
-      block_aa<KIND, WHAT, D>(S + (sa >> 2) * RS, sa & 3, blk);
-#pragma unroll
-      for (int q = 0; q < D * D; ++q) sum[q] += blk[q];
-    }
-#pragma unroll
-    for (int off = DL / 2; off > 0; off >>= 1)
-#pragma unroll
-      for (int k = 0; k < D * D; ++k) sum[k] += __shfl_xor(sum[k], off, DL);
-    const int32_t bq = s_diag[j];
-    if (bq >= 0)
-#pragma unroll
-      for (int k = 0; k < D * D; ++k)
-        if (k % DL == l) store_block<D, LAYOUT>(vals, g, s_bptr, j, bq, k, sum[k]);
   }
 }
 
@@ -471,7 +473,7 @@ extern "C" int fe_assemble_fused(fe_handle* h, const fe_mesh* m, const fe_graph*
              "fe_assemble_fused: bad schedule");
   if (g->n_blk == 0 || g->nnzb == 0) return FE_OK;
   FE_REQUIRE(m->conn && m->x && m->y && g->bptr && g->inc_ptr && g->bseg_ptr && g->bseg && g->blk &&
-                 g->welem_ptr && g->welem && g->inc_sa,
+                 g->welem_ptr && g->welem && g->inc_sa && g->bdiag,
              "fe_assemble_fused: null mesh or graph array (element schedule from fe_schedule_elements)");
   FE_REQUIRE(m->n_nodes == g->n_nodes, "fe_assemble_fused: mesh and graph node counts differ");
   switch (m->kind) {

@@ -244,6 +244,11 @@ class DeviceGraph:
         self.inc = ctx.tensor(pat.inc)
         self.bseg_ptr = ctx.tensor(pat.bseg_ptr)
         self.bseg = ctx.tensor(pat.bseg.view(np.int16))
+        rows = np.repeat(np.arange(self.n_nodes, dtype=np.int32), np.diff(pat.bptr))
+        bdiag = np.full(max(self.n_nodes, 1), -1, dtype=np.int32)
+        q = np.nonzero(pat.bcol == rows)[0]
+        bdiag[rows[q]] = q
+        self.bdiag = ctx.tensor(bdiag)
         self.set_schedule(pat.blk, pat.max_inc)
         self._padded = None
 
@@ -272,7 +277,7 @@ class DeviceGraph:
                             self.threads,
                             _vp(self.bptr), _vp(self.bcol), _vp(self.inc_ptr), _vp(self.inc),
                             _vp(self.bseg_ptr), _vp(self.bseg), _vp(self.blk), self.max_elem, 0,
-                            _vp(self.welem_ptr), _vp(self.welem), _vp(self.inc_sa))
+                            _vp(self.welem_ptr), _vp(self.welem), _vp(self.inc_sa), _vp(self.bdiag))
 
 
 def pad_pattern(bptr: np.ndarray, bcol: np.ndarray, align: int = 4):
@@ -373,6 +378,29 @@ class BsrMatrix:
 # ------------------------------------------------------------------------------------------------
 # kernels
 # ------------------------------------------------------------------------------------------------
+def element_fields(mesh: DeviceMesh, u: torch.Tensor):
+    """Per-element fields of the solution u (device FP64, interleaved DOFs): elasticity ->
+    (strain (ns, E), stress (ns, E), energy (E,)) with ns = 3 (tri) or 6 (tet), Voigt order;
+    magnetic -> (B (2, E), None, None)."""
+    ctx = mesh.ctx
+    E = mesh.n_elem
+    if u.dtype != torch.float64 or u.numel() != mesh.n_nodes * mesh.dof:
+        raise ValueError('u must be the FP64 solution vector of this mesh')
+    u = u.contiguous()
+    dev = ctx.torch_device
+    if mesh.kind == 'tri_magnetic':
+        b = torch.empty((2, E), dtype=torch.float64, device=dev)
+        _hip.call('fe_element_fields', ctx.h, ctypes.byref(mesh.struct()), _vp(u), _vp(b), None, None)
+        return b, None, None
+    ns = 3 if mesh.kind == 'tri_elastic' else 6
+    strain = torch.empty((ns, E), dtype=torch.float64, device=dev)
+    stress = torch.empty((ns, E), dtype=torch.float64, device=dev)
+    energy = torch.empty(E, dtype=torch.float64, device=dev)
+    _hip.call('fe_element_fields', ctx.h, ctypes.byref(mesh.struct()), _vp(u), _vp(strain),
+              _vp(stress), _vp(energy))
+    return strain, stress, energy
+
+
 def element_matrices(mesh: DeviceMesh, what: str = 'stiffness', sym: bool = False) -> torch.Tensor:
     """Entry-major element matrices ke[k, e] (k over m*m, or the m(m+1)/2 upper triangle)."""
     ctx = mesh.ctx

@@ -1,8 +1,11 @@
-"""Result containers (finite_elements/results.py:23-54, 57-152, 490-714).
+"""Result containers (finite_elements/results.py:23-54, 57-152, 490-830).
 
 ``Result(mesh, result_vector)`` is what ``solve()`` returns.  The elasticity and magnetic result
 classes read the first n entries of ``result_vector`` through the interleaved DOF map, as the
-reference's do; plotting and VTK export are not part of this package.
+reference's do.  Per-element strain, stress, energy and the magnetic field are computed on the GPU
+by ``fe_element_fields`` (one thread per element); the dictionaries keyed by element objects that
+the reference returns are built from those arrays.  Plotting and VTK export are not part of this
+package.
 """
 
 from __future__ import annotations
@@ -12,6 +15,21 @@ from typing import List
 import numpy as np
 
 from .mesh import ArrayMesh
+
+
+def _device_fields(mesh, kind, u, plane_mode):
+    """(fields (ns, E), stress or None, energy or None) of u on the GPU, as numpy arrays."""
+    from . import device
+    from .analysis import mesh_arrays
+    _, coords, conn, mats = mesh_arrays(mesh)
+    ctx = device.Context.get(0)
+    dm = device.DeviceMesh(ctx, kind, coords, conn, mats, plane_mode)
+    out = device.element_fields(dm, ctx.tensor(np.ascontiguousarray(u, dtype=np.float64)))
+    return tuple(None if t is None else t.cpu().numpy() for t in out)
+
+
+def _elements(mesh):
+    return [el for group in mesh.elements_groups for el in group.elements]
 
 
 class Result:
@@ -32,6 +50,9 @@ class ElasticityResults(Result):
         self.plane_strain = plane_strain
         self.plane_stress = plane_stress
         self._displacement_vectors_per_node = None
+        self._fields = None
+        self._strain = self._stress = self._energy_per_element = None
+        self._displacements_per_element = None
 
     def displacements_array(self):
         """(N, dim) displacements: node i's entries are result_vector[i*dim : (i+1)*dim]."""
@@ -49,6 +70,59 @@ class ElasticityResults(Result):
         return self._displacement_vectors_per_node
 
 
+    # ---- per-element fields (finite_elements/results.py:689-830) ---------------------------
+    def element_fields(self):
+        """(strain (E, ns), stress (E, ns), energy (E,)) arrays, Voigt order (tri: exx, eyy, gxy;
+        tet: exx, eyy, ezz, gxy, gyz, gzx), elements group by group."""
+        if self._fields is None:
+            from .elements import _plane_mode
+            kind = {2: 'tri_elastic', 3: 'tet_elastic'}[self.dimension]
+            mode = _plane_mode(self.plane_strain, self.plane_stress) if kind == 'tri_elastic' else 0
+            n = len(self.mesh.nodes) * self.dimension
+            strain, stress, energy = _device_fields(self.mesh, kind, self.result_vector[:n], mode)
+            self._fields = (strain.T.copy(), stress.T.copy(), energy)
+        return self._fields
+
+    @property
+    def displacements_per_element(self):
+        """{element: [u of vertex 0 (dim values), vertex 1, ...]} (results.py:689-715)."""
+        if self._displacements_per_element is None:
+            idx = self.mesh.node_to_index
+            d = self.dimension
+            u = self.result_vector
+            self._displacements_per_element = {
+                el: [float(u[idx[p] * d + c]) for p in el.points for c in range(d)]
+                for el in _elements(self.mesh)}
+        return self._displacements_per_element
+
+    @property
+    def strain(self):
+        """{element: strain vector B u_e} (results.py:784-830)."""
+        if self._strain is None:
+            self._strain = dict(zip(_elements(self.mesh), self.element_fields()[0]))
+        return self._strain
+
+    @property
+    def stress(self):
+        """{element: stress vector D B u_e} (results.py:784-830)."""
+        if self._stress is None:
+            self._stress = dict(zip(_elements(self.mesh), self.element_fields()[1]))
+        return self._stress
+
+    @property
+    def energy_per_element(self):
+        """{element: 1/2 u_e^T Ke u_e} (results.py:753-770)."""
+        if self._energy_per_element is None:
+            self._energy_per_element = {el: float(w) for el, w in
+                                        zip(_elements(self.mesh), self.element_fields()[2])}
+        return self._energy_per_element
+
+    @property
+    def energy(self):
+        """Total strain energy, the sum of the per-element energies (results.py:716-751)."""
+        return float(np.sum(self.element_fields()[2]))
+
+
 class ElasticityResults2D(ElasticityResults):
     pass
 
@@ -58,21 +132,26 @@ class ElasticityResults3D(ElasticityResults):
 
 
 class MagneticResults(Result):
-    """B = (sum c_i A_i, -sum b_i A_i) per element (finite_elements/results.py:121-152)."""
+    """B = (sum c_i A_i, -sum b_i A_i) per element (finite_elements/results.py:121-152), computed on
+    the GPU by ``fe_element_fields``."""
 
     def __init__(self, mesh, result_vector):
         super().__init__(mesh, result_vector)
+        self._field = None
         self.magnetic_field_per_element = self._magnetic_field_per_element()
 
+    def magnetic_field_array(self):
+        """(E, 2) field B per element, elements group by group."""
+        if self._field is None:
+            n = len(self.mesh.nodes)
+            b, _, _ = _device_fields(self.mesh, 'tri_magnetic', self.result_vector[:n], 0)
+            self._field = b.T.copy()
+        return self._field
+
     def _magnetic_field_per_element(self):
-        out = {}
-        idx = self.mesh.node_to_index
-        a = self.result_vector
-        for group in self.mesh.elements_groups:
-            for element in group.elements:
-                ff = element.form_functions
-                ids = [idx[p] for p in element.points]
-                bx = sum(ff[k][2] * a[ids[k]] for k in range(3))
-                by = -sum(ff[k][1] * a[ids[k]] for k in range(3))
-                out[element] = (float(bx), float(by))
-        return out
+        return {el: (float(b[0]), float(b[1])) for el, b in
+                zip(_elements(self.mesh), self.magnetic_field_array())}
+
+    def magnetic_field_norm(self):
+        """|B| per element (results.py:108-119)."""
+        return list(np.hypot(*self.magnetic_field_array().T))

@@ -107,6 +107,11 @@ class FESystem:
     def element_matrices(self, what: str = 'stiffness', sym: bool = False) -> torch.Tensor:
         return device.element_matrices(self.mesh, what, sym)
 
+    def element_fields(self, u):
+        """Per-element strain, stress and energy of the solution u (magnetic: the field B)."""
+        u_t = u if isinstance(u, torch.Tensor) else self.ctx.tensor(np.asarray(u, dtype=np.float64))
+        return device.element_fields(self.mesh, u_t)
+
     def assemble_two_phase(self, what: str = 'stiffness') -> torch.Tensor:
         """Element kernel (symmetric storage) + sorted segmented reduction -> CSR values."""
         if self.gather is None:

@@ -108,6 +108,7 @@ typedef struct {
   const int32_t* welem_ptr;
   const int32_t* welem;
   const uint16_t* inc_sa;
+  const int32_t* bdiag; /* per node, the block index of its diagonal entry (-1: node without elements) */
 } fe_graph;
 
 /* Block sparse matrix (node-block rows) on the device; precision 0 = FP64, 1 = FP32; layout
@@ -230,6 +231,13 @@ int fe_assemble_gather(fe_handle* h, const double* ke, const int32_t* gather, co
    order.  No element matrix touches HBM; no atomics; bitwise deterministic.  what = FE_STIFFNESS or
    FE_MASS; layout = FE_LAYOUT_CSR or FE_LAYOUT_BSR.  Replaces k_matrix_data + csr_matrix
    (finite_elements/analysis.py:324-339, 661) and m_matrix_data (357-365). */
+/* Per-element fields of a solution u (device, interleaved DOFs), component-major out[k * n_elem + e]
+   (replaces ElasticityResults.strain / stress / energy_per_element, finite_elements/results.py:689-830,
+   and MagneticResults' field, results.py:121-152).  Elasticity: strain and stress in Voigt order (tri
+   3: exx, eyy, gxy; tet 6: exx, eyy, ezz, gxy, gyz, gzx) and energy = 1/2 u_e^T Ke u_e; any of the
+   three may be NULL.  Magnetic: B = (Bx, By) in strain (2 components), stress and energy NULL. */
+int fe_element_fields(fe_handle* h, const fe_mesh* mesh, const double* u, double* strain,
+                      double* stress, double* energy);
 int fe_assemble_fused(fe_handle* h, const fe_mesh* mesh, const fe_graph* graph, int what, int layout,
                       double* vals);
 /* Permute node-chunk values between the CSR and BSR layouts (out-of-place). */

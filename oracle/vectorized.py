@@ -200,3 +200,33 @@ def dirichlet_matrix(K, fixed_dofs):
     keep[np.asarray(fixed_dofs, dtype=np.int64)] = 0.0
     Dk = sparse.diags(keep)
     return (Dk @ K @ Dk + sparse.diags(1.0 - keep)).tocsr()
+
+
+def element_fields(kind, coords, conn, materials, u, plane_mode=0):
+    """Per-element fields of a solution u (interleaved DOFs), as the reference's results.py computes
+    them: strain = B u_e, stress = D B u_e (results.py:810-830), energy = 1/2 u_e^T Ke u_e
+    (elements.py:275-292, results.py:738-770) with u_e the element's DOFs in vertex-major order
+    (results.py:689-715); magnetic: B = (sum c_i A_i, -sum b_i A_i) (results.py:121-152).
+    Returns (strain (E, ns), stress (E, ns) or None, energy (E,) or None)."""
+    conn = np.asarray(conn, dtype=np.int64)
+    u = np.asarray(u, dtype=np.float64)
+    mats = {k: np.broadcast_to(np.asarray(v, dtype=np.float64), (len(conn),)) for k, v in materials.items()}
+    if kind == 'tri_magnetic':
+        g = tri_grads(coords, conn)  # (E, 3, 2): (b_i, c_i)
+        a = u[conn]
+        B = np.stack([np.einsum('ei,ei->e', g[:, :, 1], a), -np.einsum('ei,ei->e', g[:, :, 0], a)], 1)
+        return B, None, None
+    dof = 2 if kind == 'tri_elastic' else 3
+    ue = u[(conn[:, :, None] * dof + np.arange(dof)).reshape(len(conn), -1)]
+    if kind == 'tri_elastic':
+        Bm = tri_b_matrix(coords, conn)
+        D = tri_d(mats['E'], mats['nu'], plane_mode == 1)
+    else:
+        Bm = tet_b_matrix(coords, conn)
+        D = tet_d(mats['E'], mats['nu'])
+    strain = np.einsum('eij,ej->ei', Bm, ue)
+    stress = np.einsum('eij,ej->ei', D, strain)
+    m = ue.shape[1]
+    ke = element_matrices(kind, coords, conn, materials, plane_mode).reshape(-1, m, m)
+    energy = 0.5 * np.einsum('ei,eij,ej->e', ue, ke, ue)
+    return strain, stress, energy
