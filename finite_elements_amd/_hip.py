@@ -139,6 +139,9 @@ SIGNATURES = {
     'fe_lobpcg_workspace': (_int, [_vp, _P(FeBsr), _P(FeLobpcgOpts), _P(_i64)]),
     'fe_lobpcg': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64,
                          _P(FeLobpcgOpts), _P(FeLobpcgStats)]),
+    'fe_lobpcg_coarse_workspace': (_int, [_vp, _P(FeBsr), _P(FeCoarse), _P(FeLobpcgOpts), _P(_i64)]),
+    'fe_lobpcg_coarse': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp, _vp, _P(FeCoarse), _vp, _i64, _vp, _vp,
+                                _vp, _vp, _vp, _vp, _i64, _P(FeLobpcgOpts), _P(FeLobpcgStats)]),
     'fe_sym_eig': (_int, [_vp, _int, _int, _vp, _vp, _vp]),
     'fe_gnn_num_params': (_int, [_int]),
     'fe_gnn_features': (_int, [_vp, _P(FeBsr), _vp, _vp, _vp]),

@@ -26,12 +26,13 @@ constexpr int kMaxBasis = 3 * kMaxBlock;
 constexpr int kGramRows = 16;
 constexpr int kThreads = 256;
 constexpr int kRRThreads = 1024;
+constexpr int kRefresh = 32;  // iterations between explicit K X, M X, K P, M P
 constexpr int kNumBlocks = 15;  // X0 X1 KX0 KX1 MX0 MX1 P0 P1 KP0 KP1 MP0 MP1 W KW MW
 enum { B_X = 0, B_KX = 2, B_MX = 4, B_P = 6, B_KP = 8, B_MP = 10, B_W = 12, B_KW = 13, B_MW = 14 };
 
 struct LobState {
   int32_t it, done, cur, init;
-  int32_t ns, kept, nconv, pad;
+  int32_t ns, kept, nconv, restart;  // restart: drop P from the next iteration's basis
   double lambda[kMaxBlock];
   double resid[kMaxBlock];
 };
@@ -63,12 +64,19 @@ __global__ __launch_bounds__(kThreads) void k_lob_resid(LobParams p, const doubl
   const int v = blockIdx.y;
   __shared__ int s_skip, s_cur;
   __shared__ double s_lam[kMaxBlock];
+  __shared__ int s_lock[kMaxBlock];
   __shared__ double s_rr[kThreads], s_bb[kThreads];
   if (threadIdx.x == 0) {
     s_skip = st[v].done;
     s_cur = st[v].cur;
   }
-  if (threadIdx.x < p.m) s_lam[threadIdx.x] = st[v].lambda[threadIdx.x];
+  if (threadIdx.x < p.m) {
+    s_lam[threadIdx.x] = st[v].lambda[threadIdx.x];
+    // soft locking: a wanted pair already converged ten times below the tolerance (at the last
+    // check) contributes no search direction; it stays in X and in the Rayleigh-Ritz problem
+    s_lock[threadIdx.x] = !st[v].init && threadIdx.x < p.n_wanted &&
+                          st[v].resid[threadIdx.x] <= 0.1 * p.tol;
+  }
   __syncthreads();
   if (s_skip) return;
   const int m = p.m;
@@ -101,7 +109,7 @@ __global__ __launch_bounds__(kThreads) void k_lob_resid(LobParams p, const doubl
 #pragma unroll
           for (int f = 0; f < D; ++f) w += Tv[(i * D + e) * D + f] * R[f];
         }
-        if (fixed && fixed[i * D + e]) w = 0.0;
+        if ((fixed && fixed[i * D + e]) || s_lock[j]) w = 0.0;
         W[(i * D + e) * m + j] = w;
       }
     }
@@ -122,6 +130,130 @@ __global__ __launch_bounds__(kThreads) void k_lob_resid(LobParams p, const doubl
 }
 
 // --------------------------------------------------------------------------------------------------
+// K1b (two-level preconditioner): W += P A_c^-1 P^T R, for every column of the block.  R is
+// recomputed from K X, M X and lambda; P rows of constrained DOFs are zero, so W stays zero there.
+// --------------------------------------------------------------------------------------------------
+// r_c = P^T R: one workgroup per (aggregate, variant); thread (group g, column j) sums over the
+// aggregate's nodes g, g + npg, ..., then the groups are added in a fixed order.
+template <int D, int NC>
+__global__ __launch_bounds__(kThreads) void k_lob_crestrict(LobParams p, fe_coarse C, int64_t ld,
+                                                            const LobState* st, const double* blk,
+                                                            double* __restrict__ rc) {
+  const int v = blockIdx.y;
+  const int64_t agg = blockIdx.x;
+  __shared__ int s_skip, s_cur;
+  __shared__ double s_lam[kMaxBlock];
+  __shared__ double s_acc[kThreads][NC];
+  if (threadIdx.x == 0) {
+    s_skip = st[v].done;
+    s_cur = st[v].cur;
+  }
+  if (threadIdx.x < p.m) s_lam[threadIdx.x] = st[v].lambda[threadIdx.x];
+  __syncthreads();
+  if (s_skip) return;
+  const int m = p.m, npg = kThreads / m;
+  const int g = threadIdx.x / m, j = threadIdx.x % m;
+  const double* KX = lob_block(const_cast<double*>(blk), p, v, B_KX + s_cur);
+  const double* MX = lob_block(const_cast<double*>(blk), p, v, B_MX + s_cur);
+  double acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = 0.0;
+  if (g < npg) {
+    const double lam = s_lam[j];
+    const int32_t a0 = C.agg_ptr[agg], a1 = C.agg_ptr[agg + 1];
+    for (int32_t k = a0 + g; k < a1; k += npg) {
+      const int64_t i = C.agg_nodes[k];
+#pragma unroll
+      for (int e = 0; e < D; ++e) {
+        const int64_t at = (i * D + e) * m + j;
+        const double r = KX[at] - lam * MX[at];
+        const double* Pr = C.P + (i * D + e) * NC;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c] += Pr[c] * r;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) s_acc[threadIdx.x][c] = acc[c];
+  __syncthreads();
+  if (threadIdx.x < m * NC) {
+    const int jj = threadIdx.x % m, c = threadIdx.x / m;
+    double sum = 0.0;
+    for (int q = 0; q < npg; ++q) sum += s_acc[q * m + jj][c];
+    rc[((int64_t)v * ld + agg * NC + c) * m + jj] = sum;
+  }
+}
+
+// z_c = A_c^-1 r_c per variant: a workgroup computes 64 rows x m columns, staging 32-wide slabs of
+// A_c^-1 and r_c in LDS.
+constexpr int kCgRows = 64, kCgK = 32;
+__global__ __launch_bounds__(kThreads) void k_lob_cgemm(LobParams p, int64_t ld, const LobState* st,
+                                                        const double* __restrict__ Ainv,
+                                                        int64_t ainv_stride,
+                                                        const double* __restrict__ rc,
+                                                        double* __restrict__ zc) {
+  const int v = blockIdx.y;
+  if (st[v].done) return;
+  __shared__ double s_a[kCgRows][kCgK + 1];
+  __shared__ double s_r[kCgK][kMaxBlock];
+  const int m = p.m;
+  const int64_t r0 = (int64_t)blockIdx.x * kCgRows;
+  const double* A = Ainv + (int64_t)v * ainv_stride;
+  const double* R = rc + (int64_t)v * ld * m;
+  constexpr int kPer = (kCgRows * kMaxBlock + kThreads - 1) / kThreads;  // outputs per thread (max)
+  double acc[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) acc[q] = 0.0;
+  for (int64_t k0 = 0; k0 < ld; k0 += kCgK) {
+    for (int t = threadIdx.x; t < kCgRows * kCgK; t += kThreads) {
+      const int rr = t / kCgK, kk = t % kCgK;
+      s_a[rr][kk] = A[(r0 + rr) * ld + k0 + kk];
+    }
+    for (int t = threadIdx.x; t < kCgK * m; t += kThreads) s_r[t / m][t % m] = R[(k0 + t / m) * m + t % m];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int o = threadIdx.x + q * kThreads;
+      if (o < kCgRows * m) {
+        const int rr = o / m, jj = o % m;
+        double s = acc[q];
+#pragma unroll 8
+        for (int kk = 0; kk < kCgK; ++kk) s += s_a[rr][kk] * s_r[kk][jj];
+        acc[q] = s;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int o = threadIdx.x + q * kThreads;
+    if (o < kCgRows * m) zc[((int64_t)v * ld + r0 + o / m) * m + o % m] = acc[q];
+  }
+}
+
+// W += P z_c: one thread per (DOF row, column).
+template <int D, int NC>
+__global__ __launch_bounds__(kThreads) void k_lob_cprolong(LobParams p, fe_coarse C, int64_t ld,
+                                                           const LobState* st,
+                                                           const double* __restrict__ zc, double* blk) {
+  const int v = blockIdx.y;
+  if (st[v].done) return;
+  const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= p.nm) return;
+  const int m = p.m;
+  const int64_t row = t / m;
+  const int j = (int)(t % m);
+  const int64_t i = row / D;
+  const int64_t agg = C.node_agg[i];
+  const double* Pr = C.P + row * NC;
+  const double* z = zc + ((int64_t)v * ld + agg * NC) * m + j;
+  double s = 0.0;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) s += Pr[c] * z[c * m];
+  lob_block(blk, p, v, B_W)[t] += s;
+}
+
+// --------------------------------------------------------------------------------------------------
 // K2: K Y and M Y for the block Y = X (start) or W, one thread per (node row, column)
 // --------------------------------------------------------------------------------------------------
 template <int D>
@@ -129,7 +261,7 @@ __global__ __launch_bounds__(kThreads) void k_lob_spmm(LobParams p, const int32_
                                                        const int32_t* __restrict__ bcol,
                                                        const double* __restrict__ Kv, int64_t k_stride,
                                                        const double* __restrict__ Mv, int64_t m_stride,
-                                                       const LobState* st, double* blk, int start) {
+                                                       const LobState* st, double* blk, int src) {
   const int v = blockIdx.y;
   __shared__ int s_skip, s_cur;
   if (threadIdx.x == 0) {
@@ -143,9 +275,13 @@ __global__ __launch_bounds__(kThreads) void k_lob_spmm(LobParams p, const int32_
   const int g = threadIdx.x / m, j = threadIdx.x % m;
   const int64_t i = (int64_t)blockIdx.x * npg + g;
   if (g >= npg || i >= p.nb) return;
-  const double* Y = lob_block(blk, p, v, start ? B_X + s_cur : B_W);
-  double* KY = lob_block(blk, p, v, start ? B_KX + s_cur : B_KW);
-  double* MY = lob_block(blk, p, v, start ? B_MX + s_cur : B_MW);
+  // src 0: W -> KW, MW; 1: the current X -> KX, MX; 2: the current P -> KP, MP
+  const int yb = src == 0 ? B_W : (src == 1 ? B_X : B_P) + s_cur;
+  const int kb = src == 0 ? B_KW : (src == 1 ? B_KX : B_KP) + s_cur;
+  const int mb = src == 0 ? B_MW : (src == 1 ? B_MX : B_MP) + s_cur;
+  const double* Y = lob_block(blk, p, v, yb);
+  double* KY = lob_block(blk, p, v, kb);
+  double* MY = lob_block(blk, p, v, mb);
   const double* K = Kv + (int64_t)v * k_stride;
   const double* M = Mv + (int64_t)v * m_stride;
   double ak[D], am[D];
@@ -321,13 +457,30 @@ __device__ void jacobi_eig(double* A, double* V, int r, int max_sweeps) {
   const int re = r + (r & 1), npair = re / 2;
   __syncthreads();
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    // stop when every off-diagonal entry is small relative to its own diagonal pair (the criterion
+    // that gives small eigenpairs of a unit-diagonal SPD matrix high relative accuracy), with an
+    // absolute floor of 1e-15 of the matrix norm for (near-)singular diagonals
+    double fro = 0.0;
+    for (int idx = tid; idx < r * r; idx += nth) fro += A[idx] * A[idx];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) fro += __shfl_xor(fro, o, 64);
+    if ((tid & 63) == 0) s_dia[tid >> 6] = fro;
+    __syncthreads();
+    if (tid == 0) {
+      double f2 = 0.0;
+      for (int w = 0; w < (nth + 63) / 64; ++w) f2 += s_dia[w];
+      s_off[0] = f2;
+    }
+    __syncthreads();
+    const double floor2 = 1e-30 * s_off[0];
+    __syncthreads();
     double off = 0.0, dia = 0.0;
     for (int idx = tid; idx < r * r; idx += nth) {
-      const double a = A[idx];
-      if (idx / r == idx % r)
-        dia += a * a;
-      else
-        off += a * a;
+      const int a_ = idx / r, b_ = idx % r;
+      if (a_ != b_) {
+        const double a = A[idx];
+        off += a * a / (fabs(A[a_ * r + a_] * A[b_ * r + b_]) + floor2);
+      }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -345,7 +498,7 @@ __device__ void jacobi_eig(double* A, double* V, int r, int max_sweeps) {
         o2 += s_off[w];
         d2 += s_dia[w];
       }
-      s_stop = !(o2 > 1e-30 * d2);
+      s_stop = !(o2 > 1e-24);
     }
     __syncthreads();
     if (s_stop) break;
@@ -515,6 +668,11 @@ __global__ __launch_bounds__(kRRThreads) void k_lob_rr(LobParams p, LobState* st
     }
     s_keep = kept;
     s->kept = kept;
+    // directions dropped for near-dependence (soft-locked W columns aside): the next iteration
+    // restarts the search from [X W] without P
+    int zero = 0;
+    for (int k = 0; k < ns; ++k) zero += !(s_scale[k] > 0.0);
+    s->restart = (kept + zero < ns);
   }
   __syncthreads();
   const int rk = s_keep;
@@ -627,7 +785,7 @@ __global__ void k_lob_flip(LobParams p, LobState* st) {
     s->ns = 2 * p.m;
   } else {
     s->it += 1;
-    s->ns = 3 * p.m;
+    s->ns = s->restart ? 2 * p.m : 3 * p.m;
   }
 }
 
@@ -670,9 +828,10 @@ __global__ void k_lob_init(LobParams p, LobState* st) {
   s->ns = p.m;
   s->kept = 0;
   s->nconv = 0;
+  s->restart = 0;
   for (int j = 0; j < kMaxBlock; ++j) {
     s->lambda[j] = 0.0;
-    s->resid[j] = 0.0;
+    s->resid[j] = HUGE_VAL;  // not yet measured: no pair is locked
   }
 }
 
@@ -711,6 +870,8 @@ struct LobWork {
   double* C;
   double* Z;
   double* T1;
+  double* rc;  // two-level: [nvar][ld][m] restricted residuals and coarse solutions
+  double* zc;
   int64_t bytes;
 };
 
@@ -738,7 +899,7 @@ LobParams make_params(const fe_bsr* K, const fe_lobpcg_opts* o) {
   return p;
 }
 
-LobWork layout_lob(char* base, const LobParams& p) {
+LobWork layout_lob(char* base, const LobParams& p, int64_t ld = 0) {
   LobWork w{};
   int64_t off = 0;
   auto take = [&](int64_t bytes) {
@@ -756,6 +917,10 @@ LobWork layout_lob(char* base, const LobParams& p) {
   w.C = (double*)take(nv * NS * p.m * (int64_t)sizeof(double));
   w.Z = (double*)take(nv * NS * NS * (int64_t)sizeof(double));
   w.T1 = (double*)take(nv * NS * NS * (int64_t)sizeof(double));
+  if (ld > 0) {
+    w.rc = (double*)take(nv * ld * p.m * (int64_t)sizeof(double));
+    w.zc = (double*)take(nv * ld * p.m * (int64_t)sizeof(double));
+  }
   w.bytes = off;
   return w;
 }
@@ -776,13 +941,33 @@ int check_opts(const fe_bsr* K, const fe_lobpcg_opts* o) {
   return FE_OK;
 }
 
+// Two-level preconditioner of the iteration (NULL C: block Jacobi T only).
+struct LobCoarse {
+  const fe_coarse* C;
+  const double* Ainv;
+  int64_t ainv_stride;
+  int64_t ld;
+};
+
+template <int D, int NC>
+void launch_lob_coarse(hipStream_t st, const LobParams& p, const LobCoarse& cc, const LobWork& w) {
+  const int nv = p.nvar;
+  hipLaunchKernelGGL((k_lob_crestrict<D, NC>), dim3((unsigned)cc.C->n_agg, nv), dim3(kThreads), 0, st, p,
+                     *cc.C, cc.ld, w.st, w.blk, w.rc);
+  hipLaunchKernelGGL(k_lob_cgemm, dim3((unsigned)(cc.ld / kCgRows), nv), dim3(kThreads), 0, st, p, cc.ld,
+                     w.st, cc.Ainv, cc.ainv_stride, w.rc, w.zc);
+  hipLaunchKernelGGL((k_lob_cprolong<D, NC>), dim3((unsigned)ceil_div(p.nm, kThreads), nv), dim3(kThreads),
+                     0, st, p, *cc.C, cc.ld, w.st, w.zc, w.blk);
+}
+
 template <int D>
 int run_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
                const uint8_t* fixed, double* X, double* lam, double* res, int32_t* iters,
                int32_t* vstatus, const LobWork& w, const LobParams& p, const fe_lobpcg_opts* o,
-               fe_lobpcg_stats* stats) {
+               fe_lobpcg_stats* stats, const LobCoarse& cc) {
   hipStream_t st = h->stream;
   const int nv = p.nvar;
+  if (cc.C) FE_CHECK_HIP(hipMemsetAsync(w.rc, 0, (size_t)nv * cc.ld * p.m * sizeof(double), st));
   const dim3 gvar((unsigned)ceil_div(nv, 64));
   hipLaunchKernelGGL(k_lob_init, gvar, dim3(64), 0, st, p, w.st);
   FE_CHECK_LAUNCH();
@@ -828,10 +1013,20 @@ int run_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
   hipEvent_t ev[2];
   FE_CHECK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
   FE_CHECK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  // The K- and M-images of X and P are carried by recombination; every kRefresh iterations they are
+  // recomputed from X and P so that rounding cannot accumulate into a residual floor.
+  int64_t host_it = 0;
   auto enqueue = [&](int slot) -> int {
-    for (int it = 0; it < chunk; ++it) {
+    for (int it = 0; it < chunk; ++it, ++host_it) {
+      if (host_it % kRefresh == kRefresh - 1) {
+        hipLaunchKernelGGL((k_lob_spmm<D>), gspmm, dim3(kThreads), 0, st, p, K->bptr, K->bcol, Kv,
+                           o->k_stride, Mv, o->m_stride, w.st, w.blk, 1);
+        hipLaunchKernelGGL((k_lob_spmm<D>), gspmm, dim3(kThreads), 0, st, p, K->bptr, K->bcol, Kv,
+                           o->k_stride, Mv, o->m_stride, w.st, w.blk, 2);
+      }
       hipLaunchKernelGGL((k_lob_resid<D>), gres, dim3(kThreads), 0, st, p, T, o->t_stride, fixed,
                          w.st, w.blk, w.rpart);
+      if (cc.C) launch_lob_coarse<D, D == 3 ? 6 : D == 2 ? 3 : 1>(st, p, cc, w);
       hipLaunchKernelGGL((k_lob_spmm<D>), gspmm, dim3(kThreads), 0, st, p, K->bptr, K->bcol, Kv,
                          o->k_stride, Mv, o->m_stride, w.st, w.blk, 0);
       int rc2 = rr_update();
@@ -873,6 +1068,40 @@ int run_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
   return FE_OK;
 }
 
+
+int check_lob_coarse(const fe_bsr* K, const fe_coarse* C, const char* who) {
+  FE_REQUIRE(C && C->P && C->node_agg && C->agg_ptr && C->agg_nodes, "%s: null coarse space", who);
+  FE_REQUIRE(C->dof == K->dof && C->n_nodes == K->n_brows, "%s: coarse space and K disagree", who);
+  FE_REQUIRE(C->nc == (K->dof == 3 ? 6 : K->dof == 2 ? 3 : 1), "%s: nc must be 6, 3 or 1 for dof 3, 2, 1",
+             who);
+  FE_REQUIRE(C->n_agg >= 1 && C->n_agg < (int64_t)1 << 31, "%s: bad aggregate count", who);
+  return FE_OK;
+}
+
+int lobpcg_common(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
+                  const uint8_t* fixed, double* X, double* lambda, double* resid, int32_t* iters,
+                  int32_t* vstatus, void* work, int64_t work_bytes, const fe_lobpcg_opts* o,
+                  fe_lobpcg_stats* stats, const LobCoarse& cc, const char* who) {
+  int rc = check_opts(K, o);
+  if (rc) return rc;
+  FE_REQUIRE(M && X && lambda && resid && iters && vstatus && work && stats, "%s: null argument", who);
+  FE_REQUIRE(M->precision == 0 && M->layout == FE_LAYOUT_PLANES && M->dof == K->dof &&
+                 M->n_brows == K->n_brows && M->nnzb == K->nnzb && M->row_align == 0,
+             "%s: M must be FP64 FE_LAYOUT_PLANES on the pattern of K", who);
+  FE_REQUIRE(o->k_stride >= 0 && o->m_stride >= 0 && o->t_stride >= 0, "%s: strides must be non-negative",
+             who);
+  *stats = fe_lobpcg_stats{};
+  const LobParams p = make_params(K, o);
+  const LobWork w = layout_lob((char*)work, p, cc.ld);
+  FE_REQUIRE(work_bytes >= w.bytes, "%s: workspace too small (%lld < %lld)", who, (long long)work_bytes,
+             (long long)w.bytes);
+  switch (K->dof) {
+    case 1: return run_lobpcg<1>(h, K, M, T, fixed, X, lambda, resid, iters, vstatus, w, p, o, stats, cc);
+    case 2: return run_lobpcg<2>(h, K, M, T, fixed, X, lambda, resid, iters, vstatus, w, p, o, stats, cc);
+    default: return run_lobpcg<3>(h, K, M, T, fixed, X, lambda, resid, iters, vstatus, w, p, o, stats, cc);
+  }
+}
+
 }  // namespace
 
 extern "C" int fe_lobpcg_workspace(fe_handle* h, const fe_bsr* K, const fe_lobpcg_opts* o,
@@ -890,25 +1119,38 @@ extern "C" int fe_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const d
                          int32_t* iters, int32_t* vstatus, void* work, int64_t work_bytes,
                          const fe_lobpcg_opts* o, fe_lobpcg_stats* stats) {
   FE_ENTER(h);
+  return lobpcg_common(h, K, M, T, fixed, X, lambda, resid, iters, vstatus, work, work_bytes, o, stats,
+                       LobCoarse{nullptr, nullptr, 0, 0}, "fe_lobpcg");
+}
+
+extern "C" int fe_lobpcg_coarse_workspace(fe_handle* h, const fe_bsr* K, const fe_coarse* C,
+                                          const fe_lobpcg_opts* o, int64_t* bytes) {
+  FE_ENTER(h);
   int rc = check_opts(K, o);
   if (rc) return rc;
-  FE_REQUIRE(M && X && lambda && resid && iters && vstatus && work && stats,
-             "fe_lobpcg: null argument");
-  FE_REQUIRE(M->precision == 0 && M->layout == FE_LAYOUT_PLANES && M->dof == K->dof &&
-                 M->n_brows == K->n_brows && M->nnzb == K->nnzb && M->row_align == 0,
-             "fe_lobpcg: M must be FP64 FE_LAYOUT_PLANES on the pattern of K");
-  FE_REQUIRE(o->k_stride >= 0 && o->m_stride >= 0 && o->t_stride >= 0,
-             "fe_lobpcg: strides must be non-negative");
-  *stats = fe_lobpcg_stats{};
-  const LobParams p = make_params(K, o);
-  const LobWork w = layout_lob((char*)work, p);
-  FE_REQUIRE(work_bytes >= w.bytes, "fe_lobpcg: workspace too small (%lld < %lld)",
-             (long long)work_bytes, (long long)w.bytes);
-  switch (K->dof) {
-    case 1: return run_lobpcg<1>(h, K, M, T, fixed, X, lambda, resid, iters, vstatus, w, p, o, stats);
-    case 2: return run_lobpcg<2>(h, K, M, T, fixed, X, lambda, resid, iters, vstatus, w, p, o, stats);
-    default: return run_lobpcg<3>(h, K, M, T, fixed, X, lambda, resid, iters, vstatus, w, p, o, stats);
-  }
+  rc = check_lob_coarse(K, C, "fe_lobpcg_coarse_workspace");
+  if (rc) return rc;
+  FE_REQUIRE(bytes, "fe_lobpcg_coarse_workspace: null argument");
+  *bytes = layout_lob(nullptr, make_params(K, o), fe_coarse_ld(C->n_agg * C->nc)).bytes;
+  return FE_OK;
+}
+
+extern "C" int fe_lobpcg_coarse(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
+                                const uint8_t* fixed, const fe_coarse* C, const double* Ainv,
+                                int64_t ainv_stride, double* X, double* lambda, double* resid,
+                                int32_t* iters, int32_t* vstatus, void* work, int64_t work_bytes,
+                                const fe_lobpcg_opts* o, fe_lobpcg_stats* stats) {
+  FE_ENTER(h);
+  int rc = check_opts(K, o);
+  if (rc) return rc;
+  rc = check_lob_coarse(K, C, "fe_lobpcg_coarse");
+  if (rc) return rc;
+  const int64_t ld = fe_coarse_ld(C->n_agg * C->nc);
+  FE_REQUIRE(Ainv, "fe_lobpcg_coarse: null coarse inverse");
+  FE_REQUIRE(o->n_variants == 1 || ainv_stride >= ld * ld,
+             "fe_lobpcg_coarse: ainv_stride must be >= ld * ld for several variants");
+  return lobpcg_common(h, K, M, T, fixed, X, lambda, resid, iters, vstatus, work, work_bytes, o, stats,
+                       LobCoarse{C, Ainv, ainv_stride, ld}, "fe_lobpcg_coarse");
 }
 
 extern "C" int fe_sym_eig(fe_handle* h, int n_mat, int r, const double* A, double* w, double* V) {

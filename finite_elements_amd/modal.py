@@ -39,14 +39,16 @@ def lobpcg(K: Union[BsrMatrix, Sequence[BsrMatrix]], M: Union[BsrMatrix, Sequenc
            n_wanted: int, block: Optional[int] = None, fixed: Optional[torch.Tensor] = None,
            precond: Union[str, None, torch.Tensor, Sequence[torch.Tensor]] = 'block_jacobi',
            largest: bool = False, tol: float = 1e-6, max_iter: int = 2000, X0=None, seed: int = 0,
-           check_every: int = 8):
+           check_every: int = 8, coarse=None):
     """Lowest (or largest) eigenpairs of one or several variants K_v phi = lambda M_v phi.
 
     K, M: one matrix or a list of variants on one pattern (FP64; converted to the plane layout).  A
     single M is shared by all variants.  fixed: uint8 DOF mask of constrained DOFs (their rows and
     columns of K and M must already be decoupled, e.g. by ``device.apply_dirichlet``); the iteration
     stays in the free subspace.  precond: 'block_jacobi' (blocks of each K_v), None, or explicit
-    FP64 blocks [n_brows][d][d] (one tensor or one per variant).  Returns (eigvals [nvar][n_wanted]
+    FP64 blocks [n_brows][d][d] (one tensor or one per variant).  coarse: a ``DeviceCoarse`` of the
+    constrained mesh (``FESystem.coarse_space``): the preconditioner becomes two-level, T + P A_c^-1 P^T
+    with A_c = P^T K_v P per variant (smallest eigenvalues only).  Returns (eigvals [nvar][n_wanted]
     numpy, eigvecs [nvar][n][n_wanted] torch FP64, info).
     """
     Ks = [K] if isinstance(K, BsrMatrix) else list(K)
@@ -109,12 +111,26 @@ def lobpcg(K: Union[BsrMatrix, Sequence[BsrMatrix]], M: Union[BsrMatrix, Sequenc
     ms = _hip.FeBsr(g.n_nodes, g.nnzb, d, 0, _hip.FE_LAYOUT_PLANES, 0, _vp(g.bptr), _vp(g.bcol),
                     _vp(Mall))
     nbytes = ctypes.c_int64()
-    _hip.call('fe_lobpcg_workspace', ctx.h, ctypes.byref(ks), ctypes.byref(opts), ctypes.byref(nbytes))
-    work = torch.empty(nbytes.value, dtype=torch.uint8, device=ctx.torch_device)
     stats = _hip.FeLobpcgStats()
-    _hip.call('fe_lobpcg', ctx.h, ctypes.byref(ks), ctypes.byref(ms), _vp(T), _vp(fixed), _vp(X),
-              _vp(lam), _vp(res), _vp(iters), _vp(vstat), _vp(work), work.numel(), ctypes.byref(opts),
-              ctypes.byref(stats))
+    if coarse is None:
+        _hip.call('fe_lobpcg_workspace', ctx.h, ctypes.byref(ks), ctypes.byref(opts), ctypes.byref(nbytes))
+        work = torch.empty(nbytes.value, dtype=torch.uint8, device=ctx.torch_device)
+        _hip.call('fe_lobpcg', ctx.h, ctypes.byref(ks), ctypes.byref(ms), _vp(T), _vp(fixed), _vp(X),
+                  _vp(lam), _vp(res), _vp(iters), _vp(vstat), _vp(work), work.numel(), ctypes.byref(opts),
+                  ctypes.byref(stats))
+    else:
+        if largest:
+            raise ValueError('lobpcg: the two-level preconditioner approximates K^-1 (smallest modes)')
+        # per-variant coarse inverses A_c^-1 = (P^T K_v P)^-1, FP64
+        Ainv = torch.stack([coarse.operator(A, invert=True)[0] for A in Ks])
+        cs = coarse.struct()
+        _hip.call('fe_lobpcg_coarse_workspace', ctx.h, ctypes.byref(ks), ctypes.byref(cs),
+                  ctypes.byref(opts), ctypes.byref(nbytes))
+        work = torch.empty(nbytes.value, dtype=torch.uint8, device=ctx.torch_device)
+        _hip.call('fe_lobpcg_coarse', ctx.h, ctypes.byref(ks), ctypes.byref(ms), _vp(T), _vp(fixed),
+                  ctypes.byref(cs), _vp(Ainv), coarse.ld * coarse.ld, _vp(X), _vp(lam), _vp(res),
+                  _vp(iters), _vp(vstat), _vp(work), work.numel(), ctypes.byref(opts),
+                  ctypes.byref(stats))
     lam_h = lam.cpu().numpy().reshape(nvar, block)
     res_h = res.cpu().numpy().reshape(nvar, block)
     info = {'iterations': iters.cpu().numpy(), 'variant_status': vstat.cpu().numpy(),
@@ -152,8 +168,11 @@ def sym_eig(A: torch.Tensor):
 
 
 def modal_analysis(analysis, order: str, k: int, method: str = 'lobpcg_gpu', tol: float = 1e-7,
-                   max_iter: int = 5000, block: Optional[int] = None, seed: int = 0):
-    """(eigvals, eigvecs.T) of the analysis's K, M (finite_elements/analysis.py:741-796)."""
+                   max_iter: int = 5000, block: Optional[int] = None, seed: int = 0,
+                   two_level: bool = True):
+    """(eigvals, eigvecs.T) of the analysis's K, M (finite_elements/analysis.py:741-796).  For the
+    smallest modes of a constrained elasticity problem the iteration is preconditioned by the
+    two-level method (block Jacobi + rigid-body coarse correction) unless ``two_level`` is False."""
     if order not in ('largest', 'smallest'):
         raise ValueError("Order parameter should be either 'largest' or 'smallest'")
     if method != 'lobpcg_gpu':
@@ -170,8 +189,9 @@ def modal_analysis(analysis, order: str, k: int, method: str = 'lobpcg_gpu', tol
         fixed, _ = sysm.dirichlet(pos)
         KD = device.apply_dirichlet(K, fixed)
         MD = device.apply_dirichlet(Mm, fixed)
+        coarse = sysm.coarse_space(pos) if (two_level and len(pos) and sysm.dof > 1) else None
         vals, vecs, info = lobpcg(KD, MD, k, block=block, fixed=fixed, precond='block_jacobi',
-                                  tol=tol, max_iter=max_iter, seed=seed)
+                                  tol=tol, max_iter=max_iter, seed=seed, coarse=coarse)
     analysis.last_modal_info = info
     if info['status'] != 0:
         import warnings

@@ -383,6 +383,15 @@ int fe_lobpcg_workspace(fe_handle* h, const fe_bsr* K, const fe_lobpcg_opts* opt
 int fe_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T, const uint8_t* fixed,
               double* X, double* lambda, double* resid, int32_t* iters, int32_t* variant_status,
               void* work, int64_t work_bytes, const fe_lobpcg_opts* opts, fe_lobpcg_stats* stats);
+/* fe_lobpcg with the two-level preconditioner T + P A_c^-1 P^T (no counterpart in the reference, whose
+   eigsh factorises K): C is the coarse space of the constrained mesh (shared by every variant), Ainv
+   the per-variant FP64 coarse inverses [ld][ld] from fe_coarse_operator, ainv_stride doubles apart. */
+int fe_lobpcg_coarse_workspace(fe_handle* h, const fe_bsr* K, const fe_coarse* C,
+                               const fe_lobpcg_opts* opts, int64_t* bytes);
+int fe_lobpcg_coarse(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
+                     const uint8_t* fixed, const fe_coarse* C, const double* Ainv, int64_t ainv_stride,
+                     double* X, double* lambda, double* resid, int32_t* iters, int32_t* variant_status,
+                     void* work, int64_t work_bytes, const fe_lobpcg_opts* opts, fe_lobpcg_stats* stats);
 /* Batched symmetric eigen-decomposition (parallel cyclic Jacobi, one workgroup per matrix, FP64):
    A [n_mat][r][r] (symmetrised on read), w [n_mat][r] eigenvalues (unordered), V [n_mat][r][r]
    eigenvectors in columns; r <= 96.  The Rayleigh-Ritz kernel of fe_lobpcg uses the same routine. */

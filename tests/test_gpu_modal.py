@@ -80,6 +80,32 @@ def test_lobpcg_smallest_clamped_vs_eigsh(ctx, n, k, block):
     _check_pairs(vals[0], vecs[0].cpu().numpy(), Ko, Mo, pr.fixed_dofs, ref_vals, ref_vecs)
 
 
+def test_lobpcg_two_level_vs_eigsh(ctx):
+    """The two-level preconditioner (block Jacobi + rigid-body coarse correction, one coarse inverse
+    per variant) gives the same pairs as eigsh in fewer iterations than block Jacobi alone."""
+    _say('lobpcg two-level')
+    seeds = (5000, 5001)
+    pr0, s = _modal_system(8, seeds[0])
+    fixed, _ = s.dirichlet(pr0.fixed_dofs)
+    MD = device.apply_dirichlet(s.mass_bsr(), fixed)
+    KDs, prs = [], []
+    for sd in seeds:
+        pr = meshgen.cube_modal(n=8, seed=sd)
+        s.set_materials(pr.materials)
+        KDs.append(device.apply_dirichlet(s.stiffness_bsr(), fixed))
+        prs.append(pr)
+    coarse = s.coarse_space(pr0.fixed_dofs, target=27)
+    v2, x2, i2 = modal.lobpcg(KDs, MD, 12, block=16, fixed=fixed, tol=1e-7, max_iter=3000, coarse=coarse)
+    v1, _, i1 = modal.lobpcg(KDs, MD, 12, block=16, fixed=fixed, tol=1e-7, max_iter=3000)
+    _say(f'  iterations two-level {i2["iterations"]}, block Jacobi {i1["iterations"]}')
+    assert i2['status'] == 0 and i1['status'] == 0
+    assert np.all(i2['iterations'] < i1['iterations'])
+    for v, pr in enumerate(prs):
+        Ko, Mo = _oracle_pencil(pr)
+        ref_vals, _ = eigen.smallest_clamped(Ko, Mo, pr.fixed_dofs, 12)
+        _check_pairs(v2[v], x2[v].cpu().numpy(), Ko, Mo, pr.fixed_dofs, ref_vals)
+
+
 def test_lobpcg_largest_unconstrained_vs_eigsh(ctx):
     _say('lobpcg largest unconstrained')
     pr, s = _modal_system(4, 5001)
