@@ -270,9 +270,10 @@ __global__ __launch_bounds__(kThreads) void k_lob_spmm(LobParams p, const int32_
   }
   __syncthreads();
   if (s_skip) return;
-  const int m = p.m;
-  const int npg = kThreads / m;
-  const int g = threadIdx.x / m, j = threadIdx.x % m;
+  // a thread owns 4 consecutive columns of one node row (m is a multiple of 4)
+  const int m = p.m, tpr = m / 4;
+  const int npg = kThreads / tpr;
+  const int g = threadIdx.x / tpr, j0 = (threadIdx.x % tpr) * 4;
   const int64_t i = (int64_t)blockIdx.x * npg + g;
   if (g >= npg || i >= p.nb) return;
   // src 0: W -> KW, MW; 1: the current X -> KX, MX; 2: the current P -> KP, MP
@@ -284,28 +285,45 @@ __global__ __launch_bounds__(kThreads) void k_lob_spmm(LobParams p, const int32_
   double* MY = lob_block(blk, p, v, mb);
   const double* K = Kv + (int64_t)v * k_stride;
   const double* M = Mv + (int64_t)v * m_stride;
-  double ak[D], am[D];
+  double ak[D][4], am[D][4];
 #pragma unroll
-  for (int e = 0; e < D; ++e) ak[e] = am[e] = 0.0;
+  for (int e = 0; e < D; ++e)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) ak[e][u] = am[e][u] = 0.0;
   const int32_t q0 = bptr[i], q1 = bptr[i + 1];
   for (int32_t q = q0; q < q1; ++q) {
     const int64_t c = bcol[q];
-    double y[D];
+    double y[D][4];
 #pragma unroll
-    for (int f = 0; f < D; ++f) y[f] = Y[(c * D + f) * m + j];
+    for (int f = 0; f < D; ++f) {
+      const double2* src = reinterpret_cast<const double2*>(Y + (c * D + f) * m + j0);
+      const double2 a = src[0], b = src[1];
+      y[f][0] = a.x;
+      y[f][1] = a.y;
+      y[f][2] = b.x;
+      y[f][3] = b.y;
+    }
 #pragma unroll
     for (int e = 0; e < D; ++e)
 #pragma unroll
       for (int f = 0; f < D; ++f) {
         const int64_t at = (int64_t)(e * D + f) * p.nnzb + q;
-        ak[e] += K[at] * y[f];
-        am[e] += M[at] * y[f];
+        const double kv = K[at], mv = M[at];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          ak[e][u] += kv * y[f][u];
+          am[e][u] += mv * y[f][u];
+        }
       }
   }
 #pragma unroll
   for (int e = 0; e < D; ++e) {
-    KY[(i * D + e) * m + j] = ak[e];
-    MY[(i * D + e) * m + j] = am[e];
+    double2* dk = reinterpret_cast<double2*>(KY + (i * D + e) * m + j0);
+    double2* dm = reinterpret_cast<double2*>(MY + (i * D + e) * m + j0);
+    dk[0] = make_double2(ak[e][0], ak[e][1]);
+    dk[1] = make_double2(ak[e][2], ak[e][3]);
+    dm[0] = make_double2(am[e][0], am[e][1]);
+    dm[1] = make_double2(am[e][2], am[e][3]);
   }
 }
 
@@ -457,30 +475,13 @@ __device__ void jacobi_eig(double* A, double* V, int r, int max_sweeps) {
   const int re = r + (r & 1), npair = re / 2;
   __syncthreads();
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-    // stop when every off-diagonal entry is small relative to its own diagonal pair (the criterion
-    // that gives small eigenpairs of a unit-diagonal SPD matrix high relative accuracy), with an
-    // absolute floor of 1e-15 of the matrix norm for (near-)singular diagonals
-    double fro = 0.0;
-    for (int idx = tid; idx < r * r; idx += nth) fro += A[idx] * A[idx];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) fro += __shfl_xor(fro, o, 64);
-    if ((tid & 63) == 0) s_dia[tid >> 6] = fro;
-    __syncthreads();
-    if (tid == 0) {
-      double f2 = 0.0;
-      for (int w = 0; w < (nth + 63) / 64; ++w) f2 += s_dia[w];
-      s_off[0] = f2;
-    }
-    __syncthreads();
-    const double floor2 = 1e-30 * s_off[0];
-    __syncthreads();
     double off = 0.0, dia = 0.0;
     for (int idx = tid; idx < r * r; idx += nth) {
-      const int a_ = idx / r, b_ = idx % r;
-      if (a_ != b_) {
-        const double a = A[idx];
-        off += a * a / (fabs(A[a_ * r + a_] * A[b_ * r + b_]) + floor2);
-      }
+      const double a = A[idx];
+      if (idx / r == idx % r)
+        dia += a * a;
+      else
+        off += a * a;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -498,7 +499,7 @@ __device__ void jacobi_eig(double* A, double* V, int r, int max_sweeps) {
         o2 += s_off[w];
         d2 += s_dia[w];
       }
-      s_stop = !(o2 > 1e-24);
+      s_stop = !(o2 > 1e-30 * d2);
     }
     __syncthreads();
     if (s_stop) break;
@@ -755,25 +756,43 @@ __global__ __launch_bounds__(kThreads) void k_lob_update(LobParams p, const LobS
   const double* C = Cg + (int64_t)v * 3 * m * m;
   for (int idx = threadIdx.x; idx < ns * m; idx += kThreads) sC[idx] = C[idx];
   __syncthreads();
-  const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (t >= p.nm) return;
-  const int64_t row = t / m;
-  const int j = (int)(t % m);
+  // a thread owns 4 consecutive columns of one DOF row
+  const int tpr = m / 4, rpb = kThreads / tpr;
+  const int g = threadIdx.x / tpr, j0 = (threadIdx.x % tpr) * 4;
+  const int64_t row = (int64_t)blockIdx.x * rpb + g;
+  if (g >= rpb || row >= p.n) return;
   for (int which = 0; which < 3; ++which) {
-    const double* X = lob_block(blk, p, v, B_X + 2 * which + cur);
-    const double* P = lob_block(blk, p, v, B_P + 2 * which + cur);
-    const double* W = lob_block(blk, p, v, B_W + which);
-    double xa = 0.0, pa = 0.0;
-    for (int a = 0; a < m; ++a) xa += X[row * m + a] * sC[a * m + j];
+    const double* X = lob_block(blk, p, v, B_X + 2 * which + cur) + row * m;
+    const double* P = lob_block(blk, p, v, B_P + 2 * which + cur) + row * m;
+    const double* W = lob_block(blk, p, v, B_W + which) + row * m;
+    double xa[4] = {0.0, 0.0, 0.0, 0.0}, pa[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int a = 0; a < m; a += 2) {
+      const double2 x = *reinterpret_cast<const double2*>(X + a);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) xa[u] += x.x * sC[a * m + j0 + u] + x.y * sC[(a + 1) * m + j0 + u];
+    }
     if (ns > m)
-      for (int a = 0; a < m; ++a) pa += W[row * m + a] * sC[(m + a) * m + j];
+      for (int a = 0; a < m; a += 2) {
+        const double2 w = *reinterpret_cast<const double2*>(W + a);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          pa[u] += w.x * sC[(m + a) * m + j0 + u] + w.y * sC[(m + a + 1) * m + j0 + u];
+      }
     if (ns > 2 * m)
-      for (int a = 0; a < m; ++a) pa += P[row * m + a] * sC[(2 * m + a) * m + j];
-    lob_block(blk, p, v, B_X + 2 * which + 1 - cur)[t] = xa + pa;
-    lob_block(blk, p, v, B_P + 2 * which + 1 - cur)[t] = pa;
+      for (int a = 0; a < m; a += 2) {
+        const double2 q = *reinterpret_cast<const double2*>(P + a);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          pa[u] += q.x * sC[(2 * m + a) * m + j0 + u] + q.y * sC[(2 * m + a + 1) * m + j0 + u];
+      }
+    double* Xn = lob_block(blk, p, v, B_X + 2 * which + 1 - cur) + row * m + j0;
+    double* Pn = lob_block(blk, p, v, B_P + 2 * which + 1 - cur) + row * m + j0;
+    reinterpret_cast<double2*>(Xn)[0] = make_double2(xa[0] + pa[0], xa[1] + pa[1]);
+    reinterpret_cast<double2*>(Xn)[1] = make_double2(xa[2] + pa[2], xa[3] + pa[3]);
+    reinterpret_cast<double2*>(Pn)[0] = make_double2(pa[0], pa[1]);
+    reinterpret_cast<double2*>(Pn)[1] = make_double2(pa[2], pa[3]);
   }
 }
-
 // K7: the new blocks become current; the basis gains W (and P after the first iteration).
 __global__ void k_lob_flip(LobParams p, LobState* st) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -980,13 +999,13 @@ int run_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
     for (int b = B_P; b < B_W; ++b)
       FE_CHECK_HIP(hipMemsetAsync(w.blk + ((int64_t)v * kNumBlocks + b) * p.nm, 0,
                                   p.nm * sizeof(double), st));
-  const int npg = kThreads / p.m;
+  const int npg = kThreads / (p.m / 4);  // node rows per workgroup of the block SpMM
   const dim3 gspmm((unsigned)ceil_div(p.nb, npg), nv);
   const dim3 gres((unsigned)p.nchunk_r, nv);
   const dim3 ggram((unsigned)(p.nchunk_g * p.ntg), nv);
   const int NS = 3 * p.m;
   const dim3 ggsum((unsigned)ceil_div(2 * NS * NS, kThreads), nv);
-  const dim3 gupd((unsigned)ceil_div(p.nm, kThreads), nv);
+  const dim3 gupd((unsigned)ceil_div(p.n, kThreads / (p.m / 4)), nv);
   const size_t rr_lds = 2 * (size_t)NS * NS * sizeof(double);
   FE_CHECK_HIP(hipFuncSetAttribute((const void*)k_lob_rr, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)rr_lds));

@@ -1,6 +1,8 @@
 """GPU modal path: the batched Jacobi eigensolver against numpy, and batched LOBPCG against the
 eigenvalue oracle (oracle/eigen.py: the reference's eigsh calls) on the oracle's own assembled K, M."""
 
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -8,6 +10,8 @@ import torch
 from finite_elements_amd import device, meshgen, modal
 from finite_elements_amd.system import FESystem
 from oracle import eigen, vectorized as vz
+
+from tests.conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
@@ -104,6 +108,31 @@ def test_lobpcg_two_level_vs_eigsh(ctx):
         Ko, Mo = _oracle_pencil(pr)
         ref_vals, _ = eigen.smallest_clamped(Ko, Mo, pr.fixed_dofs, 12)
         _check_pairs(v2[v], x2[v].cpu().numpy(), Ko, Mo, pr.fixed_dofs, ref_vals)
+
+
+def test_modal48_against_golden_eigsh(ctx):
+    """Full-size 48^3 variants (352,947 DOF) against the FP64 eigsh values the oracle wrote
+    (tests/golden/make_modal_golden.py): the 20 lowest eigenvalues within 1e-8 relative."""
+    path = os.path.join(GOLDEN, 'modal48_eigs.npz')
+    if not os.path.exists(path):
+        pytest.skip('tests/golden/modal48_eigs.npz not generated')
+    g = np.load(path, allow_pickle=False)
+    seeds = [int(x) for x in g['seeds']]
+    _say(f'lobpcg 48^3 golden seeds {seeds}')
+    pr0, s = _modal_system(48, seeds[0])
+    fixed, _ = s.dirichlet(pr0.fixed_dofs)
+    MD = device.apply_dirichlet(s.mass_bsr(), fixed)
+    KDs = []
+    for sd in seeds:
+        s.set_materials(meshgen.cube_modal(n=48, seed=sd).materials)
+        KDs.append(device.apply_dirichlet(s.stiffness_bsr(), fixed))
+    coarse = s.coarse_space(pr0.fixed_dofs)
+    vals, _, info = modal.lobpcg(KDs, MD, 20, fixed=fixed, tol=1e-6, max_iter=3000, coarse=coarse)
+    _say(f'  iterations {info["iterations"]}, status {info["status"]}')
+    assert info['status'] == 0
+    rel = np.abs(vals - g['eigenvalues']) / g['eigenvalues']
+    _say(f'  max relative difference to eigsh {rel.max():.2e}')
+    assert rel.max() < 1e-8
 
 
 def test_lobpcg_largest_unconstrained_vs_eigsh(ctx):
