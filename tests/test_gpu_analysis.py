@@ -8,7 +8,7 @@ import pytest
 import scipy.sparse as sp
 
 import finite_elements_amd as fe
-from finite_elements_amd import conditions, loads, mesh
+from finite_elements_amd import conditions, loads, mesh, meshgen
 from finite_elements_amd.analysis import FiniteElementAnalysis
 from oracle import reference as ref
 from tests.conftest import rowscaled_err
@@ -67,7 +67,7 @@ def build(key):
         bcs = [conditions.NodeBoundaryCondition(n, 0, 1) for n in m.nodes if abs(n[0] - 9) < 1e-6]
         return FiniteElementAnalysis(m, el_loads, [], [], [], [], bcs, [], [])
     if key == 'cantilever':
-        pr = fe.meshgen.cantilever_2d()
+        pr = meshgen.cantilever_2d()
         am = mesh.ArrayMesh(pr.coords, pr.conn, 'tri_elastic', pr.materials)
         nodes = am.nodes
         nl = [loads.NodeLoad(nodes[d // 2], pr.f[d], d % 2 + 1) for d in np.nonzero(pr.f)[0]]
@@ -120,6 +120,24 @@ def test_solve_result_vector(ctx, golden, key):
     lam_err = np.linalg.norm(x[n:] - xg[n:]) / max(np.linalg.norm(xg[n:]), 1e-300)
     print(f'  displacement rel err {u_err:.2e}, multiplier rel err {lam_err:.2e}', flush=True)
     assert u_err < U_ERR[key] and lam_err < max(U_ERR[key], 1e-8)
+
+
+@pytest.mark.parametrize('key', ['ex2', 'cantilever'])
+def test_solve_two_level(ctx, golden, key):
+    """solve(precond='two_level', precision='mixed'): block Jacobi + rigid-body coarse correction with
+    FP32 iterations, against the same golden result vector."""
+    print(f'analysis solve two-level {key}', flush=True)
+    g = golden('script_problems.npz')
+    an = build(key)
+    tol = SOLVE_TOL[key]
+    res = an.solve(tol=tol, precond='two_level', precision='mixed')
+    x = np.asarray(res.result_vector)
+    xg = g[f'{key}_x']
+    info = an.last_solve_info
+    print(f'  {info}', flush=True)
+    assert info['status'] == 0 and info['relres'] <= tol
+    n = an.n_dofs
+    assert np.linalg.norm(x[:n] - xg[:n]) / np.linalg.norm(xg[:n]) < U_ERR[key]
 
 
 def test_element_objects_and_matrix_data(ctx):
