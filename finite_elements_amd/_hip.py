@@ -80,7 +80,8 @@ class FePcgSegStats(ctypes.Structure):
 class FeCoarse(ctypes.Structure):
     _fields_ = [('n_nodes', _i64), ('n_agg', _i64), ('n_pairs', _i64), ('dof', _i32), ('nc', _i32),
                 ('node_agg', _vp), ('agg_ptr', _vp), ('agg_nodes', _vp), ('pair_ptr', _vp),
-                ('pair_ij', _vp), ('pair_q', _vp), ('pair_row', _vp), ('P', _vp), ('P32', _vp)]
+                ('pair_ij', _vp), ('pair_q', _vp), ('pair_row', _vp), ('P', _vp), ('P32', _vp),
+                ('xhat', _vp), ('xhat32', _vp), ('fixed_bits', _vp), ('T', _vp), ('T32', _vp)]
 
 
 class FeLobpcgOpts(ctypes.Structure):
@@ -147,7 +148,7 @@ SIGNATURES = {
     'fe_gnn_features': (_int, [_vp, _P(FeBsr), _vp, _vp, _vp]),
     'fe_gnn_forward': (_int, [_vp, _i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     'fe_gnn_loss_workspace': (_int, [_i64, _P(_i64)]),
-    'fe_gnn_loss_grad': (_int, [_vp, _P(FeBsr), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'fe_gnn_loss_grad': (_int, [_vp, _P(FeBsr), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _int]),
     'fe_gnn_backward_workspace': (_int, [_i64, _P(_i64)]),
     'fe_gnn_backward': (_int, [_vp, _i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     'fe_gnn_adam': (_int, [_vp, _int, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,

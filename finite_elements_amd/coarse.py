@@ -8,8 +8,12 @@ rigid-body modes are evaluated at its nodes:
 * 2D elasticity: 2 translations and 1 rotation;
 * scalar problems: the constant.
 
-Rows of constrained DOFs are zeroed, and each aggregate's modes are orthonormalised (QR, rank
-revealed). The result is the tentative prolongator P, with one d x nc block per node. The coarse
+Rows of constrained DOFs are zeroed, and each aggregate's modes are orthonormalised by an nc x nc
+transform T (from the eigen-decomposition of the modes' Gram matrix, rank revealed): P = Z T with
+one d x nc block per node. Because Z at a node is a fixed function of the node's centred and scaled
+coordinates x^, the solver evaluates P on the fly from x^ and T instead of reading it: P^T r sums the
+rigid-body moments of r over an aggregate and applies T^T; P z applies T to z and evaluates the
+rigid-body motion at the node. The coarse
 operator A_c = P^T A P is assembled on the GPU per solve (``fe_coarse_operator``) from the block
 pairs grouped here by coarse pair. The preconditioner is M = M_bj + P A_c^-1 P^T, block Jacobi plus
 the coarse correction: SPD, fixed, and cheap to apply (``fe_pcg_coarse``).
@@ -105,25 +109,31 @@ class CoarseSpace:
         if fixed_dofs is not None and len(fixed_dofs):
             fixed[np.asarray(fixed_dofs, dtype=np.int64)] = True
         fixed = fixed.reshape(n, dof)
-        # prolongator blocks, orthonormal per aggregate
+        # prolongator P = Z(x^) T, orthonormal per aggregate: T = V_kept diag(lambda_kept)^-1/2 from
+        # the eigen-decomposition of the masked modes' Gram matrix Z^T Z
         P = np.zeros((n, dof, self.nc))
+        xhat = np.zeros_like(coords)
+        T = np.zeros((self.n_agg, self.nc, self.nc))
         self.rank = np.zeros(self.n_agg, dtype=np.int32)
         for g in range(self.n_agg):
             nodes = order[self.agg_ptr[g]:self.agg_ptr[g + 1]]
             x = coords[nodes] - coords[nodes].mean(axis=0)
             scale = max(np.abs(x).max(), 1e-300)
-            Z = rigid_modes(x / scale, dof)
+            xhat[nodes] = x / scale
+            Z = rigid_modes(xhat[nodes], dof)
             Z[fixed[nodes]] = 0.0
             M = Z.reshape(-1, self.nc)
-            Q, R = np.linalg.qr(M)
-            dgn = np.abs(np.diag(R))
-            keep = dgn > 1e-10 * max(dgn.max(), 1e-300)
-            Q[:, ~keep] = 0.0
-            Q = Q.reshape(len(nodes), dof, self.nc)
-            Q[fixed[nodes]] = 0.0  # exactly zero on constrained DOFs (QR leaves rounding there)
+            lam, V = np.linalg.eigh(M.T @ M)
+            keep = lam > 1e-20 * max(lam.max(), 1e-300)
+            T[g][:, keep] = V[:, keep] / np.sqrt(lam[keep])
             self.rank[g] = int(keep.sum())
-            P[nodes] = Q
+            P[nodes] = (M @ T[g]).reshape(len(nodes), dof, self.nc)
         self.P = P
+        self.T = T
+        self.xhat = xhat
+        # per node, bit e set when DOF e is constrained
+        self.fixed_bits = (fixed.astype(np.uint8) << np.arange(dof, dtype=np.uint8)[None, :]).sum(
+            axis=1).astype(np.uint8)
         self.n_coarse = self.n_agg * self.nc
         # coarse pair map: every block (i, j) of the pattern contributes to pair (agg i, agg j)
         bptr = np.asarray(bptr, dtype=np.int64)

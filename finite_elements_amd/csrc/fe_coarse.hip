@@ -8,6 +8,10 @@
 //   k_coarse_restrict one workgroup per aggregate: r_c = P^T r
 //   k_coarse_gemv     one wavefront per coarse row: z = A_c^-1 r_c
 //   k_coarse_prolong  one thread per DOF: u += P z
+// With P = Z(x^) T given by the scaled node coordinates x^ and the aggregates' transforms T, the
+// restriction sums the rigid-body moments of r (force and moment about the aggregate's centre) and
+// applies T^T, and the prolongation applies T to z and evaluates the rigid-body motion at the node
+// (k_coarse_restrict_rm, k_coarse_prolong_rm): 15 bytes per node are read instead of P's d*nc values.
 // The dense inverse has a leading dimension padded to a multiple of 64 (identity in the padding).
 #include <algorithm>
 
@@ -306,6 +310,116 @@ __global__ __launch_bounds__(256) void k_coarse_gemv(int64_t n, int64_t ld, cons
   if (lane == 0) y[row] = (T)acc;
 }
 
+// Rigid-body moments of a nodal vector r (constrained DOFs read as zero) at scaled coordinates x:
+// the columns of Z are the translations and the rotations omega x x about the axes.
+template <int D>
+__device__ inline void rigid_moments(const double x[3], const double (&rv)[D], double* w) {
+  if constexpr (D == 3) {
+    w[0] += rv[0];
+    w[1] += rv[1];
+    w[2] += rv[2];
+    w[3] += x[1] * rv[2] - x[2] * rv[1];
+    w[4] += x[2] * rv[0] - x[0] * rv[2];
+    w[5] += x[0] * rv[1] - x[1] * rv[0];
+  } else if constexpr (D == 2) {
+    w[0] += rv[0];
+    w[1] += rv[1];
+    w[2] += x[0] * rv[1] - x[1] * rv[0];
+  } else {
+    w[0] += rv[0];
+  }
+}
+
+// r_c = P^T r with P = Z(x^) T: one workgroup per aggregate sums the moments w, then r_c = T^T w.
+template <typename T, int D, int NC>
+__global__ __launch_bounds__(kRestrictThreads) void k_coarse_restrict_rm(fe_coarse C, const T* __restrict__ xh,
+                                                                         const T* __restrict__ Tm,
+                                                                         const T* __restrict__ r,
+                                                                         T* __restrict__ rc,
+                                                                         const int32_t* state, int mode) {
+  __shared__ double red[NC][kRestrictThreads / 64];
+  __shared__ double s_w[NC];
+  __shared__ int s_skip;
+  if (threadIdx.x == 0) s_skip = state && !coarse_active(state, mode);
+  __syncthreads();
+  if (s_skip) return;
+  const int g = blockIdx.x;
+  double acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = 0.0;
+  for (int32_t t = C.agg_ptr[g] + threadIdx.x; t < C.agg_ptr[g + 1]; t += blockDim.x) {
+    const int64_t i = C.agg_nodes[t];
+    const int bits = C.fixed_bits[i];
+    const double x[3] = {(double)xh[i * 3], (double)xh[i * 3 + 1], (double)xh[i * 3 + 2]};
+    double rv[D];
+#pragma unroll
+    for (int e = 0; e < D; ++e) rv[e] = ((bits >> e) & 1) ? 0.0 : (double)r[i * D + e];
+    rigid_moments<D>(x, rv, acc);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    acc[c] = wave_sum(acc[c]);
+    if (lane == 0) red[c][wid] = acc[c];
+  }
+  __syncthreads();
+  if (threadIdx.x < NC) {
+    double v = 0.0;
+    for (int q = 0; q < kRestrictThreads / 64; ++q) v += red[threadIdx.x][q];
+    s_w[threadIdx.x] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NC) {
+    const int c = threadIdx.x;
+    double v = 0.0;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) v += (double)Tm[((int64_t)g * NC + k) * NC + c] * s_w[k];
+    rc[(int64_t)g * NC + c] = (T)v;
+  }
+}
+
+// u += P z with P = Z(x^) T: one thread per node applies T to its aggregate's z and adds the rigid
+// motion at x^ to the free DOFs.
+template <typename T, int D, int NC>
+__global__ __launch_bounds__(256) void k_coarse_prolong_rm(fe_coarse C, const T* __restrict__ xh,
+                                                           const T* __restrict__ Tm,
+                                                           const T* __restrict__ z, T* __restrict__ u,
+                                                           const int32_t* state, int mode) {
+  __shared__ int s_skip;
+  if (threadIdx.x == 0) s_skip = state && !coarse_active(state, mode);
+  __syncthreads();
+  if (s_skip) return;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= C.n_nodes) return;
+  const int64_t g = C.node_agg[i];
+  const T* Tg = Tm + g * NC * NC;
+  const T* zg = z + g * NC;
+  T y[NC];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    T v = T(0);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v += Tg[k * NC + c] * zg[c];
+    y[k] = v;
+  }
+  const T x = xh[i * 3], yy = xh[i * 3 + 1], zz = xh[i * 3 + 2];
+  T du[D];
+  if constexpr (D == 3) {
+    du[0] = y[0] + zz * y[4] - yy * y[5];
+    du[1] = y[1] - zz * y[3] + x * y[5];
+    du[2] = y[2] + yy * y[3] - x * y[4];
+  } else if constexpr (D == 2) {
+    du[0] = y[0] - yy * y[2];
+    du[1] = y[1] + x * y[2];
+  } else {
+    du[0] = y[0];
+  }
+  const int bits = C.fixed_bits[i];
+#pragma unroll
+  for (int e = 0; e < D; ++e)
+    if (!((bits >> e) & 1)) u[i * D + e] += du[e];
+}
+
 template <typename T, int D, int NC>
 __global__ __launch_bounds__(256) void k_coarse_prolong(fe_coarse C, const T* __restrict__ P,
                                                         const T* __restrict__ z, T* __restrict__ u,
@@ -349,16 +463,27 @@ int fe_coarse_correction(hipStream_t stream, const fe_coarse* C, int fp32, const
   const int64_t nct = C->n_agg * C->nc, ld = fe_coarse_ld(nct);
   const dim3 gr((unsigned)C->n_agg), gg((unsigned)ceil_div(nct, 4)),
       gp((unsigned)ceil_div(C->n_nodes * C->dof, 256));
-#define FE_CC(T, D, NC)                                                                            \
+  const bool rm = fp32 ? (C->xhat32 && C->T32 && C->fixed_bits) : (C->xhat && C->T && C->fixed_bits);
+  const dim3 gpn((unsigned)ceil_div(C->n_nodes, 256));
+#define FE_CC(V, D, NC)                                                                            \
   {                                                                                                \
-    const T* P = fp32 ? (const T*)C->P32 : (const T*)C->P;                                         \
-    hipLaunchKernelGGL((k_coarse_restrict<T, D, NC>), gr, dim3(kRestrictThreads), 0, stream, *C, P,  \
-                       (const T*)r,                                                                 \
-                       (T*)rc, state, mode);                                                        \
-    hipLaunchKernelGGL((k_coarse_gemv<T>), gg, dim3(256), 0, stream, nct, ld, (const T*)Ainv,       \
-                       (const T*)rc, (T*)zc, state, mode);                                         \
-    hipLaunchKernelGGL((k_coarse_prolong<T, D, NC>), gp, dim3(256), 0, stream, *C, P, (const T*)zc, \
-                       (T*)u, state, mode);                                                        \
+    const V* P = fp32 ? (const V*)C->P32 : (const V*)C->P;                                         \
+    const V* xh = fp32 ? (const V*)C->xhat32 : (const V*)C->xhat;                                  \
+    const V* Tm = fp32 ? (const V*)C->T32 : (const V*)C->T;                                        \
+    if (rm)                                                                                        \
+      hipLaunchKernelGGL((k_coarse_restrict_rm<V, D, NC>), gr, dim3(kRestrictThreads), 0, stream, *C, \
+                         xh, Tm, (const V*)r, (V*)rc, state, mode);                                \
+    else                                                                                           \
+      hipLaunchKernelGGL((k_coarse_restrict<V, D, NC>), gr, dim3(kRestrictThreads), 0, stream, *C, P, \
+                         (const V*)r, (V*)rc, state, mode);                                        \
+    hipLaunchKernelGGL((k_coarse_gemv<V>), gg, dim3(256), 0, stream, nct, ld, (const V*)Ainv,       \
+                       (const V*)rc, (V*)zc, state, mode);                                         \
+    if (rm)                                                                                        \
+      hipLaunchKernelGGL((k_coarse_prolong_rm<V, D, NC>), gpn, dim3(256), 0, stream, *C, xh, Tm,      \
+                         (const V*)zc, (V*)u, state, mode);                                        \
+    else                                                                                           \
+      hipLaunchKernelGGL((k_coarse_prolong<V, D, NC>), gp, dim3(256), 0, stream, *C, P, (const V*)zc, \
+                         (V*)u, state, mode);                                                      \
   }
   if (fp32) {
     if (C->dof == 1) FE_CC(float, 1, 1) else if (C->dof == 2) FE_CC(float, 2, 3) else FE_CC(float, 3, 6)

@@ -7,9 +7,12 @@
 //   block     M_i = diag(s_i) K_ii^{-1} diag(s_i)       (SPD by congruence, so PCG stays valid)
 //
 // Training loss for a residual/error pair (r, e) of a plain PCG run (A e = r), with z = M r:
-//   L = 1 - (r.z)^2 / ((z.Az)(r.e))
-// the fraction of the error's energy left after one optimally damped preconditioned Richardson step
-// (scale-free in M).  dL/dz = -2 (r.z)/((z.Az)(r.e)) * (r - (r.z)/(z.Az) Az).
+//   kind 0: L = 1 - (r.z)^2 / ((z.Az)(r.e))
+//     the fraction of the error's energy left after one optimally damped preconditioned Richardson
+//     step (scale-free in M).  dL/dz = -2 (r.z)/((z.Az)(r.e)) * (r - (r.z)/(z.Az) Az).
+//   kind 1: L = |e - z|_A^2 / |e|_A^2 = 1 - 2 (r.z)/(r.e) + (z.Az)/(r.e)
+//     the same step undamped: it also sees the top of the spectrum of MA (a step that overshoots
+//     rough errors costs energy).  dL/dz = 2 (Az - r)/(r.e).
 // Weight gradients are reduced per workgroup in registers/LDS and summed in a fixed order by a second
 // kernel: no atomics, bitwise reproducible.
 #include "fe_internal.h"
@@ -227,7 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_gnn_dots(int64_t n, const double* __
 
 // Sum the dot partials in fixed order; scal = {r.z, z.Az, r.e, loss}; loss accumulated into *loss_acc.
 __global__ void k_gnn_dots_final(int nblocks, const double* __restrict__ partials,
-                                 double* __restrict__ scal, double* __restrict__ loss_acc) {
+                                 double* __restrict__ scal, double* __restrict__ loss_acc, int kind) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double v[3] = {0, 0, 0};
   for (int k = 0; k < 3; ++k)
@@ -235,7 +238,9 @@ __global__ void k_gnn_dots_final(int nblocks, const double* __restrict__ partial
   scal[0] = v[0];
   scal[1] = v[1];
   scal[2] = v[2];
-  const double loss = (v[1] > 0 && v[2] > 0) ? 1.0 - v[0] * v[0] / (v[1] * v[2]) : 0.0;
+  const double loss = !(v[1] > 0 && v[2] > 0) ? 0.0
+                      : kind == 0          ? 1.0 - v[0] * v[0] / (v[1] * v[2])
+                                           : 1.0 - 2.0 * v[0] / v[2] + v[1] / v[2];
   scal[3] = loss;
   *loss_acc += loss;
 }
@@ -246,13 +251,13 @@ template <int D>
 __global__ void k_gnn_dlo(int64_t nb, const double* __restrict__ scal, const double* __restrict__ r,
                           const double* __restrict__ az, const double* __restrict__ y,
                           const double* __restrict__ binv, const float* __restrict__ o,
-                          float* __restrict__ dlo) {
+                          float* __restrict__ dlo, int kind) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= nb) return;
   const double rz = scal[0], zaz = scal[1], re = scal[2];
   if (!(zaz > 0 && re > 0)) return;
-  const double cr = -2.0 * rz / (zaz * re);
-  const double ca = 2.0 * rz * rz / (zaz * zaz * re);
+  const double cr = kind == 0 ? -2.0 * rz / (zaz * re) : -2.0 / re;
+  const double ca = kind == 0 ? 2.0 * rz * rz / (zaz * zaz * re) : 2.0 / re;
   double s[D], g[D], sg[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) {
@@ -472,8 +477,9 @@ extern "C" int fe_gnn_forward(fe_handle* h, int64_t n_nodes, int dof, const int3
 
 extern "C" int fe_gnn_loss_grad(fe_handle* h, const fe_bsr* A, const double* binv, const double* M,
                                 const float* o, const double* r, const double* e, const double* zc,
-                                double* work, double* loss_acc, float* dlo) {
+                                double* work, double* loss_acc, float* dlo, int kind) {
   FE_ENTER(h);
+  FE_REQUIRE(kind == 0 || kind == 1, "fe_gnn_loss_grad: kind must be 0 (damped) or 1 (undamped)");
   FE_REQUIRE(A && binv && M && o && r && e && work && loss_acc && dlo, "fe_gnn_loss_grad: null argument");
   FE_REQUIRE(A->row_align == 0, "fe_gnn_loss_grad: the matrix's own pattern expected (row_align 0)");
   FE_REQUIRE(A->precision == 0 && A->layout == FE_LAYOUT_PLANES,
@@ -493,10 +499,11 @@ extern "C" int fe_gnn_loss_grad(fe_handle* h, const fe_bsr* A, const double* bin
   int rc = fe_spmv(h, A, z, az);
   if (rc) return rc;
   hipLaunchKernelGGL(k_gnn_dots, dim3(kRedBlocks), dim3(kBlock), 0, h->stream, n, r, z, az, e, partials);
-  hipLaunchKernelGGL(k_gnn_dots_final, dim3(1), dim3(64), 0, h->stream, kRedBlocks, partials, scal, loss_acc);
-  if (D == 1) hipLaunchKernelGGL(k_gnn_dlo<1>, g1(nb), dim3(256), 0, h->stream, nb, scal, r, az, y, binv, o, dlo);
-  else if (D == 2) hipLaunchKernelGGL(k_gnn_dlo<2>, g1(nb), dim3(256), 0, h->stream, nb, scal, r, az, y, binv, o, dlo);
-  else hipLaunchKernelGGL(k_gnn_dlo<3>, g1(nb), dim3(256), 0, h->stream, nb, scal, r, az, y, binv, o, dlo);
+  hipLaunchKernelGGL(k_gnn_dots_final, dim3(1), dim3(64), 0, h->stream, kRedBlocks, partials, scal, loss_acc,
+                     kind);
+  if (D == 1) hipLaunchKernelGGL(k_gnn_dlo<1>, g1(nb), dim3(256), 0, h->stream, nb, scal, r, az, y, binv, o, dlo, kind);
+  else if (D == 2) hipLaunchKernelGGL(k_gnn_dlo<2>, g1(nb), dim3(256), 0, h->stream, nb, scal, r, az, y, binv, o, dlo, kind);
+  else hipLaunchKernelGGL(k_gnn_dlo<3>, g1(nb), dim3(256), 0, h->stream, nb, scal, r, az, y, binv, o, dlo, kind);
   FE_CHECK_LAUNCH();
   return FE_OK;
 }

@@ -520,6 +520,13 @@ class DeviceCoarse:
         self.pair_row = t(cs.pair_row)
         self.P = t(np.ascontiguousarray(cs.P.reshape(-1)))
         self.P32 = _to_f32(ctx, self.P)
+        xh = np.zeros((cs.P.shape[0], 3))
+        xh[:, :cs.xhat.shape[1]] = cs.xhat
+        self.xhat = t(np.ascontiguousarray(xh.reshape(-1)))
+        self.xhat32 = _to_f32(ctx, self.xhat)
+        self.fixed_bits = t(np.ascontiguousarray(cs.fixed_bits))
+        self.T = t(np.ascontiguousarray(cs.T.reshape(-1)))
+        self.T32 = _to_f32(ctx, self.T)
         self.n_coarse = cs.n_coarse
         self.ld = int(_hip.load().fe_coarse_ld(cs.n_coarse))
 
@@ -527,7 +534,9 @@ class DeviceCoarse:
         cs = self.host
         return _hip.FeCoarse(cs.P.shape[0], cs.n_agg, cs.n_pairs, cs.dof, cs.nc, _vp(self.node_agg),
                              _vp(self.agg_ptr), _vp(self.agg_nodes), _vp(self.pair_ptr), _vp(self.pair_ij),
-                             _vp(self.pair_q), _vp(self.pair_row), _vp(self.P), _vp(self.P32))
+                             _vp(self.pair_q), _vp(self.pair_row), _vp(self.P), _vp(self.P32),
+                             _vp(self.xhat), _vp(self.xhat32), _vp(self.fixed_bits), _vp(self.T),
+                             _vp(self.T32))
 
     def operator(self, A64: BsrMatrix, invert: bool = True):
         """(A_c or A_c^-1 [ld][ld] FP64, FP32 copy of the inverse or None) for the FP64 plane matrix A64."""

@@ -145,7 +145,7 @@ class LearnedPreconditioner:
         zcs = sample.zcs if sample.zcs is not None else [None] * len(sample.pairs)
         for (r, e), zc in zip(sample.pairs, zcs):
             _hip.call('fe_gnn_loss_grad', ctx.h, ctypes.byref(s), _vp(sample.binv), _vp(M), _vp(o),
-                      _vp(r), _vp(e), _vp(zc), _vp(work), _vp(loss), _vp(dlo))
+                      _vp(r), _vp(e), _vp(zc), _vp(work), _vp(loss), _vp(dlo), self.loss_kind)
         dlo.mul_(1.0 / len(sample.pairs))
         nb = ctypes.c_int64()
         _hip.call('fe_gnn_backward_workspace', N, ctypes.byref(nb))
@@ -154,9 +154,17 @@ class LearnedPreconditioner:
                   _vp(feat), _vp(acts), _vp(aggs), _vp(dlo), _vp(bwork), _vp(grad))
         return float(loss.item()) / len(sample.pairs)
 
+    # loss of fe_gnn_loss_grad: 0 = energy left after an optimally damped step, 1 = undamped step
+    LOSSES = {'damped': 0, 'undamped': 1}
+    loss_kind = 0
+
     def train(self, samples: Sequence['TrainingSystem'], epochs: int = 20, lr: float = 3e-3,
-              seed: int = 0, log=None) -> List[float]:
-        """Adam over the systems (one step per system), returns the mean loss per epoch."""
+              seed: int = 0, log=None, loss: str = 'damped') -> List[float]:
+        """Adam over the systems (one step per system), returns the mean loss per epoch.  loss:
+        'damped' (1 - (r.z)^2/((z.Az)(r.e))) or 'undamped' (|e - z|_A^2 / |e|_A^2)."""
+        if loss not in self.LOSSES:
+            raise ValueError("loss must be 'damped' or 'undamped'")
+        self.loss_kind = self.LOSSES[loss]
         ctx = samples[0].KD.graph.ctx
         P = self.params(ctx)
         n = P.numel()
@@ -178,7 +186,7 @@ class LearnedPreconditioner:
                 log(f'epoch {ep}: loss {history[-1]:.5f}')
         self.sync_to_host()
         self.meta.update({'epochs': epochs, 'lr': lr, 'loss_history': history,
-                          'n_systems': len(samples)})
+                          'n_systems': len(samples), 'loss': loss})
         return history
 
 

@@ -31,6 +31,7 @@ def main(argv=None):
     ap.add_argument('--probes', type=int, default=2, help='random-error pairs per training system')
     ap.add_argument('--block-jacobi-only', action='store_true',
                     help='train the network as plain block Jacobi (no coarse correction)')
+    ap.add_argument('--loss', default='damped', choices=('damped', 'undamped'))
     ap.add_argument('--eval-n', type=int, default=32)
     ap.add_argument('--eval-seeds', type=int, default=3)
     ap.add_argument('--out', default=DEFAULT_OUT)
@@ -51,7 +52,7 @@ def main(argv=None):
     log(f'{len(systems)} training systems in {time.perf_counter() - t0:.1f} s')
     model = learned.LearnedPreconditioner(3, seed=0)
     t1 = time.perf_counter()
-    hist = model.train(systems, epochs=args.epochs, lr=args.lr, log=log)
+    hist = model.train(systems, epochs=args.epochs, lr=args.lr, log=log, loss=args.loss)
     train_s = time.perf_counter() - t1
     log(f'trained in {train_s:.1f} s; loss {hist[0]:.5f} -> {hist[-1]:.5f}')
     evals = []

@@ -328,6 +328,16 @@ typedef struct {
   const int32_t* pair_row;   /* node row of each pair_q entry */
   const double* P;           /* [n_nodes][dof][nc] */
   const float* P32;          /* FP32 copy for mixed precision (may be NULL otherwise) */
+  /* P = Z(x^) T, optional (all NULL: the correction reads P): x^ the node's coordinates centred on
+     its aggregate's mean and scaled by its largest extent, Z the rigid-body modes (3D: translations,
+     rotations omega x x^ about x, y, z; 2D: translations, rotation; scalar: 1) with the rows of
+     constrained DOFs zeroed (fixed_bits: bit e of node i set when DOF i*dof+e is constrained), T the
+     aggregate's nc x nc orthonormalising transform.  The correction then evaluates P on the fly. */
+  const double* xhat;        /* [n_nodes][3] (2D: z = 0) */
+  const float* xhat32;
+  const uint8_t* fixed_bits; /* [n_nodes] */
+  const double* T;           /* [n_agg][nc][nc] */
+  const float* T32;
 } fe_coarse;
 /* Leading dimension of the dense coarse matrices: n_agg * nc rounded up to a multiple of 64. */
 int64_t fe_coarse_ld(int64_t n_coarse);
@@ -404,7 +414,8 @@ int fe_sym_eig(fe_handle* h, int n_mat, int r, const double* A, double* w, doubl
    Parameters are one FP32 vector of fe_gnn_num_params(dof) entries.
    acts: [3][n_nodes][32] layer outputs; aggs: [n_nodes][8 + 32 + 32] neighbour means (for backward).
    fe_gnn_loss_grad accumulates, for one residual/error pair (r, e) with A e = r and z = M r + zc, the
-   loss L = 1 - (r.z)^2 / ((z.Az)(r.e)) into *loss_acc and dL/do into dlo.  zc (may be NULL) is a part
+   loss into *loss_acc and dL/do into dlo: kind 0, L = 1 - (r.z)^2 / ((z.Az)(r.e)) (energy left after
+   an optimally damped step); kind 1, L = |e - z|_A^2 / |e|_A^2 (the undamped step).  zc (may be NULL) is a part
    of the preconditioned residual that does not depend on the network: the coarse correction
    P A_c^-1 P^T r when the learned blocks are the smoother of the two-level preconditioner. */
 int fe_gnn_num_params(int dof); /* returns the count (>= 0) or FE_ERR_ARG */
@@ -415,7 +426,7 @@ int fe_gnn_forward(fe_handle* h, int64_t n_nodes, int dof, const int32_t* bptr, 
 int fe_gnn_loss_workspace(int64_t n_dofs, int64_t* doubles);
 int fe_gnn_loss_grad(fe_handle* h, const fe_bsr* A, const double* binv, const double* M,
                      const float* o, const double* r, const double* e, const double* zc, double* work,
-                     double* loss_acc, float* dlo);
+                     double* loss_acc, float* dlo, int kind);
 int fe_gnn_backward_workspace(int64_t n_nodes, int64_t* floats);
 int fe_gnn_backward(fe_handle* h, int64_t n_nodes, int dof, const int32_t* bptr, const int32_t* bcol,
                     const float* params, const float* feat, const float* acts, const float* aggs,

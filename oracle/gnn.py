@@ -84,9 +84,11 @@ def forward(params, feat, avg, d):
     return h @ Wo.T + bo
 
 
-def loss(params, feat, avg, binv, A, pairs, d, zc=None):
+def loss(params, feat, avg, binv, A, pairs, d, zc=None, kind=0):
     """Mean loss over the (r, e) pairs; binv (N, d, d); A scipy (n, n).  zc: optional per-pair vectors
-    added to z = M r (a network-independent part of the preconditioner, e.g. a coarse correction)."""
+    added to z = M r (a network-independent part of the preconditioner, e.g. a coarse correction).
+    kind 0: 1 - (r.z)^2/((z.Az)(r.e)), the energy left after an optimally damped step; kind 1:
+    |e - z|_A^2 / |e|_A^2, the energy left after the undamped step, evaluated as written."""
     o = forward(params, feat, avg, d)
     s = torch.exp(torch.clamp(o, -4.0, 4.0))  # (N, d)
     B = torch.as_tensor(binv, dtype=torch.float64)
@@ -101,6 +103,11 @@ def loss(params, feat, avg, binv, A, pairs, d, zc=None):
         if zc is not None:
             z = z + torch.as_tensor(zc[k], dtype=torch.float64)
         az = torch.sparse.mm(At, z[:, None])[:, 0]
-        rz = torch.dot(r, z)
-        total = total + 1.0 - rz * rz / (torch.dot(z, az) * torch.dot(r, e))
+        if kind == 0:
+            rz = torch.dot(r, z)
+            total = total + 1.0 - rz * rz / (torch.dot(z, az) * torch.dot(r, e))
+        else:
+            ez = e - z
+            aez = torch.sparse.mm(At, ez[:, None])[:, 0]
+            total = total + torch.dot(ez, aez) / torch.dot(e, torch.sparse.mm(At, e[:, None])[:, 0])
     return total / len(pairs), o

@@ -31,9 +31,10 @@ def main():
         vals, _ = eigen.smallest_clamped(K, M, pr.fixed_dofs, 20)
         out.append(vals)
         print(f'seed {sd}: {time.time() - t0:.0f} s, {vals[:4]}', flush=True)
-    path = os.path.join(ROOT, 'tests', 'golden', f'modal{n}_eigs.npz')
-    np.savez(path, seeds=np.array(seeds), eigenvalues=np.array(out), n=np.int64(n))
-    print('wrote', path)
+        # written after every variant so that a long run keeps what it has finished
+        path = os.path.join(ROOT, 'tests', 'golden', f'modal{n}_eigs.npz')
+        np.savez(path, seeds=np.array(seeds[:len(out)]), eigenvalues=np.array(out), n=np.int64(n))
+        print('wrote', path, flush=True)
 
 
 if __name__ == '__main__':

@@ -70,3 +70,58 @@ def test_two_level_oracle_reduces_iterations():
     ref = vz.reduced_solve(K, pr.f, pr.fixed_dofs)
     assert np.linalg.norm(x - ref) <= 1e-6 * np.linalg.norm(ref)
     assert it_2 < 0.6 * it_bj, (it_2, it_bj)
+
+
+def _moment_restrict(cs, r):
+    """P^T r the way the moment kernels form it: w = (sum r, sum x^ x r) per aggregate over free
+    DOFs, then r_c = T^T w."""
+    d = cs.dof
+    free = ((cs.fixed_bits[:, None] >> np.arange(d)[None, :]) & 1) == 0
+    rv = np.where(free, r.reshape(-1, d), 0.0)
+    x = cs.xhat
+    if d == 3:
+        m = np.concatenate([rv, np.cross(x, rv)], axis=1)
+    else:
+        m = np.concatenate([rv, (x[:, 0] * rv[:, 1] - x[:, 1] * rv[:, 0])[:, None]], axis=1)
+    w = np.zeros((cs.n_agg, cs.nc))
+    np.add.at(w, cs.node_agg, m)
+    return np.einsum('gkc,gk->gc', cs.T, w).reshape(-1)
+
+
+def _moment_prolong(cs, z):
+    """P z the way the moment kernels form it: y = T z per aggregate, u = t + omega x x^ on free DOFs."""
+    d = cs.dof
+    y = np.einsum('gkc,gc->gk', cs.T, z.reshape(cs.n_agg, cs.nc))[cs.node_agg]
+    x = cs.xhat
+    if d == 3:
+        u = y[:, :3] + np.cross(y[:, 3:], x)
+    else:
+        u = y[:, :2] + y[:, 2:3] * np.stack([-x[:, 1], x[:, 0]], axis=1)
+    free = ((cs.fixed_bits[:, None] >> np.arange(d)[None, :]) & 1) == 0
+    return np.where(free, u, 0.0).reshape(-1)
+
+
+def test_moment_form_of_prolongator():
+    """P = Z(x^) T: restriction by rigid-body moments and prolongation by rigid motions reproduce
+    P^T r and P z of the assembled prolongator (3D, and 2D plane elasticity with rotations)."""
+    rng = np.random.default_rng(3)
+    pr, K, bptr, bcol = _cube(6)
+    cs = CoarseSpace(pr.coords, bptr, bcol, 3, pr.fixed_dofs, target=27)
+    assert cs.xhat.shape == pr.coords.shape and np.abs(cs.xhat).max() <= 1.0 + 1e-15
+    P = oc.prolongator(cs)
+    r = rng.standard_normal(P.shape[0])
+    z = rng.standard_normal(P.shape[1])
+    np.testing.assert_allclose(_moment_restrict(cs, r), P.T @ r, rtol=0, atol=1e-12 * np.abs(r).sum())
+    np.testing.assert_allclose(_moment_prolong(cs, z), P @ z, rtol=0, atol=1e-12 * np.abs(z).sum())
+    # 2D: a plate with its left edge clamped
+    n = 9
+    g = np.stack(np.meshgrid(np.linspace(0, 2, n), np.linspace(0, 1, n), indexing='ij'), -1).reshape(-1, 2)
+    nn = g.shape[0]
+    fixed = np.flatnonzero(np.repeat(g[:, 0] == 0, 2))
+    ptr = np.arange(nn + 1)
+    cs2 = CoarseSpace(g, ptr, np.arange(nn), 2, fixed, target=9)
+    P2 = oc.prolongator(cs2)
+    r2 = rng.standard_normal(P2.shape[0])
+    z2 = rng.standard_normal(P2.shape[1])
+    np.testing.assert_allclose(_moment_restrict(cs2, r2), P2.T @ r2, rtol=0, atol=1e-12 * np.abs(r2).sum())
+    np.testing.assert_allclose(_moment_prolong(cs2, z2), P2 @ z2, rtol=0, atol=1e-12 * np.abs(z2).sum())

@@ -65,14 +65,15 @@ def test_features_and_forward_match_oracle(ctx):
     assert np.all(np.linalg.eigvalsh(0.5 * (Mh + Mh.transpose(0, 2, 1))) > 0)
 
 
-@pytest.mark.parametrize('two_level', [False, True])
-def test_gradients_match_oracle(ctx, two_level):
-    _say(f'gnn gradients (two_level={two_level})')
+@pytest.mark.parametrize('two_level,kind', [(False, 0), (True, 0), (True, 1)])
+def test_gradients_match_oracle(ctx, two_level, kind):
+    _say(f'gnn gradients (two_level={two_level}, loss kind {kind})')
     pr, s, KD, b, fixed = _system(4)
     rng = np.random.default_rng(6)
     p = learned.init_params(3, seed=4)
     p[-(3 * 32 + 3):] = rng.normal(0, 0.3, 3 * 32 + 3)
     model = learned.LearnedPreconditioner(3, params=p)
+    model.loss_kind = kind
     coarse = s.coarse_space(pr.fixed_dofs, target=8) if two_level else None
     solver = s.solver(KD, 'two_level' if two_level else 'block_jacobi', 'fp64', coarse=coarse)
     xstar, _ = solver.solve(b, tol=1e-12)
@@ -94,7 +95,8 @@ def test_gradients_match_oracle(ctx, two_level):
     P = torch.tensor(p, dtype=torch.float64, requires_grad=True)
     host_pairs = [(r.cpu().numpy(), e.cpu().numpy()) for r, e in pairs]
     host_zc = [z.cpu().numpy() for z in zcs] if two_level else None
-    L, _ = ognn.loss(P, f_ref, avg, binv.cpu().numpy().reshape(-1, 3, 3), A, host_pairs, 3, zc=host_zc)
+    L, _ = ognn.loss(P, f_ref, avg, binv.cpu().numpy().reshape(-1, 3, 3), A, host_pairs, 3, zc=host_zc,
+                     kind=kind)
     L.backward()
     g_ref = P.grad.numpy()
     g_gpu = grad.cpu().numpy()

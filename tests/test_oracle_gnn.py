@@ -75,6 +75,37 @@ def test_loss_bounds_and_damping_invariance():
     assert abs(L2.item() - L.item()) < 1e-12
 
 
+def test_undamped_loss_pins():
+    """kind 1 is |e - z|_A^2/|e|_A^2: zero for the exact inverse, one for z = 0 (all outputs at the
+    lower clamp give z -> e^-8 M_bj r, nearly zero), equal to 1 - 2(r.z)/(r.e) + (z.Az)/(r.e) (the
+    form the kernel uses), and not invariant to a uniform rescaling of M (unlike kind 0)."""
+    blocks, A, bptr, bcol = _block_diag_system(seed=11)
+    feat = gnn.features(bptr, bcol, blocks, np.zeros(A.shape[0], bool), 3)
+    avg = gnn.neighbour_mean(bptr, bcol)
+    L0, _ = gnn.loss(_params(3, 0.0), feat, avg, np.linalg.inv(blocks), A, _pairs(A, 3), 3, kind=1)
+    assert abs(L0.item()) < 1e-12
+    rng = np.random.default_rng(12)
+    binv = np.linalg.inv(blocks) + 0.05 * rng.standard_normal(blocks.shape)
+    binv = 0.5 * (binv + binv.transpose(0, 2, 1))
+    p = _params(3, 1.0, seed=13)
+    pairs = _pairs(A, 3)
+    L1, o = gnn.loss(p, feat, avg, binv, A, pairs, 3, kind=1)
+    s = np.exp(np.clip(o.detach().numpy(), -4, 4))
+    total = 0.0
+    for r, e in pairs:
+        z = (s * np.einsum('nij,nj->ni', binv, s * r.reshape(-1, 3))).reshape(-1)
+        total += 1 - 2 * (r @ z) / (r @ e) + (z @ (A @ z)) / (r @ e)
+    assert abs(L1.item() - total / len(pairs)) < 1e-12
+    q = p.detach().clone()
+    q[-3:] += 0.7
+    L2, _ = gnn.loss(q, feat, avg, binv, A, pairs, 3, kind=1)
+    assert abs(L2.item() - L1.item()) > 1e-3
+    qlow = p.detach().clone()
+    qlow[-3:] -= 1e3
+    Llow, _ = gnn.loss(qlow, feat, avg, binv, A, pairs, 3, kind=1)
+    assert abs(Llow.item() - 1.0) < 1e-3
+
+
 def test_features_scale_free():
     blocks, A, bptr, bcol = _block_diag_system(seed=6)
     fixed = np.zeros(A.shape[0], bool)
