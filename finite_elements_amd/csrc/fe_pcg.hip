@@ -14,7 +14,9 @@
 // Reliable updates (FP64 true residual) end every chunk of iterations for the segments that asked for
 // one: when the recursive residual reached the tolerance (convergence is always judged on the true
 // FP64 residual), at the iteration limit, or (mixed precision) when the residual fell by ru_delta
-// since the last update.  r = b - A64 x, u = M r, then one matrix pass computes w = A u and s = A p
+// since the last update.  The first two stop the segment's iterations until the update; the third
+// only marks it pending and the segment iterates on to the end of the chunk, so no launched
+// iteration idles.  r = b - A64 x, u = M r, then one matrix pass computes w = A u and s = A p
 // together and the exact alpha of the continued direction from (u,w), (u,s), (p,s): the Krylov
 // direction is kept, not restarted.  When the true residual disagrees with the recursive one the
 // segment restarts from the true residual; ten updates without progress stop it as not converged.
@@ -41,7 +43,7 @@ struct PcgState {
   int32_t mixed;
   int32_t stall;    // reliable updates since the true residual last improved
   int32_t restarts;
-  int32_t pad1;
+  int32_t ru_pending;  // mixed: the residual fell by ru_delta; update at the end of the chunk
 };
 
 struct PcgStatus {
@@ -197,9 +199,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(const Tile* __restrict__ til
 }
 
 // One workgroup per segment: the Chronopoulos-Gear scalars for the next iteration.
+// last: this is the chunk's last iteration (a pending reliable update is requested now).
 __global__ __launch_bounds__(kRed) void k_cg_reduce(const int32_t* __restrict__ seg_tiles, int ntiles,
                                                     const double* __restrict__ partials,
-                                                    PcgState* st) {
+                                                    PcgState* st, int last) {
   PcgState* s = st + blockIdx.x;
   const int t0 = seg_tiles[blockIdx.x], t1 = seg_tiles[blockIdx.x + 1];
   if (!seg_running(s)) return;  // uniform: only thread 0 writes the state, after the sums
@@ -215,13 +218,13 @@ __global__ __launch_bounds__(kRed) void k_cg_reduce(const int32_t* __restrict__ 
   }
   s->gamma_prev = s->gamma;
   s->gamma = g;
-  // convergence claims, the iteration limit and (mixed precision) the reliable-update threshold
-  // hand over to the true-residual check at the end of the chunk
-  if (v[2] <= s->tol2bb || s->iter >= s->max_iter ||
-      (s->mixed && v[2] < s->ru_delta2 * s->rr_ref)) {
+  // convergence claims and the iteration limit hand over to the true-residual check at the end of
+  // the chunk; (mixed precision) the reliable-update threshold marks the update pending
+  if (v[2] <= s->tol2bb || s->iter >= s->max_iter) {
     s->need_ru = 1;
     return;
   }
+  if (s->mixed && v[2] < s->ru_delta2 * s->rr_ref) s->ru_pending = 1;
   const double beta = g / s->gamma_prev;
   const double den = dl - beta * g / s->alpha;
   if (!(den > 0.0)) {
@@ -234,6 +237,8 @@ __global__ __launch_bounds__(kRed) void k_cg_reduce(const int32_t* __restrict__ 
   }
   s->beta = beta;
   s->alpha = g / den;
+  // the update replaces r, u, w, s and recomputes beta and alpha from gamma_prev, as at a hand-over
+  if (last && s->ru_pending) s->need_ru = 1;
 }
 
 // --------------------------------------------------------------------------------------------------
@@ -445,6 +450,7 @@ __global__ __launch_bounds__(kRed) void k_ru_finish(const int32_t* __restrict__ 
   s->rr_ref = s->rr_true;
   s->n_ru += 1;
   s->need_ru = 0;
+  s->ru_pending = 0;
 }
 
 // --------------------------------------------------------------------------------------------------
@@ -691,7 +697,8 @@ int run_pcg(fe_handle* h, const Problem& pb, const Work& w, const fe_pcg_opts* o
         if (rc3) return rc3;
       }
       launch_spmv(cs, 0);
-      hipLaunchKernelGGL(k_cg_reduce, gseg, dim3(kRed), 0, cs, w.seg_tiles, nt, w.partials, w.st);
+      hipLaunchKernelGGL(k_cg_reduce, gseg, dim3(kRed), 0, cs, w.seg_tiles, nt, w.partials, w.st,
+                         (int)(it == chunk - 1));
     }
     {
       int rc4 = launch_ru(cs);
