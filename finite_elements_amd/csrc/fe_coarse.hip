@@ -128,6 +128,27 @@ __device__ void tile_gemm_store(double (*sX)[kNB + 1], double (*sY)[kNB + 1], do
     }
 }
 
+// acc = X * Y for the 4 x 4 outputs (rows tr.., columns tc..) of this thread, 256 threads.
+__device__ inline void tile_gemm_acc(double (*sX)[kNB + 1], double (*sY)[kNB + 1], double (&acc)[4][4]) {
+  const int tr = (threadIdx.x / 16) * 4, tc = (threadIdx.x % 16) * 4;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+  for (int kk = 0; kk < kNB; ++kk) {
+    double x[4], y[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      x[a] = sX[tr + a][kk];
+      y[a] = sY[kk][tc + a];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] += x[a] * y[b];
+  }
+}
+
 __device__ void load_tile(const double* __restrict__ A, int64_t ld, int bi, int bj,
                           double (*s)[kNB + 1]) {
   for (int idx = threadIdx.x; idx < kNB * kNB; idx += blockDim.x) {
@@ -136,76 +157,116 @@ __device__ void load_tile(const double* __restrict__ A, int64_t ld, int bi, int 
   }
 }
 
-// step k, part 0 (one workgroup of 1024 threads): S = inv(A_kk) by in-place Gauss-Jordan.  Thread
-// (g, c) keeps rows 4g..4g+3 of column c in registers; each elimination step broadcasts the pivot row
-// and the pivot column through LDS buffers that alternate between steps, so one barrier per step
-// separates their writes from the reads.
-constexpr int kInvThreads = 1024;
+// In-place Gauss-Jordan inversion of a 64 x 64 tile by 256 threads: thread (g, c) keeps rows
+// 16g..16g+15 of column c in a[].  Each elimination step broadcasts the pivot row, the pivot column and
+// the pivot's reciprocal (formed once, by the pivot's owner) through LDS buffers that alternate
+// between steps, so one barrier per step separates their writes from the reads.  No pivoting: the
+// diagonal blocks met by the blocked elimination of an SPD matrix are SPD.
+struct GjLds {
+  double row[2][kNB], col[2][kNB], piv[2];
+};
 
-__global__ __launch_bounds__(kInvThreads) void k_gj_inv(int64_t ld, int k, const double* __restrict__ A,
-                                                        double* __restrict__ S) {
-  __shared__ double sRow[2][kNB], sCol[2][kNB];
-  const int c = threadIdx.x % kNB, rg = threadIdx.x / kNB;
-  double a[4];
+__device__ void gj_invert_tile(double (&a)[16], GjLds& L) {
+  const int c = threadIdx.x % kNB, g = threadIdx.x / kNB;
+  for (int pb = 0; pb < kNB / 16; ++pb) {
 #pragma unroll
-  for (int q = 0; q < 4; ++q) a[q] = A[(int64_t)(k * kNB + rg * 4 + q) * ld + k * kNB + c];
-  for (int p = 0; p < kNB; ++p) {
-    const int buf = p & 1, qp = p & 3;
-    if (rg == (p >> 2)) sRow[buf][c] = qp == 0 ? a[0] : qp == 1 ? a[1] : qp == 2 ? a[2] : a[3];
-    if (c == p)
+    for (int q = 0; q < 16; ++q) {
+      const int p = pb * 16 + q, buf = q & 1;
+      if (g == pb) {
+        L.row[buf][c] = a[q];
+        if (c == p) L.piv[buf] = 1.0 / a[q];
+      }
+      if (c == p)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) sCol[buf][rg * 4 + q] = a[q];
-    __syncthreads();
-    const double piv = 1.0 / sRow[buf][p];
-    const double prow = (c == p ? 1.0 : sRow[buf][c]) * piv;
+        for (int i = 0; i < 16; ++i) L.col[buf][g * 16 + i] = a[i];
+      __syncthreads();
+      const double prow = (c == p ? 1.0 : L.row[buf][c]) * L.piv[buf];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int r = rg * 4 + q;
-      a[q] = (r == p) ? prow : (c == p ? 0.0 : a[q]) - sCol[buf][r] * prow;
+      for (int i = 0; i < 16; ++i) {
+        const int r = g * 16 + i;
+        a[i] = (r == p) ? prow : (c == p ? 0.0 : a[i]) - L.col[buf][r] * prow;
+      }
     }
   }
-#pragma unroll
-  for (int q = 0; q < 4; ++q) S[(rg * 4 + q) * kNB + c] = a[q];
 }
 
-// step k, part 1 (one workgroup per block column j != k): A_kj <- S A_kj.
-__global__ __launch_bounds__(256) void k_gj_row(int64_t ld, int k, double* __restrict__ A,
-                                                const double* __restrict__ S) {
+constexpr int kGjThreads = 256;
+
+// S = inv(A_00) (the first pivot block; later ones are inverted inside k_gj_update).
+__global__ __launch_bounds__(kGjThreads) void k_gj_inv(int64_t ld, const double* __restrict__ A,
+                                                       double* __restrict__ S) {
+  __shared__ GjLds L;
+  const int c = threadIdx.x % kNB, g = threadIdx.x / kNB;
+  double a[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) a[i] = A[(int64_t)(g * 16 + i) * ld + c];
+  gj_invert_tile(a, L);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) S[(g * 16 + i) * kNB + c] = a[i];
+}
+
+// Step k of the blocked elimination reads A (the matrix after step k-1) and writes B; S = inv(A_kk).
+// Part 1 (workgroup per block column j): B_kj = S A_kj, B_kk = S.
+__global__ __launch_bounds__(kGjThreads) void k_gj_row(int64_t ld, int k, const double* __restrict__ A,
+                                                       double* __restrict__ B,
+                                                       const double* __restrict__ S) {
   __shared__ double sP[kNB][kNB + 1], sT[kNB][kNB + 1];
   const int j = blockIdx.x;
-  if (j == k) return;
+  if (j == k) {
+    for (int idx = threadIdx.x; idx < kNB * kNB; idx += blockDim.x)
+      B[(int64_t)(k * kNB + idx / kNB) * ld + k * kNB + idx % kNB] = S[idx];
+    return;
+  }
   for (int idx = threadIdx.x; idx < kNB * kNB; idx += blockDim.x) sP[idx / kNB][idx % kNB] = S[idx];
   load_tile(A, ld, k, j, sT);
   __syncthreads();
-  tile_gemm_store(sP, sT, A + (int64_t)(k * kNB) * ld + j * kNB, ld, 1.0, false);
+  tile_gemm_store(sP, sT, B + (int64_t)(k * kNB) * ld + j * kNB, ld, 1.0, false);
 }
 
-// step k, part 2 (workgroup per block (i, j), i, j != k): A_ij -= A_ik A_kj.
-__global__ __launch_bounds__(256) void k_gj_update(int64_t ld, int k, double* __restrict__ A) {
+// Part 2 (workgroup per block (i, j), i != k): B_ij = A_ij - A_ik B_kj (j != k), B_ik = -A_ik S.
+// The workgroup of block (k+1, k+1) then inverts its new block into Snext, the next step's S.
+__global__ __launch_bounds__(kGjThreads) void k_gj_update(int64_t ld, int k, const double* __restrict__ A,
+                                                          double* __restrict__ B,
+                                                          const double* __restrict__ S,
+                                                          double* __restrict__ Snext) {
   __shared__ double sX[kNB][kNB + 1], sY[kNB][kNB + 1];
+  __shared__ GjLds L;
   const int nb = (int)(ld / kNB);
   const int i = blockIdx.x / nb, j = blockIdx.x % nb;
-  if (i == k || j == k) return;
+  if (i == k) return;
   load_tile(A, ld, i, k, sX);
-  load_tile(A, ld, k, j, sY);
-  __syncthreads();
-  tile_gemm_store(sX, sY, A + (int64_t)(i * kNB) * ld + j * kNB, ld, -1.0, true);
-}
-
-// step k, part 3 (workgroup per block row i): A_ik <- -A_ik S for i != k; A_kk <- S.
-__global__ __launch_bounds__(256) void k_gj_col(int64_t ld, int k, double* __restrict__ A,
-                                                const double* __restrict__ S) {
-  __shared__ double sX[kNB][kNB + 1], sY[kNB][kNB + 1];
-  const int i = blockIdx.x;
-  if (i == k) {
-    for (int idx = threadIdx.x; idx < kNB * kNB; idx += blockDim.x)
-      A[(int64_t)(k * kNB + idx / kNB) * ld + k * kNB + idx % kNB] = S[idx];
+  if (j == k) {
+    for (int idx = threadIdx.x; idx < kNB * kNB; idx += blockDim.x) sY[idx / kNB][idx % kNB] = S[idx];
+    __syncthreads();
+    tile_gemm_store(sX, sY, B + (int64_t)(i * kNB) * ld + k * kNB, ld, -1.0, false);
     return;
   }
-  load_tile(A, ld, i, k, sX);
-  for (int idx = threadIdx.x; idx < kNB * kNB; idx += blockDim.x) sY[idx / kNB][idx % kNB] = S[idx];
+  load_tile(B, ld, k, j, sY);
   __syncthreads();
-  tile_gemm_store(sX, sY, A + (int64_t)(i * kNB) * ld + k * kNB, ld, -1.0, false);
+  double acc[4][4];
+  tile_gemm_acc(sX, sY, acc);
+  const int tr = (threadIdx.x / 16) * 4, tc = (threadIdx.x % 16) * 4;
+  const bool pivot_next = (i == k + 1 && j == k + 1);
+  if (pivot_next) __syncthreads();  // sX is reused for the new block below
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int64_t o = (int64_t)(i * kNB + tr + a) * ld + j * kNB + tc + b;
+      const double v = A[o] - acc[a][b];
+      B[o] = v;
+      if (pivot_next) sX[tr + a][tc + b] = v;
+    }
+  if (!pivot_next) return;
+  // the next pivot block (this workgroup's own output): invert it into Snext
+  __syncthreads();
+  const int c = threadIdx.x % kNB, g = threadIdx.x / kNB;
+  double t[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) t[q] = sX[g * 16 + q][c];
+  gj_invert_tile(t, L);
+#pragma unroll
+  for (int q = 0; q < 16; ++q) Snext[(g * 16 + q) * kNB + c] = t[q];
 }
 
 // Symmetrise the inverse (average with its transpose) and make the FP32 copy.
@@ -524,13 +585,21 @@ extern "C" int fe_coarse_operator(fe_handle* h, const fe_bsr* A, const fe_coarse
                        ld, Ac);
   FE_CHECK_LAUNCH();
   if (!invert) return FE_OK;
+  // blocked Gauss-Jordan, the matrix alternating between Ac and the scratch copy; the two pivot
+  // inverses alternate between the two tiles after it
   const int nb = (int)(ld / kNB);
+  double* Sb[2] = {scratch + ld * ld, scratch + ld * ld + kNB * kNB};
+  double* cur = Ac;
+  double* nxt = scratch;
+  hipLaunchKernelGGL(k_gj_inv, dim3(1), dim3(kGjThreads), 0, st, ld, Ac, Sb[0]);
   for (int k = 0; k < nb; ++k) {
-    hipLaunchKernelGGL(k_gj_inv, dim3(1), dim3(kInvThreads), 0, st, ld, k, Ac, scratch);
-    hipLaunchKernelGGL(k_gj_row, dim3((unsigned)nb), dim3(256), 0, st, ld, k, Ac, scratch);
-    hipLaunchKernelGGL(k_gj_update, dim3((unsigned)(nb * nb)), dim3(256), 0, st, ld, k, Ac);
-    hipLaunchKernelGGL(k_gj_col, dim3((unsigned)nb), dim3(256), 0, st, ld, k, Ac, scratch);
+    hipLaunchKernelGGL(k_gj_row, dim3((unsigned)nb), dim3(kGjThreads), 0, st, ld, k, cur, nxt, Sb[k & 1]);
+    hipLaunchKernelGGL(k_gj_update, dim3((unsigned)(nb * nb)), dim3(kGjThreads), 0, st, ld, k, cur, nxt,
+                       Sb[k & 1], Sb[(k + 1) & 1]);
+    std::swap(cur, nxt);
   }
+  if (cur != Ac) FE_CHECK_HIP(hipMemcpyAsync(Ac, cur, (size_t)ld * ld * sizeof(double),
+                                             hipMemcpyDeviceToDevice, st));
   hipLaunchKernelGGL(k_sym_cast, dim3((unsigned)ceil_div(nct * nct, 256)), dim3(256), 0, st, nct, ld,
                      Ac, Ac32);
   FE_CHECK_LAUNCH();

@@ -543,7 +543,8 @@ class DeviceCoarse:
         ctx = self.ctx
         Ac = torch.empty(self.ld * self.ld, dtype=torch.float64, device=ctx.torch_device)
         A32 = torch.empty(self.ld * self.ld, dtype=torch.float32, device=ctx.torch_device) if invert else None
-        scratch = torch.empty(64 * 64, dtype=torch.float64, device=ctx.torch_device)
+        scratch = torch.empty(self.ld * self.ld + 2 * 64 * 64, dtype=torch.float64,
+                              device=ctx.torch_device) if invert else None
         a = A64.struct()
         c = self.struct()
         _hip.call('fe_coarse_operator', ctx.h, ctypes.byref(a), ctypes.byref(c), int(invert), _vp(Ac),

@@ -343,7 +343,7 @@ typedef struct {
 int64_t fe_coarse_ld(int64_t n_coarse);
 /* A_c = P^T A P (FP64 planes A) into Ac [ld][ld] (identity in the padding); with invert, Ac is
    replaced by A_c^-1 (blocked Gauss-Jordan, FP64, symmetrised) and Ac32 (may be NULL) receives its
-   FP32 copy; scratch holds 64 * 64 doubles. */
+   FP32 copy; scratch holds ld * ld + 2 * 64 * 64 doubles. */
 int fe_coarse_operator(fe_handle* h, const fe_bsr* A, const fe_coarse* C, int invert, double* Ac,
                        float* Ac32, double* scratch);
 /* u += P Ainv P^T r in FP64 (precision 0) or FP32 (1); work: 2 * round_up(n_coarse * elem, 256) bytes. */
