@@ -121,7 +121,9 @@ __global__ __launch_bounds__(kVecNodes * D) void k_cg_vec(const Tile* __restrict
                                                           T* __restrict__ u, const T* __restrict__ w,
                                                           const T* __restrict__ M,
                                                           const PcgState* st) {
-  __shared__ T s_r[kVecNodes * D];
+  constexpr int kPasses = tile_rows<lanes_for<D>()>() / kVecNodes;
+  static_assert(kPasses * kVecNodes == tile_rows<lanes_for<D>()>(), "tile = whole passes");
+  __shared__ T s_r[kPasses][kVecNodes * D];
   __shared__ int s_skip;
   __shared__ double s_ab[2];
   const int tid_ = tile_id();
@@ -137,25 +139,30 @@ __global__ __launch_bounds__(kVecNodes * D) void k_cg_vec(const Tile* __restrict
   const double alpha = s_ab[0];
   const T alpha_t = (T)alpha, beta = (T)s_ab[1];
   const int64_t kend = (int64_t)tl.row1 * D;
-  for (int64_t base = (int64_t)tl.row0 * D; base < kend; base += kVecNodes * D) {
-    const int64_t k = base + threadIdx.x;
-    T rk = T(0);
+  // the tile's passes unrolled: every pass's loads are issued before the one barrier that publishes
+  // the new residuals for the block-Jacobi product
+  T rk[kPasses];
+#pragma unroll
+  for (int ps = 0; ps < kPasses; ++ps) {
+    const int64_t k = ((int64_t)tl.row0 + ps * kVecNodes) * D + threadIdx.x;
+    rk[ps] = T(0);
     if (k < kend) {
       const T pk = u[k] + beta * p[k];
       const T sk = w[k] + beta * s[k];
       p[k] = pk;
       s[k] = sk;
       x[k] += alpha * (double)pk;
-      rk = r[k] - alpha_t * sk;
-      r[k] = rk;
+      rk[ps] = r[k] - alpha_t * sk;
+      r[k] = rk[ps];
     }
-    s_r[threadIdx.x] = rk;
-    __syncthreads();
-    if (k < kend) {
-      const int d = threadIdx.x % D;
-      u[k] = precond_dof<T, D, PC>(M, k / D, d, k, &s_r[threadIdx.x - d], rk);
-    }
-    __syncthreads();
+    s_r[ps][threadIdx.x] = rk[ps];
+  }
+  __syncthreads();
+  const int d = threadIdx.x % D;
+#pragma unroll
+  for (int ps = 0; ps < kPasses; ++ps) {
+    const int64_t k = ((int64_t)tl.row0 + ps * kVecNodes) * D + threadIdx.x;
+    if (k < kend) u[k] = precond_dof<T, D, PC>(M, k / D, d, k, &s_r[ps][threadIdx.x - d], rk[ps]);
   }
 }
 
@@ -176,20 +183,34 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(const Tile* __restrict__ til
   if (s_skip) return;
   const int lane = threadIdx.x % G;
   double v[3] = {0.0, 0.0, 0.0};
-  for (int64_t i = tl.row0 + threadIdx.x / G; i < tl.row1; i += kBlock / G) {
-    T acc[D];
-    if constexpr (VEC)
-      bsr_row_vec<T, D, G>(bptr, bcol, vals, nnzb, u, i, lane, acc);
-    else
+  auto row_out = [&](int64_t i, const T (&acc)[D]) {
+    const int64_t k = i * D + lane;
+    const T wk = pick<T, D>(acc, lane);
+    w[k] = wk;
+    const double uk = u[k], rk = r[k];
+    v[0] += rk * uk;
+    v[1] += (double)wk * uk;
+    v[2] += rk * rk;
+  };
+  if constexpr (VEC) {
+    // a tile is a fixed number of passes: unrolled, so that the loads of every pass are in flight
+    // together
+    constexpr int kRowsPerPass = kBlock / G;
+    constexpr int kPasses = tile_rows<lanes_for<D>()>() / kRowsPerPass;
+    static_assert(kPasses * kRowsPerPass == tile_rows<lanes_for<D>()>(), "tile = whole passes");
+#pragma unroll
+    for (int pass = 0; pass < kPasses; ++pass) {
+      const int64_t i = tl.row0 + pass * kRowsPerPass + threadIdx.x / G;
+      const bool ok = i < tl.row1;
+      T acc[D];
+      bsr_row_vec<T, D, G>(bptr, bcol, vals, nnzb, u, i, lane, acc, ok);
+      if (ok && lane < D) row_out(i, acc);
+    }
+  } else {
+    for (int64_t i = tl.row0 + threadIdx.x / G; i < tl.row1; i += kBlock / G) {
+      T acc[D];
       bsr_row<T, D, G, FE_LAYOUT_PLANES>(bptr, bcol, vals, nnzb, u, i, lane, acc);
-    if (lane < D) {
-      const int64_t k = i * D + lane;
-      const T wk = pick<T, D>(acc, lane);
-      w[k] = wk;
-      const double uk = u[k], rk = r[k];
-      v[0] += rk * uk;
-      v[1] += (double)wk * uk;
-      v[2] += rk * rk;
+      if (lane < D) row_out(i, acc);
     }
   }
   block_sum<3>(v);

@@ -90,16 +90,17 @@ struct Vec16<double> {
 // holds a multiple of 4; padding blocks are zero with the row as column).  G lanes per row; a lane
 // takes W consecutive blocks per step, one 16-byte load per value plane and one for the columns.
 // The summation order depends on the row alone (not on where the row sits in the arrays).
+// active false: an empty row (the lanes still take part in the group's shuffles).
 template <typename T, int D, int G>
 __device__ inline void bsr_row_vec(const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
                                    const T* __restrict__ vals, int64_t nnzb, const T* __restrict__ x,
-                                   int64_t i, int lane, T (&acc)[D]) {
+                                   int64_t i, int lane, T (&acc)[D], bool active = true) {
   using V = typename Vec16<T>::V;
   using I = typename Vec16<T>::I;
   constexpr int W = Vec16<T>::W;
 #pragma unroll
   for (int d = 0; d < D; ++d) acc[d] = T(0);
-  const int32_t s = bptr[i], e = bptr[i + 1];
+  const int32_t s = active ? bptr[i] : 0, e = active ? bptr[i + 1] : 0;
   for (int32_t q = s + lane * W; q < e; q += G * W) {
     const I cols = *reinterpret_cast<const I*>(bcol + q);
     V a[D * D];
