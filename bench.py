@@ -54,14 +54,17 @@ def fused_assembly_bytes(sysm):
     coords = 8 * N * (3 if sysm.kind == 'tet_elastic' else 2)
     conn = 4 * npe * E
     mats = 8 * n_mat * E
-    # node pointers, block pointers and block contribution map, the schedule and its element lists,
-    # the packed incidence codes
+    # node pointers, diagonal positions, block pointers and block contribution map, the packed
+    # incidence codes, the schedule's per-workgroup headers, element lists and their node ids (read
+    # in place of the connectivity)
     g = sysm.graph
-    graph = (4 * (N + 1) * 2 + 4 * (p.nnzb + 1) + 2 * p.n_inc * npe + 4 * (g.n_blk + 1) * 2
-             + 4 * int(g.welem.numel()) + 2 * p.n_inc)
+    n_welem = int(g.welem.numel())
+    graph = (4 * (N + 1) * 2 + 4 * N + 4 * (p.nnzb + 1) + 2 * p.n_inc * npe + 2 * p.n_inc
+             + 48 * g.n_blk + 4 * n_welem)
+    welem_conn = 4 * npe * n_welem
     vals = 8 * p.nnzb * d * d
     compulsory = coords + conn + mats + vals
-    return coords + conn + mats + graph + vals, compulsory
+    return coords + welem_conn + mats + graph + vals, compulsory
 
 
 def pcg_spmv_bytes(sysm, value_bytes):
@@ -95,7 +98,8 @@ def one_solve(sysm, pr, method, fixed, g, f_t, profile=False):
     KD, b = sysm.prepare(K, fixed, g, f_t)
     M = None
     if method.get('model') is not None:
-        M = method['model'].preconditioner(sysm, KD, fixed)
+        # network inference, run by the solver's set-up beside the coarse inverse
+        M = lambda: method['model'].preconditioner(sysm, KD, fixed)
     solver = sysm.solver(KD, precond=method['precond'], precision=method['precision'], M64=M,
                          coarse=method.get('coarse'))
     u, info = solver.solve(b, tol=1e-8, max_iter=MAX_ITER, profile=profile)

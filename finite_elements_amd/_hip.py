@@ -53,7 +53,9 @@ class FeGraph(ctypes.Structure):
     _fields_ = [('n_nodes', _i64), ('nnzb', _i64), ('n_inc', _i64), ('n_blk', _i64),
                 ('max_inc', _i32), ('threads', _i32), ('bptr', _vp), ('bcol', _vp), ('inc_ptr', _vp),
                 ('inc', _vp), ('bseg_ptr', _vp), ('bseg', _vp), ('blk', _vp), ('max_elem', _i32),
-                ('pad', _i32), ('welem_ptr', _vp), ('welem', _vp), ('inc_sa', _vp), ('bdiag', _vp)]
+                ('pad', _i32), ('welem_ptr', _vp), ('welem', _vp), ('inc_sa', _vp), ('bdiag', _vp),
+                ('wg_hdr', _vp), ('welem_conn', _vp), ('n_welem', _i64), ('max_nodes', _i32),
+                ('pad2', _i32)]
 
 
 class FeBsr(ctypes.Structure):

@@ -81,13 +81,13 @@ constexpr int record_size() {
 
 // Record of element e, field k stored at S[k].
 template <int KIND, int WHAT>
-__device__ inline void element_record(const fe_mesh& m, int64_t e, double* S) {
+__device__ inline void element_record(const fe_mesh& m, int64_t e, const int (&nid)[KindTraits<KIND>::NPE],
+                                     double* S) {
   constexpr int NPE = KindTraits<KIND>::NPE;
-  const int64_t ne = m.n_elem;
   double px[NPE], py[NPE], pz[NPE];
 #pragma unroll
   for (int b = 0; b < NPE; ++b) {
-    const int n = m.conn[b * ne + e];
+    const int n = nid[b];
     px[b] = m.x[n];
     py[b] = m.y[n];
     if (NPE == 4) pz[b] = m.z[n];
@@ -248,47 +248,15 @@ __device__ inline void store_block(double* __restrict__ vals, const fe_graph& g,
   }
 }
 
+// Phase 2 of the fused assembly for one node range whose records and maps are in LDS.
 template <int KIND, int WHAT, int LAYOUT, int NT>
-__global__ __launch_bounds__(NT) void k_assemble_fused(fe_mesh m, fe_graph g,
-                                                       double* __restrict__ vals) {
+__device__ inline void fused_phase2(const fe_graph& g, double* __restrict__ vals, const double* S,
+                                    const int32_t* s_bptr, const int32_t* s_iptr, const int32_t* s_diag,
+                                    const int32_t* s_bsp, const uint16_t* s_bseg, const uint16_t* s_sa,
+                                    int32_t B0, int32_t nbq, int32_t nn) {
   constexpr int D = KindTraits<KIND>::D;
   constexpr int RS = record_size<KIND, WHAT>();
   constexpr int DL = 4;  // lanes per diagonal block
-  constexpr int NPE = KindTraits<KIND>::NPE;
-  extern __shared__ double S[];  // [max_elem][RS] records (RS odd: few bank conflicts), then the maps
-  __shared__ int32_t s_bptr[kFusedMaxNodes + 1];
-  __shared__ int32_t s_iptr[kFusedMaxNodes + 1];
-  __shared__ int32_t s_diag[kFusedMaxNodes];
-  const int stride = g.max_elem;
-  int32_t* s_bsp = reinterpret_cast<int32_t*>(S + (size_t)RS * stride);  // NPE * max_inc + 1
-  uint16_t* s_bseg = reinterpret_cast<uint16_t*>(s_bsp + NPE * g.max_inc + 1);  // NPE * max_inc
-  uint16_t* s_sa = s_bseg + NPE * g.max_inc;                                     // max_inc
-  const int64_t w = xcd_remap(blockIdx.x, gridDim.x);
-  const int32_t n0 = g.blk[w], n1 = g.blk[w + 1];
-  const int32_t nn = n1 - n0;
-  const int32_t E0 = g.welem_ptr[w];
-  const int32_t nel = g.welem_ptr[w + 1] - E0;
-  const int32_t I0 = g.inc_ptr[n0], ninc = g.inc_ptr[n1] - I0;
-  const int32_t B0 = g.bptr[n0], B1 = g.bptr[n1];
-  const int32_t nbq = B1 - B0;
-  const int32_t c_lo = g.bseg_ptr[B0];
-  const int32_t nseg = g.bseg_ptr[B1] - c_lo;
-  // a range beyond the schedule's stated maxima would overrun the LDS: such a workgroup writes nothing
-  if (nn > kFusedMaxNodes || nel > stride || ninc > g.max_inc || nbq > NPE * g.max_inc ||
-      nseg > NPE * g.max_inc)
-    return;
-  // Stage row pointers, incidence codes and the block-level contribution map of the range.
-  for (int t = threadIdx.x; t <= nn; t += NT) {
-    s_bptr[t] = g.bptr[n0 + t];
-    s_iptr[t] = g.inc_ptr[n0 + t] - I0;
-    if (t < nn) s_diag[t] = g.bdiag[n0 + t];
-  }
-  for (int t = threadIdx.x; t < ninc; t += NT) s_sa[t] = g.inc_sa[I0 + t];
-  for (int t = threadIdx.x; t <= nbq; t += NT) s_bsp[t] = g.bseg_ptr[B0 + t] - c_lo;
-  for (int t = threadIdx.x; t < nseg; t += NT) s_bseg[t] = g.bseg[c_lo + t];
-  // Phase 1: the geometry record of every distinct element of the range.
-  for (int t = threadIdx.x; t < nel; t += NT) element_record<KIND, WHAT>(m, g.welem[E0 + t], S + t * RS);
-  __syncthreads();
   // Phase 2, one loop over two kinds of work item (whole wavefronts take one kind):
   //  * items [0, nbq): one thread per off-diagonal block entry (node j, neighbour q), summing block
   //    (a, b) of the elements holding both nodes in ascending element order; consecutive threads own
@@ -347,12 +315,184 @@ __global__ __launch_bounds__(NT) void k_assemble_fused(fe_mesh m, fe_graph g,
   }
 }
 
+template <int KIND, int WHAT, int LAYOUT, int NT>
+__global__ __launch_bounds__(NT) void k_assemble_fused(fe_mesh m, fe_graph g,
+                                                       double* __restrict__ vals) {
+  constexpr int D = KindTraits<KIND>::D;
+  constexpr int RS = record_size<KIND, WHAT>();
+  constexpr int DL = 4;  // lanes per diagonal block
+  constexpr int NPE = KindTraits<KIND>::NPE;
+  extern __shared__ double S[];  // [max_elem][RS] records (RS odd: few bank conflicts), then the maps
+  __shared__ int32_t s_bptr[kFusedMaxNodes + 1];
+  __shared__ int32_t s_iptr[kFusedMaxNodes + 1];
+  __shared__ int32_t s_diag[kFusedMaxNodes];
+  const int stride = g.max_elem;
+  int32_t* s_bsp = reinterpret_cast<int32_t*>(S + (size_t)RS * stride);  // NPE * max_inc + 1
+  uint16_t* s_bseg = reinterpret_cast<uint16_t*>(s_bsp + NPE * g.max_inc + 1);  // NPE * max_inc
+  uint16_t* s_sa = s_bseg + NPE * g.max_inc;                                     // max_inc
+  const int64_t w = xcd_remap(blockIdx.x, gridDim.x);
+  int32_t n0, n1, E0, nel, I0, ninc, B0, nbq, c_lo, nseg;
+  if (g.wg_hdr) {
+    // one record (uniform loads) instead of a chain of dependent index loads
+    const int32_t* hd = g.wg_hdr + w * 12;
+    n0 = hd[0], n1 = hd[1], E0 = hd[2], nel = hd[3], I0 = hd[4], ninc = hd[5];
+    B0 = hd[6], nbq = hd[7], c_lo = hd[8], nseg = hd[9];
+  } else {
+    n0 = g.blk[w], n1 = g.blk[w + 1];
+    E0 = g.welem_ptr[w];
+    nel = g.welem_ptr[w + 1] - E0;
+    I0 = g.inc_ptr[n0], ninc = g.inc_ptr[n1] - I0;
+    B0 = g.bptr[n0];
+    nbq = g.bptr[n1] - B0;
+    c_lo = g.bseg_ptr[B0];
+    nseg = g.bseg_ptr[B0 + nbq] - c_lo;
+  }
+  const int32_t nn = n1 - n0;
+  // a range beyond the schedule's stated maxima would overrun the LDS: such a workgroup writes nothing
+  if (nn > kFusedMaxNodes || nel > stride || ninc > g.max_inc || nbq > NPE * g.max_inc ||
+      nseg > NPE * g.max_inc)
+    return;
+  // Stage row pointers, incidence codes and the block-level contribution map of the range.
+  for (int t = threadIdx.x; t <= nn; t += NT) {
+    s_bptr[t] = g.bptr[n0 + t];
+    s_iptr[t] = g.inc_ptr[n0 + t] - I0;
+    if (t < nn) s_diag[t] = g.bdiag[n0 + t];
+  }
+  for (int t = threadIdx.x; t < ninc; t += NT) s_sa[t] = g.inc_sa[I0 + t];
+  for (int t = threadIdx.x; t <= nbq; t += NT) s_bsp[t] = g.bseg_ptr[B0 + t] - c_lo;
+  for (int t = threadIdx.x; t < nseg; t += NT) s_bseg[t] = g.bseg[c_lo + t];
+  // Phase 1: the geometry record of every distinct element of the range.
+  for (int t = threadIdx.x; t < nel; t += NT) {
+    const int64_t e = g.welem[E0 + t];
+    int nid[NPE];
+#pragma unroll
+    for (int b = 0; b < NPE; ++b)
+      nid[b] = g.welem_conn ? g.welem_conn[b * g.n_welem + E0 + t] : m.conn[b * m.n_elem + e];
+    element_record<KIND, WHAT>(m, e, nid, S + t * RS);
+  }
+  __syncthreads();
+  fused_phase2<KIND, WHAT, LAYOUT, NT>(g, vals, S, s_bptr, s_iptr, s_diag, s_bsp, s_bseg, s_sa, B0, nbq, nn);
+}
+
+// Persistent form of the fused assembly: a workgroup walks several node ranges and, while it
+// assembles one, already has the next range's maps, element ids and element nodes in flight (loaded
+// into registers, written to LDS at the start of the next round), so the index loads of a range no
+// longer sit on its critical path.  Needs the per-workgroup headers and element node lists, a range
+// of at most NT - 1 nodes, at most NT incidences and at most NT distinct elements.  The ranges of XCD
+// x (workgroups x, x + 8, ...) are the x-th eighth of the schedule, taken in turn by its workgroups.
+template <int KIND>
+struct PipeRegs {
+  static constexpr int NPE = KindTraits<KIND>::NPE;
+  static constexpr int QB = NPE + 1;  // block-map entries per thread: NPE * max_inc + 1 <= QB * NT
+  int32_t bptr, iptr, diag, sa, bsp[QB], bseg[QB], e, nid[NPE];
+};
+
+struct RangeHdr {
+  int32_t n0, n1, E0, nel, I0, ninc, B0, nbq, c_lo, nseg;
+};
+
+__device__ inline RangeHdr load_hdr(const fe_graph& g, int64_t r) {
+  const int32_t* hd = g.wg_hdr + r * 12;
+  return RangeHdr{hd[0], hd[1], hd[2], hd[3], hd[4], hd[5], hd[6], hd[7], hd[8], hd[9]};
+}
+
+template <int KIND, int NT>
+__device__ inline void load_range(const fe_graph& g, const RangeHdr& h, PipeRegs<KIND>& p) {
+  constexpr int NPE = KindTraits<KIND>::NPE;
+  const int t = threadIdx.x;
+  const int32_t nn = h.n1 - h.n0;
+  if (t <= nn) {
+    p.bptr = g.bptr[h.n0 + t];
+    p.iptr = g.inc_ptr[h.n0 + t] - h.I0;
+  }
+  if (t < nn) p.diag = g.bdiag[h.n0 + t];
+  if (t < h.ninc) p.sa = g.inc_sa[h.I0 + t];
+#pragma unroll
+  for (int q = 0; q < PipeRegs<KIND>::QB; ++q) {
+    const int idx = t + q * NT;
+    if (idx <= h.nbq) p.bsp[q] = g.bseg_ptr[h.B0 + idx] - h.c_lo;
+    if (idx < h.nseg) p.bseg[q] = g.bseg[h.c_lo + idx];
+  }
+  if (t < h.nel) {
+    p.e = g.welem[h.E0 + t];
+#pragma unroll
+    for (int b = 0; b < NPE; ++b) p.nid[b] = g.welem_conn[b * g.n_welem + h.E0 + t];
+  }
+}
+
+template <int KIND, int WHAT, int LAYOUT, int NT>
+__global__ __launch_bounds__(NT) void k_assemble_pipe(fe_mesh m, fe_graph g, double* __restrict__ vals) {
+  constexpr int RS = record_size<KIND, WHAT>();
+  constexpr int NPE = KindTraits<KIND>::NPE;
+  constexpr int QB = PipeRegs<KIND>::QB;
+  extern __shared__ double S[];  // [max_elem][RS] records, then the maps (as k_assemble_fused)
+  __shared__ int32_t s_bptr[NT], s_iptr[NT], s_diag[NT];
+  int32_t* s_bsp = reinterpret_cast<int32_t*>(S + (size_t)RS * g.max_elem);
+  uint16_t* s_bseg = reinterpret_cast<uint16_t*>(s_bsp + NPE * g.max_inc + 1);
+  uint16_t* s_sa = s_bseg + NPE * g.max_inc;
+  const int x = blockIdx.x % 8, L = gridDim.x / 8;
+  const int64_t lo = g.n_blk * x / 8, hi = g.n_blk * (x + 1) / 8;
+  int64_t r = lo + blockIdx.x / 8;
+  if (r >= hi) return;
+  RangeHdr h = load_hdr(g, r);
+  PipeRegs<KIND> p;
+  load_range<KIND, NT>(g, h, p);
+  const int t = threadIdx.x;
+  for (; r < hi; r += L) {
+    const RangeHdr c = h;  // the range assembled in this round
+    const int32_t nn = c.n1 - c.n0;
+    // a range beyond the limits the launch checked would overrun the LDS: it is left unwritten
+    // (uniform over the workgroup, the rounds and barriers go on)
+    const bool ok = nn < NT && c.nel <= g.max_elem && c.ninc <= g.max_inc && c.ninc <= NT &&
+                    c.nbq <= NPE * g.max_inc && c.nseg <= NPE * g.max_inc;
+    const bool more = r + L < hi;
+    if (more) h = load_hdr(g, r + L);
+    __syncthreads();  // the previous round's phase 2 has finished with the LDS
+    if (ok) {
+      if (t <= nn) {
+        s_bptr[t] = p.bptr;
+        s_iptr[t] = p.iptr;
+      }
+      if (t < nn) s_diag[t] = p.diag;
+      if (t < c.ninc) s_sa[t] = (uint16_t)p.sa;
+#pragma unroll
+      for (int q = 0; q < QB; ++q) {
+        const int idx = t + q * NT;
+        if (idx <= c.nbq) s_bsp[idx] = p.bsp[q];
+        if (idx < c.nseg) s_bseg[idx] = (uint16_t)p.bseg[q];
+      }
+      if (t < c.nel) element_record<KIND, WHAT>(m, p.e, p.nid, S + t * RS);
+    }
+    // the next range's loads fly while this one is assembled
+    if (more) load_range<KIND, NT>(g, h, p);
+    __syncthreads();
+    if (ok)
+      fused_phase2<KIND, WHAT, LAYOUT, NT>(g, vals, S, s_bptr, s_iptr, s_diag, s_bsp, s_bseg, s_sa,
+                                           c.B0, c.nbq, nn);
+  }
+}
+
 template <int KIND, int WHAT, int LAYOUT>
 int launch_fused_nt(fe_handle* h, const fe_mesh* m, const fe_graph* g, double* vals, size_t lds) {
   dim3 grid((unsigned)g->n_blk);
   const int nt = g->threads > 0 ? g->threads : 256;
+  // the persistent form when the schedule fits it (see k_assemble_pipe)
+  const bool pipe = g->wg_hdr && g->welem_conn && g->max_nodes > 0 && g->max_nodes < nt &&
+                    g->max_inc <= nt && g->max_elem <= nt && g->n_blk >= 8;
 #define FE_LAUNCH_NT(NT)                                                                        \
   do {                                                                                          \
+    if (pipe) {                                                                                 \
+      auto kern = k_assemble_pipe<KIND, WHAT, LAYOUT, NT>;                                      \
+      if (lds > 64 * 1024)                                                                      \
+        FE_CHECK_HIP(hipFuncSetAttribute((const void*)kern,                                     \
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+      int per_cu = 0;                                                                           \
+      FE_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, lds));       \
+      int64_t nwg = (int64_t)std::max(per_cu, 1) * std::max(h->num_cus, 8);                     \
+      nwg = std::min<int64_t>(nwg, g->n_blk) / 8 * 8;                                            \
+      hipLaunchKernelGGL(kern, dim3((unsigned)nwg), dim3(NT), lds, h->stream, *m, *g, vals);    \
+      break;                                                                                    \
+    }                                                                                           \
     auto kern = k_assemble_fused<KIND, WHAT, LAYOUT, NT>;                                       \
     if (lds > 64 * 1024)                                                                        \
       FE_CHECK_HIP(hipFuncSetAttribute((const void*)kern,                                       \

@@ -40,16 +40,25 @@ class Context:
 
     _by_device = {}
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, stream: Optional[torch.cuda.Stream] = None):
         require_gpu()
         _hip.load()
         self.device = device
         self.torch_device = torch.device('cuda', device)
         with torch.cuda.device(device):
-            self.stream = torch.cuda.current_stream(device)
+            self.stream = torch.cuda.current_stream(device) if stream is None else stream
         h = ctypes.c_void_p()
         _hip.call('fe_create', ctypes.byref(h), device, ctypes.c_void_p(self.stream.cuda_stream))
         self.h = h
+        self._side = None
+
+    def side(self) -> 'Context':
+        """A second handle on a stream of its own (default priority), for set-up work that may run
+        beside the main stream's; created once."""
+        if self._side is None:
+            with torch.cuda.device(self.device):
+                self._side = Context(self.device, torch.cuda.Stream(self.device))
+        return self._side
 
     @classmethod
     def get(cls, device: int = 0) -> 'Context':
@@ -249,8 +258,21 @@ class DeviceGraph:
         q = np.nonzero(pat.bcol == rows)[0]
         bdiag[rows[q]] = q
         self.bdiag = ctx.tensor(bdiag)
+        self._conn = None
         self.set_schedule(pat.blk, pat.max_inc)
         self._padded = None
+
+    def attach_connectivity(self, conn: np.ndarray):
+        """Host connectivity (E, npe) of the mesh the schedule serves: every schedule then carries
+        the node ids of its element lists (one load level less in the fused assembly)."""
+        self._conn = np.ascontiguousarray(np.asarray(conn, dtype=np.int32))
+        self._set_welem_conn()
+
+    def _set_welem_conn(self):
+        if self._conn is None or self._welem_host.size == 0:
+            self.welem_conn = None
+            return
+        self.welem_conn = self.ctx.tensor(np.ascontiguousarray(self._conn[self._welem_host].T))
 
     def padded(self) -> 'PaddedGraph':
         """The padded pattern of the same rows (row_align = 4), built once."""
@@ -271,13 +293,30 @@ class DeviceGraph:
         self.welem_ptr = self.ctx.tensor(wptr)
         self.welem = self.ctx.tensor(welem if welem.size else np.zeros(1, np.int32))
         self.inc_sa = self.ctx.tensor((inc_sa if inc_sa.size else np.zeros(1, np.uint16)).view(np.int16))
+        # per-workgroup header: every index the workgroup starts from, in one 48-byte record
+        p = self.pattern
+        n0, n1 = blk[:-1].astype(np.int64), blk[1:].astype(np.int64)
+        hdr = np.zeros((self.n_blk, 12), dtype=np.int32)
+        if self.n_blk:
+            B0, B1 = p.bptr[n0], p.bptr[n1]
+            hdr[:, 0], hdr[:, 1] = n0, n1
+            hdr[:, 2], hdr[:, 3] = wptr[:-1], np.diff(wptr)
+            hdr[:, 4], hdr[:, 5] = p.inc_ptr[n0], p.inc_ptr[n1] - p.inc_ptr[n0]
+            hdr[:, 6], hdr[:, 7] = B0, B1 - B0
+            hdr[:, 8], hdr[:, 9] = p.bseg_ptr[B0], p.bseg_ptr[B1] - p.bseg_ptr[B0]
+        self.wg_hdr = self.ctx.tensor(hdr.reshape(-1) if hdr.size else np.zeros(12, np.int32))
+        self.max_nodes = int(np.diff(blk).max()) if self.n_blk else 0
+        self._welem_host = welem
+        self._set_welem_conn()
 
     def struct(self) -> _hip.FeGraph:
         return _hip.FeGraph(self.n_nodes, self.nnzb, int(self.inc.numel()), self.n_blk, self.max_inc,
                             self.threads,
                             _vp(self.bptr), _vp(self.bcol), _vp(self.inc_ptr), _vp(self.inc),
                             _vp(self.bseg_ptr), _vp(self.bseg), _vp(self.blk), self.max_elem, 0,
-                            _vp(self.welem_ptr), _vp(self.welem), _vp(self.inc_sa), _vp(self.bdiag))
+                            _vp(self.welem_ptr), _vp(self.welem), _vp(self.inc_sa), _vp(self.bdiag),
+                            _vp(self.wg_hdr), _vp(self.welem_conn), int(self._welem_host.size),
+                            self.max_nodes, 0)
 
 
 def pad_pattern(bptr: np.ndarray, bcol: np.ndarray, align: int = 4):
@@ -538,9 +577,10 @@ class DeviceCoarse:
                              _vp(self.xhat), _vp(self.xhat32), _vp(self.fixed_bits), _vp(self.T),
                              _vp(self.T32))
 
-    def operator(self, A64: BsrMatrix, invert: bool = True):
-        """(A_c or A_c^-1 [ld][ld] FP64, FP32 copy of the inverse or None) for the FP64 plane matrix A64."""
-        ctx = self.ctx
+    def operator(self, A64: BsrMatrix, invert: bool = True, ctx: Optional[Context] = None):
+        """(A_c or A_c^-1 [ld][ld] FP64, FP32 copy of the inverse or None) for the FP64 plane matrix A64,
+        computed on ``ctx``'s stream (default: the coarse space's context)."""
+        ctx = self.ctx if ctx is None else ctx
         Ac = torch.empty(self.ld * self.ld, dtype=torch.float64, device=ctx.torch_device)
         A32 = torch.empty(self.ld * self.ld, dtype=torch.float32, device=ctx.torch_device) if invert else None
         scratch = torch.empty(self.ld * self.ld + 2 * 64 * 64, dtype=torch.float64,
@@ -573,7 +613,9 @@ class PcgSolver:
 
     ``segments``: node-row offsets (n_seg + 1 entries) of independent blocks of a block-diagonal
     system (e.g. many plates assembled together); each block then iterates and stops on its own
-    (``fe_pcg_batched``).  ``graph``: replay each chunk of iterations from a captured HIP graph."""
+    (``fe_pcg_batched``).  ``graph``: replay each chunk of iterations from a captured HIP graph.
+    ``M64``: block preconditioner [n_nodes][d][d] FP64, a callable returning it, or None (the blocks
+    of A64)."""
 
     def __init__(self, A64: BsrMatrix, precond: str = 'jacobi', mixed: bool = False,
                  M64: Optional[torch.Tensor] = None, A32: Optional[BsrMatrix] = None,
@@ -585,16 +627,6 @@ class PcgSolver:
         self.A64 = A64
         self.precond = precond
         self.mixed = bool(mixed)
-        self.M64 = precond_setup(A64, 'jacobi' if precond == 'jacobi' else 'block_jacobi') \
-            if M64 is None else M64
-        # the iteration matrix: a copy on the padded pattern (16-byte loads in the SpMV), FP32 with
-        # mixed precision; A64 keeps the true residual
-        if self.mixed:
-            self.A32 = A64.padded(torch.float32) if A32 is None else A32
-            self.M32 = _to_f32(self.ctx, self.M64)
-        else:
-            self.A32 = A64.padded(torch.float64) if A32 is None else A32
-            self.M32 = None
         self.check_every = check_every
         self.ru_delta = ru_delta
         self.graph = bool(graph)
@@ -603,8 +635,36 @@ class PcgSolver:
         if coarse is not None:
             if segments is not None:
                 raise ValueError('the two-level preconditioner is for one system (no segments)')
-            # coarse operator and its inverse: part of the preconditioner set-up of every solver
-            self.Ainv64, self.Ainv32 = coarse.operator(A64, invert=True)
+            # the coarse operator and its inverse (part of the preconditioner set-up of every
+            # solver) run on the side stream, beside the block preconditioner and the iteration
+            # matrix built below on the main stream
+            side = self.ctx.side()
+            ready = torch.cuda.Event()
+            ready.record(self.ctx.stream)
+            side.stream.wait_event(ready)
+            with torch.cuda.stream(side.stream):
+                self.Ainv64, self.Ainv32 = coarse.operator(A64, invert=True, ctx=side)
+            coarse_done = torch.cuda.Event()
+            coarse_done.record(side.stream)
+        # block preconditioner: given, computed by a callable (e.g. a learned model's inference), or
+        # the blocks of A64
+        if M64 is None:
+            M64 = precond_setup(A64, 'jacobi' if precond == 'jacobi' else 'block_jacobi')
+        elif callable(M64):
+            M64 = M64()
+        self.M64 = M64
+        # the iteration matrix: a copy on the padded pattern (16-byte loads in the SpMV), FP32 with
+        # mixed precision; A64 keeps the true residual
+        if self.mixed:
+            self.A32 = A64.padded(torch.float32) if A32 is None else A32
+            self.M32 = _to_f32(self.ctx, self.M64)
+        else:
+            self.A32 = A64.padded(torch.float64) if A32 is None else A32
+            self.M32 = None
+        if coarse is not None:
+            self.ctx.stream.wait_event(coarse_done)
+            for t in (self.Ainv64, self.Ainv32):
+                t.record_stream(self.ctx.stream)
         if segments is not None:
             seg = np.ascontiguousarray(np.asarray(segments, dtype=np.int32))
             if seg.ndim != 1 or seg.size < 2 or seg[0] != 0 or seg[-1] != A64.graph.n_nodes:

@@ -37,6 +37,7 @@ class FESystem:
         self.pattern = Pattern(conn, self.dof, self.n_nodes, sym=True, with_gather=with_gather,
                                max_inc=max_inc)
         self.graph = DeviceGraph(self.ctx, self.pattern)
+        self.graph.attach_connectivity(conn)
         self.mesh = DeviceMesh(self.ctx, kind, coords, conn, materials, plane_mode)
         self._coords, self._conn = coords, conn
         if with_gather:
@@ -158,7 +159,9 @@ class FESystem:
                coarse=None) -> PcgSolver:
         """PCG for the Dirichlet-applied KD.  precond 'jacobi', 'block_jacobi' or 'two_level'
         (block Jacobi -- or the learned blocks M64 -- plus the coarse correction of ``coarse``, a
-        ``DeviceCoarse`` from ``coarse_space``)."""
+        ``DeviceCoarse`` from ``coarse_space``).  M64 may be a callable returning the blocks: it is
+        called after the coarse inverse has been started on the side stream, so that both run
+        together."""
         if precision not in ('fp64', 'mixed'):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         if precond not in ('jacobi', 'block_jacobi', 'two_level'):

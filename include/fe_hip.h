@@ -109,6 +109,15 @@ typedef struct {
   const int32_t* welem;
   const uint16_t* inc_sa;
   const int32_t* bdiag; /* per node, the block index of its diagonal entry (-1: node without elements) */
+  /* Optional (NULL: derived on the device from the arrays above).  wg_hdr: per workgroup, 12 int32
+     {n0, n1, E0, nel, I0, ninc, B0, nbq, c0, nseg, 0, 0}: its node range [n0, n1), welem list start
+     and length, first incidence and count, first block and count, first bseg entry and count.
+     welem_conn: the node ids of the welem list's elements, [npe][n_welem] (n_welem = welem_ptr[n_blk]). */
+  const int32_t* wg_hdr;
+  const int32_t* welem_conn;
+  int64_t n_welem;
+  int32_t max_nodes; /* largest node count of one workgroup's range (0: not given) */
+  int32_t pad2;
 } fe_graph;
 
 /* Block sparse matrix (node-block rows) on the device; precision 0 = FP64, 1 = FP32; layout

@@ -113,6 +113,33 @@ def test_fused_schedules_agree(ctx, max_inc, threads):
     assert np.array_equal(s.stiffness(_hip.FE_LAYOUT_PLANES).cpu().numpy(), ref_planes)
 
 
+@pytest.mark.parametrize('kind', ['tet_elastic', 'tri_elastic', 'tri_magnetic'])
+@pytest.mark.parametrize('max_inc,threads', [(48, 64), (96, 128), (192, 256), (384, 512)])
+def test_persistent_assembly_equals_one_range_per_workgroup(ctx, kind, max_inc, threads):
+    """The persistent fused assembly (headers + element node lists present) writes the same bits as
+    the one-range-per-workgroup kernel (node lists withheld), stiffness and mass, every layout."""
+    _say(f'persistent assembly {kind} max_inc={max_inc} threads={threads}')
+    rng = np.random.default_rng(21)
+    c, t, mats = _perturbed(kind, rng)
+    s = FESystem(kind, c, t, mats, plane_mode=_hip.FE_PLANE_STRESS if kind == 'tri_elastic' else 0)
+    s.set_schedule(max_inc, threads)
+    g = s.graph
+    assert g.welem_conn is not None and g.max_nodes < threads and g.max_elem <= threads
+    whats = ['stiffness'] + ([] if kind == 'tri_magnetic' else ['mass'])
+    layouts = [_hip.FE_LAYOUT_CSR, _hip.FE_LAYOUT_BSR, _hip.FE_LAYOUT_PLANES]
+    got = {(w, l): (s.stiffness(l) if w == 'stiffness' else s.mass(l)).cpu().numpy()
+           for w in whats for l in layouts}
+    saved = g.welem_conn
+    g.welem_conn = None
+    try:
+        for w in whats:
+            for l in layouts:
+                ref = (s.stiffness(l) if w == 'stiffness' else s.mass(l)).cpu().numpy()
+                assert np.array_equal(got[(w, l)], ref), (w, l)
+    finally:
+        g.welem_conn = saved
+
+
 def test_autotuner_picks_a_valid_schedule(ctx):
     _say('assembly autotuner')
     c, t = meshgen.kuhn_tets(8)
