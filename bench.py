@@ -221,7 +221,7 @@ def summarize_pmc(fetch_dir, write_dir):
            'kernels': {}}
     for name in sorted(fetch):
         if not any(k in name for k in ('k_cg_spmv', 'k_cg_vec', 'k_cg_reduce', 'k_assemble_fused',
-                                        'k_ru_', 'k_spmv_bsr')):
+                                        'k_assemble_pipe', 'k_ru_', 'k_spmv_bsr')):
             continue
         # dispatches that did the work (launches of finished solves return at once)
         fa = np.array(fetch[name])
@@ -234,6 +234,28 @@ def summarize_pmc(fetch_dir, write_dir):
     with open(TRAFFIC_JSON, 'w') as fh:
         json.dump(out, fh, indent=1, sort_keys=True)
     print(json.dumps(out, indent=1, sort_keys=True))
+
+
+def summarize_trace(csv_path, top=25):
+    """Per-kernel summary of a rocprofv3 kernel trace (csv): calls, mean duration, the 'active'
+    dispatches (longer than half the kernel's median: launches that found their solve finished return
+    at once) and their mean, total time; printed as the table the profile summaries hold."""
+    import csv
+    import collections
+    rows = collections.defaultdict(list)
+    with open(csv_path) as fh:
+        for r in csv.DictReader(fh):
+            name = r['Kernel_Name'].replace('void ', '').replace('(anonymous namespace)::', '')
+            rows[name.split('(')[0]].append((int(r['End_Timestamp']) - int(r['Start_Timestamp'])) / 1e3)
+    out = []
+    for k, v in rows.items():
+        v = np.array(v)
+        act = v[v >= 0.5 * np.median(v)]
+        out.append((v.sum() / 1e3, k, len(v), v.mean(), len(act), act.mean()))
+    print(f'{"kernel":60s} {"calls":>6s} {"avg_us":>9s} {"active_n":>8s} {"active_avg_us":>13s} '
+          f'{"total_ms":>9s}')
+    for tot, k, n, avg, na, aavg in sorted(out, reverse=True)[:top]:
+        print(f'{k:60s} {n:6d} {avg:9.2f} {na:8d} {aavg:13.2f} {tot:9.2f}')
 
 
 def measured_traffic(kernel):
@@ -316,15 +338,22 @@ def main():
     ap.add_argument('--no-learned', action='store_true')
     ap.add_argument('--weights', default=None)
     ap.add_argument('--no-tune', action='store_true', help='keep the default assembly schedule')
+    ap.add_argument('--schedule', nargs=2, type=int, metavar=('THREADS', 'MAX_INC'),
+                    help='use this assembly schedule instead of the autotuner')
     ap.add_argument('--max-iter', type=int, default=40000,
                     help='PCG iteration cap (lowered only for short counter-collection runs)')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='first run a plain 512 MiB device copy (reference for HBM byte counters)')
     ap.add_argument('--summarize-pmc', nargs=2, metavar=('FETCH_DIR', 'WRITE_DIR'),
                     help='summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) and exit')
+    ap.add_argument('--summarize-trace', metavar='KERNEL_TRACE_CSV',
+                    help='summarise a rocprofv3 --kernel-trace csv per kernel and exit')
     args = ap.parse_args()
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc)
+        return
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace)
         return
     global MAX_ITER
     MAX_ITER = args.max_iter
@@ -332,6 +361,7 @@ def main():
         raise SystemExit('bench.py measures one GPU: this build has no multi-GPU path (the sharding '
                          'design is described in DESIGN.md); run with --gpus 1')
     import torch
+    from finite_elements_amd import _hip
     from finite_elements_amd.system import FESystem
 
     if args.workload == 'modal48':
@@ -353,7 +383,10 @@ def main():
     if args.copy_calibration:
         copy_calibration()
     # ---- assembly: autotune the row-block schedule, then time the best ----
-    if args.no_tune:
+    if args.schedule:
+        sysm.set_schedule(args.schedule[1], args.schedule[0])
+        best, tune_times = (args.schedule[0], args.schedule[1]), {}
+    elif args.no_tune:
         best, tune_times = (sysm.graph.threads, sysm.graph.max_inc), {}
     else:
         best, tune_times = sysm.tune_assembly()
@@ -361,11 +394,18 @@ def main():
         ', '.join(f'{k}:{v:.3f}ms' for k, v in sorted(tune_times.items())))
     asm_ms = time_assembly(sysm)
     asm_bytes, asm_compulsory = fused_assembly_bytes(sysm)
-    asm = {'kernel': 'k_assemble_fused', 'schedule': {'threads': best[0], 'max_inc': best[1]},
+    g_ = sysm.graph
+    persistent = (g_.welem_conn is not None and 0 < g_.max_nodes < best[0] and g_.max_inc <= best[0]
+                  and g_.max_elem <= best[0] and g_.n_blk >= 8)
+    asm = {'kernel': 'k_assemble_pipe' if persistent else 'k_assemble_fused',
+           'schedule': {'threads': best[0], 'max_inc': best[1]},
            'ms': asm_ms, 'bytes': asm_bytes,
            'gbps': asm_bytes / asm_ms / 1e6, 'frac': asm_bytes / asm_ms / 1e6 / HBM_PEAK_GBS,
            'compulsory_bytes': asm_compulsory,
-           'compulsory_gbps': asm_compulsory / asm_ms / 1e6}
+           'compulsory_gbps': asm_compulsory / asm_ms / 1e6,
+           'traffic': measured_traffic(f"{'k_assemble_pipe' if persistent else 'k_assemble_fused'}"
+                                       f"<{sysm.mesh.kind_id}, {_hip.FE_STIFFNESS}, "
+                                       f"{_hip.FE_LAYOUT_PLANES}, {best[0]}>")}
     log(f'assembly {asm_ms:.3f} ms, {asm["gbps"]:.0f} GB/s algorithmic')
 
     model = None
