@@ -256,7 +256,9 @@ __global__ __launch_bounds__(kThreads) void k_lob_cprolong(LobParams p, fe_coars
 // --------------------------------------------------------------------------------------------------
 // K2: K Y and M Y for the block Y = X (start) or W, one thread per (node row, column)
 // --------------------------------------------------------------------------------------------------
-template <int D>
+// LAYOUT: FE_LAYOUT_BSR keeps a block's d*d values together, so the lanes of a row walk their blocks
+// through contiguous memory (the planes layout spreads each step over d*d lines of the L1).
+template <int D, int LAYOUT>
 __global__ __launch_bounds__(kThreads) void k_lob_spmm(LobParams p, const int32_t* __restrict__ bptr,
                                                        const int32_t* __restrict__ bcol,
                                                        const double* __restrict__ Kv, int64_t k_stride,
@@ -307,7 +309,8 @@ __global__ __launch_bounds__(kThreads) void k_lob_spmm(LobParams p, const int32_
     for (int e = 0; e < D; ++e)
 #pragma unroll
       for (int f = 0; f < D; ++f) {
-        const int64_t at = (int64_t)(e * D + f) * p.nnzb + q;
+        const int64_t at = LAYOUT == FE_LAYOUT_PLANES ? (int64_t)(e * D + f) * p.nnzb + q
+                                                      : (int64_t)q * D * D + e * D + f;
         const double kv = K[at], mv = M[at];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -948,8 +951,8 @@ int check_opts(const fe_bsr* K, const fe_lobpcg_opts* o) {
   FE_REQUIRE(K && o, "fe_lobpcg: null argument");
   FE_REQUIRE(K->dof >= 1 && K->dof <= 3, "fe_lobpcg: dof must be 1..3");
   FE_REQUIRE(K->row_align == 0, "fe_lobpcg: the matrix's own pattern expected (row_align 0)");
-  FE_REQUIRE(K->precision == 0 && K->layout == FE_LAYOUT_PLANES,
-             "fe_lobpcg: matrices must be FP64 in FE_LAYOUT_PLANES");
+  FE_REQUIRE(K->precision == 0 && (K->layout == FE_LAYOUT_PLANES || K->layout == FE_LAYOUT_BSR),
+             "fe_lobpcg: matrices must be FP64 in FE_LAYOUT_PLANES or FE_LAYOUT_BSR");
   FE_REQUIRE(o->block >= 4 && o->block <= kMaxBlock && o->block % 4 == 0,
              "fe_lobpcg: block must be a multiple of 4 in [4, %d]", kMaxBlock);
   FE_REQUIRE(o->n_wanted >= 1 && o->n_wanted <= o->block, "fe_lobpcg: need 1 <= n_wanted <= block");
@@ -979,7 +982,7 @@ void launch_lob_coarse(hipStream_t st, const LobParams& p, const LobCoarse& cc, 
                      0, st, p, *cc.C, cc.ld, w.st, w.zc, w.blk);
 }
 
-template <int D>
+template <int D, int LAYOUT>
 int run_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
                const uint8_t* fixed, double* X, double* lam, double* res, int32_t* iters,
                int32_t* vstatus, const LobWork& w, const LobParams& p, const fe_lobpcg_opts* o,
@@ -1022,7 +1025,7 @@ int run_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
     return FE_OK;
   };
   // start: K X, M X, Rayleigh-Ritz on X alone
-  hipLaunchKernelGGL((k_lob_spmm<D>), gspmm, dim3(kThreads), 0, st, p, K->bptr, K->bcol, Kv,
+  hipLaunchKernelGGL((k_lob_spmm<D, LAYOUT>), gspmm, dim3(kThreads), 0, st, p, K->bptr, K->bcol, Kv,
                      o->k_stride, Mv, o->m_stride, w.st, w.blk, 1);
   FE_CHECK_LAUNCH();
   int rc = rr_update();
@@ -1038,15 +1041,15 @@ int run_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
   auto enqueue = [&](int slot) -> int {
     for (int it = 0; it < chunk; ++it, ++host_it) {
       if (host_it % kRefresh == kRefresh - 1) {
-        hipLaunchKernelGGL((k_lob_spmm<D>), gspmm, dim3(kThreads), 0, st, p, K->bptr, K->bcol, Kv,
+        hipLaunchKernelGGL((k_lob_spmm<D, LAYOUT>), gspmm, dim3(kThreads), 0, st, p, K->bptr, K->bcol, Kv,
                            o->k_stride, Mv, o->m_stride, w.st, w.blk, 1);
-        hipLaunchKernelGGL((k_lob_spmm<D>), gspmm, dim3(kThreads), 0, st, p, K->bptr, K->bcol, Kv,
+        hipLaunchKernelGGL((k_lob_spmm<D, LAYOUT>), gspmm, dim3(kThreads), 0, st, p, K->bptr, K->bcol, Kv,
                            o->k_stride, Mv, o->m_stride, w.st, w.blk, 2);
       }
       hipLaunchKernelGGL((k_lob_resid<D>), gres, dim3(kThreads), 0, st, p, T, o->t_stride, fixed,
                          w.st, w.blk, w.rpart);
       if (cc.C) launch_lob_coarse<D, D == 3 ? 6 : D == 2 ? 3 : 1>(st, p, cc, w);
-      hipLaunchKernelGGL((k_lob_spmm<D>), gspmm, dim3(kThreads), 0, st, p, K->bptr, K->bcol, Kv,
+      hipLaunchKernelGGL((k_lob_spmm<D, LAYOUT>), gspmm, dim3(kThreads), 0, st, p, K->bptr, K->bcol, Kv,
                          o->k_stride, Mv, o->m_stride, w.st, w.blk, 0);
       int rc2 = rr_update();
       if (rc2) return rc2;
@@ -1104,9 +1107,9 @@ int lobpcg_common(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* 
   int rc = check_opts(K, o);
   if (rc) return rc;
   FE_REQUIRE(M && X && lambda && resid && iters && vstatus && work && stats, "%s: null argument", who);
-  FE_REQUIRE(M->precision == 0 && M->layout == FE_LAYOUT_PLANES && M->dof == K->dof &&
+  FE_REQUIRE(M->precision == 0 && M->layout == K->layout && M->dof == K->dof &&
                  M->n_brows == K->n_brows && M->nnzb == K->nnzb && M->row_align == 0,
-             "%s: M must be FP64 FE_LAYOUT_PLANES on the pattern of K", who);
+             "%s: M must be FP64 in the layout of K, on the pattern of K", who);
   FE_REQUIRE(o->k_stride >= 0 && o->m_stride >= 0 && o->t_stride >= 0, "%s: strides must be non-negative",
              who);
   *stats = fe_lobpcg_stats{};
@@ -1114,11 +1117,18 @@ int lobpcg_common(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* 
   const LobWork w = layout_lob((char*)work, p, cc.ld);
   FE_REQUIRE(work_bytes >= w.bytes, "%s: workspace too small (%lld < %lld)", who, (long long)work_bytes,
              (long long)w.bytes);
+#define FE_RUN(DD)                                                                                   \
+  return K->layout == FE_LAYOUT_BSR                                                                  \
+             ? run_lobpcg<DD, FE_LAYOUT_BSR>(h, K, M, T, fixed, X, lambda, resid, iters, vstatus, w, p, \
+                                             o, stats, cc)                                           \
+             : run_lobpcg<DD, FE_LAYOUT_PLANES>(h, K, M, T, fixed, X, lambda, resid, iters, vstatus, w, \
+                                                p, o, stats, cc)
   switch (K->dof) {
-    case 1: return run_lobpcg<1>(h, K, M, T, fixed, X, lambda, resid, iters, vstatus, w, p, o, stats, cc);
-    case 2: return run_lobpcg<2>(h, K, M, T, fixed, X, lambda, resid, iters, vstatus, w, p, o, stats, cc);
-    default: return run_lobpcg<3>(h, K, M, T, fixed, X, lambda, resid, iters, vstatus, w, p, o, stats, cc);
+    case 1: FE_RUN(1);
+    case 2: FE_RUN(2);
+    default: FE_RUN(3);
   }
+#undef FE_RUN
 }
 
 }  // namespace

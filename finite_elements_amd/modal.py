@@ -35,11 +35,18 @@ def _planes64(A: BsrMatrix) -> BsrMatrix:
     return A
 
 
+def _bsr64(A: BsrMatrix) -> torch.Tensor:
+    """The FP64 values in the block-contiguous layout the eigen-solve's block SpMM reads."""
+    if A.layout == _hip.FE_LAYOUT_BSR and A.precision == 0:
+        return A.vals
+    return A.convert(_hip.FE_LAYOUT_BSR, torch.float64).vals
+
+
 def lobpcg(K: Union[BsrMatrix, Sequence[BsrMatrix]], M: Union[BsrMatrix, Sequence[BsrMatrix]],
            n_wanted: int, block: Optional[int] = None, fixed: Optional[torch.Tensor] = None,
            precond: Union[str, None, torch.Tensor, Sequence[torch.Tensor]] = 'block_jacobi',
            largest: bool = False, tol: float = 1e-6, max_iter: int = 2000, X0=None, seed: int = 0,
-           check_every: int = 8, coarse=None):
+           check_every: int = 8, coarse=None, iteration_layout: int = _hip.FE_LAYOUT_BSR):
     """Lowest (or largest) eigenpairs of one or several variants K_v phi = lambda M_v phi.
 
     K, M: one matrix or a list of variants on one pattern (FP64; converted to the plane layout).  A
@@ -48,8 +55,9 @@ def lobpcg(K: Union[BsrMatrix, Sequence[BsrMatrix]], M: Union[BsrMatrix, Sequenc
     stays in the free subspace.  precond: 'block_jacobi' (blocks of each K_v), None, or explicit
     FP64 blocks [n_brows][d][d] (one tensor or one per variant).  coarse: a ``DeviceCoarse`` of the
     constrained mesh (``FESystem.coarse_space``): the preconditioner becomes two-level, T + P A_c^-1 P^T
-    with A_c = P^T K_v P per variant (smallest eigenvalues only).  Returns (eigvals [nvar][n_wanted]
-    numpy, eigvecs [nvar][n][n_wanted] torch FP64, info).
+    with A_c = P^T K_v P per variant (smallest eigenvalues only).  iteration_layout: the value
+    layout the block SpMM reads (FE_LAYOUT_BSR, or FE_LAYOUT_PLANES; same results bit for bit).
+    Returns (eigvals [nvar][n_wanted] numpy, eigvecs [nvar][n][n_wanted] torch FP64, info).
     """
     Ks = [K] if isinstance(K, BsrMatrix) else list(K)
     Ms = [M] if isinstance(M, BsrMatrix) else list(M)
@@ -72,8 +80,13 @@ def lobpcg(K: Union[BsrMatrix, Sequence[BsrMatrix]], M: Union[BsrMatrix, Sequenc
         if A.graph is not g or A.dof != d:
             raise ValueError('lobpcg: all matrices must share one pattern')
     nv_vals = Ks[0].vals.numel()
-    Kall = Ks[0].vals if nvar == 1 else torch.stack([A.vals for A in Ks])
-    Mall = Ms[0].vals if len(Ms) == 1 else torch.stack([A.vals for A in Ms])
+    # the iteration reads K and M block-contiguous; the set-up below (block Jacobi, the coarse
+    # operators) reads the plane layout
+    if iteration_layout not in (_hip.FE_LAYOUT_BSR, _hip.FE_LAYOUT_PLANES):
+        raise ValueError('lobpcg: iteration_layout must be FE_LAYOUT_BSR or FE_LAYOUT_PLANES')
+    vals = _bsr64 if iteration_layout == _hip.FE_LAYOUT_BSR else (lambda A: A.vals)
+    Kall = vals(Ks[0]) if nvar == 1 else torch.stack([vals(A) for A in Ks])
+    Mall = vals(Ms[0]) if len(Ms) == 1 else torch.stack([vals(A) for A in Ms])
     k_stride = nv_vals if nvar > 1 else 0
     m_stride = nv_vals if len(Ms) > 1 else 0
     # preconditioner blocks
@@ -106,10 +119,8 @@ def lobpcg(K: Union[BsrMatrix, Sequence[BsrMatrix]], M: Union[BsrMatrix, Sequenc
     vstat = torch.zeros(nvar, dtype=torch.int32, device=ctx.torch_device)
     opts = _hip.FeLobpcgOpts(nvar, block, n_wanted, int(bool(largest)), int(max_iter), int(check_every),
                              float(tol), k_stride, m_stride, t_stride)
-    ks = _hip.FeBsr(g.n_nodes, g.nnzb, d, 0, _hip.FE_LAYOUT_PLANES, 0, _vp(g.bptr), _vp(g.bcol),
-                    _vp(Kall))
-    ms = _hip.FeBsr(g.n_nodes, g.nnzb, d, 0, _hip.FE_LAYOUT_PLANES, 0, _vp(g.bptr), _vp(g.bcol),
-                    _vp(Mall))
+    ks = _hip.FeBsr(g.n_nodes, g.nnzb, d, 0, iteration_layout, 0, _vp(g.bptr), _vp(g.bcol), _vp(Kall))
+    ms = _hip.FeBsr(g.n_nodes, g.nnzb, d, 0, iteration_layout, 0, _vp(g.bptr), _vp(g.bcol), _vp(Mall))
     nbytes = ctypes.c_int64()
     stats = _hip.FeLobpcgStats()
     if coarse is None:

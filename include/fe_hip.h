@@ -372,7 +372,8 @@ int fe_pcg_coarse(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void
    finite_elements/analysis.py:741-796).  Variants share the pattern of K; variant v uses the values
    at K->vals + v * k_stride (M likewise; stride 0 = shared) and the block-Jacobi blocks
    T + v * t_stride ([n_brows][dof][dof], FP64; T = NULL: no preconditioner, the choice for
-   largest = 1).  Matrices FP64 in FE_LAYOUT_PLANES.  X: [n_variants][n][block] row-major, the initial
+   largest = 1).  Matrices FP64, K and M in the same layout: FE_LAYOUT_BSR (faster: a row's blocks are
+   read contiguously) or FE_LAYOUT_PLANES.  X: [n_variants][n][block] row-major, the initial
    block on entry (zero on fixed DOFs), the Ritz vectors on return (M-orthonormal, ascending eigenvalue
    for largest = 0).  lambda/resid: [n_variants][block]; resid = ||K x - l M x|| / (|l| ||M x||).
    A variant stops when its n_wanted first pairs reach tol. fixed (n bytes, may be NULL) zeroes the

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 import torch
 
-from finite_elements_amd import device, meshgen, modal
+from finite_elements_amd import _hip, device, meshgen, modal
 from finite_elements_amd.system import FESystem
 from oracle import eigen, vectorized as vz
 
@@ -173,3 +173,8 @@ def test_lobpcg_batched_variants_share_mass(ctx):
     v1, _, _ = modal.lobpcg(KDs[1], MD, 8, block=12, fixed=fixed, tol=1e-7, max_iter=3000,
                             X0=None, seed=0)
     np.testing.assert_allclose(v1[0], vals[1], rtol=1e-9)
+    # the plane layout of K and M gives the same iteration bit for bit (same per-row sum order)
+    vp_, xp_, ip_ = modal.lobpcg(KDs, MD, 8, block=12, fixed=fixed, tol=1e-7, max_iter=3000,
+                                 iteration_layout=_hip.FE_LAYOUT_PLANES)
+    assert np.array_equal(vp_, vals) and torch.equal(xp_, vecs)
+    assert np.array_equal(ip_['iterations'], info['iterations'])
