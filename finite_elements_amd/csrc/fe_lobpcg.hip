@@ -759,41 +759,41 @@ __global__ __launch_bounds__(kThreads) void k_lob_update(LobParams p, const LobS
   const double* C = Cg + (int64_t)v * 3 * m * m;
   for (int idx = threadIdx.x; idx < ns * m; idx += kThreads) sC[idx] = C[idx];
   __syncthreads();
-  // a thread owns 4 consecutive columns of one DOF row
-  const int tpr = m / 4, rpb = kThreads / tpr;
+  // a thread owns 4 consecutive columns of two consecutive DOF rows (each coefficient read from LDS
+  // serves both rows)
+  const int tpr = m / 4, gpb = kThreads / tpr, rpb = 2 * gpb;
   const int g = threadIdx.x / tpr, j0 = (threadIdx.x % tpr) * 4;
-  const int64_t row = (int64_t)blockIdx.x * rpb + g;
-  if (g >= rpb || row >= p.n) return;
+  const int64_t row = (int64_t)blockIdx.x * rpb + 2 * g;
+  if (g >= gpb || row >= p.n) return;
+  const bool two = row + 1 < p.n;
   for (int which = 0; which < 3; ++which) {
     const double* X = lob_block(blk, p, v, B_X + 2 * which + cur) + row * m;
     const double* P = lob_block(blk, p, v, B_P + 2 * which + cur) + row * m;
     const double* W = lob_block(blk, p, v, B_W + which) + row * m;
-    double xa[4] = {0.0, 0.0, 0.0, 0.0}, pa[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int a = 0; a < m; a += 2) {
-      const double2 x = *reinterpret_cast<const double2*>(X + a);
+    double xa[2][4] = {}, pa[2][4] = {};
+    auto accum = [&](const double* Y, int cbase, double (&acc)[2][4]) {
+      for (int a = 0; a < m; a += 2) {
+        const double2 y0 = *reinterpret_cast<const double2*>(Y + a);
+        const double2 y1 = two ? *reinterpret_cast<const double2*>(Y + m + a) : make_double2(0.0, 0.0);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) xa[u] += x.x * sC[a * m + j0 + u] + x.y * sC[(a + 1) * m + j0 + u];
+        for (int u = 0; u < 4; ++u) {
+          const double c0 = sC[(cbase + a) * m + j0 + u], c1 = sC[(cbase + a + 1) * m + j0 + u];
+          acc[0][u] += y0.x * c0 + y0.y * c1;
+          acc[1][u] += y1.x * c0 + y1.y * c1;
+        }
+      }
+    };
+    accum(X, 0, xa);
+    if (ns > m) accum(W, m, pa);
+    if (ns > 2 * m) accum(P, 2 * m, pa);
+    for (int r = 0; r < (two ? 2 : 1); ++r) {
+      double* Xn = lob_block(blk, p, v, B_X + 2 * which + 1 - cur) + (row + r) * m + j0;
+      double* Pn = lob_block(blk, p, v, B_P + 2 * which + 1 - cur) + (row + r) * m + j0;
+      reinterpret_cast<double2*>(Xn)[0] = make_double2(xa[r][0] + pa[r][0], xa[r][1] + pa[r][1]);
+      reinterpret_cast<double2*>(Xn)[1] = make_double2(xa[r][2] + pa[r][2], xa[r][3] + pa[r][3]);
+      reinterpret_cast<double2*>(Pn)[0] = make_double2(pa[r][0], pa[r][1]);
+      reinterpret_cast<double2*>(Pn)[1] = make_double2(pa[r][2], pa[r][3]);
     }
-    if (ns > m)
-      for (int a = 0; a < m; a += 2) {
-        const double2 w = *reinterpret_cast<const double2*>(W + a);
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-          pa[u] += w.x * sC[(m + a) * m + j0 + u] + w.y * sC[(m + a + 1) * m + j0 + u];
-      }
-    if (ns > 2 * m)
-      for (int a = 0; a < m; a += 2) {
-        const double2 q = *reinterpret_cast<const double2*>(P + a);
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-          pa[u] += q.x * sC[(2 * m + a) * m + j0 + u] + q.y * sC[(2 * m + a + 1) * m + j0 + u];
-      }
-    double* Xn = lob_block(blk, p, v, B_X + 2 * which + 1 - cur) + row * m + j0;
-    double* Pn = lob_block(blk, p, v, B_P + 2 * which + 1 - cur) + row * m + j0;
-    reinterpret_cast<double2*>(Xn)[0] = make_double2(xa[0] + pa[0], xa[1] + pa[1]);
-    reinterpret_cast<double2*>(Xn)[1] = make_double2(xa[2] + pa[2], xa[3] + pa[3]);
-    reinterpret_cast<double2*>(Pn)[0] = make_double2(pa[0], pa[1]);
-    reinterpret_cast<double2*>(Pn)[1] = make_double2(pa[2], pa[3]);
   }
 }
 // K7: the new blocks become current; the basis gains W (and P after the first iteration).
@@ -1008,7 +1008,7 @@ int run_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
   const dim3 ggram((unsigned)(p.nchunk_g * p.ntg), nv);
   const int NS = 3 * p.m;
   const dim3 ggsum((unsigned)ceil_div(2 * NS * NS, kThreads), nv);
-  const dim3 gupd((unsigned)ceil_div(p.n, kThreads / (p.m / 4)), nv);
+  const dim3 gupd((unsigned)ceil_div(p.n, 2 * (kThreads / (p.m / 4))), nv);
   const size_t rr_lds = 2 * (size_t)NS * NS * sizeof(double);
   FE_CHECK_HIP(hipFuncSetAttribute((const void*)k_lob_rr, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)rr_lds));
