@@ -412,7 +412,9 @@ class FiniteElementAnalysis(FiniteElements):
         in the reference's condition order.  method: 'gpu' (PCG on the MI355X).  precision: 'fp64' or
         'mixed' (FP32 SpMV/preconditioner inside FP64 reliable updates).  precond: 'jacobi',
         'block_jacobi', 'two_level' (block Jacobi plus a coarse correction over aggregated rigid-body
-        modes) or 'learned' (needs ``learned``, a trained ``LearnedPreconditioner``).
+        modes), 'learned' (the learned blocks in place of block Jacobi; needs ``learned``, a trained
+        ``LearnedPreconditioner``) or 'learned_two_level' (the learned blocks plus the coarse
+        correction).
         Convergence is judged on the true FP64 residual of the eliminated system.
         """
         if method != 'gpu':
@@ -429,11 +431,12 @@ class FiniteElementAnalysis(FiniteElements):
         KD, b = sysm.prepare(K, fixed, g, f_t)
         M64 = None
         pc = precond
-        if precond == 'learned':
+        if precond in ('learned', 'learned_two_level'):
             if learned is None:
-                raise ValueError("precond='learned' needs a trained LearnedPreconditioner")
-            M64 = learned.preconditioner(sysm, KD, fixed)
-            pc = 'block_jacobi'
+                raise ValueError(f"precond={precond!r} needs a trained LearnedPreconditioner")
+            # inference runs inside the solver's set-up, beside the coarse inverse
+            M64 = lambda: learned.preconditioner(sysm, KD, fixed)  # noqa: E731
+            pc = 'block_jacobi' if precond == 'learned' else 'two_level'
         coarse = sysm.coarse_space(pos) if pc == 'two_level' else None
         solver = sysm.solver(KD, pc, precision, M64=M64, coarse=coarse)
         u, info = solver.solve(b, tol=tol, max_iter=max_iter)

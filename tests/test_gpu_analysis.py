@@ -140,6 +140,34 @@ def test_solve_two_level(ctx, golden, key):
     assert np.linalg.norm(x[:n] - xg[:n]) / np.linalg.norm(xg[:n]) < U_ERR[key]
 
 
+def test_solve_learned_two_level(ctx):
+    """solve(precond='learned_two_level', precision='mixed') with the shipped tet weights (the bench's
+    headline method) on a clamped lognormal-field 8^3 cube, against the oracle's direct solve."""
+    import os
+    from finite_elements_amd.learned import LearnedPreconditioner
+    from oracle import vectorized as vz
+    print('analysis solve learned two-level cube8', flush=True)
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 'finite_elements_amd',
+                        'weights', 'learned_block_jacobi_tet.npz')
+    model = LearnedPreconditioner.load(path)
+    pr = meshgen.cube_elasticity(n=8)
+    am = mesh.ArrayMesh(pr.coords, pr.conn, 'tet_elastic', pr.materials)
+    nodes = am.nodes
+    nl = [loads.NodeLoad(nodes[d // 3], pr.f[d], d % 3 + 1) for d in np.nonzero(pr.f)[0]]
+    bcs = [conditions.NodeBoundaryCondition(nodes[d // 3], 0.0, d % 3 + 1) for d in pr.fixed_dofs]
+    an = FiniteElementAnalysis(am, [], [], nl, [], [], bcs, [], [], plane_strain=True, plane_stress=False)
+    res = an.solve(tol=1e-10, precond='learned_two_level', precision='mixed', learned=model)
+    info = an.last_solve_info
+    print(f'  {info}', flush=True)
+    assert info['status'] == 0 and info['relres'] <= 1e-10
+    K = vz.assemble_csr(pr.kind, pr.coords, pr.conn, pr.materials)
+    ref_u = vz.reduced_solve(K, pr.f, pr.fixed_dofs)
+    x = np.asarray(res.result_vector)[:ref_u.size]
+    assert np.linalg.norm(x - ref_u) / np.linalg.norm(ref_u) < 1e-7
+    with pytest.raises(ValueError):
+        an.solve(tol=1e-10, precond='learned_two_level')
+
+
 def test_element_objects_and_matrix_data(ctx):
     print('element objects', flush=True)
     an = build('ex2')
