@@ -2,7 +2,8 @@
 // modes, its inverse, and the coarse correction u += P A_c^-1 P^T r.
 //
 //   k_coarse_ac       one workgroup per coarse block pair (I, J): the sum over the fine blocks (i, j)
-//                     with agg(i) = I, agg(j) = J of Z_i^T A_ij Z_j, in a fixed order (no atomics)
+//                     with agg(i) = I, agg(j) = J of P_i^T A_ij P_j, in a fixed order (no atomics);
+//                     k_coarse_ac_rm forms the same from x^ and T (T_I^T (sum Z_i^T A_ij Z_j) T_J)
 //   k_gj_*            in-place blocked Gauss-Jordan inversion of the dense SPD A_c (FP64, 64 x 64
 //                     blocks, no pivoting: every pivot block is a Schur complement of an SPD matrix)
 //   k_coarse_restrict one workgroup per aggregate: r_c = P^T r
@@ -87,6 +88,104 @@ __global__ __launch_bounds__(kAcThreads) void k_coarse_ac(fe_coarse C, int64_t n
     double v = 0.0;
 #pragma unroll
     for (int w = 0; w < kAcThreads / 64; ++w) v += red[k][w];
+    // a coarse column that is zero (rank-deficient aggregate) gets a decoupled unit diagonal
+    if (I == J && x == y && v == 0.0) v = 1.0;
+    Ac[((int64_t)I * NC + x) * ld + (int64_t)J * NC + y] = v;
+  }
+}
+
+// Rigid-body modes at a node's scaled coordinates x: column c < D is the translation e_c, the
+// others the rotations omega x x (3D: about x, y, z; 2D: about z).
+template <int D, int NC>
+__device__ inline void rigid_modes_at(const double x[3], double (&Z)[D][NC]) {
+#pragma unroll
+  for (int e = 0; e < D; ++e)
+#pragma unroll
+    for (int c = 0; c < D; ++c) Z[e][c] = (e == c) ? 1.0 : 0.0;
+  if constexpr (D == 3) {
+    Z[0][3] = 0.0, Z[1][3] = -x[2], Z[2][3] = x[1];
+    Z[0][4] = x[2], Z[1][4] = 0.0, Z[2][4] = -x[0];
+    Z[0][5] = -x[1], Z[1][5] = x[0], Z[2][5] = 0.0;
+  } else if constexpr (D == 2) {
+    Z[0][2] = -x[1], Z[1][2] = x[0];
+  }
+}
+
+// A_c = P^T A P with P = Z(x^) T: one workgroup per coarse pair (I, J) sums W = Z_i^T A_ij Z_j over the
+// pair's fine blocks (ascending order; constrained rows and columns of A_ij read as zero), then
+// forms T_I^T W T_J.  Reads the block values and the nodes' x^ and constraint bits, not P.
+template <int D, int NC>
+__global__ __launch_bounds__(kAcThreads) void k_coarse_ac_rm(fe_coarse C, int64_t nnzb,
+                                                             const int32_t* __restrict__ bcol,
+                                                             const double* __restrict__ vals, int64_t ld,
+                                                             double* __restrict__ Ac) {
+  __shared__ double red[NC * NC][kAcThreads / 64];
+  __shared__ double s_W[NC * NC];
+  const int64_t pr = blockIdx.x;
+  const int32_t I = C.pair_ij[2 * pr], J = C.pair_ij[2 * pr + 1];
+  double acc[NC * NC];
+#pragma unroll
+  for (int k = 0; k < NC * NC; ++k) acc[k] = 0.0;
+  for (int32_t t = C.pair_ptr[pr] + threadIdx.x; t < C.pair_ptr[pr + 1]; t += kAcThreads) {
+    const int64_t q = C.pair_q[t];
+    const int64_t i = C.pair_row[t], j = bcol[q];
+    const int bi = C.fixed_bits[i], bj = C.fixed_bits[j];
+    double a[D][D];
+#pragma unroll
+    for (int e = 0; e < D; ++e)
+#pragma unroll
+      for (int f = 0; f < D; ++f)
+        a[e][f] = (((bi >> e) | (bj >> f)) & 1) ? 0.0 : vals[(int64_t)(e * D + f) * nnzb + q];
+    const double xi[3] = {C.xhat[i * 3], C.xhat[i * 3 + 1], C.xhat[i * 3 + 2]};
+    const double xj[3] = {C.xhat[j * 3], C.xhat[j * 3 + 1], C.xhat[j * 3 + 2]};
+    double Zi[D][NC], Zj[D][NC];
+    rigid_modes_at<D, NC>(xi, Zi);
+    rigid_modes_at<D, NC>(xj, Zj);
+    // tz = A_ij Z_j, then acc += Z_i^T tz
+    double tz[D][NC];
+#pragma unroll
+    for (int e = 0; e < D; ++e)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        double v = 0.0;
+#pragma unroll
+        for (int f = 0; f < D; ++f) v += a[e][f] * Zj[f][c];
+        tz[e][c] = v;
+      }
+#pragma unroll
+    for (int x = 0; x < NC; ++x)
+#pragma unroll
+      for (int y = 0; y < NC; ++y) {
+        double v = 0.0;
+#pragma unroll
+        for (int e = 0; e < D; ++e) v += Zi[e][x] * tz[e][y];
+        acc[x * NC + y] += v;
+      }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NC * NC; ++k) {
+    const double v = wave_sum(acc[k]);
+    if (lane == 0) red[k][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NC * NC) {
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < kAcThreads / 64; ++w) v += red[threadIdx.x][w];
+    s_W[threadIdx.x] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NC * NC) {
+    const int x = threadIdx.x / NC, y = threadIdx.x % NC;
+    const double* TI = C.T + (int64_t)I * NC * NC;
+    const double* TJ = C.T + (int64_t)J * NC * NC;
+    double v = 0.0;
+    for (int k = 0; k < NC; ++k) {
+      double wk = 0.0;
+      for (int l = 0; l < NC; ++l) wk += s_W[k * NC + l] * TJ[l * NC + y];
+      v += TI[k * NC + x] * wk;
+    }
     // a coarse column that is zero (rank-deficient aggregate) gets a decoupled unit diagonal
     if (I == J && x == y && v == 0.0) v = 1.0;
     Ac[((int64_t)I * NC + x) * ld + (int64_t)J * NC + y] = v;
@@ -574,10 +673,16 @@ extern "C" int fe_coarse_operator(fe_handle* h, const fe_bsr* A, const fe_coarse
   hipStream_t st = h->stream;
   FE_CHECK_HIP(hipMemsetAsync(Ac, 0, (size_t)ld * ld * sizeof(double), st));
   const double* v = (const double*)A->vals;
+  const bool rm = C->xhat && C->T && C->fixed_bits;
 #define FE_AC(D, NC)                                                                               \
-  hipLaunchKernelGGL((k_coarse_ac<D, NC>), dim3((unsigned)C->n_pairs), dim3(kAcThreads), 0, st, *C, \
-                     A->nnzb,                                                                        \
-                     A->bcol, v, ld, Ac);
+  {                                                                                                \
+    if (rm)                                                                                        \
+      hipLaunchKernelGGL((k_coarse_ac_rm<D, NC>), dim3((unsigned)C->n_pairs), dim3(kAcThreads), 0, st, \
+                         *C, A->nnzb, A->bcol, v, ld, Ac);                                         \
+    else                                                                                           \
+      hipLaunchKernelGGL((k_coarse_ac<D, NC>), dim3((unsigned)C->n_pairs), dim3(kAcThreads), 0, st,   \
+                         *C, A->nnzb, A->bcol, v, ld, Ac);                                         \
+  }
   if (C->dof == 1) FE_AC(1, 1) else if (C->dof == 2) FE_AC(2, 3) else FE_AC(3, 6)
 #undef FE_AC
   if (ld > nct)

@@ -65,6 +65,22 @@ def test_coarse_operator_and_inverse(ctx, case):
     u32 = torch.zeros(A.shape[0], dtype=torch.float32, device=ctx.torch_device)
     dc.apply(dc.operator(KD)[1], ctx.tensor(r).float(), u32)
     assert np.abs(u32.cpu().numpy() - want).max() <= 2e-3 * np.abs(want).max()
+    # the stored-P form of the correction (the moment form's arrays withheld) gives the same result
+    saved = (dc.xhat, dc.xhat32, dc.T, dc.T32)
+    dc.xhat = dc.xhat32 = dc.T = dc.T32 = None
+    try:
+        Ac_p, _ = dc.operator(KD, invert=False)
+        Ac_p = Ac_p.cpu().numpy().reshape(dc.ld, dc.ld)
+        assert np.abs(Ac_p[:n, :n] - Ac[:n, :n]).max() <= 1e-12 * np.abs(ref).max()
+        up = ctx.tensor(np.zeros(A.shape[0]))
+        dc.apply(dc.operator(KD)[0], ctx.tensor(r), up)
+        assert np.abs(up.cpu().numpy() - want).max() <= 1e-10 * np.abs(want).max()
+        assert np.abs(up.cpu().numpy() - u.cpu().numpy()).max() <= 1e-12 * np.abs(want).max()
+        up32 = torch.zeros(A.shape[0], dtype=torch.float32, device=ctx.torch_device)
+        dc.apply(dc.operator(KD)[1], ctx.tensor(r).float(), up32)
+        assert np.abs(up32.cpu().numpy() - u32.cpu().numpy()).max() <= 1e-5 * np.abs(want).max()
+    finally:
+        dc.xhat, dc.xhat32, dc.T, dc.T32 = saved
 
 
 @pytest.mark.parametrize('precision', ['fp64', 'mixed'])
