@@ -536,28 +536,37 @@ __device__ void jacobi_eig(double* A, double* V, int r, int max_sweeps) {
         s_s[tid] = s;
       }
       __syncthreads();
-      for (int idx = tid; idx < npair * r; idx += nth) {
-        const int k = idx / r, j = idx % r;
-        const int qq = s_q[k];
-        const double s = s_s[k];
-        if (qq >= r || s == 0.0) continue;
-        const int pp = s_p[k];
-        const double c = s_c[k];
-        const double ap = A[pp * r + j], aq = A[qq * r + j];
-        A[pp * r + j] = c * ap - s * aq;
-        A[qq * r + j] = s * ap + c * aq;
+      // A <- J^T A J in one pass: each 2 x 2 block (rows {p_k, q_k}, columns {p_l, q_l}) gets the row
+      // rotation of pair k, then the column rotation of pair l (the same operations, in the same
+      // order, as rotating all rows and then all columns); V <- V J alongside
+      for (int idx = tid; idx < 64 * npair; idx += nth) {
+        const int k = idx >> 6, l = idx & 63;
+        if (l >= npair) continue;
+        const int p1 = s_p[k], q1 = s_q[k], p2 = s_p[l], q2 = s_q[l];
+        const bool hq1 = q1 < r, hq2 = q2 < r;
+        const double c1 = s_c[k], s1 = s_s[k], c2 = s_c[l], s2 = s_s[l];
+        const double a_pp = A[p1 * r + p2], a_pq = hq2 ? A[p1 * r + q2] : 0.0;
+        const double a_qp = hq1 ? A[q1 * r + p2] : 0.0;
+        const double a_qq = (hq1 && hq2) ? A[q1 * r + q2] : 0.0;
+        // rows
+        const double r_pp = c1 * a_pp - s1 * a_qp, r_pq = c1 * a_pq - s1 * a_qq;
+        const double r_qp = s1 * a_pp + c1 * a_qp, r_qq = s1 * a_pq + c1 * a_qq;
+        // columns
+        A[p1 * r + p2] = c2 * r_pp - s2 * r_pq;
+        if (hq2) A[p1 * r + q2] = s2 * r_pp + c2 * r_pq;
+        if (hq1) {
+          A[q1 * r + p2] = c2 * r_qp - s2 * r_qq;
+          if (hq2) A[q1 * r + q2] = s2 * r_qp + c2 * r_qq;
+        }
       }
-      __syncthreads();
-      for (int idx = tid; idx < npair * r; idx += nth) {
-        const int k = idx / r, i = idx % r;
+      for (int idx = tid; idx < 128 * npair; idx += nth) {
+        const int k = idx >> 7, i = idx & 127;
+        if (i >= r) continue;
         const int qq = s_q[k];
         const double s = s_s[k];
         if (qq >= r || s == 0.0) continue;
         const int pp = s_p[k];
         const double c = s_c[k];
-        const double ap = A[i * r + pp], aq = A[i * r + qq];
-        A[i * r + pp] = c * ap - s * aq;
-        A[i * r + qq] = s * ap + c * aq;
         const double vp = V[i * r + pp], vq = V[i * r + qq];
         V[i * r + pp] = c * vp - s * vq;
         V[i * r + qq] = s * vp + c * vq;
