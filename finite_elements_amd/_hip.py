@@ -83,7 +83,9 @@ class FeCoarse(ctypes.Structure):
     _fields_ = [('n_nodes', _i64), ('n_agg', _i64), ('n_pairs', _i64), ('dof', _i32), ('nc', _i32),
                 ('node_agg', _vp), ('agg_ptr', _vp), ('agg_nodes', _vp), ('pair_ptr', _vp),
                 ('pair_ij', _vp), ('pair_q', _vp), ('pair_row', _vp), ('P', _vp), ('P32', _vp),
-                ('xhat', _vp), ('xhat32', _vp), ('fixed_bits', _vp), ('T', _vp), ('T32', _vp)]
+                ('xhat', _vp), ('xhat32', _vp), ('fixed_bits', _vp), ('T', _vp), ('T32', _vp),
+                ('n_seg', _i32), ('max_block', _i32), ('blk_total', _i64), ('agg_seg', _vp),
+                ('seg_of_agg', _vp), ('blk_ofs', _vp)]
 
 
 class FeLobpcgOpts(ctypes.Structure):
@@ -139,6 +141,10 @@ SIGNATURES = {
     'fe_pcg_coarse_workspace': (_int, [_vp, _P(FeBsr), _P(FeCoarse), _P(FePcgOpts), _P(_i64)]),
     'fe_pcg_coarse': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp, _vp, _P(FeCoarse), _vp, _vp, _vp, _vp,
                              _vp, _i64, _P(FePcgOpts), _P(FePcgStats)]),
+    'fe_pcg_batched_coarse_workspace': (_int, [_vp, _P(FeBsr), _i32, _vp, _P(FeCoarse), _P(FePcgOpts),
+                                               _P(_i64)]),
+    'fe_pcg_batched_coarse': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp, _vp, _i32, _vp, _P(FeCoarse), _vp, _vp,
+                                     _vp, _vp, _vp, _i64, _P(FePcgOpts), _P(FePcgStats), _vp]),
     'fe_lobpcg_workspace': (_int, [_vp, _P(FeBsr), _P(FeLobpcgOpts), _P(_i64)]),
     'fe_lobpcg': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64,
                          _P(FeLobpcgOpts), _P(FeLobpcgStats)]),

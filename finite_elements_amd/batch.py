@@ -81,16 +81,24 @@ class MeshBatch:
         KD, b = s.prepare(K, fixed, g, self._f)
         return K, KD, b
 
+    def coarse_space(self, max_block: int = 120):
+        """Block-diagonal coarse space of the batch: aggregated rigid-body modes, one coarse block of
+        at most max_block DOF per mesh (built once; the batched two-level solver's)."""
+        return self.sysm.coarse_space(self.fixed_dofs, node_offsets=self.node_offsets, max_block=max_block)
+
     def solve(self, tol: float = 1e-8, precision: str = 'fp64', precond: str = 'block_jacobi',
               graph: bool = True, max_iter: Optional[int] = None, check_every: int = 16):
         """Assemble and solve every mesh; returns (u on the device, info with per-mesh arrays).
 
-        precision 'mixed' iterates in FP32 inside FP64 reliable updates and finishes every mesh whose
-        FP32 iteration fails (long thin plates: condition number times the FP32 unit roundoff near
-        one) with FP64 PCG; 'fp64' iterates in FP64 throughout."""
+        precond 'block_jacobi', 'jacobi' or 'two_level' (block Jacobi plus each mesh's own coarse
+        correction over its aggregated rigid-body modes).  precision 'mixed' iterates in FP32 inside
+        FP64 reliable updates and finishes every mesh whose FP32 iteration fails (with block Jacobi:
+        long thin plates, whose condition number times the FP32 unit roundoff is near one) with FP64
+        PCG; 'fp64' iterates in FP64 throughout."""
         K, KD, b = self.assemble()
+        coarse = self.coarse_space() if precond == 'two_level' else None
         solver = self.sysm.solver(KD, precond, precision, segments=self.node_offsets, graph=graph,
-                                  check_every=check_every)
+                                  check_every=check_every, coarse=coarse)
         u, info = solver.solve(b, tol=tol, max_iter=max_iter)
         return u, info
 

@@ -82,7 +82,11 @@ class CoarseSpace:
 
     def __init__(self, coords: np.ndarray, bptr: np.ndarray, bcol: np.ndarray, dof: int,
                  fixed_dofs: Optional[np.ndarray] = None, target: Optional[int] = None,
-                 node_offsets: Optional[np.ndarray] = None):
+                 node_offsets: Optional[np.ndarray] = None, max_block: Optional[int] = None):
+        """node_offsets: node ranges of independent meshes (aggregates never cross them).  max_block:
+        with node_offsets, a block-diagonal space (one coarse block per range, for the batched
+        solver) whose blocks have at most max_block coarse DOF; a range's aggregates are enlarged
+        beyond ``target`` nodes until its block fits."""
         coords = np.asarray(coords, dtype=np.float64)
         n = coords.shape[0]
         self.dof = dof
@@ -94,12 +98,29 @@ class CoarseSpace:
         offs = np.array([0, n]) if node_offsets is None else np.asarray(node_offsets, dtype=np.int64)
         agg = np.empty(n, dtype=np.int64)
         base = 0
+        agg_seg = [0]
         for a, b in zip(offs[:-1], offs[1:]):
             if b > a:
-                loc = spatial_aggregates(coords[a:b], target)
+                t = target
+                loc = spatial_aggregates(coords[a:b], t)
+                while max_block is not None and (int(loc.max()) + 1) * self.nc > max_block:
+                    t = int(t * 1.25) + 1
+                    loc = spatial_aggregates(coords[a:b], t)
                 agg[a:b] = loc + base
                 base += int(loc.max()) + 1
+            agg_seg.append(base)
         self.n_agg = base
+        # block-diagonal layout (batched solver): aggregates of range s are [agg_seg[s], agg_seg[s+1])
+        if max_block is not None and node_offsets is not None:
+            self.agg_seg = np.asarray(agg_seg, dtype=np.int32)
+            counts = np.diff(self.agg_seg).astype(np.int64) * self.nc
+            self.seg_of_agg = np.repeat(np.arange(counts.size, dtype=np.int32), np.diff(self.agg_seg))
+            self.blk_ofs = np.zeros(counts.size + 1, dtype=np.int64)
+            self.blk_ofs[1:] = np.cumsum(counts * counts)
+            self.max_block = int(counts.max()) if counts.size else 0
+        else:
+            self.agg_seg = self.seg_of_agg = self.blk_ofs = None
+            self.max_block = 0
         self.node_agg = agg.astype(np.int32)
         order = np.argsort(agg, kind='stable')
         self.agg_nodes = order.astype(np.int32)

@@ -188,7 +188,16 @@ __global__ __launch_bounds__(kAcThreads) void k_coarse_ac_rm(fe_coarse C, int64_
     }
     // a coarse column that is zero (rank-deficient aggregate) gets a decoupled unit diagonal
     if (I == J && x == y && v == 0.0) v = 1.0;
-    Ac[((int64_t)I * NC + x) * ld + (int64_t)J * NC + y] = v;
+    if (C.n_seg) {
+      // block-diagonal space: the pair's segment block, row-major n_s x n_s (a pair across two
+      // segments would be a coupling the space does not represent: it is dropped)
+      const int sg = C.seg_of_agg[I], a0 = C.agg_seg[sg];
+      const int64_t ns = (int64_t)NC * (C.agg_seg[sg + 1] - a0);
+      if (C.seg_of_agg[J] == sg)
+        Ac[C.blk_ofs[sg] + ((int64_t)(I - a0) * NC + x) * ns + (int64_t)(J - a0) * NC + y] = v;
+    } else {
+      Ac[((int64_t)I * NC + x) * ld + (int64_t)J * NC + y] = v;
+    }
   }
 }
 
@@ -368,6 +377,42 @@ __global__ __launch_bounds__(kGjThreads) void k_gj_update(int64_t ld, int k, con
   for (int q = 0; q < 16; ++q) Snext[(g * 16 + q) * kNB + c] = t[q];
 }
 
+// Block-diagonal space: one workgroup per segment inverts its block (n_s <= 128, SPD, no pivoting) by
+// in-place Gauss-Jordan in LDS, then writes the symmetrised inverse and its FP32 copy.
+constexpr int kSegMax = 128;
+constexpr int kSegThreads = 1024;
+
+__global__ __launch_bounds__(kSegThreads) void k_coarse_inv_seg(fe_coarse C, double* __restrict__ A,
+                                                               float* __restrict__ A32) {
+  extern __shared__ double sA[];  // [n][n]
+  __shared__ double s_row[kSegMax], s_col[kSegMax];
+  const int sg = blockIdx.x;
+  const int n = C.nc * (C.agg_seg[sg + 1] - C.agg_seg[sg]);
+  if (n == 0) return;
+  double* Ag = A + C.blk_ofs[sg];
+  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) sA[idx] = Ag[idx];
+  __syncthreads();
+  for (int p = 0; p < n; ++p) {
+    const double piv = 1.0 / sA[p * n + p];
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+      s_row[j] = (j == p ? 1.0 : sA[p * n + j]) * piv;
+      s_col[j] = sA[j * n + p];
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
+      const int i = idx / n, j = idx - i * n;
+      sA[idx] = (i == p) ? s_row[j] : (j == p ? 0.0 : sA[idx]) - s_col[i] * s_row[j];
+    }
+    __syncthreads();
+  }
+  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
+    const int i = idx / n, j = idx - i * n;
+    const double v = 0.5 * (sA[idx] + sA[j * n + i]);
+    Ag[idx] = v;
+    if (A32) A32[C.blk_ofs[sg] + idx] = (float)v;
+  }
+}
+
 // Symmetrise the inverse (average with its transpose) and make the FP32 copy.
 __global__ void k_sym_cast(int64_t n, int64_t ld, double* __restrict__ A, float* __restrict__ A32) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -470,6 +515,29 @@ __global__ __launch_bounds__(256) void k_coarse_gemv(int64_t n, int64_t ld, cons
   if (lane == 0) y[row] = (T)acc;
 }
 
+// Block-diagonal space: z_s = A_s^-1 r_s, one workgroup per segment; thread i forms row i from column i
+// of the symmetric block (coalesced over i).
+template <typename T>
+__global__ __launch_bounds__(kSegMax) void k_coarse_gemv_seg(fe_coarse C, const T* __restrict__ Ainv,
+                                                            const T* __restrict__ x, T* __restrict__ y,
+                                                            const int32_t* state, int state_stride,
+                                                            int mode) {
+  __shared__ T sx[kSegMax];
+  const int sg = blockIdx.x;
+  if (state && !coarse_active(state + (int64_t)sg * state_stride, mode)) return;  // uniform
+  const int a0 = C.agg_seg[sg];
+  const int n = C.nc * (C.agg_seg[sg + 1] - a0);
+  const int64_t c0 = (int64_t)a0 * C.nc;
+  if (threadIdx.x < n) sx[threadIdx.x] = x[c0 + threadIdx.x];
+  __syncthreads();
+  const int i = threadIdx.x;
+  if (i >= n) return;
+  const T* Ag = Ainv + C.blk_ofs[sg];
+  double acc = 0.0;
+  for (int j = 0; j < n; ++j) acc += (double)Ag[(int64_t)j * n + i] * (double)sx[j];
+  y[c0 + i] = (T)acc;
+}
+
 // Rigid-body moments of a nodal vector r (constrained DOFs read as zero) at scaled coordinates x:
 // the columns of Z are the translations and the rotations omega x x about the axes.
 template <int D>
@@ -496,14 +564,18 @@ __global__ __launch_bounds__(kRestrictThreads) void k_coarse_restrict_rm(fe_coar
                                                                          const T* __restrict__ Tm,
                                                                          const T* __restrict__ r,
                                                                          T* __restrict__ rc,
-                                                                         const int32_t* state, int mode) {
+                                                                         const int32_t* state,
+                                                                         int state_stride, int mode) {
   __shared__ double red[NC][kRestrictThreads / 64];
   __shared__ double s_w[NC];
   __shared__ int s_skip;
-  if (threadIdx.x == 0) s_skip = state && !coarse_active(state, mode);
+  const int g = blockIdx.x;
+  if (threadIdx.x == 0) {
+    const int64_t sg = C.n_seg ? C.seg_of_agg[g] : 0;  // the aggregate's segment
+    s_skip = state && !coarse_active(state + sg * state_stride, mode);
+  }
   __syncthreads();
   if (s_skip) return;
-  const int g = blockIdx.x;
   double acc[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) acc[c] = 0.0;
@@ -544,14 +616,18 @@ template <typename T, int D, int NC>
 __global__ __launch_bounds__(256) void k_coarse_prolong_rm(fe_coarse C, const T* __restrict__ xh,
                                                            const T* __restrict__ Tm,
                                                            const T* __restrict__ z, T* __restrict__ u,
-                                                           const int32_t* state, int mode) {
+                                                           const int32_t* state, int state_stride,
+                                                           int mode) {
   __shared__ int s_skip;
-  if (threadIdx.x == 0) s_skip = state && !coarse_active(state, mode);
+  // one system: one state for the workgroup; a batch: the node's segment (ranges of a workgroup may
+  // belong to several)
+  if (threadIdx.x == 0) s_skip = !C.n_seg && state && !coarse_active(state, mode);
   __syncthreads();
   if (s_skip) return;
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= C.n_nodes) return;
   const int64_t g = C.node_agg[i];
+  if (C.n_seg && state && !coarse_active(state + (int64_t)C.seg_of_agg[g] * state_stride, mode)) return;
   const T* Tg = Tm + g * NC * NC;
   const T* zg = z + g * NC;
   T y[NC];
@@ -608,8 +684,17 @@ int check_coarse(const fe_coarse* C, const char* who) {
   FE_REQUIRE(C->node_agg && C->agg_ptr && C->agg_nodes && C->pair_ptr && C->pair_ij && C->pair_q &&
                  C->pair_row && C->P,
              "%s: null coarse array", who);
-  FE_REQUIRE(C->n_agg * C->nc <= 16384, "%s: coarse dimension %lld above 16384", who,
-             (long long)(C->n_agg * C->nc));
+  if (C->n_seg) {
+    FE_REQUIRE(C->n_seg > 0 && C->agg_seg && C->seg_of_agg && C->blk_ofs && C->xhat && C->T &&
+                   C->fixed_bits,
+               "%s: a block-diagonal coarse space needs agg_seg, seg_of_agg, blk_ofs, xhat, T and "
+               "fixed_bits", who);
+    FE_REQUIRE(C->max_block >= 1 && C->max_block <= kSegMax && C->blk_total >= 0,
+               "%s: coarse blocks of 1..%d rows expected", who, kSegMax);
+  } else {
+    FE_REQUIRE(C->n_agg * C->nc <= 16384, "%s: coarse dimension %lld above 16384", who,
+               (long long)(C->n_agg * C->nc));
+  }
   return FE_OK;
 }
 
@@ -619,7 +704,7 @@ int check_coarse(const fe_coarse* C, const char* who) {
 // state: the segment's {done, need_ru} words (or null); mode as in coarse_active.
 int fe_coarse_correction(hipStream_t stream, const fe_coarse* C, int fp32, const void* Ainv,
                          const void* r, void* u, void* rc, void* zc, const int32_t* state,
-                         int mode) {
+                         int state_stride, int mode) {
   const int64_t nct = C->n_agg * C->nc, ld = fe_coarse_ld(nct);
   const dim3 gr((unsigned)C->n_agg), gg((unsigned)ceil_div(nct, 4)),
       gp((unsigned)ceil_div(C->n_nodes * C->dof, 256));
@@ -632,15 +717,19 @@ int fe_coarse_correction(hipStream_t stream, const fe_coarse* C, int fp32, const
     const V* Tm = fp32 ? (const V*)C->T32 : (const V*)C->T;                                        \
     if (rm)                                                                                        \
       hipLaunchKernelGGL((k_coarse_restrict_rm<V, D, NC>), gr, dim3(kRestrictThreads), 0, stream, *C, \
-                         xh, Tm, (const V*)r, (V*)rc, state, mode);                                \
+                         xh, Tm, (const V*)r, (V*)rc, state, state_stride, mode);                  \
     else                                                                                           \
       hipLaunchKernelGGL((k_coarse_restrict<V, D, NC>), gr, dim3(kRestrictThreads), 0, stream, *C, P, \
                          (const V*)r, (V*)rc, state, mode);                                        \
-    hipLaunchKernelGGL((k_coarse_gemv<V>), gg, dim3(256), 0, stream, nct, ld, (const V*)Ainv,       \
-                       (const V*)rc, (V*)zc, state, mode);                                         \
+    if (C->n_seg)                                                                                  \
+      hipLaunchKernelGGL((k_coarse_gemv_seg<V>), dim3((unsigned)C->n_seg), dim3(kSegMax), 0, stream,  \
+                         *C, (const V*)Ainv, (const V*)rc, (V*)zc, state, state_stride, mode);     \
+    else                                                                                           \
+      hipLaunchKernelGGL((k_coarse_gemv<V>), gg, dim3(256), 0, stream, nct, ld, (const V*)Ainv,     \
+                         (const V*)rc, (V*)zc, state, mode);                                       \
     if (rm)                                                                                        \
       hipLaunchKernelGGL((k_coarse_prolong_rm<V, D, NC>), gpn, dim3(256), 0, stream, *C, xh, Tm,      \
-                         (const V*)zc, (V*)u, state, mode);                                        \
+                         (const V*)zc, (V*)u, state, state_stride, mode);                          \
     else                                                                                           \
       hipLaunchKernelGGL((k_coarse_prolong<V, D, NC>), gp, dim3(256), 0, stream, *C, P, (const V*)zc, \
                          (V*)u, state, mode);                                                      \
@@ -668,10 +757,10 @@ extern "C" int fe_coarse_operator(fe_handle* h, const fe_bsr* A, const fe_coarse
              "fe_coarse_operator: A must be FP64 in FE_LAYOUT_PLANES");
   FE_REQUIRE(A->dof == C->dof && A->n_brows == C->n_nodes, "fe_coarse_operator: A and C disagree");
   FE_REQUIRE(A->row_align == 0, "fe_coarse_operator: the pattern of the coarse pair map expected");
-  FE_REQUIRE(Ac && (!invert || scratch), "fe_coarse_operator: null output");
+  FE_REQUIRE(Ac && (!invert || scratch || C->n_seg), "fe_coarse_operator: null output");
   const int64_t nct = C->n_agg * C->nc, ld = fe_coarse_ld(nct);
   hipStream_t st = h->stream;
-  FE_CHECK_HIP(hipMemsetAsync(Ac, 0, (size_t)ld * ld * sizeof(double), st));
+  FE_CHECK_HIP(hipMemsetAsync(Ac, 0, (size_t)(C->n_seg ? C->blk_total : ld * ld) * sizeof(double), st));
   const double* v = (const double*)A->vals;
   const bool rm = C->xhat && C->T && C->fixed_bits;
 #define FE_AC(D, NC)                                                                               \
@@ -685,6 +774,19 @@ extern "C" int fe_coarse_operator(fe_handle* h, const fe_bsr* A, const fe_coarse
   }
   if (C->dof == 1) FE_AC(1, 1) else if (C->dof == 2) FE_AC(2, 3) else FE_AC(3, 6)
 #undef FE_AC
+  if (C->n_seg) {
+    // block-diagonal: one workgroup per segment inverts its block in LDS
+    FE_CHECK_LAUNCH();
+    if (!invert) return FE_OK;
+    const size_t lds = (size_t)C->max_block * C->max_block * sizeof(double);
+    if (lds > 64 * 1024)
+      FE_CHECK_HIP(hipFuncSetAttribute((const void*)k_coarse_inv_seg,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_coarse_inv_seg, dim3((unsigned)C->n_seg), dim3(kSegThreads), lds, st, *C, Ac,
+                       Ac32);
+    FE_CHECK_LAUNCH();
+    return FE_OK;
+  }
   if (ld > nct)
     hipLaunchKernelGGL(k_identity_pad, dim3((unsigned)ceil_div(ld - nct, 64)), dim3(64), 0, st, nct,
                        ld, Ac);
@@ -724,5 +826,5 @@ extern "C" int fe_coarse_apply(fe_handle* h, const fe_coarse* C, int precision, 
   char* w = (char*)work;
   void* rcv = w;
   void* zcv = w + ((nct * el + 255) / 256) * 256;
-  return fe_coarse_correction(h->stream, C, precision, Ainv, r, u, rcv, zcv, nullptr, 2);
+  return fe_coarse_correction(h->stream, C, precision, Ainv, r, u, rcv, zcv, nullptr, 0, 2);
 }

@@ -21,10 +21,12 @@ struct fe_handle {
 int fe_set_error(int code, const char* fmt, ...);
 
 // Coarse correction u += P Ainv P^T r on `stream` (fe_coarse.hip); rc, zc: n_agg * nc scratch values.
-// state (may be null): the {done, need_ru} words of the solve; mode 0 runs while it iterates, 1 during
-// a reliable update, 2 always.
+// state (may be null): the {done, need_ru} words of the solve's first segment, state_stride int32
+// apart for the next segments (block-diagonal space); mode 0 runs while a segment iterates, 1 during
+// its reliable update, 2 always.
 int fe_coarse_correction(hipStream_t stream, const fe_coarse* C, int fp32, const void* Ainv,
-                         const void* r, void* u, void* rc, void* zc, const int32_t* state, int mode);
+                         const void* r, void* u, void* rc, void* zc, const int32_t* state,
+                         int state_stride, int mode);
 
 #define FE_CHECK_HIP(call)                                                              \
   do {                                                                                  \

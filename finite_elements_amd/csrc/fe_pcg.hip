@@ -54,6 +54,10 @@ struct Tile {
   int32_t seg, row0, row1, pad;
 };
 
+// int32 words between consecutive segments' states (the coarse correction's per-segment checks)
+constexpr int kStateStride = (int)(sizeof(PcgState) / sizeof(int32_t));
+static_assert(sizeof(PcgState) % sizeof(int32_t) == 0, "PcgState is whole int32 words");
+
 constexpr int kStallLimit = 10;
 constexpr int kMaxRestarts32 = 3;   // FP32: more restarts than this and the segment goes to FP64
 constexpr int kMaxRestarts64 = 50;
@@ -678,7 +682,7 @@ int run_pcg(fe_handle* h, const Problem& pb, const Work& w, const fe_pcg_opts* o
     hipLaunchKernelGGL((k_ru_prep<T, D, PC>), gt, bvec, 0, q, w.tiles, w.r64, r, u, p, sv, M, w.st);
     if (pb.C) {
       int rc3 = fe_coarse_correction(q, pb.C, (int)(sizeof(T) == 4), pb.Ainv, r, u, w.rc, w.zc,
-                                     &w.st[0].done, 1);
+                                     &w.st[0].done, kStateStride, 1);
       if (rc3) return rc3;
     }
     hipLaunchKernelGGL((k_ru_spmv<T, D, G>), gt, dim3(kBlock), 0, q, w.tiles, nt, nnzb_it, bptr_it,
@@ -714,7 +718,7 @@ int run_pcg(fe_handle* h, const Problem& pb, const Work& w, const fe_pcg_opts* o
       hipLaunchKernelGGL((k_cg_vec<T, D, PC>), gt, bvec, 0, cs, w.tiles, pb.x, p, sv, r, u, wv, M, w.st);
       if (pb.C) {
         int rc3 = fe_coarse_correction(cs, pb.C, (int)(sizeof(T) == 4), pb.Ainv, r, u, w.rc, w.zc,
-                                       &w.st[0].done, 0);
+                                       &w.st[0].done, kStateStride, 0);
         if (rc3) return rc3;
       }
       launch_spmv(cs, 0);
@@ -1045,6 +1049,7 @@ extern "C" int fe_pcg_coarse(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32,
   int rc = check_pcg_args(A64, A32, M64, M32, o, "fe_pcg_coarse");
   if (rc) return rc;
   FE_REQUIRE(C->dof == A64->dof && C->n_nodes == A64->n_brows, "fe_pcg_coarse: coarse space and A disagree");
+  FE_REQUIRE(C->n_seg == 0, "fe_pcg_coarse: a block-diagonal coarse space goes with fe_pcg_batched_coarse");
   FE_REQUIRE(o->mixed ? (Ainv32 && C->P32) : (Ainv64 != nullptr),
              "fe_pcg_coarse: missing coarse inverse (FP32 inverse and P32 for mixed precision)");
   FE_REQUIRE(A64->n_brows < ((int64_t)1 << 31), "fe_pcg_coarse: too many node rows");
@@ -1053,4 +1058,40 @@ extern "C" int fe_pcg_coarse(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32,
   const int32_t seg[2] = {0, (int32_t)A64->n_brows};
   return pcg_common(h, A64, A32, M64, M32, 1, seg, b, x, work, work_bytes, o, s, nullptr,
                     "fe_pcg_coarse", C, Ainv64, Ainv32);
+}
+
+extern "C" int fe_pcg_batched_coarse_workspace(fe_handle* h, const fe_bsr* A, int32_t n_seg,
+                                               const int32_t* seg_nodes, const fe_coarse* C,
+                                               const fe_pcg_opts* opts, int64_t* bytes) {
+  FE_ENTER(h);
+  FE_REQUIRE(A && C && opts && bytes, "fe_pcg_batched_coarse_workspace: null argument");
+  FE_REQUIRE(A->dof >= 1 && A->dof <= 3, "fe_pcg_batched_coarse_workspace: dof must be 1..3");
+  int rc = check_segments(A, n_seg, seg_nodes, "fe_pcg_batched_coarse_workspace");
+  if (rc) return rc;
+  const int64_t ntiles = count_tiles(n_seg, seg_nodes, tile_rows_dof(A->dof));
+  *bytes = layout_work(nullptr, A->n_brows, A->dof, (int)sizeof(double), n_seg, ntiles, C->n_agg * C->nc)
+               .bytes;
+  return FE_OK;
+}
+
+extern "C" int fe_pcg_batched_coarse(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M64,
+                                     const void* M32, int32_t n_seg, const int32_t* seg_nodes,
+                                     const fe_coarse* C, const void* Ainv64, const void* Ainv32,
+                                     const double* b, double* x, void* work, int64_t work_bytes,
+                                     const fe_pcg_opts* o, fe_pcg_stats* s, fe_pcg_seg_stats* seg_stats) {
+  FE_ENTER(h);
+  FE_REQUIRE(b && x && work && o && s && C, "fe_pcg_batched_coarse: null argument");
+  int rc = check_pcg_args(A64, A32, M64, M32, o, "fe_pcg_batched_coarse");
+  if (rc) return rc;
+  rc = check_segments(A64, n_seg, seg_nodes, "fe_pcg_batched_coarse");
+  if (rc) return rc;
+  FE_REQUIRE(C->dof == A64->dof && C->n_nodes == A64->n_brows,
+             "fe_pcg_batched_coarse: coarse space and A disagree");
+  FE_REQUIRE(C->n_seg == n_seg && C->agg_seg && C->seg_of_agg && C->blk_ofs && C->xhat && C->T &&
+                 C->fixed_bits && (!o->mixed || (C->xhat32 && C->T32)),
+             "fe_pcg_batched_coarse: a block-diagonal coarse space with one block per segment expected");
+  FE_REQUIRE(o->mixed ? (Ainv32 != nullptr) : (Ainv64 != nullptr),
+             "fe_pcg_batched_coarse: missing coarse inverse (FP32 for mixed precision)");
+  return pcg_common(h, A64, A32, M64, M32, n_seg, seg_nodes, b, x, work, work_bytes, o, s, seg_stats,
+                    "fe_pcg_batched_coarse", C, Ainv64, Ainv32);
 }

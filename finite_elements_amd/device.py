@@ -568,6 +568,17 @@ class DeviceCoarse:
         self.T32 = _to_f32(ctx, self.T)
         self.n_coarse = cs.n_coarse
         self.ld = int(_hip.load().fe_coarse_ld(cs.n_coarse))
+        # block-diagonal space of a batch: one dense coarse block per mesh
+        if cs.agg_seg is not None:
+            self.n_seg = cs.agg_seg.size - 1
+            self.agg_seg = t(cs.agg_seg)
+            self.seg_of_agg = t(cs.seg_of_agg if cs.seg_of_agg.size else np.zeros(1, np.int32))
+            self.blk_ofs = t(cs.blk_ofs)
+            self.blk_total = int(cs.blk_ofs[-1])
+        else:
+            self.n_seg = 0
+            self.agg_seg = self.seg_of_agg = self.blk_ofs = None
+            self.blk_total = 0
 
     def struct(self) -> _hip.FeCoarse:
         cs = self.host
@@ -575,16 +586,19 @@ class DeviceCoarse:
                              _vp(self.agg_ptr), _vp(self.agg_nodes), _vp(self.pair_ptr), _vp(self.pair_ij),
                              _vp(self.pair_q), _vp(self.pair_row), _vp(self.P), _vp(self.P32),
                              _vp(self.xhat), _vp(self.xhat32), _vp(self.fixed_bits), _vp(self.T),
-                             _vp(self.T32))
+                             _vp(self.T32), self.n_seg, self.host.max_block, self.blk_total,
+                             _vp(self.agg_seg), _vp(self.seg_of_agg), _vp(self.blk_ofs))
 
     def operator(self, A64: BsrMatrix, invert: bool = True, ctx: Optional[Context] = None):
         """(A_c or A_c^-1 [ld][ld] FP64, FP32 copy of the inverse or None) for the FP64 plane matrix A64,
         computed on ``ctx``'s stream (default: the coarse space's context)."""
         ctx = self.ctx if ctx is None else ctx
-        Ac = torch.empty(self.ld * self.ld, dtype=torch.float64, device=ctx.torch_device)
-        A32 = torch.empty(self.ld * self.ld, dtype=torch.float32, device=ctx.torch_device) if invert else None
+        # one system: [ld][ld]; a batch: the per-mesh blocks (blk_total values, no scratch)
+        size = self.blk_total if self.n_seg else self.ld * self.ld
+        Ac = torch.empty(max(size, 1), dtype=torch.float64, device=ctx.torch_device)
+        A32 = torch.empty(max(size, 1), dtype=torch.float32, device=ctx.torch_device) if invert else None
         scratch = torch.empty(self.ld * self.ld + 2 * 64 * 64, dtype=torch.float64,
-                              device=ctx.torch_device) if invert else None
+                              device=ctx.torch_device) if (invert and not self.n_seg) else None
         a = A64.struct()
         c = self.struct()
         _hip.call('fe_coarse_operator', ctx.h, ctypes.byref(a), ctypes.byref(c), int(invert), _vp(Ac),
@@ -633,8 +647,10 @@ class PcgSolver:
         self.coarse = coarse
         self.Ainv64 = self.Ainv32 = None
         if coarse is not None:
-            if segments is not None:
-                raise ValueError('the two-level preconditioner is for one system (no segments)')
+            nseg = 1 if segments is None else len(segments) - 1
+            if (segments is None) != (coarse.n_seg == 0) or (coarse.n_seg and coarse.n_seg != nseg):
+                raise ValueError('a batched solve needs the block-diagonal coarse space of its meshes '
+                                 '(one block per segment), a single system the plain one')
             # the coarse operator and its inverse (part of the preconditioner set-up of every
             # solver) run on the side stream, beside the block preconditioner and the iteration
             # matrix built below on the main stream
@@ -698,15 +714,19 @@ class PcgSolver:
         a32 = self.A32.struct() if self.A32 is not None else None
         nbytes = ctypes.c_int64()
         cstruct = self.coarse.struct() if self.coarse is not None else None
-        if cstruct is not None:
+        if self.segments is not None:
+            nseg = self.segments.size - 1
+            segp = self.segments.ctypes.data_as(ctypes.c_void_p)
+        if cstruct is not None and self.segments is None:
             _hip.call('fe_pcg_coarse_workspace', ctx.h, ctypes.byref(a64), ctypes.byref(cstruct),
                       ctypes.byref(opts), ctypes.byref(nbytes))
+        elif cstruct is not None:
+            _hip.call('fe_pcg_batched_coarse_workspace', ctx.h, ctypes.byref(a64), nseg, segp,
+                      ctypes.byref(cstruct), ctypes.byref(opts), ctypes.byref(nbytes))
         elif self.segments is None:
             _hip.call('fe_pcg_workspace', ctx.h, ctypes.byref(a64), ctypes.byref(opts),
                       ctypes.byref(nbytes))
         else:
-            nseg = self.segments.size - 1
-            segp = self.segments.ctypes.data_as(ctypes.c_void_p)
             _hip.call('fe_pcg_batched_workspace', ctx.h, ctypes.byref(a64), nseg, segp,
                       ctypes.byref(opts), ctypes.byref(nbytes))
         if self._work is None or self._work.numel() < nbytes.value:
@@ -714,7 +734,7 @@ class PcgSolver:
         stats = _hip.FePcgStats()
         a32p = ctypes.byref(a32) if a32 is not None else None
         seg_info = None
-        if cstruct is not None:
+        if cstruct is not None and self.segments is None:
             _hip.call('fe_pcg_coarse', ctx.h, ctypes.byref(a64), a32p, _vp(self.M64), _vp(self.M32),
                       ctypes.byref(cstruct), _vp(self.Ainv64), _vp(self.Ainv32), _vp(b), _vp(x),
                       _vp(self._work), self._work.numel(), ctypes.byref(opts), ctypes.byref(stats))
@@ -724,9 +744,15 @@ class PcgSolver:
                       ctypes.byref(stats))
         else:
             seg_stats = (_hip.FePcgSegStats * nseg)()
-            _hip.call('fe_pcg_batched', ctx.h, ctypes.byref(a64), a32p, _vp(self.M64), _vp(self.M32),
-                      nseg, segp, _vp(b), _vp(x), _vp(self._work), self._work.numel(),
-                      ctypes.byref(opts), ctypes.byref(stats), ctypes.cast(seg_stats, ctypes.c_void_p))
+            if cstruct is not None:
+                _hip.call('fe_pcg_batched_coarse', ctx.h, ctypes.byref(a64), a32p, _vp(self.M64),
+                          _vp(self.M32), nseg, segp, ctypes.byref(cstruct), _vp(self.Ainv64),
+                          _vp(self.Ainv32), _vp(b), _vp(x), _vp(self._work), self._work.numel(),
+                          ctypes.byref(opts), ctypes.byref(stats), ctypes.cast(seg_stats, ctypes.c_void_p))
+            else:
+                _hip.call('fe_pcg_batched', ctx.h, ctypes.byref(a64), a32p, _vp(self.M64), _vp(self.M32),
+                          nseg, segp, _vp(b), _vp(x), _vp(self._work), self._work.numel(),
+                          ctypes.byref(opts), ctypes.byref(stats), ctypes.cast(seg_stats, ctypes.c_void_p))
             seg_info = {f: np.array([getattr(q, f) for q in seg_stats])
                         for f in ('iterations', 'reliable_updates', 'status', 'restarts', 'fallback',
                                   'relres', 'relres_rec', 'bnorm')}

@@ -347,12 +347,25 @@ typedef struct {
   const uint8_t* fixed_bits; /* [n_nodes] */
   const double* T;           /* [n_agg][nc][nc] */
   const float* T32;
+  /* Block-diagonal coarse space of a batch of independent meshes (n_seg = 0: one system).  Segment s
+     (the batched PCG's segment s) owns aggregates [agg_seg[s], agg_seg[s+1]); the coarse operator
+     and its inverse are one dense symmetric block per segment, n_s = nc * (its aggregate count)
+     square with n_s <= 128, row-major at offset blk_ofs[s]; seg_of_agg[g] is aggregate g's segment.
+     Needs xhat, T and fixed_bits (the correction evaluates P from them). */
+  int32_t n_seg;
+  int32_t max_block;         /* largest n_s */
+  int64_t blk_total;         /* blk_ofs[n_seg] */
+  const int32_t* agg_seg;    /* [n_seg + 1] */
+  const int32_t* seg_of_agg; /* [n_agg] */
+  const int64_t* blk_ofs;    /* [n_seg + 1] */
 } fe_coarse;
 /* Leading dimension of the dense coarse matrices: n_agg * nc rounded up to a multiple of 64. */
 int64_t fe_coarse_ld(int64_t n_coarse);
 /* A_c = P^T A P (FP64 planes A) into Ac [ld][ld] (identity in the padding); with invert, Ac is
    replaced by A_c^-1 (blocked Gauss-Jordan, FP64, symmetrised) and Ac32 (may be NULL) receives its
-   FP32 copy; scratch holds ld * ld + 2 * 64 * 64 doubles. */
+   FP32 copy; scratch holds ld * ld + 2 * 64 * 64 doubles.  Block-diagonal space (n_seg > 0): Ac and
+   Ac32 hold the per-segment blocks (blk_ofs[n_seg] values), each inverted by one workgroup in LDS;
+   scratch is not used. */
 int fe_coarse_operator(fe_handle* h, const fe_bsr* A, const fe_coarse* C, int invert, double* Ac,
                        float* Ac32, double* scratch);
 /* u += P Ainv P^T r in FP64 (precision 0) or FP32 (1); work: 2 * round_up(n_coarse * elem, 256) bytes. */
@@ -360,12 +373,21 @@ int fe_coarse_apply(fe_handle* h, const fe_coarse* C, int precision, const void*
                     void* u, void* work);
 /* PCG with the two-level preconditioner (one system): as fe_pcg, with the coarse correction added to
    the block-Jacobi (or learned) M after every update of the residual.  Ainv64/Ainv32 from
-   fe_coarse_operator (Ainv32 for opts->mixed). */
+   fe_coarse_operator (Ainv32 for opts->mixed).  fe_pcg_batched_coarse: the same for the segments of
+   fe_pcg_batched, with a block-diagonal coarse space whose segments are the PCG's (no counterpart in
+   the reference, whose plates are solved one by one by spsolve). */
 int fe_pcg_coarse_workspace(fe_handle* h, const fe_bsr* A, const fe_coarse* C, const fe_pcg_opts* opts,
                             int64_t* bytes);
 int fe_pcg_coarse(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M64, const void* M32,
                   const fe_coarse* C, const void* Ainv64, const void* Ainv32, const double* b, double* x,
                   void* work, int64_t work_bytes, const fe_pcg_opts* opts, fe_pcg_stats* stats);
+int fe_pcg_batched_coarse_workspace(fe_handle* h, const fe_bsr* A, int32_t n_seg, const int32_t* seg_nodes,
+                                    const fe_coarse* C, const fe_pcg_opts* opts, int64_t* bytes);
+int fe_pcg_batched_coarse(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M64,
+                          const void* M32, int32_t n_seg, const int32_t* seg_nodes, const fe_coarse* C,
+                          const void* Ainv64, const void* Ainv32, const double* b, double* x, void* work,
+                          int64_t work_bytes, const fe_pcg_opts* opts, fe_pcg_stats* stats,
+                          fe_pcg_seg_stats* seg_stats);
 
 /* ---- eigen-solve ---------------------------------------------------------------------------------
    Batched LOBPCG for K x = lambda M x (replaces the eigsh / dense-inverse eigs calls of

@@ -125,3 +125,29 @@ def test_moment_form_of_prolongator():
     z2 = rng.standard_normal(P2.shape[1])
     np.testing.assert_allclose(_moment_restrict(cs2, r2), P2.T @ r2, rtol=0, atol=1e-12 * np.abs(r2).sum())
     np.testing.assert_allclose(_moment_prolong(cs2, z2), P2 @ z2, rtol=0, atol=1e-12 * np.abs(z2).sum())
+
+
+def test_block_diagonal_space_of_a_batch():
+    """Aggregates of a batch stay inside their mesh, every mesh's block fits max_block, and P^T A P
+    of the block-diagonal system is block diagonal with the layout the batched solver reads."""
+    from finite_elements_amd.batch import concatenate
+    plates = [pr for pr in meshgen.plates(count=60, seed=7) if pr.meta['nx'] <= 80][:5]
+    kind, plane, coords, conn, mats, fixed, vals, f, offs = concatenate(plates)
+    K = vz.assemble_csr(kind, coords, conn, mats, plane_mode=plane).tocsr()
+    nn = coords.shape[0]
+    rows = np.repeat(np.arange(K.shape[0]), np.diff(K.indptr)) // 2
+    Gn = sp.csr_matrix((np.ones(rows.size), (rows, K.indices // 2)), shape=(nn, nn))
+    Gn.sum_duplicates()
+    Gn.sort_indices()
+    cs = CoarseSpace(coords, Gn.indptr, Gn.indices, 2, fixed, node_offsets=offs, max_block=36)
+    assert cs.agg_seg.size == len(plates) + 1 and cs.max_block <= 36
+    for s in range(len(plates)):
+        nodes = np.arange(offs[s], offs[s + 1])
+        assert np.all(cs.seg_of_agg[cs.node_agg[nodes]] == s)
+        assert set(np.unique(cs.node_agg[nodes])) == set(range(cs.agg_seg[s], cs.agg_seg[s + 1]))
+    n_s = np.diff(cs.agg_seg) * cs.nc
+    assert np.array_equal(cs.blk_ofs, np.concatenate([[0], np.cumsum(n_s * n_s)]))
+    Ac, P = oc.coarse_operator(cs, vz.dirichlet_matrix(K, fixed))
+    for s in range(len(plates)):
+        c0, c1 = cs.agg_seg[s] * cs.nc, cs.agg_seg[s + 1] * cs.nc
+        assert not Ac[c0:c1, :c0].any() and not Ac[c0:c1, c1:].any()

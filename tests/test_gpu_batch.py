@@ -120,3 +120,59 @@ def test_batch_4096_plates(ctx):
         free[pr.fixed_dofs] = False
         res = np.linalg.norm((pr.f - K @ parts[g])[free]) / np.linalg.norm(pr.f[free])
         assert res <= 2e-8, (g, res)
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_batched_two_level(ctx, precision):
+    """Block-diagonal coarse space of a batch: every plate's coarse block and its inverse against the
+    oracle's P^T A P of the whole batch (which must be block diagonal), each plate's solution against
+    its direct solve, and fewer iterations than block Jacobi on the longest plates."""
+    from oracle import coarse as oc
+    _say(f'batched two-level {precision}')
+    plates = _small_plates(400, 5)[:12]
+    assert len(plates) >= 6
+    mb = MeshBatch(plates)
+    dc = mb.coarse_space(max_block=60)
+    cs = dc.host
+    assert dc.n_seg == len(plates) and cs.max_block <= 60
+    # aggregates stay inside their plate
+    offs = mb.node_offsets
+    for s in range(len(plates)):
+        nodes = np.flatnonzero(cs.seg_of_agg[cs.node_agg] == s)
+        assert nodes.min() >= offs[s] and nodes.max() < offs[s + 1]
+    if precision == 'fp64':
+        import scipy.sparse as sp
+        from finite_elements_amd import _hip
+        K, KD, b = mb.assemble()
+        d = KD.dof
+        g = KD.graph.pattern
+        bv = KD.convert(_hip.FE_LAYOUT_BSR).vals.cpu().numpy().reshape(-1, d, d)
+        Ah = sp.bsr_matrix((bv, g.bcol, g.bptr), shape=(g.n_nodes * d,) * 2).tocsr()
+        ref, P = oc.coarse_operator(cs, Ah)
+        Ac, _ = dc.operator(KD, invert=False)
+        Ac = Ac.cpu().numpy()
+        inv64, inv32 = dc.operator(KD, invert=True)
+        inv64 = inv64.cpu().numpy()
+        for s in range(len(plates)):
+            c0, c1 = cs.agg_seg[s] * cs.nc, cs.agg_seg[s + 1] * cs.nc
+            n = c1 - c0
+            blk = Ac[cs.blk_ofs[s]:cs.blk_ofs[s + 1]].reshape(n, n)
+            want = ref[c0:c1, c0:c1]
+            assert np.abs(blk - want).max() <= 1e-12 * np.abs(want).max()
+            # nothing couples two plates
+            assert not ref[c0:c1, :c0].any() and not ref[c0:c1, c1:].any()
+            binv = inv64[cs.blk_ofs[s]:cs.blk_ofs[s + 1]].reshape(n, n)
+            err = np.abs(binv - np.linalg.inv(want)).max() / np.abs(np.linalg.inv(want)).max()
+            assert err <= 1e-12 * np.linalg.cond(want)
+    u, info = mb.solve(tol=1e-10, precision=precision, precond='two_level')
+    seg = info['segments']
+    _, info_bj = mb.solve(tol=1e-10, precision=precision, precond='block_jacobi')
+    seg_bj = info_bj['segments']
+    _say(f'  iterations two-level {seg["iterations"].max()} (sum {seg["iterations"].sum()}), '
+         f'block Jacobi {seg_bj["iterations"].max()} (sum {seg_bj["iterations"].sum()}), '
+         f'fallbacks {seg["fallback"].sum()}')
+    assert np.all(seg['status'] == 0) and np.all(seg['relres'] <= 1e-10)
+    assert seg['iterations'].max() < seg_bj['iterations'].max()
+    for pr, up in zip(plates, mb.split(u)):
+        ref_u, _ = _oracle_solution(pr)
+        assert np.linalg.norm(up - ref_u) <= 1e-6 * np.linalg.norm(ref_u)
