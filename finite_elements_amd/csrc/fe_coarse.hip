@@ -516,9 +516,10 @@ __global__ __launch_bounds__(256) void k_coarse_gemv(int64_t n, int64_t ld, cons
 }
 
 // Block-diagonal space: z_s = A_s^-1 r_s, one workgroup per segment; thread i forms row i from column i
-// of the symmetric block (coalesced over i).
+// of the symmetric block (coalesced over i).  The blocks are read in FP64 at every precision: a thin
+// mesh's coarse block is too ill-conditioned for an FP32 inverse.
 template <typename T>
-__global__ __launch_bounds__(kSegMax) void k_coarse_gemv_seg(fe_coarse C, const T* __restrict__ Ainv,
+__global__ __launch_bounds__(kSegMax) void k_coarse_gemv_seg(fe_coarse C, const double* __restrict__ Ainv,
                                                             const T* __restrict__ x, T* __restrict__ y,
                                                             const int32_t* state, int state_stride,
                                                             int mode) {
@@ -532,9 +533,9 @@ __global__ __launch_bounds__(kSegMax) void k_coarse_gemv_seg(fe_coarse C, const 
   __syncthreads();
   const int i = threadIdx.x;
   if (i >= n) return;
-  const T* Ag = Ainv + C.blk_ofs[sg];
+  const double* Ag = Ainv + C.blk_ofs[sg];
   double acc = 0.0;
-  for (int j = 0; j < n; ++j) acc += (double)Ag[(int64_t)j * n + i] * (double)sx[j];
+  for (int j = 0; j < n; ++j) acc += Ag[(int64_t)j * n + i] * (double)sx[j];
   y[c0 + i] = (T)acc;
 }
 
@@ -701,6 +702,7 @@ int check_coarse(const fe_coarse* C, const char* who) {
 }  // namespace
 
 // Launch the three coarse-correction kernels on `stream` (used by the PCG and by fe_coarse_apply).
+// Ainv: the inverse in the iteration precision; for a block-diagonal space always the FP64 blocks.
 // state: the segment's {done, need_ru} words (or null); mode as in coarse_active.
 int fe_coarse_correction(hipStream_t stream, const fe_coarse* C, int fp32, const void* Ainv,
                          const void* r, void* u, void* rc, void* zc, const int32_t* state,
@@ -723,7 +725,7 @@ int fe_coarse_correction(hipStream_t stream, const fe_coarse* C, int fp32, const
                          (const V*)r, (V*)rc, state, mode);                                        \
     if (C->n_seg)                                                                                  \
       hipLaunchKernelGGL((k_coarse_gemv_seg<V>), dim3((unsigned)C->n_seg), dim3(kSegMax), 0, stream,  \
-                         *C, (const V*)Ainv, (const V*)rc, (V*)zc, state, state_stride, mode);     \
+                         *C, (const double*)Ainv, (const V*)rc, (V*)zc, state, state_stride, mode); \
     else                                                                                           \
       hipLaunchKernelGGL((k_coarse_gemv<V>), gg, dim3(256), 0, stream, nct, ld, (const V*)Ainv,     \
                          (const V*)rc, (V*)zc, state, mode);                                       \

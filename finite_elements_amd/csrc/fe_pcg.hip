@@ -906,8 +906,9 @@ int pcg_common(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M
                               hipMemcpyHostToDevice, h->stream));
   FE_CHECK_HIP(hipMemcpyAsync(w.seg_tiles, seg_tiles.data(), seg_tiles.size() * sizeof(int32_t),
                               hipMemcpyHostToDevice, h->stream));
+  // the block-diagonal coarse space applies its FP64 blocks at every precision
   Problem pb{A64, A32 ? A32 : A64, o->mixed ? M32 : M64, b, x, nseg, ntiles, C,
-             o->mixed ? Ainv32 : Ainv64};
+             (o->mixed && !(C && C->n_seg)) ? Ainv32 : Ainv64};
   *s = fe_pcg_stats{};
   int rc = ntiles == 0 ? FE_OK
            : o->mixed  ? dispatch_dof<float>(h, pb, w, o, s)
@@ -1090,8 +1091,7 @@ extern "C" int fe_pcg_batched_coarse(fe_handle* h, const fe_bsr* A64, const fe_b
   FE_REQUIRE(C->n_seg == n_seg && C->agg_seg && C->seg_of_agg && C->blk_ofs && C->xhat && C->T &&
                  C->fixed_bits && (!o->mixed || (C->xhat32 && C->T32)),
              "fe_pcg_batched_coarse: a block-diagonal coarse space with one block per segment expected");
-  FE_REQUIRE(o->mixed ? (Ainv32 != nullptr) : (Ainv64 != nullptr),
-             "fe_pcg_batched_coarse: missing coarse inverse (FP32 for mixed precision)");
+  FE_REQUIRE(Ainv64 != nullptr, "fe_pcg_batched_coarse: missing coarse inverse (FP64 blocks)");
   return pcg_common(h, A64, A32, M64, M32, n_seg, seg_nodes, b, x, work, work_bytes, o, s, seg_stats,
                     "fe_pcg_batched_coarse", C, Ainv64, Ainv32);
 }

@@ -606,8 +606,11 @@ class DeviceCoarse:
         return Ac, A32
 
     def apply(self, Ainv: torch.Tensor, r: torch.Tensor, u: torch.Tensor):
-        """u += P Ainv P^T r (FP64 or FP32 by the dtype of r)."""
+        """u += P Ainv P^T r (FP64 or FP32 by the dtype of r; a block-diagonal space takes its FP64
+        blocks as Ainv at either precision)."""
         prec = 0 if r.dtype == torch.float64 else 1
+        if self.n_seg and Ainv.dtype != torch.float64:
+            raise ValueError('a block-diagonal coarse space applies its FP64 blocks')
         el = 8 if prec == 0 else 4
         work = torch.empty(2 * ((self.n_coarse * el + 255) // 256) * 256, dtype=torch.uint8,
                            device=r.device)

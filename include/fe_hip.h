@@ -368,14 +368,16 @@ int64_t fe_coarse_ld(int64_t n_coarse);
    scratch is not used. */
 int fe_coarse_operator(fe_handle* h, const fe_bsr* A, const fe_coarse* C, int invert, double* Ac,
                        float* Ac32, double* scratch);
-/* u += P Ainv P^T r in FP64 (precision 0) or FP32 (1); work: 2 * round_up(n_coarse * elem, 256) bytes. */
+/* u += P Ainv P^T r in FP64 (precision 0) or FP32 (1); work: 2 * round_up(n_coarse * elem, 256) bytes.
+   A block-diagonal space takes its FP64 blocks as Ainv at either precision. */
 int fe_coarse_apply(fe_handle* h, const fe_coarse* C, int precision, const void* Ainv, const void* r,
                     void* u, void* work);
 /* PCG with the two-level preconditioner (one system): as fe_pcg, with the coarse correction added to
    the block-Jacobi (or learned) M after every update of the residual.  Ainv64/Ainv32 from
    fe_coarse_operator (Ainv32 for opts->mixed).  fe_pcg_batched_coarse: the same for the segments of
-   fe_pcg_batched, with a block-diagonal coarse space whose segments are the PCG's (no counterpart in
-   the reference, whose plates are solved one by one by spsolve). */
+   fe_pcg_batched, with a block-diagonal coarse space whose segments are the PCG's; its coarse blocks
+   are applied in FP64 at every precision (Ainv64 required, Ainv32 unused) (no counterpart in the
+   reference, whose plates are solved one by one by spsolve). */
 int fe_pcg_coarse_workspace(fe_handle* h, const fe_bsr* A, const fe_coarse* C, const fe_pcg_opts* opts,
                             int64_t* bytes);
 int fe_pcg_coarse(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M64, const void* M32,
