@@ -92,7 +92,11 @@ class CoarseSpace:
         self.dof = dof
         self.nc = {1: 1, 2: 3, 3: 6}[dof]
         if target is None:
-            target = 1000 if coords.shape[1] == 3 else 81
+            # aggregates of at least 1000 nodes (3D) / 81 (2D), and for one large system a coarse
+            # dimension near 1200 (the dense coarse inverse grows with its cube, the iteration
+            # count falls with it: 64^3 tet cube 1,296 coarse DOF, 450^2 magnetostatics ~1,200)
+            base = 1000 if coords.shape[1] == 3 else 81
+            target = base if node_offsets is not None else max(base, -(-n * self.nc // 1200))
         self.target = target
         # aggregates never cross the given node ranges (independent meshes of a batch)
         offs = np.array([0, n]) if node_offsets is None else np.asarray(node_offsets, dtype=np.int64)
