@@ -560,14 +560,16 @@ __device__ inline void rigid_moments(const double x[3], const double (&rv)[D], d
 }
 
 // r_c = P^T r with P = Z(x^) T: one workgroup per aggregate sums the moments w, then r_c = T^T w.
-template <typename T, int D, int NC>
-__global__ __launch_bounds__(kRestrictThreads) void k_coarse_restrict_rm(fe_coarse C, const T* __restrict__ xh,
+// NT threads per workgroup, sized to the aggregates (1,000-node aggregates of one large system,
+// ~100-node aggregates of a batch of small meshes).
+template <typename T, int D, int NC, int NT>
+__global__ __launch_bounds__(NT) void k_coarse_restrict_rm(fe_coarse C, const T* __restrict__ xh,
                                                                          const T* __restrict__ Tm,
                                                                          const T* __restrict__ r,
                                                                          T* __restrict__ rc,
                                                                          const int32_t* state,
                                                                          int state_stride, int mode) {
-  __shared__ double red[NC][kRestrictThreads / 64];
+  __shared__ double red[NC][NT / 64];
   __shared__ double s_w[NC];
   __shared__ int s_skip;
   const int g = blockIdx.x;
@@ -598,7 +600,7 @@ __global__ __launch_bounds__(kRestrictThreads) void k_coarse_restrict_rm(fe_coar
   __syncthreads();
   if (threadIdx.x < NC) {
     double v = 0.0;
-    for (int q = 0; q < kRestrictThreads / 64; ++q) v += red[threadIdx.x][q];
+    for (int q = 0; q < NT / 64; ++q) v += red[threadIdx.x][q];
     s_w[threadIdx.x] = v;
   }
   __syncthreads();
@@ -712,14 +714,21 @@ int fe_coarse_correction(hipStream_t stream, const fe_coarse* C, int fp32, const
       gp((unsigned)ceil_div(C->n_nodes * C->dof, 256));
   const bool rm = fp32 ? (C->xhat32 && C->T32 && C->fixed_bits) : (C->xhat && C->T && C->fixed_bits);
   const dim3 gpn((unsigned)ceil_div(C->n_nodes, 256));
+  const int64_t avg_agg = C->n_nodes / std::max<int64_t>(1, C->n_agg);  // nodes per aggregate
 #define FE_CC(V, D, NC)                                                                            \
   {                                                                                                \
     const V* P = fp32 ? (const V*)C->P32 : (const V*)C->P;                                         \
     const V* xh = fp32 ? (const V*)C->xhat32 : (const V*)C->xhat;                                  \
     const V* Tm = fp32 ? (const V*)C->T32 : (const V*)C->T;                                        \
-    if (rm)                                                                                        \
-      hipLaunchKernelGGL((k_coarse_restrict_rm<V, D, NC>), gr, dim3(kRestrictThreads), 0, stream, *C, \
-                         xh, Tm, (const V*)r, (V*)rc, state, state_stride, mode);                  \
+    if (rm && avg_agg >= 600)                                                                      \
+      hipLaunchKernelGGL((k_coarse_restrict_rm<V, D, NC, 1024>), gr, dim3(1024), 0, stream, *C, xh,   \
+                         Tm, (const V*)r, (V*)rc, state, state_stride, mode);                      \
+    else if (rm && avg_agg >= 200)                                                                 \
+      hipLaunchKernelGGL((k_coarse_restrict_rm<V, D, NC, 256>), gr, dim3(256), 0, stream, *C, xh, Tm, \
+                         (const V*)r, (V*)rc, state, state_stride, mode);                          \
+    else if (rm)                                                                                   \
+      hipLaunchKernelGGL((k_coarse_restrict_rm<V, D, NC, 128>), gr, dim3(128), 0, stream, *C, xh, Tm, \
+                         (const V*)r, (V*)rc, state, state_stride, mode);                          \
     else                                                                                           \
       hipLaunchKernelGGL((k_coarse_restrict<V, D, NC>), gr, dim3(kRestrictThreads), 0, stream, *C, P, \
                          (const V*)r, (V*)rc, state, mode);                                        \
