@@ -11,7 +11,7 @@
 //                    M-orthonormalisation of S (eigen-decomposition of the scaled S^T M S, directions
 //                    below a relative threshold dropped), both small eigenproblems by parallel cyclic
 //                    Jacobi in LDS
-//   K6 k_lob_update  X' = S C, P' = [W P] C_WP, with the K- and M-images recombined the same way
+//   K6 k_lob_update_rows X' = S C, P' = [W P] C_WP, with the K- and M-images recombined the same way
 //   K7 k_lob_flip    swap the double-buffered blocks, advance the iteration
 // Block vectors are stored row-major [n][m] (the m columns of one DOF row contiguous), so every
 // kernel reads whole rows with coalesced loads.  The host polls a status word one chunk behind.
@@ -23,7 +23,6 @@ namespace {
 
 constexpr int kMaxBlock = 32;
 constexpr int kMaxBasis = 3 * kMaxBlock;
-constexpr int kGramRows = 16;
 constexpr int kThreads = 256;
 constexpr int kRRThreads = 1024;
 constexpr int kRefresh = 32;  // iterations between explicit K X, M X, K P, M P
@@ -45,7 +44,7 @@ struct LobParams {
   int64_t nb, n, nnzb, nm;  // node rows, DOF rows, blocks, n*m
   int32_t m, n_wanted, largest, max_iter;
   double tol, drop;
-  int32_t nchunk_g, nchunk_r, ntg, nvar;
+  int32_t nchunk_g, nchunk_r, nvar, pad;
 };
 
 __device__ inline double* lob_block(double* base, const LobParams& p, int v, int idx) {
@@ -331,8 +330,16 @@ __global__ __launch_bounds__(kThreads) void k_lob_spmm(LobParams p, const int32_
 }
 
 // --------------------------------------------------------------------------------------------------
-// K3: partial Gram matrices over a range of DOF rows.  Each thread owns one 4x4 tile (ta <= tb) of
-// both S^T K S and S^T M S; rows are staged through LDS kGramRows at a time.
+// K3: partial Gram matrices over a range of DOF rows on the FP64 matrix cores.  The basis S = [X W P]
+// (ns <= 96 columns) is cut into 16-column tiles and every upper tile pair (I <= J) of S^T (K S) and
+// S^T (M S) gets one 16x16x4 MFMA accumulator.  A workgroup of four wavefronts takes one row range:
+// wavefronts 0, 1 form the K Gram, 2, 3 the M Gram; the first of each pair takes the tile pairs with
+// J <= 3 (10), the second those with J >= 4 (11).  For v_mfma_f64_16x16x4f64, lane l holds
+// A[l % 16][l / 16] and B[l / 16][l % 16], and component c of the result is D[4c + l / 16][l % 16];
+// with A = S^T and B = (K S) both operands are the element (row l / 16, column 16 T + l % 16) of a
+// 4-row step, so each lane loads its operands straight from HBM (16 consecutive doubles of a row per
+// 16 lanes) and nothing is staged in LDS.  Diagonal tiles are stored whole, the others above the
+// diagonal only.
 // --------------------------------------------------------------------------------------------------
 __device__ inline const double* basis_col_src(const double* X, const double* W, const double* P,
                                               int c, int m, int& cc) {
@@ -348,99 +355,115 @@ __device__ inline const double* basis_col_src(const double* X, const double* W, 
   return P;
 }
 
-__global__ __launch_bounds__(kThreads) void k_lob_gram(LobParams p, const LobState* st, double* blk,
-                                                       double* __restrict__ gpart) {
-  const int v = blockIdx.y;
-  const int chunk = blockIdx.x / p.ntg, tg = blockIdx.x % p.ntg;
-  __shared__ double sS[kGramRows][kMaxBasis], sK[kGramRows][kMaxBasis], sM[kGramRows][kMaxBasis];
-  __shared__ int s_skip, s_cur, s_ns;
-  if (threadIdx.x == 0) {
-    s_skip = st[v].done;
-    s_cur = st[v].cur;
-    s_ns = st[v].ns;
+constexpr int kGramTile = 16;
+constexpr int kGramTiles = kMaxBasis / kGramTile;  // 6
+constexpr int kGramThreads = 256;
+constexpr int kGramSteps = 3;
+
+typedef double GramAcc __attribute__((ext_vector_type(4)));
+
+// Tiles of one Gram wavefront: S tiles 0 .. NI-1 and image tiles J0 .. J0+NJ-1.
+template <int H>
+struct GramHalf {
+  static constexpr int NI = H ? 6 : 4, J0 = H ? 4 : 0, NJ = H ? 2 : 4;
+};
+
+template <int H>
+__device__ __forceinline__ void gram_rows(const double* __restrict__ base, int ns, int nT, int m,
+                                          const int (&bS)[3], const int (&bY)[3], int64_t nm,
+                                          int64_t lo, int64_t hi, double* __restrict__ out, int NS) {
+  constexpr int NI = GramHalf<H>::NI, J0 = GramHalf<H>::J0, NJ = GramHalf<H>::NJ;
+  const int lane = threadIdx.x & 63, rr = lane >> 4, cl = lane & 15;
+  // offsets inside the variant's blocks of this lane's column in each tile; a column past the basis
+  // is masked (never read)
+  int32_t offS[NI], offY[NJ];
+  bool okS[NI], okY[NJ];
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    const int c = i * kGramTile + cl;
+    okS[i] = c < ns;
+    offS[i] = okS[i] ? (int32_t)(bS[c / m] * nm) + c % m : 0;
   }
-  __syncthreads();
-  if (s_skip) return;
-  const int m = p.m, ns = s_ns, cur = s_cur;
-  const int nt = ns / 4;
-  const int ntiles = nt * (nt + 1) / 2;
-  const int tile = tg * kThreads + threadIdx.x;
-  int ta = -1, tb = -1;
-  if (tile < ntiles) {
-    int t = tile, a = 0;
-    while (t >= nt - a) {
-      t -= nt - a;
-      ++a;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int c = (J0 + j) * kGramTile + cl;
+    okY[j] = c < ns;
+    offY[j] = okY[j] ? (int32_t)(bY[c / m] * nm) + c % m : 0;
+  }
+  GramAcc acc[NI][NJ];
+#pragma unroll
+  for (int i = 0; i < NI; ++i)
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) acc[i][j] = GramAcc{0.0, 0.0, 0.0, 0.0};
+  // kGramSteps 4-row MFMA steps per iteration; the next iteration's operands are loaded before this
+  // one's products, so their loads are in flight while the matrix cores work
+  double sa[kGramSteps][NI], yb[kGramSteps][NJ], sn[kGramSteps][NI], yn[kGramSteps][NJ];
+  auto load = [&](int64_t r0, double(&s)[kGramSteps][NI], double(&y)[kGramSteps][NJ])
+                  __attribute__((always_inline)) {
+#pragma unroll
+    for (int k = 0; k < kGramSteps; ++k) {
+      const int64_t row = r0 + 4 * k + rr;
+      const bool rok = row < hi;
+      const int64_t ro = row * m;
+#pragma unroll
+      for (int i = 0; i < NI; ++i) s[k][i] = (rok && okS[i]) ? base[offS[i] + ro] : 0.0;
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) y[k][j] = (rok && okY[j]) ? base[offY[j] + ro] : 0.0;
     }
-    ta = a;
-    tb = a + t;
+  };
+  load(lo, sa, yb);
+  for (int64_t r0 = lo; r0 < hi; r0 += 4 * kGramSteps) {
+    load(r0 + 4 * kGramSteps, sn, yn);  // the next iteration's operands (zeros past hi)
+#pragma unroll
+    for (int k = 0; k < kGramSteps; ++k)
+#pragma unroll
+      for (int i = 0; i < NI; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j)
+          if (i <= J0 + j && J0 + j < nT)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(sa[k][i], yb[k][j], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int k = 0; k < kGramSteps; ++k) {
+#pragma unroll
+      for (int i = 0; i < NI; ++i) sa[k][i] = sn[k][i];
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) yb[k][j] = yn[k][j];
+    }
   }
-  const double* X = lob_block(blk, p, v, B_X + cur);
-  const double* KX = lob_block(blk, p, v, B_KX + cur);
-  const double* MX = lob_block(blk, p, v, B_MX + cur);
-  const double* W = lob_block(blk, p, v, B_W);
-  const double* KW = lob_block(blk, p, v, B_KW);
-  const double* MW = lob_block(blk, p, v, B_MW);
-  const double* P = lob_block(blk, p, v, B_P + cur);
-  const double* KP = lob_block(blk, p, v, B_KP + cur);
-  const double* MP = lob_block(blk, p, v, B_MP + cur);
-  const int64_t per = ceil_div(p.n, (int64_t)p.nchunk_g);
+#pragma unroll
+  for (int i = 0; i < NI; ++i)
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      if (i > J0 + j || J0 + j >= nT) continue;
+      const int col = (J0 + j) * kGramTile + cl;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int row = i * kGramTile + 4 * c + rr;
+        if (row < NS && col < NS) out[row * NS + col] = acc[i][j][c];
+      }
+    }
+}
+
+__global__ __launch_bounds__(kGramThreads, 2) void k_lob_gram(LobParams p, const LobState* st,
+                                                           const double* blk,
+                                                           double* __restrict__ gpart) {
+  const int v = blockIdx.y, chunk = blockIdx.x;
+  if (st[v].done) return;
+  const int m = p.m, ns = st[v].ns, cur = st[v].cur;
+  const int nT = (ns + kGramTile - 1) / kGramTile;
+  const int wave = threadIdx.x >> 6, img = wave >> 1, half = wave & 1;  // img 0: K, 1: M
+  if (half && nT <= GramHalf<1>::J0) return;  // no tile pair with J >= 4
+  const double* base = blk + (int64_t)v * kNumBlocks * p.nm;
+  const int bS[3] = {B_X + cur, B_W, B_P + cur};
+  const int bY[3] = {(img ? B_MX : B_KX) + cur, img ? B_MW : B_KW, (img ? B_MP : B_KP) + cur};
+  const int64_t per = ceil_div(ceil_div(p.n, (int64_t)p.nchunk_g), 4 * kGramSteps) * 4 * kGramSteps;
   const int64_t lo = min(p.n, (int64_t)chunk * per), hi = min(p.n, lo + per);
-  double ga[4][4], gb[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) ga[a][b] = gb[a][b] = 0.0;
-  for (int64_t r0 = lo; r0 < hi; r0 += kGramRows) {
-    for (int idx = threadIdx.x; idx < kGramRows * ns; idx += kThreads) {
-      const int r = idx / ns, c = idx % ns;
-      const int64_t row = r0 + r;
-      int cc;
-      double s = 0.0, k = 0.0, mm = 0.0;
-      if (row < hi) {
-        const double* src = basis_col_src(X, W, P, c, m, cc);
-        const double* srck = basis_col_src(KX, KW, KP, c, m, cc);
-        const double* srcm = basis_col_src(MX, MW, MP, c, m, cc);
-        s = src[row * m + cc];
-        k = srck[row * m + cc];
-        mm = srcm[row * m + cc];
-      }
-      sS[r][c] = s;
-      sK[r][c] = k;
-      sM[r][c] = mm;
-    }
-    __syncthreads();
-    if (ta >= 0) {
-#pragma unroll 4
-      for (int r = 0; r < kGramRows; ++r) {
-        double sa[4], kb[4], mb[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          sa[a] = sS[r][4 * ta + a];
-          kb[a] = sK[r][4 * tb + a];
-          mb[a] = sM[r][4 * tb + a];
-        }
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int b = 0; b < 4; ++b) {
-            ga[a][b] += sa[a] * kb[b];
-            gb[a][b] += sa[a] * mb[b];
-          }
-      }
-    }
-    __syncthreads();
-  }
-  if (ta < 0) return;
   const int NS = 3 * m;
-  double* out = gpart + ((int64_t)v * p.nchunk_g + chunk) * 2 * NS * NS;
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      out[(4 * ta + a) * NS + 4 * tb + b] = ga[a][b];
-      out[NS * NS + (4 * ta + a) * NS + 4 * tb + b] = gb[a][b];
-    }
+  double* out = gpart + (((int64_t)v * p.nchunk_g + chunk) * 2 + img) * NS * NS;
+  if (half)
+    gram_rows<1>(base, ns, nT, m, bS, bY, p.nm, lo, hi, out, NS);
+  else
+    gram_rows<0>(base, ns, nT, m, bS, bY, p.nm, lo, hi, out, NS);
 }
 
 // K4: fixed-order sum of the Gram partials into G[v][2][NS][NS] (both triangles).
@@ -454,9 +477,9 @@ __global__ __launch_bounds__(kThreads) void k_lob_gsum(LobParams p, const LobSta
   if (e >= 2 * NS * NS) return;
   const int mat = e / (NS * NS), a = (e % (NS * NS)) / NS, b = e % NS;
   if (a >= ns || b >= ns) return;
-  // (a, b) with a > b is only computed inside diagonal tiles; read the upper entry instead
-  const int a4 = a / 4, b4 = b / 4;
-  const int ra = (a4 > b4) ? b : a, rb = (a4 > b4) ? a : b;
+  // below the diagonal only the diagonal tiles hold entries; elsewhere read the upper entry
+  const int at = a / kGramTile, bt = b / kGramTile;
+  const int ra = (at > bt) ? b : a, rb = (at > bt) ? a : b;
   double s = 0.0;
   const double* src = gpart + (int64_t)v * p.nchunk_g * 2 * NS * NS + mat * NS * NS + ra * NS + rb;
   for (int c = 0; c < p.nchunk_g; ++c) s += src[(int64_t)c * 2 * NS * NS];
@@ -750,61 +773,86 @@ __global__ __launch_bounds__(kRRThreads) void k_lob_rr(LobParams p, LobState* st
 }
 
 // --------------------------------------------------------------------------------------------------
-// K6: X' = S C, P' = [W P] C_WP for the basis and its K-, M-images; one thread per (row, column).
-// --------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_lob_update(LobParams p, const LobState* st, double* blk,
-                                                         const double* __restrict__ Cg) {
-  const int v = blockIdx.y;
-  __shared__ double sC[kMaxBasis * kMaxBlock];
-  __shared__ int s_skip, s_cur, s_ns;
-  if (threadIdx.x == 0) {
-    s_skip = st[v].done;
-    s_cur = st[v].cur;
-    s_ns = st[v].ns;
-  }
-  __syncthreads();
-  if (s_skip) return;
-  const int m = p.m, ns = s_ns, cur = s_cur;
-  const double* C = Cg + (int64_t)v * 3 * m * m;
-  for (int idx = threadIdx.x; idx < ns * m; idx += kThreads) sC[idx] = C[idx];
-  __syncthreads();
-  // a thread owns 4 consecutive columns of two consecutive DOF rows (each coefficient read from LDS
-  // serves both rows)
-  const int tpr = m / 4, gpb = kThreads / tpr, rpb = 2 * gpb;
-  const int g = threadIdx.x / tpr, j0 = (threadIdx.x % tpr) * 4;
-  const int64_t row = (int64_t)blockIdx.x * rpb + 2 * g;
-  if (g >= gpb || row >= p.n) return;
-  const bool two = row + 1 < p.n;
-  for (int which = 0; which < 3; ++which) {
-    const double* X = lob_block(blk, p, v, B_X + 2 * which + cur) + row * m;
-    const double* P = lob_block(blk, p, v, B_P + 2 * which + cur) + row * m;
-    const double* W = lob_block(blk, p, v, B_W + which) + row * m;
-    double xa[2][4] = {}, pa[2][4] = {};
-    auto accum = [&](const double* Y, int cbase, double (&acc)[2][4]) {
-      for (int a = 0; a < m; a += 2) {
-        const double2 y0 = *reinterpret_cast<const double2*>(Y + a);
-        const double2 y1 = two ? *reinterpret_cast<const double2*>(Y + m + a) : make_double2(0.0, 0.0);
+// K6, one thread per DOF row holding all MB columns: the coefficients, uniform over the workgroup,
+// are read as scalars (the constant address space), so no coefficient goes through LDS.  Per image
+// (basis, K-image, M-image) the steps are W, P (their sum is P' = W C_W + P C_P, stored before X is
+// added) and X (X' = P' + X C_X); W and P are skipped while they are not in the basis.  The row of
+// the next step's block is loaded while the current one is multiplied.
+using ConstF64 = const __attribute__((address_space(4))) double*;
+
+template <int MB>
+__device__ __forceinline__ void upd_load(const double* __restrict__ Y, double (&y)[MB]) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const double c0 = sC[(cbase + a) * m + j0 + u], c1 = sC[(cbase + a + 1) * m + j0 + u];
-          acc[0][u] += y0.x * c0 + y0.y * c1;
-          acc[1][u] += y1.x * c0 + y1.y * c1;
-        }
-      }
-    };
-    accum(X, 0, xa);
-    if (ns > m) accum(W, m, pa);
-    if (ns > 2 * m) accum(P, 2 * m, pa);
-    for (int r = 0; r < (two ? 2 : 1); ++r) {
-      double* Xn = lob_block(blk, p, v, B_X + 2 * which + 1 - cur) + (row + r) * m + j0;
-      double* Pn = lob_block(blk, p, v, B_P + 2 * which + 1 - cur) + (row + r) * m + j0;
-      reinterpret_cast<double2*>(Xn)[0] = make_double2(xa[r][0] + pa[r][0], xa[r][1] + pa[r][1]);
-      reinterpret_cast<double2*>(Xn)[1] = make_double2(xa[r][2] + pa[r][2], xa[r][3] + pa[r][3]);
-      reinterpret_cast<double2*>(Pn)[0] = make_double2(pa[r][0], pa[r][1]);
-      reinterpret_cast<double2*>(Pn)[1] = make_double2(pa[r][2], pa[r][3]);
-    }
+  for (int a = 0; a < MB; a += 2) {
+    const double2 t = *reinterpret_cast<const double2*>(Y + a);
+    y[a] = t.x;
+    y[a + 1] = t.y;
   }
 }
+
+template <int MB>
+__device__ __forceinline__ void upd_store(double* __restrict__ Y, const double (&acc)[MB]) {
+#pragma unroll
+  for (int j = 0; j < MB; j += 2) reinterpret_cast<double2*>(Y + j)[0] = make_double2(acc[j], acc[j + 1]);
+}
+
+template <int MB>
+__global__ __launch_bounds__(kThreads) void k_lob_update_rows(LobParams p, const LobState* st,
+                                                              double* blk,
+                                                              const double* __restrict__ Cg) {
+  const int v = blockIdx.y;
+  if (st[v].done) return;
+  const int cur = st[v].cur, ns = st[v].ns;
+  const int64_t row = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (row >= p.n) return;
+  const ConstF64 C = (ConstF64)(Cg + (int64_t)v * 3 * MB * MB);
+  double* base = blk + (int64_t)v * kNumBlocks * p.nm + row * MB;
+  const int nk = ns > 2 * MB ? 3 : ns > MB ? 2 : 1;  // steps per image: [W [P]] X
+  const int nsteps = 3 * nk;
+  // step s: image s / nk, kind 0 W, 1 P, 2 X (the last step of an image is always X)
+  auto kind_of = [nk](int s) { const int k = s % nk; return k == nk - 1 ? 2 : k; };
+  auto src_of = [&](int s) -> const double* {
+    const int which = s / nk, k = kind_of(s);
+    const int b = k == 0 ? B_W + which : k == 1 ? B_P + 2 * which + cur : B_X + 2 * which + cur;
+    return base + (int64_t)b * p.nm;
+  };
+  double acc[MB], y[MB], yn[MB];
+#pragma unroll
+  for (int j = 0; j < MB; ++j) acc[j] = 0.0;
+  upd_load<MB>(src_of(0), y);
+#pragma unroll 1
+  for (int s = 0; s < nsteps; ++s) {
+    if (s + 1 < nsteps) upd_load<MB>(src_of(s + 1), yn);
+    const int which = s / nk, k = kind_of(s);
+    if (k == 2) upd_store<MB>(base + (int64_t)(B_P + 2 * which + 1 - cur) * p.nm, acc);
+    const ConstF64 Ck = C + (k == 2 ? 0 : k == 0 ? MB * MB : 2 * MB * MB);
+#pragma unroll
+    for (int a = 0; a < MB; ++a)
+#pragma unroll
+      for (int j = 0; j < MB; ++j) acc[j] = fma(y[a], Ck[a * MB + j], acc[j]);
+    if (k == 2) {
+      upd_store<MB>(base + (int64_t)(B_X + 2 * which + 1 - cur) * p.nm, acc);
+#pragma unroll
+      for (int j = 0; j < MB; ++j) acc[j] = 0.0;
+    }
+#pragma unroll
+    for (int a = 0; a < MB; ++a) y[a] = yn[a];
+  }
+}
+
+void launch_update(hipStream_t st, const LobParams& p, const LobState* s, double* blk, const double* C) {
+  const dim3 grid((unsigned)ceil_div(p.n, kThreads), p.nvar);
+  switch (p.m) {
+#define FE_LOB_UPD(MB) \
+  case MB:             \
+    hipLaunchKernelGGL(k_lob_update_rows<MB>, grid, dim3(kThreads), 0, st, p, s, blk, C); \
+    break;
+    FE_LOB_UPD(4) FE_LOB_UPD(8) FE_LOB_UPD(12) FE_LOB_UPD(16)
+    FE_LOB_UPD(20) FE_LOB_UPD(24) FE_LOB_UPD(28) FE_LOB_UPD(32)
+#undef FE_LOB_UPD
+  }
+}
+
 // K7: the new blocks become current; the basis gains W (and P after the first iteration).
 __global__ void k_lob_flip(LobParams p, LobState* st) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -919,14 +967,11 @@ LobParams make_params(const fe_bsr* K, const fe_lobpcg_opts* o) {
   p.tol = o->tol;
   p.drop = 1e-11;
   p.nvar = o->n_variants;
-  // Gram chunks: enough workgroups per variant to fill the GPU with one variant, fewer per variant
-  // when many variants run together
-  int64_t chunks = 512 / std::max(1, o->n_variants);
+  // Gram chunks: about 2,048 four-wavefront workgroups over all variants, at least 8 per variant
+  int64_t chunks = 2048 / std::max(1, o->n_variants);
   chunks = std::max<int64_t>(8, std::min<int64_t>(chunks, ceil_div(p.n, 64)));
   p.nchunk_g = (int)std::max<int64_t>(1, chunks);
   p.nchunk_r = (int)std::max<int64_t>(1, std::min<int64_t>(256, ceil_div(p.nb, 64)));
-  const int nt = 3 * p.m / 4;
-  p.ntg = (int)ceil_div(nt * (nt + 1) / 2, kThreads);
   return p;
 }
 
@@ -1014,21 +1059,20 @@ int run_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
   const int npg = kThreads / (p.m / 4);  // node rows per workgroup of the block SpMM
   const dim3 gspmm((unsigned)ceil_div(p.nb, npg), nv);
   const dim3 gres((unsigned)p.nchunk_r, nv);
-  const dim3 ggram((unsigned)(p.nchunk_g * p.ntg), nv);
+  const dim3 ggram((unsigned)p.nchunk_g, nv);
   const int NS = 3 * p.m;
   const dim3 ggsum((unsigned)ceil_div(2 * NS * NS, kThreads), nv);
-  const dim3 gupd((unsigned)ceil_div(p.n, 2 * (kThreads / (p.m / 4))), nv);
   const size_t rr_lds = 2 * (size_t)NS * NS * sizeof(double);
   FE_CHECK_HIP(hipFuncSetAttribute((const void*)k_lob_rr, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)rr_lds));
   const double* Kv = (const double*)K->vals;
   const double* Mv = (const double*)M->vals;
   auto rr_update = [&]() -> int {
-    hipLaunchKernelGGL(k_lob_gram, ggram, dim3(kThreads), 0, st, p, w.st, w.blk, w.gpart);
+    hipLaunchKernelGGL(k_lob_gram, ggram, dim3(kGramThreads), 0, st, p, w.st, w.blk, w.gpart);
     hipLaunchKernelGGL(k_lob_gsum, ggsum, dim3(kThreads), 0, st, p, w.st, w.gpart, w.G);
     hipLaunchKernelGGL(k_lob_rr, dim3(nv), dim3(kRRThreads), rr_lds, st, p, w.st, w.rpart, w.G, w.Z,
                        w.T1, w.C);
-    hipLaunchKernelGGL(k_lob_update, gupd, dim3(kThreads), 0, st, p, w.st, w.blk, w.C);
+    launch_update(st, p, w.st, w.blk, w.C);
     hipLaunchKernelGGL(k_lob_flip, gvar, dim3(64), 0, st, p, w.st);
     FE_CHECK_LAUNCH();
     return FE_OK;
