@@ -57,6 +57,8 @@ def lobpcg(K: Union[BsrMatrix, Sequence[BsrMatrix]], M: Union[BsrMatrix, Sequenc
     constrained mesh (``FESystem.coarse_space``): the preconditioner becomes two-level, T + P A_c^-1 P^T
     with A_c = P^T K_v P per variant (smallest eigenvalues only).  iteration_layout: the value
     layout the block SpMM reads (FE_LAYOUT_BSR, or FE_LAYOUT_PLANES; same results bit for bit).
+    X0: initial block [nvar][n][block] (host array); by default a standard normal block drawn on the
+    device from ``seed``.
     Returns (eigvals [nvar][n_wanted] numpy, eigvecs [nvar][n][n_wanted] torch FP64, info).
     """
     Ks = [K] if isinstance(K, BsrMatrix) else list(K)
@@ -107,10 +109,14 @@ def lobpcg(K: Union[BsrMatrix, Sequence[BsrMatrix]], M: Union[BsrMatrix, Sequenc
         t_stride = Ts[0].numel()
     # initial block
     if X0 is None:
-        rng = np.random.default_rng(seed)
-        X0 = rng.standard_normal((nvar, n, block))
-    X0 = np.asarray(X0, dtype=np.float64).reshape(nvar, n, block)
-    X = torch.as_tensor(X0, device=ctx.torch_device).contiguous().clone()
+        # standard normal start drawn on the device (seeded): 64 variants of the 48^3 cube need
+        # 5 GB of it, which a host generator takes seconds to produce
+        gen = torch.Generator(device=ctx.torch_device)
+        gen.manual_seed(int(seed))
+        X = torch.randn((nvar, n, block), generator=gen, dtype=torch.float64, device=ctx.torch_device)
+    else:
+        X0 = np.asarray(X0, dtype=np.float64).reshape(nvar, n, block)
+        X = torch.as_tensor(X0, device=ctx.torch_device).contiguous().clone()
     if fixed is not None:
         X[:, fixed.bool(), :] = 0.0
     lam = torch.zeros(nvar * block, dtype=torch.float64, device=ctx.torch_device)
