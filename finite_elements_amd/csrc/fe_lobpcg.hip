@@ -562,9 +562,8 @@ __device__ void jacobi_eig(double* A, double* V, int r, int max_sweeps) {
       // A <- J^T A J in one pass: each 2 x 2 block (rows {p_k, q_k}, columns {p_l, q_l}) gets the row
       // rotation of pair k, then the column rotation of pair l (the same operations, in the same
       // order, as rotating all rows and then all columns); V <- V J alongside
-      for (int idx = tid; idx < 64 * npair; idx += nth) {
-        const int k = idx >> 6, l = idx & 63;
-        if (l >= npair) continue;
+      for (int idx = tid; idx < npair * npair; idx += nth) {
+        const int k = idx / npair, l = idx - k * npair;
         const int p1 = s_p[k], q1 = s_q[k], p2 = s_p[l], q2 = s_q[l];
         const bool hq1 = q1 < r, hq2 = q2 < r;
         const double c1 = s_c[k], s1 = s_s[k], c2 = s_c[l], s2 = s_s[l];
@@ -582,9 +581,8 @@ __device__ void jacobi_eig(double* A, double* V, int r, int max_sweeps) {
           if (hq2) A[q1 * r + q2] = s2 * r_qp + c2 * r_qq;
         }
       }
-      for (int idx = tid; idx < 128 * npair; idx += nth) {
-        const int k = idx >> 7, i = idx & 127;
-        if (i >= r) continue;
+      for (int idx = tid; idx < r * npair; idx += nth) {
+        const int k = idx / r, i = idx - k * r;
         const int qq = s_q[k];
         const double s = s_s[k];
         if (qq >= r || s == 0.0) continue;
