@@ -237,11 +237,12 @@ __global__ __launch_bounds__(kThreads) void k_lob_cprolong(LobParams p, fe_coars
                                                            const double* __restrict__ zc, double* blk) {
   const int v = blockIdx.y;
   if (st[v].done) return;
-  const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (t >= p.nm) return;
-  const int m = p.m;
-  const int64_t row = t / m;
-  const int j = (int)(t % m);
+  // kThreads / m DOF rows per workgroup, one thread per (row, column): no 64-bit division
+  const int m = p.m, npg = kThreads / m;
+  const int g = threadIdx.x / m, j = threadIdx.x - g * m;
+  const int64_t row = (int64_t)blockIdx.x * npg + g;
+  if (g >= npg || row >= p.n) return;
+  const int64_t t = row * m + j;
   const int64_t i = row / D;
   const int64_t agg = C.node_agg[i];
   const double* Pr = C.P + row * NC;
@@ -1030,7 +1031,7 @@ void launch_lob_coarse(hipStream_t st, const LobParams& p, const LobCoarse& cc, 
                      *cc.C, cc.ld, w.st, w.blk, w.rc);
   hipLaunchKernelGGL(k_lob_cgemm, dim3((unsigned)(cc.ld / kCgRows), nv), dim3(kThreads), 0, st, p, cc.ld,
                      w.st, cc.Ainv, cc.ainv_stride, w.rc, w.zc);
-  hipLaunchKernelGGL((k_lob_cprolong<D, NC>), dim3((unsigned)ceil_div(p.nm, kThreads), nv), dim3(kThreads),
+  hipLaunchKernelGGL((k_lob_cprolong<D, NC>), dim3((unsigned)ceil_div(p.n, kThreads / p.m), nv), dim3(kThreads),
                      0, st, p, *cc.C, cc.ld, w.st, w.zc, w.blk);
 }
 
