@@ -377,19 +377,19 @@ __device__ __forceinline__ void gram_rows(const double* __restrict__ base, int n
   const int lane = threadIdx.x & 63, rr = lane >> 4, cl = lane & 15;
   // offsets inside the variant's blocks of this lane's column in each tile; a column past the basis
   // is masked (never read)
-  int32_t offS[NI], offY[NJ];
+  int64_t offS[NI], offY[NJ];
   bool okS[NI], okY[NJ];
 #pragma unroll
   for (int i = 0; i < NI; ++i) {
     const int c = i * kGramTile + cl;
     okS[i] = c < ns;
-    offS[i] = okS[i] ? (int32_t)(bS[c / m] * nm) + c % m : 0;
+    offS[i] = okS[i] ? bS[c / m] * nm + c % m : 0;
   }
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
     const int c = (J0 + j) * kGramTile + cl;
     okY[j] = c < ns;
-    offY[j] = okY[j] ? (int32_t)(bY[c / m] * nm) + c % m : 0;
+    offY[j] = okY[j] ? bY[c / m] * nm + c % m : 0;
   }
   GramAcc acc[NI][NJ];
 #pragma unroll
