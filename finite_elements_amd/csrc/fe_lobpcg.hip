@@ -237,20 +237,30 @@ __global__ __launch_bounds__(kThreads) void k_lob_cprolong(LobParams p, fe_coars
                                                            const double* __restrict__ zc, double* blk) {
   const int v = blockIdx.y;
   if (st[v].done) return;
-  // kThreads / m DOF rows per workgroup, one thread per (row, column): no 64-bit division
-  const int m = p.m, npg = kThreads / m;
-  const int g = threadIdx.x / m, j = threadIdx.x - g * m;
+  // a thread owns 4 consecutive columns of one DOF row (m is a multiple of 4): P's row is read once
+  // per 4 outputs, z and W as 16-byte pairs
+  const int m = p.m, tpr = m / 4, npg = kThreads / tpr;
+  const int g = threadIdx.x / tpr, j0 = (threadIdx.x - g * tpr) * 4;
   const int64_t row = (int64_t)blockIdx.x * npg + g;
   if (g >= npg || row >= p.n) return;
-  const int64_t t = row * m + j;
-  const int64_t i = row / D;
-  const int64_t agg = C.node_agg[i];
+  const int64_t agg = C.node_agg[row / D];
   const double* Pr = C.P + row * NC;
-  const double* z = zc + ((int64_t)v * ld + agg * NC) * m + j;
-  double s = 0.0;
+  const double* z = zc + ((int64_t)v * ld + agg * NC) * m + j0;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int c = 0; c < NC; ++c) s += Pr[c] * z[c * m];
-  lob_block(blk, p, v, B_W)[t] += s;
+  for (int c = 0; c < NC; ++c) {
+    const double pc = Pr[c];
+    const double2 a = reinterpret_cast<const double2*>(z + c * m)[0];
+    const double2 b = reinterpret_cast<const double2*>(z + c * m)[1];
+    s[0] += pc * a.x;
+    s[1] += pc * a.y;
+    s[2] += pc * b.x;
+    s[3] += pc * b.y;
+  }
+  double2* Wr = reinterpret_cast<double2*>(lob_block(blk, p, v, B_W) + row * m + j0);
+  const double2 w0 = Wr[0], w1 = Wr[1];
+  Wr[0] = make_double2(w0.x + s[0], w0.y + s[1]);
+  Wr[1] = make_double2(w1.x + s[2], w1.y + s[3]);
 }
 
 // --------------------------------------------------------------------------------------------------
@@ -1031,7 +1041,7 @@ void launch_lob_coarse(hipStream_t st, const LobParams& p, const LobCoarse& cc, 
                      *cc.C, cc.ld, w.st, w.blk, w.rc);
   hipLaunchKernelGGL(k_lob_cgemm, dim3((unsigned)(cc.ld / kCgRows), nv), dim3(kThreads), 0, st, p, cc.ld,
                      w.st, cc.Ainv, cc.ainv_stride, w.rc, w.zc);
-  hipLaunchKernelGGL((k_lob_cprolong<D, NC>), dim3((unsigned)ceil_div(p.n, kThreads / p.m), nv), dim3(kThreads),
+  hipLaunchKernelGGL((k_lob_cprolong<D, NC>), dim3((unsigned)ceil_div(p.n, kThreads / (p.m / 4)), nv), dim3(kThreads),
                      0, st, p, *cc.C, cc.ld, w.st, w.zc, w.blk);
 }
 
