@@ -976,8 +976,10 @@ LobParams make_params(const fe_bsr* K, const fe_lobpcg_opts* o) {
   p.tol = o->tol;
   p.drop = 1e-11;
   p.nvar = o->n_variants;
-  // Gram chunks: about 2,048 four-wavefront workgroups over all variants, at least 8 per variant
-  int64_t chunks = 2048 / std::max(1, o->n_variants);
+  // Gram chunks: about 2,048 four-wavefront workgroups over all variants, at least 8 and at most
+  // 512 per variant (the partials of one variant, 2 (3m)^2 doubles per chunk, are summed each
+  // iteration)
+  int64_t chunks = std::min<int64_t>(512, 2048 / std::max(1, o->n_variants));
   chunks = std::max<int64_t>(8, std::min<int64_t>(chunks, ceil_div(p.n, 64)));
   p.nchunk_g = (int)std::max<int64_t>(1, chunks);
   p.nchunk_r = (int)std::max<int64_t>(1, std::min<int64_t>(256, ceil_div(p.nb, 64)));
