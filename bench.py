@@ -384,8 +384,8 @@ def main():
     global MAX_ITER
     MAX_ITER = args.max_iter
     if args.gpus != 1:
-        raise SystemExit('bench.py measures one GPU: this build has no multi-GPU path (the sharding '
-                         'design is described in DESIGN.md); run with --gpus 1')
+        raise SystemExit('bench.py measures one GPU: there is no multi-GPU path (a system would '
+                         'shard by node ranges, batches by whole meshes or variants); run with --gpus 1')
     import torch
     from finite_elements_amd import _hip
     from finite_elements_amd.system import FESystem
