@@ -93,6 +93,55 @@ def mesh_arrays(mesh):
     return kind, coords, conn, mats
 
 
+class SolveError(RuntimeError):
+    """PCG reached neither the tolerance nor a fallback solution (the reference's spsolve raises or
+    returns the direct solution; this path never returns an unconverged vector)."""
+
+    def __init__(self, msg, info):
+        super().__init__(msg)
+        self.info = info
+
+
+def solve_reduced(sysm, KD, b, fixed, precond: str, precision: str, tol: float = 1e-8,
+                  max_iter: Optional[int] = None, model=None, coarse=None):
+    """PCG on the Dirichlet-eliminated system with plain PCG as judge and fallback.
+
+    With ``model`` (a trained ``LearnedPreconditioner``) the network's blocks replace block Jacobi
+    (inference runs inside the solver's set-up, beside the coarse inverse).  A solve that does not
+    reach the true FP64 relative residual ``tol`` is repeated by the same PCG without the network
+    (``info['fallback'] = 'plain'``), and, when that fails too or there was no network, by plain FP64
+    PCG with the same preconditioner and a larger iteration cap (``'fp64'``).  Raises ``SolveError``
+    when every attempt fails.  Returns (u, info) with ``info['attempts']`` the statuses tried."""
+    attempts = []
+
+    def run(M64, prec, cap):
+        solver = sysm.solver(KD, precond, prec, M64=M64, coarse=coarse)
+        u, info = solver.solve(b, tol=tol, max_iter=cap)
+        attempts.append({'network': M64 is not None, 'precision': prec, 'status': info['status'],
+                         'iterations': info['iterations'], 'relres': info['relres']})
+        return u, info
+
+    def ok(info):
+        return info['status'] == 0 and info['relres'] <= tol
+
+    M = None
+    if model is not None:
+        M = lambda: model.preconditioner(sysm, KD, fixed)  # noqa: E731
+    u, info = run(M, precision, max_iter)
+    info['fallback'] = None
+    if not ok(info) and model is not None:
+        u, info = run(None, precision, max_iter)
+        info['fallback'] = 'plain'
+    if not ok(info) and precision != 'fp64':
+        cap = None if max_iter is None else 4 * max_iter
+        u, info = run(None, 'fp64', cap)
+        info['fallback'] = 'fp64'
+    info['attempts'] = attempts
+    if not ok(info):
+        raise SolveError(f'PCG did not reach relres {tol:g}: {attempts}', info)
+    return u, info
+
+
 class FiniteElements:
     """Problem definition and host-side lowering (finite_elements/analysis.py:113-577)."""
 
@@ -415,7 +464,10 @@ class FiniteElementAnalysis(FiniteElements):
         modes), 'learned' (the learned blocks in place of block Jacobi; needs ``learned``, a trained
         ``LearnedPreconditioner``) or 'learned_two_level' (the learned blocks plus the coarse
         correction).
-        Convergence is judged on the true FP64 residual of the eliminated system.
+        Convergence is judged on the true FP64 residual of the eliminated system.  Plain PCG is the
+        judge and the fallback: a learned solve that misses ``tol`` is repeated without the network,
+        a mixed-precision one then in FP64, and ``SolveError`` is raised when nothing converges (an
+        unconverged vector is never returned).  ``last_solve_info`` keeps the attempts.
         """
         if method != 'gpu':
             raise ValueError("finite_elements_amd solves on the GPU only (method='gpu')")
@@ -429,25 +481,23 @@ class FiniteElementAnalysis(FiniteElements):
         fixed, g = sysm.dirichlet(pos, val)
         f_t = ctx.tensor(f)
         KD, b = sysm.prepare(K, fixed, g, f_t)
-        M64 = None
         pc = precond
+        model = None
         if precond in ('learned', 'learned_two_level'):
             if learned is None:
                 raise ValueError(f"precond={precond!r} needs a trained LearnedPreconditioner")
-            # inference runs inside the solver's set-up, beside the coarse inverse
-            M64 = lambda: learned.preconditioner(sysm, KD, fixed)  # noqa: E731
+            model = learned
             pc = 'block_jacobi' if precond == 'learned' else 'two_level'
+        if pc not in ('jacobi', 'block_jacobi', 'two_level'):
+            raise ValueError(f'unknown preconditioner {precond!r}')
         coarse = sysm.coarse_space(pos) if pc == 'two_level' else None
-        solver = sysm.solver(KD, pc, precision, M64=M64, coarse=coarse)
-        u, info = solver.solve(b, tol=tol, max_iter=max_iter)
+        u, info = solve_reduced(sysm, KD, b, fixed, pc, precision, tol=tol, max_iter=max_iter,
+                                model=model, coarse=coarse)
         from . import device
         ku = device.spmv(K, u)
         lam = f_t[pos] - ku[pos] if len(pos) else f_t[:0]
         x = np.concatenate([u.cpu().numpy(), lam.cpu().numpy()])
         self.last_solve_info = info
-        if info['status'] != 0:
-            import warnings
-            warnings.warn(f'PCG did not converge: {info}')
         return Result(self.mesh, list(x))
 
     def modal_analysis(self, order, k, **kwargs):

@@ -139,6 +139,12 @@ __global__ __launch_bounds__(kBlock) void k_gnn_layer(int64_t nb, const int32_t*
   hout[i * GW + lane] = fmaxf(acc, 0.f);
 }
 
+// s = exp(clamp(o, -4, 4)); a non-finite o gives NaN (fminf/fmaxf alone would map NaN to -4 and
+// hide a broken network behind a valid scaling): the PCG then breaks down and its caller falls back.
+__device__ inline double head_scale(float o) {
+  return o == o ? exp((double)fminf(fmaxf(o, -4.f), 4.f)) : __builtin_nan("");
+}
+
 // Head + learned blocks: o = Wo h + bo, s = exp(clamp(o)), M_i = diag(s) Binv_i diag(s).
 template <int D>
 __global__ void k_gnn_head(int64_t nb, const float* __restrict__ h, const float* __restrict__ P,
@@ -155,7 +161,7 @@ __global__ void k_gnn_head(int64_t nb, const float* __restrict__ h, const float*
 #pragma unroll 8
     for (int c = 0; c < GW; ++c) acc += Wo[k * GW + c] * h[i * GW + c];
     o[i * D + k] = acc;
-    s[k] = exp((double)fminf(fmaxf(acc, -4.f), 4.f));
+    s[k] = head_scale(acc);
   }
 #pragma unroll
   for (int a = 0; a < D; ++a)
@@ -176,7 +182,7 @@ __global__ void k_gnn_apply(int64_t nb, const double* __restrict__ M, const doub
   double s[D], u[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    s[k] = exp((double)fminf(fmaxf(o[i * D + k], -4.f), 4.f));
+    s[k] = head_scale(o[i * D + k]);
     u[k] = s[k] * r[i * D + k];
   }
 #pragma unroll
@@ -261,7 +267,7 @@ __global__ void k_gnn_dlo(int64_t nb, const double* __restrict__ scal, const dou
   double s[D], g[D], sg[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    s[k] = exp((double)fminf(fmaxf(o[i * D + k], -4.f), 4.f));
+    s[k] = head_scale(o[i * D + k]);
     g[k] = cr * r[i * D + k] + ca * az[i * D + k];
     sg[k] = s[k] * g[k];
   }

@@ -256,15 +256,10 @@ def make_training_systems(n: int = 32, seeds: Sequence[int] = range(1000, 1010),
 def solve_with_fallback(sysm, KD: BsrMatrix, b: torch.Tensor, fixed: torch.Tensor,
                         model: LearnedPreconditioner, precision: str = 'mixed', tol: float = 1e-8,
                         max_iter: Optional[int] = None, coarse=None):
-    """Learned-preconditioned PCG (two-level when ``coarse`` is given); the same PCG without the
-    network re-solves if it fails to converge."""
+    """Learned-preconditioned PCG (two-level when ``coarse`` is given) judged by its true FP64
+    residual; the same PCG without the network re-solves when it fails (``info['fallback']``), then
+    plain FP64 PCG.  Raises ``analysis.SolveError`` when no attempt converges."""
+    from .analysis import solve_reduced
     pc = 'two_level' if coarse is not None else 'block_jacobi'
-    M = model.preconditioner(sysm, KD, fixed)
-    solver = sysm.solver(KD, pc, precision, M64=M, coarse=coarse)
-    x, info = solver.solve(b, tol=tol, max_iter=max_iter)
-    info['fallback'] = False
-    if info['status'] != 0:
-        solver = sysm.solver(KD, pc, precision, coarse=coarse)
-        x, info = solver.solve(b, tol=tol, max_iter=max_iter)
-        info['fallback'] = True
-    return x, info
+    return solve_reduced(sysm, KD, b, fixed, pc, precision, tol=tol, max_iter=max_iter, model=model,
+                         coarse=coarse)

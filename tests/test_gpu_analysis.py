@@ -168,6 +168,50 @@ def test_solve_learned_two_level(ctx):
         an.solve(tol=1e-10, precond='learned_two_level')
 
 
+def _cube8_analysis():
+    pr = meshgen.cube_elasticity(n=8)
+    am = mesh.ArrayMesh(pr.coords, pr.conn, 'tet_elastic', pr.materials)
+    nodes = am.nodes
+    nl = [loads.NodeLoad(nodes[d // 3], pr.f[d], d % 3 + 1) for d in np.nonzero(pr.f)[0]]
+    bcs = [conditions.NodeBoundaryCondition(nodes[d // 3], 0.0, d % 3 + 1) for d in pr.fixed_dofs]
+    return pr, FiniteElementAnalysis(am, [], [], nl, [], [], bcs, [], [], plane_strain=True,
+                                     plane_stress=False)
+
+
+def test_learned_fallback_fires_on_nan_network(ctx):
+    """A network whose output is NaN makes the learned PCG break down (the blocks are NaN); solve()
+    must then fall back to the same PCG without the network, converge to the tolerance and match the
+    oracle's direct solve -- plain PCG is the judge and the fallback."""
+    from finite_elements_amd.learned import LearnedPreconditioner, num_params
+    from oracle import vectorized as vz
+    print('analysis solve: NaN network -> plain PCG fallback (cube8)', flush=True)
+    bad = LearnedPreconditioner(3, params=np.full(num_params(3), np.nan, dtype=np.float32))
+    pr, an = _cube8_analysis()
+    res = an.solve(tol=1e-10, precond='learned_two_level', precision='mixed', learned=bad)
+    info = an.last_solve_info
+    print(f'  attempts {info["attempts"]}', flush=True)
+    assert info['fallback'] == 'plain'
+    assert info['attempts'][0]['network'] and info['attempts'][0]['status'] != 0
+    assert not info['attempts'][1]['network']
+    assert info['status'] == 0 and info['relres'] <= 1e-10
+    K = vz.assemble_csr(pr.kind, pr.coords, pr.conn, pr.materials)
+    ref_u = vz.reduced_solve(K, pr.f, pr.fixed_dofs)
+    x = np.asarray(res.result_vector)[:ref_u.size]
+    assert np.linalg.norm(x - ref_u) / np.linalg.norm(ref_u) < 1e-7
+
+
+def test_unconverged_solve_raises(ctx):
+    """An iteration cap far below what the system needs: solve() raises SolveError (after its FP64
+    fallback also misses) instead of returning an unconverged vector."""
+    from finite_elements_amd.analysis import SolveError
+    print('analysis solve: iteration cap -> SolveError (cube8)', flush=True)
+    _, an = _cube8_analysis()
+    with pytest.raises(SolveError) as exc:
+        an.solve(tol=1e-12, precond='jacobi', precision='mixed', max_iter=3)
+    att = exc.value.info['attempts']
+    assert [a['precision'] for a in att] == ['mixed', 'fp64'] and all(a['status'] != 0 for a in att)
+
+
 def test_element_objects_and_matrix_data(ctx):
     print('element objects', flush=True)
     an = build('ex2')
