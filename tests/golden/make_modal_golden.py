@@ -1,10 +1,13 @@
 """Golden eigenvalues of the 48^3 modal cube (variant seeds given on the command line) from the oracle:
 the 20 lowest eigenvalues of the clamped pencil by scipy eigsh in shift-invert mode (oracle/eigen.py).
 
-    python tests/golden/make_modal_golden.py 5000 [5001 ...]
+    python tests/golden/make_modal_golden.py 5000 [5001 ...]          # merge into modal48_eigs.npz
+    python tests/golden/make_modal_golden.py --out part.npz 5021       # write a part file
+    python tests/golden/make_modal_golden.py --merge part1.npz ...     # merge part files
 
-Writes tests/golden/modal48_eigs.npz (seeds, eigenvalues).  Takes minutes and several GB per variant
-(sparse LU of the 346k-DOF free stiffness).
+Seeds already present in the output file are kept (a seed computed again replaces its row), so
+variants can be computed by several processes into part files and merged afterwards.  Takes about an
+hour and several GB per variant (sparse LU of the 346k-DOF free stiffness).
 """
 import os
 import sys
@@ -19,22 +22,45 @@ from finite_elements_amd import meshgen  # noqa: E402
 from oracle import eigen, vectorized as vz  # noqa: E402
 
 
+def _load(path):
+    if not os.path.exists(path):
+        return {}
+    d = np.load(path, allow_pickle=False)
+    return {int(s): np.asarray(v) for s, v in zip(d['seeds'], d['eigenvalues'])}
+
+
+def _save(path, rows, n):
+    seeds = sorted(rows)
+    np.savez(path, seeds=np.array(seeds, dtype=np.int64),
+             eigenvalues=np.array([rows[s] for s in seeds]), n=np.int64(n))
+    print('wrote', path, seeds, flush=True)
+
+
 def main():
-    seeds = [int(s) for s in sys.argv[1:]] or [5000]
+    args = sys.argv[1:]
     n = int(os.environ.get('FE_MODAL_N', '48'))
-    out = []
+    default = os.path.join(ROOT, 'tests', 'golden', f'modal{n}_eigs.npz')
+    if args and args[0] == '--merge':
+        rows = _load(default)
+        for part in args[1:]:
+            rows.update(_load(part))
+        _save(default, rows, n)
+        return
+    path = default
+    if args and args[0] == '--out':
+        path, args = args[1], args[2:]
+    seeds = [int(s) for s in args] or [5000]
+    rows = _load(path)
     for sd in seeds:
         t0 = time.time()
         pr = meshgen.cube_modal(n=n, seed=sd)
         K = vz.assemble_csr(pr.kind, pr.coords, pr.conn, pr.materials, what='stiffness')
         M = vz.assemble_csr(pr.kind, pr.coords, pr.conn, pr.materials, what='mass')
         vals, _ = eigen.smallest_clamped(K, M, pr.fixed_dofs, 20)
-        out.append(vals)
+        rows[sd] = vals
         print(f'seed {sd}: {time.time() - t0:.0f} s, {vals[:4]}', flush=True)
         # written after every variant so that a long run keeps what it has finished
-        path = os.path.join(ROOT, 'tests', 'golden', f'modal{n}_eigs.npz')
-        np.savez(path, seeds=np.array(seeds[:len(out)]), eigenvalues=np.array(out), n=np.int64(n))
-        print('wrote', path, flush=True)
+        _save(path, rows, n)
 
 
 if __name__ == '__main__':

@@ -112,15 +112,20 @@ def test_lobpcg_two_level_vs_eigsh(ctx):
         _check_pairs(v2[v], x2[v].cpu().numpy(), Ko, Mo, pr.fixed_dofs, ref_vals)
 
 
+@pytest.mark.timeout(400)
 def test_modal48_against_golden_eigsh(ctx):
-    """Full-size 48^3 variants (352,947 DOF) against the FP64 eigsh values the oracle wrote
-    (tests/golden/make_modal_golden.py): the 20 lowest eigenvalues within 1e-8 relative."""
+    """The full 64-variant batch of the 48^3 modal workload (seeds 5000-5063, 352,947 DOF each) in one
+    batched LOBPCG call, as the bench runs it: every variant must reach the residual tolerance, and the
+    variants whose FP64 eigsh values the oracle wrote (tests/golden/make_modal_golden.py) must match
+    them within 1e-8 relative in all 20 lowest eigenvalues."""
     path = os.path.join(GOLDEN, 'modal48_eigs.npz')
     if not os.path.exists(path):
         pytest.skip('tests/golden/modal48_eigs.npz not generated')
     g = np.load(path, allow_pickle=False)
-    seeds = [int(x) for x in g['seeds']]
-    _say(f'lobpcg 48^3 golden seeds {seeds}')
+    golden = {int(sd): v for sd, v in zip(g['seeds'], g['eigenvalues'])}
+    seeds = list(range(5000, 5064))
+    assert set(golden) <= set(seeds)
+    _say(f'lobpcg 48^3, 64 variants in one call; golden eigsh seeds {sorted(golden)}')
     pr0, s = _modal_system(48, seeds[0])
     fixed, _ = s.dirichlet(pr0.fixed_dofs)
     MD = device.apply_dirichlet(s.mass_bsr(), fixed)
@@ -129,12 +134,18 @@ def test_modal48_against_golden_eigsh(ctx):
         s.set_materials(meshgen.cube_modal(n=48, seed=sd).materials)
         KDs.append(device.apply_dirichlet(s.stiffness_bsr(), fixed))
     coarse = s.coarse_space(pr0.fixed_dofs)
-    vals, _, info = modal.lobpcg(KDs, MD, 20, fixed=fixed, tol=1e-6, max_iter=3000, coarse=coarse)
-    _say(f'  iterations {info["iterations"]}, status {info["status"]}')
+    tol = 1e-6
+    vals, _, info = modal.lobpcg(KDs, MD, 20, fixed=fixed, tol=tol, max_iter=3000, coarse=coarse)
+    its = info['iterations']
+    _say(f'  iterations {its.min()}..{its.max()}, status {info["status"]}, max residual '
+         f'{info["resid"].max():.2e}')
     assert info['status'] == 0
-    rel = np.abs(vals - g['eigenvalues']) / g['eigenvalues']
-    _say(f'  max relative difference to eigsh {rel.max():.2e}')
-    assert rel.max() < 1e-8
+    assert info['resid'].shape == (64, 20) and info['resid'].max() <= tol
+    assert np.all(np.diff(vals, axis=1) >= 0)
+    for sd, ref in sorted(golden.items()):
+        rel = np.abs(vals[sd - 5000] - ref) / ref
+        _say(f'  seed {sd}: max relative difference to eigsh {rel.max():.2e}')
+        assert rel.max() < 1e-8
 
 
 def test_lobpcg_largest_unconstrained_vs_eigsh(ctx):
