@@ -4,9 +4,11 @@ Default workload: 3D linear-tet elasticity on a 64^3 unit cube (6 Kuhn tets per 
 274,625 nodes, 823,875 DOF, 36,168,777 nonzeros), lognormal random-field Young's modulus (mean 1e11,
 sigma 1.0, correlation length 0.1), nu = 0.3, z = 0 clamped, uniform traction on z = 1.  One step =
 fused assembly of K + Dirichlet elimination + preconditioner setup + PCG to true FP64 relative
-residual 1e-8, with the headline method (FP32 SpMV / block-Jacobi inside FP64 reliable updates).
-The plain FP64 Jacobi-PCG on the same GPU is timed beside it, and the CPU oracle (per-element Python
-loop + scipy COO->CSR + SuperLU with natural ordering, the reference's algorithm) on a bounded sample.
+residual 1e-8 with the headline method, plain PCG as judge and fallback.  Every timed solve must
+reach the tolerance (the run fails otherwise).  The plain FP64 Jacobi-PCG on the same GPU is timed
+beside it, the CPU oracle (per-element Python loop + scipy COO->CSR + SuperLU with natural ordering,
+the reference's algorithm) on a bounded sample, and the other workloads (40x10 cantilever, 450^2
+magnetostatics, 4096 plates, 64 modal variants of the 48^3 cube) as extra keys of the same line.
 
 Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cube64]
 """
@@ -91,30 +93,60 @@ def time_assembly(sysm, reps=20):
     return start.elapsed_time(stop) / reps
 
 
-def one_solve(sysm, pr, method, fixed, g, f_t, profile=False):
+class BenchFailure(SystemExit):
+    pass
+
+
+def require_converged(info, tol, what):
+    """Every solve the bench counts must have reached the tolerance on the true FP64 residual."""
+    if info['status'] != 0 or not info['relres'] <= tol:
+        raise BenchFailure(f'{what}: solve did not converge (status {info["status"]}, relres '
+                           f'{info["relres"]:.3e} > {tol:g}); no number is reported')
+
+
+def one_solve(sysm, method, fixed, g, f_t, tol=1e-8):
     """Assemble, eliminate, set up the preconditioner (network inference included for the learned
-    method) and solve; returns (u, info)."""
+    method) and solve, with plain PCG as judge and fallback; returns (u, info)."""
+    from finite_elements_amd.analysis import solve_reduced
+    K = sysm.stiffness_bsr()
+    KD, b = sysm.prepare(K, fixed, g, f_t)
+    u, info = solve_reduced(sysm, KD, b, fixed, method['precond'], method['precision'], tol=tol,
+                            max_iter=MAX_ITER, model=method.get('model'), coarse=method.get('coarse'))
+    require_converged(info, tol, method['name'])
+    return u, info
+
+
+def profile_spmv(sysm, method, fixed, g, f_t, tol=1e-8):
+    """The iteration's SpMV kernel timed by 20 back-to-back launches on the final state of one more
+    solve of the method (outside every timed region); returns (ms per launch, launches)."""
     K = sysm.stiffness_bsr()
     KD, b = sysm.prepare(K, fixed, g, f_t)
     M = None
     if method.get('model') is not None:
-        # network inference, run by the solver's set-up beside the coarse inverse
-        M = lambda: method['model'].preconditioner(sysm, KD, fixed)
+        M = lambda: method['model'].preconditioner(sysm, KD, fixed)  # noqa: E731
     solver = sysm.solver(KD, precond=method['precond'], precision=method['precision'], M64=M,
                          coarse=method.get('coarse'))
-    u, info = solver.solve(b, tol=1e-8, max_iter=MAX_ITER, profile=profile)
-    return u, info
+    _, info = solver.solve(b, tol=tol, max_iter=MAX_ITER, profile=True)
+    return info['spmv_ms'], info['spmv_samples']
 
 
-def time_method(sysm, pr, method, fixed, g, f_t, steps, warmup, profile=False):
+def summarize_infos(infos):
+    return {'iterations_min': int(min(i['iterations'] for i in infos)),
+            'iterations_max': int(max(i['iterations'] for i in infos)),
+            'relres_true_fp64_max': float(max(i['relres'] for i in infos)),
+            'fallbacks': int(sum(1 for i in infos if i.get('fallback'))),
+            'solves_checked': len(infos)}
+
+
+def time_method(sysm, method, fixed, g, f_t, steps, warmup):
     import torch
     for _ in range(warmup):
-        one_solve(sysm, pr, method, fixed, g, f_t)
+        one_solve(sysm, method, fixed, g, f_t)
     torch.cuda.synchronize()
     times, infos = [], []
     for _ in range(steps):
         ts = time.perf_counter()
-        _, info = one_solve(sysm, pr, method, fixed, g, f_t, profile=profile)
+        _, info = one_solve(sysm, method, fixed, g, f_t)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - ts)
         infos.append(info)
@@ -141,30 +173,80 @@ def _cpu_solve_once(n):
             'finite': bool(np.all(np.isfinite(x)))}
 
 
-def cpu_baseline():
+def host_description():
+    """CPU model, visible cores and the BLAS thread pools of this process."""
+    model = 'unknown'
+    try:
+        with open('/proc/cpuinfo') as fh:
+            for line in fh:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    blas = None
+    try:
+        import threadpoolctl
+        blas = [{'api': d['internal_api'], 'threads': d['num_threads']}
+                for d in threadpoolctl.threadpool_info()]
+    except ImportError:
+        pass
+    return {'cpu_model': model, 'cores_visible': len(os.sched_getaffinity(0)), 'blas_pools': blas}
+
+
+def cpu_baseline(repeats=3):
     """The reference's algorithm (oracle restatement: per-element Python loop, scipy COO->CSR,
-    natural-order SuperLU on the Lagrange system) on a ladder of small cubes, about 10-20 s of CPU
-    work; the value is the largest rung (14^3), and a power-law fit of the ladder is extrapolated to
-    64^3 (labelled as such: natural-order LU of the 64^3 system does not fit in memory)."""
+    natural-order SuperLU on the Lagrange system) on a ladder of small cubes, each rung the median of
+    `repeats` runs with BLAS held to one thread (the loop and SuperLU are single-threaded), about
+    20-30 s of CPU work; the value is the largest rung (14^3), and a power-law fit of the ladder is
+    extrapolated to 64^3 (labelled as such: natural-order LU of the 64^3 system does not fit in
+    memory)."""
+    import threadpoolctl
     sizes = [int(v) for v in os.environ.get('FE_CPU_LADDER', '8,10,12,14').split(',')]
-    ladder = [_cpu_solve_once(n) for n in sizes]
+    ladder = []
+    with threadpoolctl.threadpool_limits(1):
+        for n in sizes:
+            runs = [_cpu_solve_once(n) for _ in range(repeats)]
+            med = {k: float(np.median([r[k] for r in runs])) for k in ('assemble_s', 'solve_s', 'total_s')}
+            ladder.append({'n': n, 'dofs': runs[0]['dofs'], 'elements': runs[0]['elements'], **med,
+                           'runs_total_s': [r['total_s'] for r in runs],
+                           'finite': all(r['finite'] for r in runs)})
+            log(f'CPU oracle {n}^3: median {med["total_s"]:.2f} s of {repeats}')
     top = ladder[-1]
     # assembly and LU solve extrapolated apart (their exponents differ)
     d = np.log([r['dofs'] for r in ladder])
     fits = {k: np.polyfit(d, np.log([r[k] for r in ladder]), 1) for k in ('assemble_s', 'solve_s')}
     extrap = float(sum(np.exp(b + a * np.log(823875)) for a, b in fits.values()))
-    cores = len(os.sched_getaffinity(0))
+    host = host_description()
     return {'value': 1.0 / top['total_s'], 'unit': 'solves/s', 'cores': 1, 'kind': 'port',
-            'sample': (f"{top['n']}^3 cube ({top['dofs']} DOF, {top['elements']} tets): oracle "
-                       f"per-element Python loop + scipy COO->CSR ({top['assemble_s']:.2f} s) + spsolve "
-                       f"NATURAL on the Lagrange system ({top['solve_s']:.2f} s); single-threaded "
-                       f'Python/SuperLU; {cores} host cores visible'),
-            'ladder': [{k: r[k] for k in ('n', 'dofs', 'total_s', 'assemble_s', 'solve_s')}
-                       for r in ladder],
+            'sample': (f"{top['n']}^3 cube ({top['dofs']} DOF, {top['elements']} tets), median of "
+                       f"{repeats}: oracle per-element Python loop + scipy COO->CSR "
+                       f"({top['assemble_s']:.2f} s) + spsolve NATURAL on the Lagrange system "
+                       f"({top['solve_s']:.2f} s); single-threaded Python/SuperLU, BLAS limited to 1 "
+                       f"thread; {host['cores_visible']} host cores visible ({host['cpu_model']})"),
+            'host': host,
+            'ladder': ladder,
             'fit_exponents': {k: float(v[0]) for k, v in fits.items()},
             'extrapolated_64cube_s': extrap,
             'extrapolation_note': 'power-law fit of the ladder, not a measurement',
             'finite': all(r['finite'] for r in ladder)}
+
+
+def cpu_strong(n=32, iter_cap=None):
+    """The multi-core CPU baseline (oracle/strong_cpu.py: vectorised assembly + Jacobi-CG with
+    threaded sparse products) on an n^3 cube of the same lognormal-field family, to relres 1e-8
+    (or `iter_cap` iterations, then the per-iteration time is what is reported)."""
+    from finite_elements_amd import meshgen
+    from oracle import strong_cpu
+    threads = min(16, len(os.sched_getaffinity(0)))
+    pr = meshgen.cube_elasticity(n=n, seed=4)
+    r = strong_cpu.solve_cube(pr, tol=1e-8, threads=threads, iter_cap=iter_cap)
+    total = r['assemble_s'] + r['eliminate_s'] + r['cg_s']
+    return {'value': 1.0 / total, 'unit': 'solves/s', 'cores': threads, 'kind': 'port',
+            'sample': (f'{n}^3 cube ({r["dofs"]} DOF): vectorised numpy assembly '
+                       f'({r["assemble_s"]:.2f} s) + Jacobi-CG in FP64 with {threads}-thread sparse '
+                       f'products ({r["iterations"]} iterations, {r["cg_s"]:.2f} s, relres '
+                       f'{r["relres"]:.1e})'), **r}
 
 
 def copy_calibration(reps=3):
@@ -272,7 +354,8 @@ def modal_bench(n=48, n_var=64, n_modes=20, tol=1e-6):
     """The batched eigen-solve workload: 64 lognormal-stiffness variants of the 48^3 clamped Kuhn-tet
     cube, 20 lowest modes each, by one batched LOBPCG call preconditioned with the two-level method
     (one coarse inverse per variant).  Timed: per-variant assembly + Dirichlet decoupling of K, the
-    coarse operators and the eigen-solve (the shared mass matrix is assembled once, untimed)."""
+    coarse operators and the eigen-solve (the shared mass matrix is assembled once, untimed).  Every
+    variant must reach the residual tolerance."""
     import torch
     from finite_elements_amd import device, meshgen, modal
     from finite_elements_amd.system import FESystem
@@ -292,24 +375,27 @@ def modal_bench(n=48, n_var=64, n_modes=20, tol=1e-6):
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     its = info['iterations']
+    max_resid = float(np.max(info['resid'][:, :n_modes]))
+    if info['status'] != 0 or not max_resid <= tol:
+        raise BenchFailure(f'modal48: LOBPCG did not converge (status {info["status"]}, max residual '
+                           f'{max_resid:.2e})')
     return {'metric': f'variants/s to {n_modes} lowest modes (rel. residual {tol:g})',
             'value': n_var / el, 'unit': 'variants/s', 'seconds': el, 'n_variants': n_var,
             'dofs': int(sysm.n), 'block': info['block'], 'coarse_dofs': int(coarse.n_coarse),
             'iterations_min': int(its.min()), 'iterations_max': int(its.max()),
-            'status': int(info['status']), 'max_resid': float(np.max(info['resid'])),
+            'status': int(info['status']), 'max_resid': max_resid,
             'lowest_modes_variant0': [float(x) for x in vals[0][:4]]}
 
 
-def plates_bench(count=4096, precision='fp64', precond='two_level'):
+def plates_bench(count=4096, precision='fp64', precond='two_level', compare=True):
     """The batched-plates workload: 4096 uneven plane-stress plates (seed 3) concatenated into one
     block-diagonal system, one autotuned fused assembly and one batched PCG (every plate its own
     segment, chunks replayed from a HIP graph) to true FP64 relres 1e-8 per plate.  Headline: the
     two-level preconditioner (block Jacobi plus each plate's own rigid-body coarse correction) in
-    FP64; beside it the same in mixed precision (FP32 iterations inside FP64 reliable updates; the
-    long thin plates still fail in FP32 and are finished by the FP64 fallback, which makes it slower)
-    and plain FP64 block Jacobi.  Timed: assembly + Dirichlet elimination + set-up (coarse blocks and
-    their inverses included) + solve, after one untimed warm-up; the coarse space itself (aggregates,
-    host) is built once per mesh, before."""
+    FP64; beside it the same in mixed precision and plain FP64 block Jacobi.  Timed: assembly +
+    Dirichlet elimination + set-up (coarse blocks and their inverses included) + solve, after one
+    untimed warm-up; the coarse space itself (aggregates, host) is built once per mesh, before.
+    Every plate of every timed run must reach the tolerance."""
     import torch
     from finite_elements_amd import meshgen
     from finite_elements_amd.batch import MeshBatch
@@ -328,27 +414,103 @@ def plates_bench(count=4096, precision='fp64', precond='two_level'):
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
         seg = info['segments']
-        return {'seconds': el, 'plates_per_s': count / el, 'precision': prec, 'precond': pc,
-                'iterations_min': int(seg['iterations'].min()),
-                'iterations_max': int(seg['iterations'].max()),
-                'not_converged': int((seg['status'] != 0).sum()), 'fallbacks': int(info['fallbacks']),
-                'max_relres': float(seg['relres'].max())}
+        out = {'seconds': el, 'plates_per_s': count / el, 'precision': prec, 'precond': pc,
+               'iterations_min': int(seg['iterations'].min()),
+               'iterations_max': int(seg['iterations'].max()),
+               'not_converged': int((seg['status'] != 0).sum()), 'fallbacks': int(info['fallbacks']),
+               'max_relres': float(seg['relres'].max())}
+        if out['not_converged'] or not out['max_relres'] <= 1e-8:
+            raise BenchFailure(f'plates {prec} {pc}: {out["not_converged"]} plates unconverged, '
+                               f'max relres {out["max_relres"]:.2e}')
+        log(f'plates {prec} {pc}: {el:.3f} s, iterations {out["iterations_min"]}..'
+            f'{out["iterations_max"]}, fallbacks {out["fallbacks"]}')
+        return out
 
     head = run(precision, precond)
-    log(f'plates {precision} {precond}: {head["seconds"]:.3f} s, iterations {head["iterations_min"]}..'
-        f'{head["iterations_max"]}, fallbacks {head["fallbacks"]}')
-    other = run('mixed' if precision == 'fp64' else 'fp64', precond)
-    log(f'plates {other["precision"]} {precond}: {other["seconds"]:.3f} s, iterations '
-        f'{other["iterations_min"]}..{other["iterations_max"]}, fallbacks {other["fallbacks"]}')
-    base = run('fp64', 'block_jacobi')
-    log(f'plates fp64 block_jacobi: {base["seconds"]:.3f} s, iterations {base["iterations_min"]}..'
-        f'{base["iterations_max"]}')
-    return {'metric': 'plates/s to true FP64 rel-res 1e-8 (every plate)', 'value': head['plates_per_s'],
-            'unit': 'plates/s', 'seconds': head['seconds'], 'plates': count, 'precision': precision,
-            'precond': precond, 'dofs': int(mb.sysm.n), 'iterations_min': head['iterations_min'],
-            'iterations_max': head['iterations_max'], 'not_converged': head['not_converged'],
-            'fallbacks': head['fallbacks'], 'max_relres': head['max_relres'],
-            f'{other["precision"]}_{precond}': other, 'fp64_block_jacobi': base}
+    res = {'metric': 'plates/s to true FP64 rel-res 1e-8 (every plate)', 'value': head['plates_per_s'],
+           'unit': 'plates/s', 'seconds': head['seconds'], 'plates': count, 'precision': precision,
+           'precond': precond, 'dofs': int(mb.sysm.n), 'iterations_min': head['iterations_min'],
+           'iterations_max': head['iterations_max'], 'not_converged': head['not_converged'],
+           'fallbacks': head['fallbacks'], 'max_relres': head['max_relres']}
+    if compare:
+        other = run('mixed' if precision == 'fp64' else 'fp64', precond)
+        res[f'{other["precision"]}_{precond}'] = other
+        res['fp64_block_jacobi'] = run('fp64', 'block_jacobi')
+    return res
+
+
+def cantilever_bench(repeats=5):
+    """The 40x10 plane-stress cantilever through the drop-in API (FiniteElementAnalysis built from
+    reference-style node loads and conditions, .solve() to a Result with the multipliers): median
+    latency of `repeats` solves after a warm-up, the tip deflection against the oracle's direct solve
+    of the reference algorithm, and that oracle's own time on the host (full size: it is small)."""
+    import torch
+    from finite_elements_amd import conditions, loads, meshgen
+    from finite_elements_amd.analysis import FiniteElementAnalysis
+    from finite_elements_amd.mesh import ArrayMesh
+    from oracle import reference as ref
+    pr = meshgen.cantilever_2d()
+    am = ArrayMesh(pr.coords, pr.conn, 'tri_elastic', pr.materials)
+    nodes = am.nodes
+    nl = [loads.NodeLoad(nodes[d // 2], pr.f[d], d % 2 + 1) for d in np.nonzero(pr.f)[0]]
+    bcs = [conditions.NodeBoundaryCondition(nodes[d // 2], 0.0, d % 2 + 1) for d in pr.fixed_dofs]
+
+    def gpu_once():
+        an = FiniteElementAnalysis(am, [], [], nl, [], [], bcs, [], [], plane_strain=False,
+                                   plane_stress=True)
+        res = an.solve(tol=1e-10, precond='block_jacobi', precision='fp64')
+        torch.cuda.synchronize()
+        return np.asarray(res.result_vector), an.last_solve_info
+
+    gpu_once()
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        x, info = gpu_once()
+        times.append(time.perf_counter() - t0)
+    cpu_times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        prob = ref.problem_from_arrays(pr.kind, pr.coords, pr.conn, pr.materials, plane_strain=False,
+                                       plane_stress=True, fixed_dofs=pr.fixed_dofs, f=pr.f)
+        x_ref = np.asarray(ref.solve(prob))
+        cpu_times.append(time.perf_counter() - t0)
+    tip = 2 * pr.meta['tip_node'] + 1
+    err = float(np.linalg.norm(x[:pr.n] - x_ref[:pr.n]) / np.linalg.norm(x_ref[:pr.n]))
+    return {'metric': 'drop-in solve() latency, 40x10 cantilever (assembly + lowering + PCG to 1e-10)',
+            'value': float(np.median(times)) * 1e3, 'unit': 'ms', 'higher_is_better': False,
+            'iterations': info['iterations'], 'relres_true_fp64': info['relres'],
+            'tip_deflection': float(x[tip]), 'tip_deflection_oracle': float(x_ref[tip]),
+            'u_rel_err_vs_oracle_direct': err,
+            'cpu_oracle_ms': float(np.median(cpu_times)) * 1e3,
+            'cpu_oracle': 'reference algorithm restated (per-element loop + spsolve NATURAL), 1 core'}
+
+
+def magnetic_bench(steps=5):
+    """The 450^2 magnetostatics: the two-level mixed PCG (block Jacobi + constant-mode coarse
+    correction, FP32 iterations inside FP64 reliable updates) against plain FP64 Jacobi PCG on the
+    same GPU, each the median of `steps` checked solves (assembly included)."""
+    import torch
+    from finite_elements_amd.system import FESystem
+    pr, desc = make_problem('magnetic450')
+    sysm = FESystem(pr.kind, pr.coords, pr.conn, pr.materials)
+    fixed, g = sysm.dirichlet(pr.fixed_dofs, pr.fixed_values)
+    f_t = sysm.ctx.tensor(pr.f)
+    coarse = sysm.coarse_space(pr.fixed_dofs)
+    head = {'name': 'magnetic two_level_mixed', 'precond': 'two_level', 'precision': 'mixed',
+            'coarse': coarse}
+    plain = {'name': 'magnetic jacobi_fp64', 'precond': 'jacobi', 'precision': 'fp64'}
+    ht, hi = time_method(sysm, head, fixed, g, f_t, steps, 1)
+    pt, pi = time_method(sysm, plain, fixed, g, f_t, max(2, steps // 2), 1)
+    torch.cuda.synchronize()
+    asm_ms = time_assembly(sysm)
+    h_ms, p_ms = float(np.median(ht)) * 1e3, float(np.median(pt)) * 1e3
+    log(f'magnetic450: two-level mixed {h_ms:.2f} ms, plain FP64 Jacobi {p_ms:.2f} ms')
+    return {'metric': 'solves/s to FP64 rel-res 1e-8, 450^2 magnetostatics', 'value': 1e3 / h_ms,
+            'unit': 'solves/s', 'workload': desc, 'dofs': int(sysm.n), 'method': 'two_level_mixed',
+            'ms': h_ms, **summarize_infos(hi), 'coarse_dofs': int(coarse.n_coarse),
+            'plain_fp64_jacobi': {'ms': p_ms, **summarize_infos(pi)},
+            'speedup_vs_plain': p_ms / h_ms, 'assembly_ms': asm_ms}
 
 
 def main():
@@ -361,6 +523,12 @@ def main():
     ap.add_argument('--workload', default='cube64')
     ap.add_argument('--plain-steps', type=int, default=2)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--cpu-strong-n', type=int, default=32,
+                    help='cube size of the multi-core CPU baseline (0: skip it)')
+    ap.add_argument('--configs', default='cantilever,magnetic450,plates4096,modal48',
+                    help='other workloads reported as extra keys of the line ("none": skip)')
+    ap.add_argument('--headline', default='auto', choices=('auto', 'learned', 'two_level'),
+                    help='auto: the learned two-level method when weights exist')
     ap.add_argument('--no-learned', action='store_true')
     ap.add_argument('--weights', default=None)
     ap.add_argument('--no-tune', action='store_true', help='keep the default assembly schedule')
@@ -374,12 +542,19 @@ def main():
                     help='summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) and exit')
     ap.add_argument('--summarize-trace', metavar='KERNEL_TRACE_CSV',
                     help='summarise a rocprofv3 --kernel-trace csv per kernel and exit')
+    ap.add_argument('--cpu-only', choices=('ladder', 'strong'),
+                    help='run only a CPU baseline (ladder: FE_CPU_LADDER rungs; strong: '
+                         '--cpu-strong-n) and print its JSON')
     args = ap.parse_args()
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc)
         return
     if args.summarize_trace:
         summarize_trace(args.summarize_trace)
+        return
+    if args.cpu_only:
+        out = cpu_baseline() if args.cpu_only == 'ladder' else cpu_strong(args.cpu_strong_n)
+        print(json.dumps(out), flush=True)
         return
     global MAX_ITER
     MAX_ITER = args.max_iter
@@ -391,12 +566,13 @@ def main():
     from finite_elements_amd.system import FESystem
 
     if args.workload == 'modal48':
-        out = modal_bench()
-        print(json.dumps(out), flush=True)
+        print(json.dumps(modal_bench()), flush=True)
         return
     if args.workload == 'plates4096':
-        out = plates_bench()
-        print(json.dumps(out), flush=True)
+        print(json.dumps(plates_bench()), flush=True)
+        return
+    if args.workload == 'magnetic450_only':
+        print(json.dumps(magnetic_bench()), flush=True)
         return
     log(f'building workload {args.workload}')
     t0 = time.perf_counter()
@@ -423,21 +599,23 @@ def main():
     g_ = sysm.graph
     persistent = (g_.welem_conn is not None and 0 < g_.max_nodes < best[0] and g_.max_inc <= best[0]
                   and g_.max_elem <= best[0] and g_.n_blk >= 8)
-    asm = {'kernel': 'k_assemble_pipe' if persistent else 'k_assemble_fused',
+    asm_kernel = 'k_assemble_pipe' if persistent else 'k_assemble_fused'
+    asm = {'kernel': asm_kernel,
            'schedule': {'threads': best[0], 'max_inc': best[1]},
            'ms': asm_ms, 'bytes': asm_bytes,
            'gbps': asm_bytes / asm_ms / 1e6, 'frac': asm_bytes / asm_ms / 1e6 / HBM_PEAK_GBS,
            'compulsory_bytes': asm_compulsory,
            'compulsory_gbps': asm_compulsory / asm_ms / 1e6,
-           'traffic': measured_traffic(f"{'k_assemble_pipe' if persistent else 'k_assemble_fused'}"
-                                       f"<{sysm.mesh.kind_id}, {_hip.FE_STIFFNESS}, "
+           'compulsory_frac': asm_compulsory / asm_ms / 1e6 / HBM_PEAK_GBS,
+           'traffic': measured_traffic(f"{asm_kernel}<{sysm.mesh.kind_id}, {_hip.FE_STIFFNESS}, "
                                        f"{_hip.FE_LAYOUT_PLANES}, {best[0]}>")}
     log(f'assembly {asm_ms:.3f} ms, {asm["gbps"]:.0f} GB/s algorithmic')
 
     model = None
     wpath = args.weights or os.path.join(ROOT, 'finite_elements_amd', 'weights',
                                          'learned_block_jacobi_tet.npz')
-    if sysm.kind == 'tet_elastic' and os.path.exists(wpath) and not args.no_learned:
+    if (sysm.kind == 'tet_elastic' and os.path.exists(wpath) and not args.no_learned
+            and args.headline != 'two_level'):
         from finite_elements_amd.learned import LearnedPreconditioner
         model = LearnedPreconditioner.load(wpath)
         log(f'learned preconditioner weights: {os.path.relpath(wpath, ROOT)}')
@@ -450,38 +628,37 @@ def main():
     plain = {'name': 'jacobi_fp64', 'precond': 'jacobi', 'precision': 'fp64'}
     headline = learned_m if model is not None else two
 
-    # ---- plain FP64 Jacobi-PCG (the in-GPU baseline) ----
-    plain_s, plain_info = None, None
+    # ---- plain FP64 Jacobi-PCG (the in-GPU baseline) and the ablations ----
+    plain_s, plain_infos = None, None
     if args.plain_steps > 0:
-        plain_times, plain_infos = time_method(sysm, pr, plain, fixed, g, f_t, args.plain_steps, 1)
+        plain_times, plain_infos = time_method(sysm, plain, fixed, g, f_t, args.plain_steps, 1)
         plain_s = float(np.median(plain_times))
-        plain_info = plain_infos[-1]
-        log(f'plain FP64 Jacobi: {plain_s * 1e3:.1f} ms, {plain_info["iterations"]} it, '
-            f'relres {plain_info["relres"]:.2e}')
+        log(f'plain FP64 Jacobi: {plain_s * 1e3:.1f} ms, {plain_infos[-1]["iterations"]} it, '
+            f'relres {plain_infos[-1]["relres"]:.2e}')
     ablations = {}
     if args.plain_steps > 0:
         for m in ([two] if model is not None else []) + [block]:
-            bt, bi = time_method(sysm, pr, m, fixed, g, f_t, args.plain_steps, 1)
+            bt, bi = time_method(sysm, m, fixed, g, f_t, args.plain_steps, 1)
             ablations[m['name']] = {'solves_per_s': 1.0 / float(np.median(bt)),
-                                    'ms': float(np.median(bt)) * 1e3, 'iterations': bi[-1]['iterations'],
-                                    'relres_true_fp64': bi[-1]['relres']}
+                                    'ms': float(np.median(bt)) * 1e3, **summarize_infos(bi)}
             log(f'{m["name"]}: {ablations[m["name"]]["ms"]:.1f} ms, {bi[-1]["iterations"]} it')
 
-    # ---- headline method: warmup then K timed steps ----
+    # ---- headline method: warmup then K timed steps, every one checked ----
     for _ in range(args.warmup):
-        one_solve(sysm, pr, headline, fixed, g, f_t)
+        one_solve(sysm, headline, fixed, g, f_t)
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     infos, step_s = [], []
-    for k in range(args.steps):
+    for _ in range(args.steps):
         ts = time.perf_counter()
-        # the last step also times the iteration's SpMV kernel (20 launches after convergence)
-        _, info = one_solve(sysm, pr, headline, fixed, g, f_t, profile=(k == args.steps - 1))
+        _, info = one_solve(sysm, headline, fixed, g, f_t)
         torch.cuda.synchronize()
         step_s.append(time.perf_counter() - ts)
         infos.append(info)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
+    ms_per_step = elapsed / args.steps * 1e3
+    value = args.steps / elapsed
     # cold variant (outside the timed region): L2 and the 256 MB Infinity Cache flushed before each
     # solve by writing a 1 GiB buffer
     cold_s = []
@@ -491,17 +668,16 @@ def main():
             flush.fill_(1.0)
             torch.cuda.synchronize()
             ts = time.perf_counter()
-            one_solve(sysm, pr, headline, fixed, g, f_t)
+            one_solve(sysm, headline, fixed, g, f_t)
             torch.cuda.synchronize()
             cold_s.append(time.perf_counter() - ts)
         del flush
-    ms_per_step = elapsed / args.steps * 1e3
-    value = args.steps / elapsed
-    info = infos[-1]
-    log(f'headline {headline["name"]}: {ms_per_step:.1f} ms/solve, {info["iterations"]} it, '
-        f'relres {info["relres"]:.2e}')
+    hs = summarize_infos(infos)
+    log(f'headline {headline["name"]}: {ms_per_step:.1f} ms/solve, {hs["iterations_min"]}..'
+        f'{hs["iterations_max"]} it, max relres {hs["relres_true_fp64_max"]:.2e}')
 
-    spmv_ms = float(np.mean([i['spmv_ms'] for i in infos if i['spmv_samples'] > 0]))
+    # ---- roofline kernel: the iteration SpMV timed after one more (untimed) solve ----
+    spmv_ms, spmv_n = profile_spmv(sysm, headline, fixed, g, f_t)
     spmv_bytes = pcg_spmv_bytes(sysm, 4)
     # the FP32 iteration SpMV on the padded pattern: 4 lanes per row (d = 3), else 2, 16-byte loads
     spmv_kernel = f'k_cg_spmv<float, {sysm.dof}, {4 if sysm.dof == 3 else 2}, true>'
@@ -509,40 +685,56 @@ def main():
                 'bound': 'hbm', 'achieved': spmv_bytes / spmv_ms / 1e6, 'peak': HBM_PEAK_GBS,
                 'unit': 'GB/s', 'frac': spmv_bytes / spmv_ms / 1e6 / HBM_PEAK_GBS,
                 'traffic': measured_traffic(spmv_kernel),
-                'bytes_per_launch': spmv_bytes, 'avg_ms': spmv_ms,
-                'samples': int(sum(i['spmv_samples'] for i in infos))}
+                'bytes_per_launch': spmv_bytes, 'avg_ms': spmv_ms, 'samples': int(spmv_n)}
 
-    cpu = None
+    configs = {}
+    wanted = [c for c in args.configs.split(',') if c and c != 'none']
+    runners = {'cantilever': cantilever_bench, 'magnetic450': magnetic_bench,
+               'plates4096': plates_bench, 'modal48': modal_bench}
+    for name in wanted:
+        if name not in runners:
+            raise SystemExit(f'unknown config {name}')
+        log(f'config {name}')
+        t_c = time.perf_counter()
+        configs[name] = runners[name]()
+        configs[name]['wall_s_including_setup'] = time.perf_counter() - t_c
+        log(f'config {name}: {configs[name]["value"]:.4g} {configs[name]["unit"]}')
+
+    cpu = strong = None
     if not args.no_cpu_baseline:
         log('CPU baseline (oracle restatement of the reference algorithm)')
         cpu = cpu_baseline()
         log(f'CPU baseline {cpu["value"]:.3f} solves/s on the sample')
+        if args.cpu_strong_n > 0:
+            strong = cpu_strong(args.cpu_strong_n)
+            log(f'strong CPU baseline: {strong["sample"]}')
 
+    method_desc = ('fused assembly + two-level (learned block Jacobi + rigid-body coarse correction) '
+                   'FP32 PCG with FP64 reliable updates, plain PCG as fallback' if model is not None else
+                   'fused assembly + two-level (block Jacobi + rigid-body coarse correction) FP32 PCG '
+                   'with FP64 reliable updates')
     out = {
         'metric': METRIC, 'value': value, 'unit': 'solves/s', 'n_gpus': 1, 'steps': args.steps,
         'warmup': args.warmup, 'ms_per_step': ms_per_step, 'higher_is_better': True, 'scaling': 'weak',
         'vs_baseline': None, 'dtype': 'f32+f64', 'data': 'synthetic',
         'config': {'workload': desc, 'dofs': int(sysm.n), 'nnz': int(sysm.pattern.nnz),
-                   'elements': int(sysm.n_elem), 'tol': 1e-8,
-                   'method': ('fused assembly + two-level (learned block Jacobi + rigid-body coarse '
-                              'correction) FP32 PCG with FP64 reliable updates' if model is not None else
-                              'fused assembly + two-level (block Jacobi + rigid-body coarse correction) '
-                              'FP32 PCG with FP64 reliable updates'),
+                   'elements': int(sysm.n_elem), 'tol': 1e-8, 'method': method_desc,
                    'coarse_dofs': int(coarse.n_coarse),
                    'step': 'assembly + Dirichlet elimination + preconditioner setup + PCG'},
-        'headline': {'method': headline['name'], 'iterations': info['iterations'],
+        'headline': {'method': headline['name'], **hs,
                      'median_ms_warm': float(np.median(step_s)) * 1e3,
                      'median_ms_cold': float(np.median(cold_s)) * 1e3 if cold_s else None,
-                     'relres_true_fp64': info['relres'], 'reliable_updates': info['reliable_updates'],
-                     'restarts': info['restarts'], 'fallbacks': info.get('fallbacks', 0)},
+                     'reliable_updates': infos[-1]['reliable_updates'],
+                     'restarts': infos[-1]['restarts']},
         'ablations': ablations,
         'plain_fp64_jacobi': None if plain_s is None else
-        {'solves_per_s': 1.0 / plain_s, 'ms': plain_s * 1e3, 'iterations': plain_info['iterations'],
-         'relres_true_fp64': plain_info['relres']},
+        {'solves_per_s': 1.0 / plain_s, 'ms': plain_s * 1e3, **summarize_infos(plain_infos)},
         'speedup_vs_plain': None if plain_s is None else value * plain_s,
         'assembly': asm,
         'roofline': roofline,
+        'configs': configs,
         'cpu_baseline': cpu,
+        'cpu_strong': strong,
     }
     print(json.dumps(out), flush=True)
 
