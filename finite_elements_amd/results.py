@@ -4,7 +4,8 @@
 classes read the first n entries of ``result_vector`` through the interleaved DOF map, as the
 reference's do.  Per-element strain, stress, energy and the magnetic field are computed on the GPU
 by ``fe_element_fields`` (one thread per element); the dictionaries keyed by element objects that
-the reference returns are built from those arrays.  Plotting and VTK export are not part of this
+the reference returns are built from those arrays.  ``generate_vtk_file`` writes the displacement
+field in the reference's legacy-VTK layout (finite_elements_amd.vtk).  Plotting is not part of this
 package.
 """
 
@@ -69,6 +70,18 @@ class ElasticityResults(Result):
                                                    enumerate(self.mesh.nodes)}
         return self._displacement_vectors_per_node
 
+
+    def generate_vtk_file(self, file_name_output: str) -> str:
+        """Mesh + displacement field as a legacy-ASCII VTK file (finite_elements/results.py:832-871):
+        '.vtk' is appended when missing; POINT_DATA holds Displacement_Magnitude and
+        Displacement_Vectors (2D vectors padded with 0), one entry per mesh node in node order."""
+        from . import vtk
+        from .analysis import mesh_arrays
+        if file_name_output[-3:] != 'vtk':
+            file_name_output += '.vtk'
+        _, coords, conn, _ = mesh_arrays(self.mesh)
+        name = getattr(self.mesh, 'name', '') or 'mesh'
+        return vtk.write_vtk(file_name_output, coords, conn, self.displacements_array(), title=name)
 
     # ---- per-element fields (finite_elements/results.py:689-830) ---------------------------
     def element_fields(self):
