@@ -30,6 +30,7 @@ FE_LAYOUT_BSR = 1
 FE_LAYOUT_PLANES = 2
 FE_PC_JACOBI = 1
 FE_PC_BLOCK_JACOBI = 2
+FE_COARSE_NOALLOC_MAX = 65536
 
 FE_PAT_NNZ, FE_PAT_NNZB, FE_PAT_NINC, FE_PAT_NCONTRIB, FE_PAT_NBLK, FE_PAT_MAXINC, FE_PAT_NROWS = range(7)
 FE_PAT_NINFO = 8
@@ -138,6 +139,7 @@ SIGNATURES = {
     'fe_coarse_ld': (_i64, [_i64]),
     'fe_coarse_operator': (_int, [_vp, _P(FeBsr), _P(FeCoarse), _int, _vp, _vp, _vp]),
     'fe_coarse_apply': (_int, [_vp, _P(FeCoarse), _int, _vp, _vp, _vp, _vp]),
+    'fe_coarse_apply_noalloc': (_int, [_vp, _P(FeCoarse), _int, _vp, _vp, _vp]),
     'fe_pcg_coarse_workspace': (_int, [_vp, _P(FeBsr), _P(FeCoarse), _P(FePcgOpts), _P(_i64)]),
     'fe_pcg_coarse': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp, _vp, _P(FeCoarse), _vp, _vp, _vp, _vp,
                              _vp, _i64, _P(FePcgOpts), _P(FePcgStats)]),

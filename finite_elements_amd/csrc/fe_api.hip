@@ -43,6 +43,13 @@ extern "C" int fe_create(fe_handle** out, int device, void* stream) {
     delete h;
     return fe_set_error((int)e, "hipHostMalloc: %s", hipGetErrorString(e));
   }
+  // device address of the allocation-free entry point's scratch array on this device
+  int rc = fe_coarse_scratch_setup(device);
+  if (rc) {
+    hipHostFree(h->pinned);
+    delete h;
+    return rc;
+  }
   *out = h;
   return FE_OK;
 }

@@ -15,6 +15,7 @@
 // (k_coarse_restrict_rm, k_coarse_prolong_rm): 15 bytes per node are read instead of P's d*nc values.
 // The dense inverse has a leading dimension padded to a multiple of 64 (identity in the padding).
 #include <algorithm>
+#include <mutex>
 
 #include "fe_sparse.h"
 
@@ -822,6 +823,69 @@ extern "C" int fe_coarse_operator(fe_handle* h, const fe_bsr* A, const fe_coarse
                      Ac, Ac32);
   FE_CHECK_LAUNCH();
   return FE_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// allocation-free coarse correction
+// ---------------------------------------------------------------------------------------------------
+// The two coarse vectors (r_c = P^T r and z_c = A_c^-1 r_c) of fe_coarse_apply_noalloc live in this
+// file-scope device array instead of a caller workspace.  Every GPU holds its own copy until the
+// process ends; its device address is looked up once per device (fe_coarse_scratch_setup, called by
+// fe_create with the device current) and only that address is handed to kernels.  Nothing in it needs
+// clearing: the restriction writes every r_c entry and the coarse solve every z_c entry.
+constexpr int64_t kNoallocScratchBytes = (int64_t)1 << 20;
+constexpr int kMaxScratchDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+static void* g_scratch_ptr[kMaxScratchDevices];
+static std::mutex g_scratch_mutex;
+
+static_assert(2 * (int64_t)FE_COARSE_NOALLOC_MAX * 8 <= kNoallocScratchBytes,
+              "FE_COARSE_NOALLOC_MAX FP64 coarse DOF must fit twice in the scratch array");
+
+int fe_coarse_scratch_setup(int device) {
+  if (device < 0 || device >= kMaxScratchDevices)
+    return fe_set_error(FE_ERR_UNSUPPORTED, "coarse scratch: device %d above %d", device,
+                        kMaxScratchDevices - 1);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[device]) return FE_OK;
+  int cur = -1;
+  FE_CHECK_HIP(hipGetDevice(&cur));
+  if (cur != device) return fe_set_error(FE_ERR_ARG, "coarse scratch: device %d is not current", device);
+  void* ptr = nullptr;
+  FE_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) return fe_set_error(FE_ERR_NOMEM, "coarse scratch: null symbol address on device %d", device);
+  g_scratch_ptr[device] = ptr;
+  return FE_OK;
+}
+
+extern "C" int fe_coarse_apply_noalloc(fe_handle* h, const fe_coarse* C, int precision,
+                                       const void* Ainv, const void* r, void* u) {
+  FE_ENTER(h);
+  int rc = check_coarse(C, "fe_coarse_apply_noalloc");
+  if (rc) return rc;
+  FE_REQUIRE(precision == 0 || precision == 1, "fe_coarse_apply_noalloc: precision must be 0 or 1");
+  FE_REQUIRE(precision == 0 || C->P32, "fe_coarse_apply_noalloc: FP32 needs P32");
+  FE_REQUIRE(Ainv && r && u, "fe_coarse_apply_noalloc: null argument");
+  const int64_t nct = C->n_agg * C->nc;
+  FE_REQUIRE(nct <= FE_COARSE_NOALLOC_MAX,
+             "fe_coarse_apply_noalloc: %lld coarse DOF above the allocation-free maximum %d "
+             "(use fe_coarse_apply with a workspace)", (long long)nct, FE_COARSE_NOALLOC_MAX);
+  void* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    base = h->device >= 0 && h->device < kMaxScratchDevices ? g_scratch_ptr[h->device] : nullptr;
+  }
+  if (!base) {
+    rc = fe_coarse_scratch_setup(h->device);  // first call on a device the handle did not set up
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    base = g_scratch_ptr[h->device];
+  }
+  const size_t el = precision ? sizeof(float) : sizeof(double);
+  char* w = (char*)base;
+  void* rcv = w;
+  void* zcv = w + ((nct * el + 255) / 256) * 256;
+  return fe_coarse_correction(h->stream, C, precision, Ainv, r, u, rcv, zcv, nullptr, 0, 2);
 }
 
 extern "C" int fe_coarse_apply(fe_handle* h, const fe_coarse* C, int precision, const void* Ainv,

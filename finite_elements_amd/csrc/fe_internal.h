@@ -28,6 +28,10 @@ int fe_coarse_correction(hipStream_t stream, const fe_coarse* C, int fp32, const
                          const void* r, void* u, void* rc, void* zc, const int32_t* state,
                          int state_stride, int mode);
 
+// Look up (once per device, with that device current) the device address of the file-scope scratch
+// array of fe_coarse_apply_noalloc (fe_coarse.hip); fe_create calls it.
+int fe_coarse_scratch_setup(int device);
+
 #define FE_CHECK_HIP(call)                                                              \
   do {                                                                                  \
     hipError_t e_ = (call);                                                             \

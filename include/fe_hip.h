@@ -372,6 +372,15 @@ int fe_coarse_operator(fe_handle* h, const fe_bsr* A, const fe_coarse* C, int in
    A block-diagonal space takes its FP64 blocks as Ainv at either precision. */
 int fe_coarse_apply(fe_handle* h, const fe_coarse* C, int precision, const void* Ainv, const void* r,
                     void* u, void* work);
+/* fe_coarse_apply without a workspace, for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+   reference.  The coarse vectors live in a static device array of the library (1 MiB per GPU, held
+   until the process ends), so n_agg * nc must be <= FE_COARSE_NOALLOC_MAX (FE_ERR_ARG otherwise);
+   the call allocates nothing.  Every call on one GPU shares that array: calls that overlap on one GPU
+   (several streams or threads) race and must be serialised by the caller. */
+#define FE_COARSE_NOALLOC_MAX 65536
+int fe_coarse_apply_noalloc(fe_handle* h, const fe_coarse* C, int precision, const void* Ainv,
+                            const void* r, void* u);
 /* PCG with the two-level preconditioner (one system): as fe_pcg, with the coarse correction added to
    the block-Jacobi (or learned) M after every update of the residual.  Ainv64/Ainv32 from
    fe_coarse_operator (Ainv32 for opts->mixed).  fe_pcg_batched_coarse: the same for the segments of
