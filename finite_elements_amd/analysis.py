@@ -22,6 +22,7 @@ import scipy.sparse as sp
 
 from . import conditions as fe_cond
 from . import core
+from . import lowering
 from .elements import (ElasticityElement, ElasticityTetrahedralElement3D,
                        ElasticityTriangularElement2D, MagneticElement2D, _plane_mode)
 from .loads import NodeLoad
@@ -165,6 +166,7 @@ class FiniteElements:
         self._positions = None
         self._system = None
         self._arrays = None
+        self._element_index = None
         self.last_solve_info = None
 
     # ---- geometry / DOF map ------------------------------------------------------------------
@@ -248,12 +250,55 @@ class FiniteElements:
             self._boundary_conditions = node_boundary_from_dict(node_boundary_conditions_to_dict(bcs))
         return self._boundary_conditions
 
+    def _elements_index(self, elements):
+        if self._element_index is None:
+            self._element_index = lowering.ElementIndex(self.mesh)
+        return self._element_index(elements)
+
+    def _node_indices(self, nodes):
+        idx = self.mesh.node_to_index
+        return np.fromiter((idx[p] for p in nodes), dtype=np.int64, count=len(nodes))
+
+    def _triangle_arrays(self):
+        """(coords, conn) when the mesh is made of triangles (the element factors of the array
+        lowering are the triangle's), else None."""
+        kind, coords, conn, _ = self._mesh_arrays()
+        return (coords, conn) if kind in ('tri_elastic', 'tri_magnetic') else None
+
+    def _collapsed_condition_arrays(self):
+        """(nodes, dims, values) of the collapsed Dirichlet conditions in condition-row order: node
+        conditions, element conditions (value * factor_i), edge conditions (value * 0.5), collapsed
+        per (node, dimension) with the last write winning (analysis.py:241-279)."""
+        tri = self._triangle_arrays()
+        if self.element_boundary_conditions and tri is None:
+            bcs = self._collapsed_boundary_conditions()
+            return (self._node_indices([bc.application for bc in bcs]),
+                    np.array([bc.dimension for bc in bcs], dtype=np.int64),
+                    np.array([bc.source_c_matrix() for bc in bcs], dtype=np.float64))
+        parts = []
+        nb = self.node_boundary_conditions
+        if nb:
+            parts.append((self._node_indices([bc.application for bc in nb]),
+                          np.array([bc.dimension for bc in nb], dtype=np.int64),
+                          np.array([+bc.value for bc in nb], dtype=np.float64)))
+        eb = self.element_boundary_conditions
+        if eb:
+            ei = self._elements_index([c.application for c in eb])
+            parts.append(lowering.element_condition_items(tri[0], tri[1], ei, [c.value for c in eb],
+                                                          [c.dimension for c in eb]))
+        gb = self.edge_boundary_conditions
+        if gb:
+            ends = [p for c in gb for p in (c.application.start, c.application.end)]
+            parts.append((self._node_indices(ends), np.repeat([c.dimension for c in gb], 2),
+                          np.repeat([c.value * 0.5 for c in gb], 2).astype(np.float64)))
+        if not parts:
+            return np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0)
+        return lowering.collapse(*(np.concatenate(x) for x in zip(*parts)))
+
     def boundary_condition_dofs(self):
         """(DOF positions, values) of the collapsed conditions, in condition-row order."""
-        bcs = self._collapsed_boundary_conditions()
-        pos = np.array([self._dof(bc.application, bc.dimension) for bc in bcs], dtype=np.int64)
-        val = np.array([bc.source_c_matrix() for bc in bcs], dtype=np.float64)
-        return pos, val
+        nodes, dims, vals = self._collapsed_condition_arrays()
+        return nodes * self.dimension + (dims - 1), vals
 
     def c_matrix_boundary_conditions(self):
         """(data, row, col) of the constraint entries (analysis.py:241-279)."""
@@ -301,20 +346,64 @@ class FiniteElements:
         n = self.n_dofs
         return val.tolist(), [n + i for i in range(len(val))]
 
+    def load_arrays(self):
+        """(DOF rows, values) of the collapsed loads in the reference's order: node loads, element
+        loads (value * A_e / sum A * factor_i), edge loads (value * 0.5), collapsed per (node,
+        dimension) with the last write winning (analysis.py:457-489, 407-447)."""
+        tri = self._triangle_arrays()
+        if self.element_loads and tri is None:
+            data, row = self.source_c_matrix_loads()
+            return np.asarray(row, dtype=np.int64), np.asarray(data, dtype=np.float64)
+        parts = []
+        nl = self.node_loads
+        if nl:
+            parts.append((self._node_indices([ld.node for ld in nl]),
+                          np.array([ld.dimension for ld in nl], dtype=np.int64),
+                          np.array([+ld.value for ld in nl], dtype=np.float64)))
+        for load in self.element_loads:
+            ei = self._elements_index(load.elements)
+            parts.append(lowering.element_load_items(tri[0], tri[1], ei, load.value, load.dimension))
+        el = self.edge_loads
+        if el:
+            ends = [p for ld in el for p in (ld.edge.start, ld.edge.end)]
+            parts.append((self._node_indices(ends), np.repeat([ld.dimension for ld in el], 2),
+                          np.repeat([ld.value * 0.5 for ld in el], 2).astype(np.float64)))
+        if not parts:
+            return np.zeros(0, np.int64), np.zeros(0)
+        nodes, dims, vals = lowering.collapse(*(np.concatenate(x) for x in zip(*parts)))
+        return nodes * self.dimension + (dims - 1), vals
+
+    def magnet_arrays(self):
+        """(node rows, values) of the magnet loads, load by load (analysis.py:556-577); scalar
+        (magnetostatic) meshes only."""
+        if not self.magnet_loads:
+            return np.zeros(0, np.int64), np.zeros(0)
+        kind, coords, conn, _ = self._mesh_arrays()
+        if kind != 'tri_magnetic':
+            raise ValueError('magnet loads apply to magnetostatic (MagneticElement2D) meshes')
+        rows, vals = [], []
+        for ml in self.magnet_loads:
+            ei = self._elements_index(ml.elements)
+            inner = self._node_indices(list(ml.non_contour_nodes))
+            m = ml.magnetization_vector
+            r, v = lowering.magnet_sources(coords, conn, ei, inner, (float(m[0]), float(m[1])))
+            rows.append(r)
+            vals.append(v)
+        return np.concatenate(rows), np.concatenate(vals)
+
     def source_c_matrix_magnet_loads(self):
-        if self.magnet_loads:
-            raise NotImplementedError('magnet loads are not supported by finite_elements_amd')
-        return [], []
+        """(data, row) lists of the magnet loads (analysis.py:556-577)."""
+        rows, vals = self.magnet_arrays()
+        return vals.tolist(), rows.tolist()
 
     def load_vector(self):
-        """DOF part (length n) of the reference's source matrix."""
+        """DOF part (length n) of the reference's source matrix: the collapsed loads, then the
+        magnet sources added one by one in the reference's order."""
         f = np.zeros(self.n_dofs, dtype=np.float64)
-        data, row = self.source_c_matrix_loads()
-        for d, r in zip(data, row):
-            f[r] += d
-        mdata, mrow = self.source_c_matrix_magnet_loads()
-        for d, r in zip(mdata, mrow):
-            f[r] += d
+        rows, vals = self.load_arrays()
+        np.add.at(f, rows, vals)
+        mrows, mvals = self.magnet_arrays()
+        np.add.at(f, mrows, mvals)
         return f
 
     # ---- element matrices ----------------------------------------------------------------------

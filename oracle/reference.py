@@ -339,6 +339,39 @@ def collapsed_node_loads(problem):
     return [(k[0], v, k[1]) for k, v in _last_write_wins(items).items()]
 
 
+def magnet_sources(elements, non_contour_nodes, magnetization):
+    """finite_elements/loads.py:129-147 and analysis.py:556-577, one magnet load: count the edges
+    of the magnet's triangles (an edge is the same in either orientation; edges taken (v0, v1),
+    (v1, v2), (v2, v0) element by element), keep those used once and not with both ends in
+    non_contour_nodes, and give both ends of each M . dl * length / 2 with dl = (-n_y, n_x), n the
+    unit normal of the edge pointing into its triangle.  elements: vertex tuples.  Returns
+    [(node, value)] in the reference's order."""
+    count = {}
+    for pts in elements:
+        for k in range(3):
+            a, b, o = pts[k], pts[(k + 1) % 3], pts[(k + 2) % 3]
+            key = frozenset((a, b))
+            if key in count:
+                count[key][1] += 1
+            else:
+                count[key] = [(a, b, o), 1]
+    out = []
+    inner = set(non_contour_nodes)
+    for (a, b, o), c in count.values():
+        if c != 1 or (a in inner and b in inner):
+            continue
+        tx, ty = b[0] - a[0], b[1] - a[1]
+        length = (tx * tx + ty * ty) ** 0.5
+        nx, ny = -ty / length, tx / length
+        if nx * (o[0] - a[0]) + ny * (o[1] - a[1]) < 0:
+            nx, ny = -nx, -ny
+        dl = (-ny, nx)
+        v = (magnetization[0] * dl[0] + magnetization[1] * dl[1]) * length / 2
+        out.append((a, v))
+        out.append((b, v))
+    return out
+
+
 def c_matrix_boundary_conditions(problem, positions, node_to_index):
     """finite_elements/analysis.py:267-279."""
     n = len(problem.nodes) * problem.dimension
