@@ -240,7 +240,9 @@ def cpu_strong(n=32, iter_cap=None):
     from oracle import strong_cpu
     threads = min(16, len(os.sched_getaffinity(0)))
     pr = meshgen.cube_elasticity(n=n, seed=4)
-    r = strong_cpu.solve_cube(pr, tol=1e-8, threads=threads, iter_cap=iter_cap)
+    r = strong_cpu.solve_cube(pr, tol=1e-8, threads=threads, iter_cap=iter_cap,
+                              progress=lambda it, rr: log(f'strong CPU {n}^3: iteration {it}, '
+                                                          f'relres {rr:.2e}'))
     total = r['assemble_s'] + r['eliminate_s'] + r['cg_s']
     return {'value': 1.0 / total, 'unit': 'solves/s', 'cores': threads, 'kind': 'port',
             'sample': (f'{n}^3 cube ({r["dofs"]} DOF): vectorised numpy assembly '

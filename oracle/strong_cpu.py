@@ -47,9 +47,10 @@ class ThreadedCsr:
         self.pool.shutdown()
 
 
-def jacobi_cg(K, b, tol=1e-8, max_iter=100000, threads=1, iter_cap=None):
+def jacobi_cg(K, b, tol=1e-8, max_iter=100000, threads=1, iter_cap=None, progress=None):
     """Jacobi-preconditioned CG from zero; returns (x, iterations, true relres, seconds per
-    iteration).  iter_cap stops early (a bounded timing sample): the relres is then not reached."""
+    iteration).  iter_cap stops early (a bounded timing sample): the relres is then not reached.
+    progress(it, relres_estimate) is called every 250 iterations."""
     op = ThreadedCsr(K, threads)
     dinv = 1.0 / K.diagonal()
     n = b.size
@@ -74,13 +75,15 @@ def jacobi_cg(K, b, tol=1e-8, max_iter=100000, threads=1, iter_cap=None):
         p += z
         rz = rz_new
         it += 1
+        if progress is not None and it % 250 == 0:
+            progress(it, float(np.sqrt(float(r @ r) / bb)))
     per_it = (time.perf_counter() - t0) / max(it, 1)
     relres = float(np.linalg.norm(b - op.matvec(x, q)) / np.sqrt(bb))
     op.close()
     return x, it, relres, per_it
 
 
-def solve_cube(pr, tol=1e-8, threads=1, iter_cap=None):
+def solve_cube(pr, tol=1e-8, threads=1, iter_cap=None, progress=None):
     """Assemble and solve one Problem; returns a dict of timings."""
     t0 = time.perf_counter()
     K = vz.assemble_csr(pr.kind, pr.coords, pr.conn, pr.materials, pr.plane_mode)
@@ -90,7 +93,8 @@ def solve_cube(pr, tol=1e-8, threads=1, iter_cap=None):
     Kff = K[free][:, free].tocsr()
     b = np.asarray(pr.f, dtype=np.float64)[free]
     t2 = time.perf_counter()
-    _, its, relres, per_it = jacobi_cg(Kff, b, tol=tol, threads=threads, iter_cap=iter_cap)
+    _, its, relres, per_it = jacobi_cg(Kff, b, tol=tol, threads=threads, iter_cap=iter_cap,
+                                       progress=progress)
     t3 = time.perf_counter()
     return {'dofs': int(K.shape[0]), 'assemble_s': t1 - t0, 'eliminate_s': t2 - t1,
             'cg_s': t3 - t2, 'iterations': its, 'relres': relres, 's_per_iteration': per_it}
