@@ -28,7 +28,7 @@ constexpr int kNB = 64;  // Gauss-Jordan block size
 // ---------------------------------------------------------------------------------------------------
 // coarse operator
 // ---------------------------------------------------------------------------------------------------
-constexpr int kAcThreads = 128;
+constexpr int kAcThreads = 256;
 
 // One workgroup per coarse pair (I, J); its threads stride over the pair's fine blocks (ascending
 // order), each accumulating Z_i^T A_ij Z_j, then a fixed-order workgroup reduction.
@@ -258,11 +258,26 @@ __device__ inline void tile_gemm_acc(double (*sX)[kNB + 1], double (*sY)[kNB + 1
   }
 }
 
+constexpr int kGjThreads = 256;
+
+// A 64 x 64 tile into LDS by the 256 threads: each thread's eight 16-byte loads are issued together
+// (the rows are 16-byte aligned: ld and the tile offsets are multiples of 64).
 __device__ void load_tile(const double* __restrict__ A, int64_t ld, int bi, int bj,
                           double (*s)[kNB + 1]) {
-  for (int idx = threadIdx.x; idx < kNB * kNB; idx += blockDim.x) {
+  constexpr int kPer = kNB * kNB / 2 / kGjThreads;
+  double2 v[kPer];
+#pragma unroll
+  for (int it = 0; it < kPer; ++it) {
+    const int idx = 2 * ((int)threadIdx.x + it * kGjThreads);
     const int r = idx / kNB, c = idx % kNB;
-    s[r][c] = A[(int64_t)(bi * kNB + r) * ld + bj * kNB + c];
+    v[it] = *reinterpret_cast<const double2*>(A + (int64_t)(bi * kNB + r) * ld + bj * kNB + c);
+  }
+#pragma unroll
+  for (int it = 0; it < kPer; ++it) {
+    const int idx = 2 * ((int)threadIdx.x + it * kGjThreads);
+    const int r = idx / kNB, c = idx % kNB;
+    s[r][c] = v[it].x;
+    s[r][c + 1] = v[it].y;
   }
 }
 
@@ -298,8 +313,6 @@ __device__ void gj_invert_tile(double (&a)[16], GjLds& L) {
     }
   }
 }
-
-constexpr int kGjThreads = 256;
 
 // S = inv(A_00) (the first pivot block; later ones are inverted inside k_gj_update).
 __global__ __launch_bounds__(kGjThreads) void k_gj_inv(int64_t ld, const double* __restrict__ A,
@@ -351,21 +364,33 @@ __global__ __launch_bounds__(kGjThreads) void k_gj_update(int64_t ld, int k, con
     return;
   }
   load_tile(B, ld, k, j, sY);
+  // this thread's 4 x 4 outputs of A_ij, loaded before the product so that their latency overlaps it
+  const int tr = (threadIdx.x / 16) * 4, tc = (threadIdx.x % 16) * 4;
+  double aij[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const double* row = A + (int64_t)(i * kNB + tr + a) * ld + j * kNB + tc;
+    const double2 lo = *reinterpret_cast<const double2*>(row);
+    const double2 hi = *reinterpret_cast<const double2*>(row + 2);
+    aij[a][0] = lo.x, aij[a][1] = lo.y, aij[a][2] = hi.x, aij[a][3] = hi.y;
+  }
   __syncthreads();
   double acc[4][4];
   tile_gemm_acc(sX, sY, acc);
-  const int tr = (threadIdx.x / 16) * 4, tc = (threadIdx.x % 16) * 4;
   const bool pivot_next = (i == k + 1 && j == k + 1);
   if (pivot_next) __syncthreads();  // sX is reused for the new block below
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 4; ++a) {
+    double v[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const int64_t o = (int64_t)(i * kNB + tr + a) * ld + j * kNB + tc + b;
-      const double v = A[o] - acc[a][b];
-      B[o] = v;
-      if (pivot_next) sX[tr + a][tc + b] = v;
+      v[b] = aij[a][b] - acc[a][b];
+      if (pivot_next) sX[tr + a][tc + b] = v[b];
     }
+    double* row = B + (int64_t)(i * kNB + tr + a) * ld + j * kNB + tc;
+    *reinterpret_cast<double2*>(row) = double2{v[0], v[1]};
+    *reinterpret_cast<double2*>(row + 2) = double2{v[2], v[3]};
+  }
   if (!pivot_next) return;
   // the next pivot block (this workgroup's own output): invert it into Snext
   __syncthreads();

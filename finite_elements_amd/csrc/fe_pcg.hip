@@ -48,6 +48,7 @@ struct PcgState {
 
 struct PcgStatus {
   int32_t active, max_iter, n_ru, pad;
+  double rr_ratio;  // largest rr / (tol^2 b.b) over the running segments (the distance to the target)
 };
 
 struct Tile {
@@ -518,22 +519,27 @@ __global__ __launch_bounds__(kRed) void k_pcg_init(const int32_t* __restrict__ s
 
 __global__ __launch_bounds__(1024) void k_pcg_status(int nseg, const PcgState* st, PcgStatus* out) {
   __shared__ int s_act[1024 / 64], s_max[1024 / 64], s_ru[1024 / 64];
+  __shared__ double s_ratio[1024 / 64];
   int act = 0, mx = 0, ru = 0;
+  double ratio = 0.0;
   for (int g = threadIdx.x; g < nseg; g += blockDim.x) {
     act += st[g].done == 0;
     mx = max(mx, st[g].iter);
     ru += st[g].n_ru;
+    if (st[g].done == 0 && st[g].tol2bb > 0.0) ratio = fmax(ratio, st[g].rr / st[g].tol2bb);
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     act += __shfl_xor(act, off, 64);
     mx = max(mx, __shfl_xor(mx, off, 64));
     ru += __shfl_xor(ru, off, 64);
+    ratio = fmax(ratio, __shfl_xor(ratio, off, 64));
   }
   if ((threadIdx.x & 63) == 0) {
     s_act[threadIdx.x >> 6] = act;
     s_max[threadIdx.x >> 6] = mx;
     s_ru[threadIdx.x >> 6] = ru;
+    s_ratio[threadIdx.x >> 6] = ratio;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -541,10 +547,12 @@ __global__ __launch_bounds__(1024) void k_pcg_status(int nseg, const PcgState* s
       act += s_act[w];
       mx = max(mx, s_max[w]);
       ru += s_ru[w];
+      ratio = fmax(ratio, s_ratio[w]);
     }
     out->active = act;
     out->max_iter = mx;
     out->n_ru = ru;
+    out->rr_ratio = ratio;
   }
 }
 
@@ -759,26 +767,37 @@ int run_pcg(fe_handle* h, const Problem& pb, const Work& w, const fe_pcg_opts* o
     FE_CHECK_HIP(hipEventRecord(ev[slot], cs));
     return FE_OK;
   };
-  // One chunk is always queued ahead of the one being polled, so the device never waits for the
-  // host; chunks queued after convergence find every segment done and return at once.
+  // Two chunks are kept in flight, so the device never waits for the host, except when the residual
+  // trend of the last two polls predicts that the chunk already queued ends the solve: then the host
+  // waits for it before queueing another (a chunk queued after convergence finds every segment done
+  // and returns at once, but its launches still cost device time).
   const int64_t max_chunks = 4 * ((int64_t)o->max_iter / chunk + 2) + 8;
   PcgStatus last{};
   int finished = 0;
-  if (rc == FE_OK) rc = enqueue_chunk(0);
-  if (rc == FE_OK) rc = enqueue_chunk(1);
-  for (int64_t c = 0; rc == FE_OK && c < max_chunks; ++c) {
-    const int slot = (int)(c & 1);
+  int64_t queued = 0, polled = 0;
+  double prev_ratio = -1.0;
+  if (rc == FE_OK) rc = enqueue_chunk((int)(queued++ & 1));
+  if (rc == FE_OK) rc = enqueue_chunk((int)(queued++ & 1));
+  while (rc == FE_OK && polled < queued && polled < max_chunks) {
+    const int slot = (int)(polled & 1);
     hipError_t e = hipEventSynchronize(ev[slot]);
     if (e != hipSuccess) {
       rc = fe_set_error((int)e, "hipEventSynchronize: %s", hipGetErrorString(e));
       break;
     }
     last = pinned[slot];
+    ++polled;
     if (last.active == 0) {
       finished = 1;
       break;
     }
-    rc = enqueue_chunk(slot);
+    const double ratio = last.rr_ratio;
+    const bool ends_next = prev_ratio > 0.0 && ratio > 0.0 && ratio < prev_ratio &&
+                           ratio * (ratio / prev_ratio) <= 1.0;
+    prev_ratio = ratio;
+    const int64_t ahead = ends_next ? 1 : 2;
+    while (rc == FE_OK && queued - polled < ahead && queued < max_chunks)
+      rc = enqueue_chunk((int)(queued++ & 1));
   }
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);

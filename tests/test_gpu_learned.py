@@ -135,9 +135,11 @@ def test_training_reduces_loss_and_reports_gain(ctx):
         gains.append(ib['iterations'] / il['iterations'])
         _say(f'  seed {seed}: block-Jacobi {ib["iterations"]} it, learned {il["iterations"]} it')
     _say(f'  iteration ratio plain/learned: {np.round(gains, 3)}')
-    # the learned preconditioner must stay a valid SPD preconditioner that converges; its gain is
-    # reported (and checked against the fallback policy) rather than assumed
-    assert min(gains) > 0.5
+    # the learned preconditioner must stay a valid SPD preconditioner that converges.  It does not
+    # gain on these fields (measured ratios 0.88-0.90: a per-node rescaling of block Jacobi has no
+    # leverage here), which is why the bench's headline is the method without it; the bound keeps
+    # the loss from growing unnoticed
+    assert min(gains) >= 0.8
 
 
 def test_two_level_training_ablation(ctx):
@@ -161,4 +163,5 @@ def test_two_level_training_ablation(ctx):
         gains.append(ib['iterations'] / il['iterations'])
         _say(f'  seed {seed}: two-level {ib["iterations"]} it, learned two-level {il["iterations"]} it')
     _say(f'  iteration ratio without/with network: {np.round(gains, 3)}')
-    assert min(gains) > 0.5
+    # measured 0.89-0.98: the network costs iterations as the two-level smoother too (see above)
+    assert min(gains) >= 0.8
