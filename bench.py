@@ -131,39 +131,47 @@ def profile_spmv(sysm, method, fixed, g, f_t, tol=1e-8):
 
 
 def spmv_cache_study(sysm, fixed, g, f_t, reps=20):
-    """The padded iteration matrices' SpMV (k_spmv_vec, the row product k_cg_spmv runs) timed with
-    HIP events: back to back (the matrix may stay in the 256 MB Infinity Cache) and one launch at a
-    time after a 1 GiB write that flushes L2 and the Infinity Cache, in FP32 (183 MB of values) and
-    FP64 (the plain Jacobi iteration's copy, too large for the cache)."""
+    """The padded iteration matrix's SpMV (k_spmv_vec, the row product k_cg_spmv runs) timed with
+    HIP events three ways, in FP32 (183 MB of values: fits the 256 MB Infinity Cache) and FP64 (the
+    plain Jacobi iteration's copy, 366 MB: does not): back to back on one copy (the PCG's regime:
+    the matrix is re-read every iteration), back to back alternating between two copies (their sum
+    exceeds the cache, so every launch streams from HBM), and single launches after a 1 GiB write
+    that flushes L2 and the cache (cold, launch latency included)."""
     import torch
     from finite_elements_amd import device
     K = sysm.stiffness_bsr()
     KD, _ = sysm.prepare(K, fixed, g, f_t)
     out = {}
     flush = torch.empty(1 << 28, dtype=torch.float32, device='cuda')
-    for name, dt in (('fp32', torch.float32), ('fp64', torch.float64)):
-        A = KD.padded(dt)
-        x = torch.ones(A.n, dtype=dt, device='cuda')
-        y = torch.empty_like(x)
-        device.spmv(A, x, y)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn):
         torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
-        for _ in range(reps):
-            device.spmv(A, x, y)
+        for k in range(reps):
+            fn(k)
         e1.record()
         torch.cuda.synchronize()
-        warm = e0.elapsed_time(e1) / reps
+        return e0.elapsed_time(e1) / reps * 1e3
+
+    for name, dt in (('fp32', torch.float32), ('fp64', torch.float64)):
+        mats = [KD.padded(dt), KD.padded(dt)]
+        x = torch.ones(mats[0].n, dtype=dt, device='cuda')
+        y = torch.empty_like(x)
+        device.spmv(mats[0], x, y)
+        warm = timed(lambda k: device.spmv(mats[0], x, y))
+        alt = timed(lambda k: device.spmv(mats[k & 1], x, y))
         cold = []
         for _ in range(reps):
             flush.fill_(1.0)
             e0.record()
-            device.spmv(A, x, y)
+            device.spmv(mats[0], x, y)
             e1.record()
             torch.cuda.synchronize()
-            cold.append(e0.elapsed_time(e1))
-        out[name] = {'warm_us': warm * 1e3, 'cold_us': float(np.median(cold)) * 1e3,
-                     'matrix_value_mb': A.vals.numel() * A.vals.element_size() / 1e6}
+            cold.append(e0.elapsed_time(e1) * 1e3)
+        out[name] = {'one_copy_us': warm, 'two_copies_us': alt, 'after_flush_us': float(np.median(cold)),
+                     'matrix_value_mb': mats[0].vals.numel() * mats[0].vals.element_size() / 1e6}
+        del mats
     del flush
     return out
 
@@ -723,8 +731,9 @@ def main():
     # the FP32 iteration SpMV on the padded pattern: 4 lanes per row (d = 3), else 2, 16-byte loads
     spmv_kernel = f'k_cg_spmv<float, {sysm.dof}, {4 if sysm.dof == 3 else 2}, true>'
     cache = spmv_cache_study(sysm, fixed, g, f_t)
-    log(f'SpMV warm/cold (1 GiB flush): FP32 {cache["fp32"]["warm_us"]:.1f}/{cache["fp32"]["cold_us"]:.1f} us, '
-        f'FP64 {cache["fp64"]["warm_us"]:.1f}/{cache["fp64"]["cold_us"]:.1f} us')
+    log('SpMV one copy / two alternating copies / after a flush: ' +
+        ', '.join(f'{k} {v["one_copy_us"]:.1f}/{v["two_copies_us"]:.1f}/{v["after_flush_us"]:.1f} us'
+                  for k, v in cache.items()))
     roofline = {'kernel': spmv_kernel,
                 'bound': 'hbm', 'achieved': spmv_bytes / spmv_ms / 1e6, 'peak': HBM_PEAK_GBS,
                 'unit': 'GB/s', 'frac': spmv_bytes / spmv_ms / 1e6 / HBM_PEAK_GBS,

@@ -129,6 +129,7 @@ SIGNATURES = {
     'fe_element_fields': (_int, [_vp, _P(FeMesh), _vp, _vp, _vp, _vp]),
     'fe_schedule_elements': (_int, [_vp, _vp, _int, _i64, _vp, _i64, _vp, _vp, _vp, _P(_i32)]),
     'fe_bsr_repack': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp]),
+    'fe_bsr_fold': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp, _vp, _vp]),
     'fe_precond_setup': (_int, [_vp, _P(FeBsr), _int, _vp]),
     'fe_pcg_workspace': (_int, [_vp, _P(FeBsr), _P(FePcgOpts), _P(_i64)]),
     'fe_pcg': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp, _vp, _vp, _vp, _vp, _i64, _P(FePcgOpts),

@@ -218,13 +218,44 @@ class FiniteElements:
         return self._system
 
     # ---- conditions ----------------------------------------------------------------------------
+    def continuity_arrays(self):
+        """(node1 indices, node2 indices, values) of the continuity conditions, in order."""
+        cc = self.continuity_conditions or []
+        if cc and self.dimension != 1:
+            raise NotImplementedError('continuity conditions are defined for scalar (magnetic) '
+                                      'problems: the reference uses node indices as DOF rows')
+        return (self._node_indices([c.node1 for c in cc]), self._node_indices([c.node2 for c in cc]),
+                np.array([float(c.value) for c in cc], dtype=np.float64))
+
     def c_matrix_continuity_conditions(self):
-        if self.continuity_conditions:
-            raise NotImplementedError(
-                'continuity conditions are not supported: the reference indexes their rows with '
-                'len(nodes)+len(node_loads)+i and node indices (analysis.py:190-197), which collide '
-                'with DOF rows')
-        return [], [], []
+        """(data, row, col) of the continuity entries (analysis.py:175-199): per condition
+        (1, 1, -value, -value) at (R, i1), (i1, R), (R, i2), (i2, R).  The reference's row index
+        len(nodes) + len(node_loads) + i falls on Dirichlet rows; R = n + n_bc + i here, after the
+        Dirichlet rows, the layout of the source vector's length n + n_continuity + n_bc
+        (analysis.py:737-739)."""
+        n1, n2, v = self.continuity_arrays()
+        base = self.n_dofs + len(self._collapsed_boundary_conditions())
+        data, row, col = [], [], []
+        for i, (a, b, val) in enumerate(zip(n1.tolist(), n2.tolist(), v.tolist())):
+            data.extend((1, 1, -val, -val))
+            row.extend((base + i, a, base + i, b))
+            col.extend((a, base + i, b, base + i))
+        return data, row, col
+
+    def continuity_fold(self):
+        """Elimination of the continuity conditions on this mesh's pattern (cached), or None."""
+        if not self.continuity_conditions:
+            return None
+        n1, n2, v = self.continuity_arrays()
+        key = (n1.tobytes(), n2.tobytes(), v.tobytes())
+        cached = getattr(self, '_fold', None)
+        if cached is None or cached[0] != key:
+            from .continuity import ContinuityFold
+            pat = self.system().pattern
+            cached = (key, ContinuityFold(pat.bptr, pat.bcol, len(self.mesh.nodes), self.dimension,
+                                          n1, n2, v))
+            self._fold = cached
+        return cached[1]
 
     def boundary_conditions_element_to_node(self):
         out = []
@@ -492,9 +523,17 @@ class FiniteElementAnalysis(FiniteElements):
     """Assembly and solve (finite_elements/analysis.py:580-830) on the MI355X path."""
 
     def create_matrix(self):
-        """Augmented [[K, C^T], [C, 0]] as scipy CSR, same pattern as the reference's (analysis.py:661)."""
-        self.c_matrix_continuity_conditions()
+        """Augmented [[K, C^T], [C, 0]] as scipy CSR, same pattern as the reference's (analysis.py:661):
+        K, then the continuity rows, then the Dirichlet rows' entries."""
         K = self._assembled_csr('stiffness')
+        if self.continuity_conditions:
+            Kc = K.tocoo()
+            cd, cr, cc = self.c_matrix_continuity_conditions()
+            bd, br, bc = self.c_matrix_boundary_conditions()
+            data = np.concatenate([Kc.data, np.asarray(cd, dtype=np.float64), np.asarray(bd, dtype=np.float64)])
+            row = np.concatenate([Kc.row, np.asarray(cr, dtype=np.int64), np.asarray(br, dtype=np.int64)])
+            col = np.concatenate([Kc.col, np.asarray(cc, dtype=np.int64), np.asarray(bc, dtype=np.int64)])
+            return sp.csr_matrix((data, (row, col)))
         pos, _ = self.boundary_condition_dofs()
         n = self.n_dofs
         nbc = len(pos)
@@ -547,7 +586,9 @@ class FiniteElementAnalysis(FiniteElements):
         """Solve and return ``Result(mesh, result_vector)`` (analysis.py:798-830).
 
         ``result_vector`` = n DOF values followed by one Lagrange multiplier per collapsed condition,
-        in the reference's condition order.  method: 'gpu' (PCG on the MI355X).  precision: 'fp64' or
+        in the reference's condition order, then one per continuity condition.  Continuity
+        conditions u(node1) = value u(node2) are eliminated (``ContinuityFold``): the folded matrix
+        T^T K T is formed on the GPU and node1 is recovered from node2 after the solve.  method: 'gpu' (PCG on the MI355X).  precision: 'fp64' or
         'mixed' (FP32 SpMV/preconditioner inside FP64 reliable updates).  precond: 'jacobi',
         'block_jacobi', 'two_level' (block Jacobi plus a coarse correction over aggregated rigid-body
         modes), 'learned' (the learned blocks in place of block Jacobi; needs ``learned``, a trained
@@ -560,16 +601,28 @@ class FiniteElementAnalysis(FiniteElements):
         """
         if method != 'gpu':
             raise ValueError("finite_elements_amd solves on the GPU only (method='gpu')")
-        self.c_matrix_continuity_conditions()
         sysm = self.system()
         self._plane_mode()
         pos, val = self.boundary_condition_dofs()
         f = self.load_vector()
         ctx = sysm.ctx
         K = sysm.stiffness_bsr()
-        fixed, g = sysm.dirichlet(pos, val)
+        fold = self.continuity_fold()
+        if fold is None:
+            fixed_dofs, fixed_vals, Kr, fr = pos, val, K, f
+        else:
+            sl = fold.slave_dofs()
+            if np.isin(sl, pos).any():
+                raise ValueError('the first node of a continuity condition also has a boundary '
+                                 'condition (over-constrained)')
+            fixed_dofs = np.concatenate([pos, sl])
+            fixed_vals = np.concatenate([val, np.zeros(sl.size)])
+            Kr, fr = fold.fold(K), fold.restrict(f)
+            if precond in ('learned', 'learned_two_level'):
+                raise ValueError('the learned preconditioner does not take continuity conditions')
+        fixed, g = sysm.dirichlet(fixed_dofs, fixed_vals)
         f_t = ctx.tensor(f)
-        KD, b = sysm.prepare(K, fixed, g, f_t)
+        KD, b = sysm.prepare(Kr, fixed, g, f_t if fold is None else ctx.tensor(fr))
         pc = precond
         model = None
         if precond in ('learned', 'learned_two_level'):
@@ -579,13 +632,21 @@ class FiniteElementAnalysis(FiniteElements):
             pc = 'block_jacobi' if precond == 'learned' else 'two_level'
         if pc not in ('jacobi', 'block_jacobi', 'two_level'):
             raise ValueError(f'unknown preconditioner {precond!r}')
-        coarse = sysm.coarse_space(pos) if pc == 'two_level' else None
+        coarse = None
+        if pc == 'two_level':
+            coarse = sysm.coarse_space(fixed_dofs, graph=None if fold is None else Kr.graph)
         u, info = solve_reduced(sysm, KD, b, fixed, pc, precision, tol=tol, max_iter=max_iter,
                                 model=model, coarse=coarse)
         from . import device
-        ku = device.spmv(K, u)
-        lam = f_t[pos] - ku[pos] if len(pos) else f_t[:0]
-        x = np.concatenate([u.cpu().numpy(), lam.cpu().numpy()])
+        if fold is None:
+            ku = device.spmv(K, u)
+            lam = f_t[pos] - ku[pos] if len(pos) else f_t[:0]
+            x = np.concatenate([u.cpu().numpy(), lam.cpu().numpy()])
+        else:
+            u = fold.expand(u)
+            r = (f_t - device.spmv(K, u)).cpu().numpy()
+            lam_cc, lam_bc = fold.multipliers(r, pos)
+            x = np.concatenate([u.cpu().numpy(), lam_bc, lam_cc])
         self.last_solve_info = info
         return Result(self.mesh, list(x))
 

@@ -40,10 +40,12 @@ class ElementBoundaryCondition(BoundaryCondition):
 
 
 class ContinuityCondition:
-    """Periodic / anti-periodic link between two nodes (finite_elements/conditions.py:196-241).
+    """Periodic / anti-periodic link between two nodes, A(node1) = value * A(node2)
+    (finite_elements/conditions.py:196-241): value 1 periodic, -1 anti-periodic.
 
-    Accepted for signature compatibility; the analysis rejects non-empty lists because the reference's
-    row indexing for these conditions (finite_elements/analysis.py:190-197) collides with DOF rows.
+    The analysis eliminates node1 in favour of node2 (``finite_elements_amd.continuity``); its
+    constraint rows follow the Dirichlet rows, where the reference's index
+    len(nodes) + len(node_loads) + i (finite_elements/analysis.py:190-197) lands on them.
     """
 
     def __init__(self, node1, node2, value):

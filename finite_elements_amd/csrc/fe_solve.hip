@@ -228,6 +228,30 @@ void launch_convert_d(fe_handle* h, const fe_bsr* A, const fe_bsr* B) {
   else launch_convert<S, T, 3>(h, A, B);
 }
 
+// --------------------------------------------------------------------------------------------------
+// folding: dst block q = sum over k in [seg[q], seg[q+1]) of coef[k] * src block src_block[k]
+// (one thread per value, the sum in the map's order)
+// --------------------------------------------------------------------------------------------------
+template <int D, int LAYOUT>
+__global__ void k_bsr_fold(int64_t nnzb_dst, int64_t nnzb_src, const int32_t* __restrict__ seg,
+                           const int32_t* __restrict__ src_block, const double* __restrict__ coef,
+                           const double* __restrict__ src, double* __restrict__ dst) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nnzb_dst * D * D) return;
+  int64_t q;
+  int k;
+  if (LAYOUT == FE_LAYOUT_PLANES) {
+    k = (int)(t / nnzb_dst);
+    q = t - (int64_t)k * nnzb_dst;
+  } else {
+    q = t / (D * D);
+    k = (int)(t - q * D * D);
+  }
+  double v = 0.0;
+  for (int32_t c = seg[q]; c < seg[q + 1]; ++c) v += coef[c] * src[vix<D, LAYOUT>(src_block[c], k, nnzb_src)];
+  dst[vix<D, LAYOUT>(q, k, nnzb_dst)] = v;
+}
+
 }  // namespace
 
 
@@ -297,6 +321,37 @@ extern "C" int fe_bsr_repack(fe_handle* h, const fe_bsr* src, const fe_bsr* dst,
   else if (src->precision == 0) launch_repack_d<double, float>(h, src, dst, src_block);
   else if (dst->precision == 0) launch_repack_d<float, double>(h, src, dst, src_block);
   else launch_repack_d<float, float>(h, src, dst, src_block);
+  FE_CHECK_LAUNCH();
+  return FE_OK;
+}
+
+extern "C" int fe_bsr_fold(fe_handle* h, const fe_bsr* src, const fe_bsr* dst, const int32_t* seg_ptr,
+                           const int32_t* src_block, const double* coef) {
+  FE_ENTER(h);
+  int rc = check_bsr(src, "fe_bsr_fold(src)");
+  if (rc) return rc;
+  rc = check_bsr(dst, "fe_bsr_fold(dst)");
+  if (rc) return rc;
+  FE_REQUIRE(src->dof == dst->dof && src->n_brows == dst->n_brows && src->layout == dst->layout &&
+                 src->precision == 0 && dst->precision == 0 && src->row_align == 0 && dst->row_align == 0,
+             "fe_bsr_fold: FP64 matrices of the same rows, dof and layout expected (unpadded)");
+  FE_REQUIRE(src->vals != dst->vals, "fe_bsr_fold: must be out-of-place");
+  if (dst->nnzb == 0) return FE_OK;
+  FE_REQUIRE(seg_ptr && src_block && coef, "fe_bsr_fold: null fold map");
+  const int D = dst->dof;
+  const int64_t total = dst->nnzb * D * D;
+  const dim3 grid((unsigned)ceil_div(total, 256)), block(256);
+  const double* a = (const double*)src->vals;
+  double* b = (double*)dst->vals;
+#define FE_FOLD(DD, L)                                                                              \
+  hipLaunchKernelGGL((k_bsr_fold<DD, L>), grid, block, 0, h->stream, dst->nnzb, src->nnzb, seg_ptr, \
+                     src_block, coef, a, b)
+  if (src->layout == FE_LAYOUT_PLANES) {
+    if (D == 1) FE_FOLD(1, FE_LAYOUT_PLANES); else if (D == 2) FE_FOLD(2, FE_LAYOUT_PLANES); else FE_FOLD(3, FE_LAYOUT_PLANES);
+  } else {
+    if (D == 1) FE_FOLD(1, FE_LAYOUT_BSR); else if (D == 2) FE_FOLD(2, FE_LAYOUT_BSR); else FE_FOLD(3, FE_LAYOUT_BSR);
+  }
+#undef FE_FOLD
   FE_CHECK_LAUNCH();
   return FE_OK;
 }

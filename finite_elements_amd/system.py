@@ -140,19 +140,22 @@ class FESystem:
         return KD, b
 
     def coarse_space(self, fixed_dofs, target: Optional[int] = None, node_offsets=None,
-                     max_block: Optional[int] = None):
+                     max_block: Optional[int] = None, graph=None):
         """Device coarse space of aggregated rigid-body modes for these constrained DOFs (cached).
         With node_offsets (the meshes of a batch) and max_block, a block-diagonal space with one coarse
-        block of at most max_block DOF per mesh, for the batched solver."""
+        block of at most max_block DOF per mesh, for the batched solver.  ``graph``: the node graph
+        of the matrix it will serve when that is not the mesh's own (a folded pattern)."""
         from .coarse import CoarseSpace
         from .device import DeviceCoarse
         fixed_dofs = np.asarray(fixed_dofs, dtype=np.int64)
         offs = None if node_offsets is None else np.asarray(node_offsets, dtype=np.int64)
-        key = (fixed_dofs.tobytes(), target, None if offs is None else offs.tobytes(), max_block)
+        pat = self.pattern if graph is None else graph.pattern
+        key = (fixed_dofs.tobytes(), target, None if offs is None else offs.tobytes(), max_block,
+               id(pat))
         cache = getattr(self, '_coarse_cache', None)
         if cache is not None and cache[0] == key:
             return cache[1]
-        cs = CoarseSpace(self._coords, self.pattern.bptr, self.pattern.bcol, self.dof, fixed_dofs, target,
+        cs = CoarseSpace(self._coords, pat.bptr, pat.bcol, self.dof, fixed_dofs, target,
                          node_offsets=offs, max_block=max_block)
         dc = DeviceCoarse(self.ctx, cs)
         self._coarse_cache = (key, dc)

@@ -273,6 +273,13 @@ int fe_bsr_convert(fe_handle* h, const fe_bsr* src, const fe_bsr* dst);
    src_block[q] < 0.  src_block: device array of dst->nnzb entries.  No counterpart in the reference:
    it builds the solver's iteration copy of the matrix. */
 int fe_bsr_repack(fe_handle* h, const fe_bsr* src, const fe_bsr* dst, const int32_t* src_block);
+/* Fold src into the pattern of dst (same rows, dof and layout, FP64, unpadded): dst block q =
+   sum over k in [seg_ptr[q], seg_ptr[q+1]) of coef[k] * src block src_block[k], in that order.
+   With the map of a set of continuity conditions u_s = v u_m (finite_elements_amd/continuity.py) it
+   forms T^T K T on the pattern where each slave node's couplings are moved to its master; replaces
+   the constraint rows of ContinuityCondition, finite_elements/analysis.py:175-199 (eliminated form). */
+int fe_bsr_fold(fe_handle* h, const fe_bsr* src, const fe_bsr* dst, const int32_t* seg_ptr,
+                const int32_t* src_block, const double* coef);
 
 /* ---- solver ------------------------------------------------------------------------------------- */
 /* Preconditioner setup from the FP64 (Dirichlet-applied) matrix: FE_PC_JACOBI writes n values,

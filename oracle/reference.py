@@ -14,6 +14,7 @@ material dict, and loads / conditions as tuples.  Functions follow the reference
   matrix_sparse      finite_elements/analysis.py:387-405
   conditions         finite_elements/analysis.py:28-68, 201-279, 509-543
   loads              finite_elements/analysis.py:71-110, 407-489; loads.py:24-35
+  continuity rows    finite_elements/analysis.py:175-199, conditions.py:196-231
   create_matrix      finite_elements/analysis.py:617-663
   create_source      finite_elements/analysis.py:665-708, 737-739
   solve              finite_elements/analysis.py:798-830
@@ -215,6 +216,7 @@ class Problem:
     edge_bcs: List = field(default_factory=list)
     plane_strain: Optional[bool] = None
     plane_stress: Optional[bool] = None
+    continuity: List = field(default_factory=list)  # (node1, node2, value): u(node1) = value u(node2)
 
     @property
     def dimension(self):
@@ -384,15 +386,35 @@ def c_matrix_boundary_conditions(problem, positions, node_to_index):
     return data, row_ind, col_ind
 
 
+def c_matrix_continuity_conditions(problem, node_to_index):
+    """finite_elements/analysis.py:175-199 with conditions.py:223-231: per condition the entries
+    (1, 1, -value, -value) at (R, i1), (i1, R), (R, i2), (i2, R).  The reference's row
+    R = len(nodes) + len(node_loads) + i lands on Dirichlet rows; here R = n + n_bc + i, after the
+    Dirichlet rows, which is the layout of its source vector length n + n_continuity + n_bc
+    (analysis.py:737-739).  Scalar problems only (the reference uses node indices as DOF rows)."""
+    if problem.continuity and problem.dimension != 1:
+        raise ValueError('continuity conditions are defined for scalar problems')
+    base = len(problem.nodes) + len(collapsed_boundary_conditions(problem))
+    row_ind, col_ind, data = [], [], []
+    for i, (node1, node2, value) in enumerate(problem.continuity):
+        data.extend((1, 1, -value, -value))
+        i1, i2 = node_to_index[node1], node_to_index[node2]
+        row_ind.extend((base + i, i1, base + i, i2))
+        col_ind.extend((i1, base + i, i2, base + i))
+    return data, row_ind, col_ind
+
+
 def create_matrix(problem):
-    """COO lists of K and the constraint entries -> scipy CSR (finite_elements/analysis.py:617-663)."""
+    """COO lists of K, the continuity entries and the Dirichlet entries -> scipy CSR
+    (finite_elements/analysis.py:617-663)."""
     positions = global_matrix_positions(problem.dimension, len(problem.nodes))
     node_to_index = problem.node_to_index()
     data, row, col = matrix_data(problem, 'k')
-    d2, r2, c2 = c_matrix_boundary_conditions(problem, positions, node_to_index)
-    data.extend(d2)
-    row.extend(r2)
-    col.extend(c2)
+    for d2, r2, c2 in (c_matrix_continuity_conditions(problem, node_to_index),
+                       c_matrix_boundary_conditions(problem, positions, node_to_index)):
+        data.extend(d2)
+        row.extend(r2)
+        col.extend(c2)
     return sparse.csr_matrix((data, (row, col)))
 
 
@@ -402,7 +424,7 @@ def create_source_matrix(problem):
     node_to_index = problem.node_to_index()
     bcs = collapsed_boundary_conditions(problem)
     n = len(problem.nodes) * problem.dimension
-    matrix = np.zeros((n + len(bcs), 1))
+    matrix = np.zeros((n + len(bcs) + len(problem.continuity), 1))
     for node, value, dim in collapsed_node_loads(problem):
         matrix[positions[(node_to_index[node], dim)]][0] += value
     for i, (_, value, _) in enumerate(bcs):
@@ -427,9 +449,10 @@ def modal_largest(problem, k):
 
 
 def problem_from_arrays(kind, coords, conn, materials, plane_strain=None, plane_stress=None,
-                        fixed_dofs=(), fixed_values=None, f=None):
+                        fixed_dofs=(), fixed_values=None, f=None, continuity=()):
     """Oracle problem from arrays: node tuples in index order, per-element materials, DOF-level
-    loads (as node loads) and fixed DOFs (as node BCs, in the given order)."""
+    loads (as node loads), fixed DOFs (as node BCs, in the given order) and continuity conditions
+    as (node1 index, node2 index, value)."""
     nodes = [tuple(float(v) for v in c) for c in np.asarray(coords)]
     d = {'tri_elastic': 2, 'tet_elastic': 3, 'tri_magnetic': 1}[kind]
     elements = []
@@ -444,5 +467,6 @@ def problem_from_arrays(kind, coords, conn, materials, plane_strain=None, plane_
     vals = np.zeros(len(fixed_dofs)) if fixed_values is None else np.asarray(fixed_values)
     for dof, v in zip(np.asarray(fixed_dofs).tolist(), vals.tolist()):
         node_bcs.append((nodes[dof // d], float(v), dof % d + 1))
+    cont = [(nodes[int(a)], nodes[int(b)], float(v)) for a, b, v in continuity]
     return Problem(kind, nodes, elements, node_loads=node_loads, node_bcs=node_bcs,
-                   plane_strain=plane_strain, plane_stress=plane_stress)
+                   plane_strain=plane_strain, plane_stress=plane_stress, continuity=cont)

@@ -193,6 +193,30 @@ def reduced_solve(K, f, fixed_dofs, fixed_values=None):
     return u
 
 
+def kkt_solve(K, f, fixed_dofs, fixed_values=None, continuity=()):
+    """The reference's Lagrange formulation solved directly (finite_elements/analysis.py:175-199,
+    241-279, 617-663, 798-830) for large scalar systems: [[K, C^T], [C, 0]] [u; lambda] = [f; g; 0]
+    with one row per fixed DOF (u_p = g_p) and, after them, one per continuity condition
+    (u_node1 - value u_node2 = 0).  SuperLU with its default ordering (the reference's NATURAL
+    ordering does not finish at this size).  Returns (u, lambda of the fixed DOFs, lambda of the
+    conditions)."""
+    n = K.shape[0]
+    fixed_dofs = np.asarray(fixed_dofs, dtype=np.int64)
+    nb = fixed_dofs.size
+    cont = np.asarray(continuity, dtype=np.float64).reshape(-1, 3)
+    nc = cont.shape[0]
+    s, m, v = cont[:, 0].astype(np.int64), cont[:, 1].astype(np.int64), cont[:, 2]
+    rows = np.concatenate([np.arange(nb), nb + np.arange(nc), nb + np.arange(nc)])
+    cols = np.concatenate([fixed_dofs, s, m])
+    vals = np.concatenate([np.ones(nb), np.ones(nc), -v])
+    C = sparse.csr_matrix((vals, (rows, cols)), shape=(nb + nc, n))
+    A = sparse.bmat([[K, C.T], [C, None]], format='csc')
+    g = np.zeros(nb) if fixed_values is None else np.asarray(fixed_values, dtype=np.float64)
+    rhs = np.concatenate([np.asarray(f, dtype=np.float64), g, np.zeros(nc)])
+    x = scipy.sparse.linalg.spsolve(A, rhs)
+    return x[:n], x[n:n + nb], x[n + nb:]
+
+
 def dirichlet_matrix(K, fixed_dofs):
     """K with fixed rows/columns replaced by identity (the eliminated system the solver sees)."""
     n = K.shape[0]
