@@ -398,10 +398,10 @@ def measured_traffic(kernel):
         return None
 
 
-def modal_bench(n=48, n_var=64, n_modes=20, tol=1e-6):
+def modal_bench(n=48, n_var=64, n_modes=20, tol=1e-6, streams=2, block=None):
     """The batched eigen-solve workload: 64 lognormal-stiffness variants of the 48^3 clamped Kuhn-tet
-    cube, 20 lowest modes each, by one batched LOBPCG call preconditioned with the two-level method
-    (one coarse inverse per variant).  Timed: per-variant assembly + Dirichlet decoupling of K, the
+    cube, 20 lowest modes each, by batched LOBPCG preconditioned with the two-level method (one
+    coarse inverse per variant), the variants split over ``streams`` concurrent calls.  Timed: per-variant assembly + Dirichlet decoupling of K, the
     coarse operators and the eigen-solve (the shared mass matrix is assembled once, untimed).  Every
     variant must reach the residual tolerance."""
     import torch
@@ -419,7 +419,8 @@ def modal_bench(n=48, n_var=64, n_modes=20, tol=1e-6):
     for mt in mats:
         sysm.set_materials(mt)
         KDs.append(device.apply_dirichlet(sysm.stiffness_bsr(), fixed))
-    vals, _, info = modal.lobpcg(KDs, MD, n_modes, fixed=fixed, tol=tol, max_iter=3000, coarse=coarse)
+    vals, _, info = modal.lobpcg(KDs, MD, n_modes, fixed=fixed, tol=tol, max_iter=3000, coarse=coarse,
+                                 streams=streams, block=block)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     its = info['iterations']

@@ -340,6 +340,118 @@ __global__ __launch_bounds__(kThreads) void k_lob_spmm(LobParams p, const int32_
   }
 }
 
+// The same product for the block-contiguous layout with the workgroup's K and M blocks staged in
+// LDS.  Its rows' blocks are one contiguous range of both value arrays, so a 128-thread workgroup
+// loads them with coalesced 8-byte loads (each block once) instead of every lane of a row loading
+// each of the 2 d^2 values of every block (d^2 scattered lines per wavefront load).  Blocks beyond
+// the LDS capacity (rows of unusually high degree) are read from HBM as before; the per-row sum
+// order is the same, so the result is bitwise that of k_lob_spmm.
+constexpr int kSpmmThreads = 128;
+constexpr int kSpmmLdsVals = 2448;  // per matrix: 272 blocks of 3 x 3 (19.1 KB); K and M: 38.3 KB
+
+template <int D>
+__device__ __forceinline__ void spmm_block(const double* __restrict__ kb, const double* __restrict__ mb,
+                                           const double (&y)[D][4], double (&ak)[D][4],
+                                           double (&am)[D][4]) {
+#pragma unroll
+  for (int e = 0; e < D; ++e)
+#pragma unroll
+    for (int f = 0; f < D; ++f) {
+      const double kv = kb[e * D + f], mv = mb[e * D + f];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        ak[e][u] += kv * y[f][u];
+        am[e][u] += mv * y[f][u];
+      }
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void spmm_load_y(const double* __restrict__ Y, int64_t c, int m, int j0,
+                                            double (&y)[D][4]) {
+#pragma unroll
+  for (int f = 0; f < D; ++f) {
+    const double2* src = reinterpret_cast<const double2*>(Y + (c * D + f) * m + j0);
+    const double2 a = src[0], b = src[1];
+    y[f][0] = a.x;
+    y[f][1] = a.y;
+    y[f][2] = b.x;
+    y[f][3] = b.y;
+  }
+}
+
+template <int D>
+__global__ __launch_bounds__(kSpmmThreads) void k_lob_spmm_lds(LobParams p, const int32_t* __restrict__ bptr,
+                                                               const int32_t* __restrict__ bcol,
+                                                               const double* __restrict__ Kv,
+                                                               int64_t k_stride,
+                                                               const double* __restrict__ Mv,
+                                                               int64_t m_stride, const LobState* st,
+                                                               double* blk, int src) {
+  constexpr int DD = D * D;
+  constexpr int kCap = kSpmmLdsVals / DD;
+  __shared__ double sK[kCap * DD], sM[kCap * DD];
+  __shared__ int32_t sCol[kCap];
+  __shared__ int s_skip, s_cur;
+  const int v = blockIdx.y;
+  if (threadIdx.x == 0) {
+    s_skip = st[v].done;
+    s_cur = st[v].cur;
+  }
+  __syncthreads();
+  if (s_skip) return;
+  const int m = p.m, tpr = m / 4;
+  const int npg = kSpmmThreads / tpr;
+  const int64_t r0 = (int64_t)blockIdx.x * npg;
+  const int64_t r1 = r0 + npg < p.nb ? r0 + npg : p.nb;
+  const int32_t qa = bptr[r0], qb = bptr[r1];
+  const int nst = qb - qa < kCap ? qb - qa : kCap;
+  const double* K = Kv + (int64_t)v * k_stride;
+  const double* M = Mv + (int64_t)v * m_stride;
+  for (int t = threadIdx.x; t < nst * DD; t += kSpmmThreads) {
+    sK[t] = K[(int64_t)qa * DD + t];
+    sM[t] = M[(int64_t)qa * DD + t];
+  }
+  for (int t = threadIdx.x; t < nst; t += kSpmmThreads) sCol[t] = bcol[qa + t];
+  __syncthreads();
+  const int g = threadIdx.x / tpr, j0 = (threadIdx.x % tpr) * 4;
+  const int64_t i = r0 + g;
+  if (g >= npg || i >= p.nb) return;
+  const int yb = src == 0 ? B_W : (src == 1 ? B_X : B_P) + s_cur;
+  const int kb = src == 0 ? B_KW : (src == 1 ? B_KX : B_KP) + s_cur;
+  const int mb = src == 0 ? B_MW : (src == 1 ? B_MX : B_MP) + s_cur;
+  const double* Y = lob_block(blk, p, v, yb);
+  double* KY = lob_block(blk, p, v, kb);
+  double* MY = lob_block(blk, p, v, mb);
+  double ak[D][4], am[D][4];
+#pragma unroll
+  for (int e = 0; e < D; ++e)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) ak[e][u] = am[e][u] = 0.0;
+  const int32_t q0 = bptr[i], q1 = bptr[i + 1];
+  const int32_t qs = q1 < qa + nst ? q1 : qa + nst;  // end of the row's staged blocks
+  for (int32_t q = q0; q < qs; ++q) {
+    const int l = q - qa;
+    double y[D][4];
+    spmm_load_y<D>(Y, sCol[l], m, j0, y);
+    spmm_block<D>(sK + l * DD, sM + l * DD, y, ak, am);
+  }
+  for (int32_t q = q0 > qs ? q0 : qs; q < q1; ++q) {
+    double y[D][4];
+    spmm_load_y<D>(Y, bcol[q], m, j0, y);
+    spmm_block<D>(K + (int64_t)q * DD, M + (int64_t)q * DD, y, ak, am);
+  }
+#pragma unroll
+  for (int e = 0; e < D; ++e) {
+    double2* dk = reinterpret_cast<double2*>(KY + (i * D + e) * m + j0);
+    double2* dm = reinterpret_cast<double2*>(MY + (i * D + e) * m + j0);
+    dk[0] = make_double2(ak[e][0], ak[e][1]);
+    dk[1] = make_double2(ak[e][2], ak[e][3]);
+    dm[0] = make_double2(am[e][0], am[e][1]);
+    dm[1] = make_double2(am[e][2], am[e][3]);
+  }
+}
+
 // --------------------------------------------------------------------------------------------------
 // K3: partial Gram matrices over a range of DOF rows on the FP64 matrix cores.  The basis S = [X W P]
 // (ns <= 96 columns) is cut into 16-column tiles and every upper tile pair (I <= J) of S^T (K S) and
@@ -1067,8 +1179,19 @@ int run_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
     for (int b = B_P; b < B_W; ++b)
       FE_CHECK_HIP(hipMemsetAsync(w.blk + ((int64_t)v * kNumBlocks + b) * p.nm, 0,
                                   p.nm * sizeof(double), st));
-  const int npg = kThreads / (p.m / 4);  // node rows per workgroup of the block SpMM
+  const double* Kv = (const double*)K->vals;
+  const double* Mv = (const double*)M->vals;
+  // block SpMM: K and M staged in LDS for the block-contiguous layout
+  const int npg = (LAYOUT == FE_LAYOUT_BSR ? kSpmmThreads : kThreads) / (p.m / 4);
   const dim3 gspmm((unsigned)ceil_div(p.nb, npg), nv);
+  auto spmm = [&](int src) {
+    if (LAYOUT == FE_LAYOUT_BSR)
+      hipLaunchKernelGGL((k_lob_spmm_lds<D>), gspmm, dim3(kSpmmThreads), 0, st, p, K->bptr, K->bcol, Kv,
+                         o->k_stride, Mv, o->m_stride, w.st, w.blk, src);
+    else
+      hipLaunchKernelGGL((k_lob_spmm<D, LAYOUT>), gspmm, dim3(kThreads), 0, st, p, K->bptr, K->bcol, Kv,
+                         o->k_stride, Mv, o->m_stride, w.st, w.blk, src);
+  };
   const dim3 gres((unsigned)p.nchunk_r, nv);
   const dim3 ggram((unsigned)p.nchunk_g, nv);
   const int NS = 3 * p.m;
@@ -1076,8 +1199,6 @@ int run_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
   const size_t rr_lds = 2 * (size_t)NS * NS * sizeof(double);
   FE_CHECK_HIP(hipFuncSetAttribute((const void*)k_lob_rr, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)rr_lds));
-  const double* Kv = (const double*)K->vals;
-  const double* Mv = (const double*)M->vals;
   auto rr_update = [&]() -> int {
     hipLaunchKernelGGL(k_lob_gram, ggram, dim3(kGramThreads), 0, st, p, w.st, w.blk, w.gpart);
     hipLaunchKernelGGL(k_lob_gsum, ggsum, dim3(kThreads), 0, st, p, w.st, w.gpart, w.G);
@@ -1089,8 +1210,7 @@ int run_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
     return FE_OK;
   };
   // start: K X, M X, Rayleigh-Ritz on X alone
-  hipLaunchKernelGGL((k_lob_spmm<D, LAYOUT>), gspmm, dim3(kThreads), 0, st, p, K->bptr, K->bcol, Kv,
-                     o->k_stride, Mv, o->m_stride, w.st, w.blk, 1);
+  spmm(1);
   FE_CHECK_LAUNCH();
   int rc = rr_update();
   if (rc) return rc;
@@ -1105,16 +1225,13 @@ int run_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
   auto enqueue = [&](int slot) -> int {
     for (int it = 0; it < chunk; ++it, ++host_it) {
       if (host_it % kRefresh == kRefresh - 1) {
-        hipLaunchKernelGGL((k_lob_spmm<D, LAYOUT>), gspmm, dim3(kThreads), 0, st, p, K->bptr, K->bcol, Kv,
-                           o->k_stride, Mv, o->m_stride, w.st, w.blk, 1);
-        hipLaunchKernelGGL((k_lob_spmm<D, LAYOUT>), gspmm, dim3(kThreads), 0, st, p, K->bptr, K->bcol, Kv,
-                           o->k_stride, Mv, o->m_stride, w.st, w.blk, 2);
+        spmm(1);
+        spmm(2);
       }
       hipLaunchKernelGGL((k_lob_resid<D>), gres, dim3(kThreads), 0, st, p, T, o->t_stride, fixed,
                          w.st, w.blk, w.rpart);
       if (cc.C) launch_lob_coarse<D, D == 3 ? 6 : D == 2 ? 3 : 1>(st, p, cc, w);
-      hipLaunchKernelGGL((k_lob_spmm<D, LAYOUT>), gspmm, dim3(kThreads), 0, st, p, K->bptr, K->bcol, Kv,
-                         o->k_stride, Mv, o->m_stride, w.st, w.blk, 0);
+      spmm(0);
       int rc2 = rr_update();
       if (rc2) return rc2;
     }

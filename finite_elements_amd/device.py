@@ -50,15 +50,17 @@ class Context:
         h = ctypes.c_void_p()
         _hip.call('fe_create', ctypes.byref(h), device, ctypes.c_void_p(self.stream.cuda_stream))
         self.h = h
-        self._side = None
+        self._sides = {}
 
-    def side(self) -> 'Context':
-        """A second handle on a stream of its own (default priority), for set-up work that may run
-        beside the main stream's; created once."""
-        if self._side is None:
+    def side(self, index: int = 0) -> 'Context':
+        """Another handle on a stream of its own (default priority), for work that may run beside
+        the main stream's; ``index`` names one of several, each created once."""
+        ctx = self._sides.get(index)
+        if ctx is None:
             with torch.cuda.device(self.device):
-                self._side = Context(self.device, torch.cuda.Stream(self.device))
-        return self._side
+                ctx = Context(self.device, torch.cuda.Stream(self.device))
+            self._sides[index] = ctx
+        return ctx
 
     @classmethod
     def get(cls, device: int = 0) -> 'Context':

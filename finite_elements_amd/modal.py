@@ -46,7 +46,8 @@ def lobpcg(K: Union[BsrMatrix, Sequence[BsrMatrix]], M: Union[BsrMatrix, Sequenc
            n_wanted: int, block: Optional[int] = None, fixed: Optional[torch.Tensor] = None,
            precond: Union[str, None, torch.Tensor, Sequence[torch.Tensor]] = 'block_jacobi',
            largest: bool = False, tol: float = 1e-6, max_iter: int = 2000, X0=None, seed: int = 0,
-           check_every: int = 8, coarse=None, iteration_layout: int = _hip.FE_LAYOUT_BSR):
+           check_every: int = 8, coarse=None, iteration_layout: int = _hip.FE_LAYOUT_BSR,
+           streams: int = 1):
     """Lowest (or largest) eigenpairs of one or several variants K_v phi = lambda M_v phi.
 
     K, M: one matrix or a list of variants on one pattern (FP64; converted to the plane layout).  A
@@ -58,7 +59,11 @@ def lobpcg(K: Union[BsrMatrix, Sequence[BsrMatrix]], M: Union[BsrMatrix, Sequenc
     with A_c = P^T K_v P per variant (smallest eigenvalues only).  iteration_layout: the value
     layout the block SpMM reads (FE_LAYOUT_BSR, or FE_LAYOUT_PLANES; same results bit for bit).
     X0: initial block [nvar][n][block] (host array); by default a standard normal block drawn on the
-    device from ``seed``.
+    device from ``seed``.  streams: 1 to 4; above 1 the variants are split into that many groups
+    iterated by concurrent calls on streams of their own (one host thread each), so that one
+    group's latency-bound Rayleigh-Ritz launches (one workgroup per variant) overlap the others'
+    bandwidth-bound kernels.  A variant's iteration is the same except that the Gram partial sums
+    are cut into more row chunks when a call holds fewer variants (rounding-level differences).
     Returns (eigvals [nvar][n_wanted] numpy, eigvecs [nvar][n][n_wanted] torch FP64, info).
     """
     Ks = [K] if isinstance(K, BsrMatrix) else list(K)
@@ -123,31 +128,83 @@ def lobpcg(K: Union[BsrMatrix, Sequence[BsrMatrix]], M: Union[BsrMatrix, Sequenc
     res = torch.zeros_like(lam)
     iters = torch.zeros(nvar, dtype=torch.int32, device=ctx.torch_device)
     vstat = torch.zeros(nvar, dtype=torch.int32, device=ctx.torch_device)
-    opts = _hip.FeLobpcgOpts(nvar, block, n_wanted, int(bool(largest)), int(max_iter), int(check_every),
-                             float(tol), k_stride, m_stride, t_stride)
-    ks = _hip.FeBsr(g.n_nodes, g.nnzb, d, 0, iteration_layout, 0, _vp(g.bptr), _vp(g.bcol), _vp(Kall))
-    ms = _hip.FeBsr(g.n_nodes, g.nnzb, d, 0, iteration_layout, 0, _vp(g.bptr), _vp(g.bcol), _vp(Mall))
-    nbytes = ctypes.c_int64()
-    stats = _hip.FeLobpcgStats()
-    if coarse is None:
-        _hip.call('fe_lobpcg_workspace', ctx.h, ctypes.byref(ks), ctypes.byref(opts), ctypes.byref(nbytes))
-        work = torch.empty(nbytes.value, dtype=torch.uint8, device=ctx.torch_device)
-        _hip.call('fe_lobpcg', ctx.h, ctypes.byref(ks), ctypes.byref(ms), _vp(T), _vp(fixed), _vp(X),
-                  _vp(lam), _vp(res), _vp(iters), _vp(vstat), _vp(work), work.numel(), ctypes.byref(opts),
-                  ctypes.byref(stats))
-    else:
+    if streams not in (1, 2, 3, 4):
+        raise ValueError('lobpcg: streams must be 1, 2, 3 or 4')
+    if coarse is not None:
         if largest:
             raise ValueError('lobpcg: the two-level preconditioner approximates K^-1 (smallest modes)')
         # per-variant coarse inverses A_c^-1 = (P^T K_v P)^-1, FP64
         Ainv = torch.stack([coarse.operator(A, invert=True)[0] for A in Ks])
         cs = coarse.struct()
-        _hip.call('fe_lobpcg_coarse_workspace', ctx.h, ctypes.byref(ks), ctypes.byref(cs),
-                  ctypes.byref(opts), ctypes.byref(nbytes))
+    # variant groups [v0, v1), each a call of its own on a handle of its own
+    ng = max(1, min(streams, nvar))
+    bounds = [(nvar * k + ng - 1) // ng for k in range(ng + 1)]
+    groups = []
+    for gi, (v0, v1) in enumerate(zip(bounds[:-1], bounds[1:])):
+        gctx = ctx if gi == 0 else ctx.side(gi - 1)
+        part = lambda t, stride: t if (stride == 0 or nvar == 1) else t[v0:v1]  # noqa: E731
+        kg, mg = part(Kall, k_stride), part(Mall, m_stride)
+        tg = None if T is None else part(T, t_stride)
+        opts = _hip.FeLobpcgOpts(v1 - v0, block, n_wanted, int(bool(largest)), int(max_iter),
+                                 int(check_every), float(tol), k_stride, m_stride, t_stride)
+        ks = _hip.FeBsr(g.n_nodes, g.nnzb, d, 0, iteration_layout, 0, _vp(g.bptr), _vp(g.bcol), _vp(kg))
+        ms = _hip.FeBsr(g.n_nodes, g.nnzb, d, 0, iteration_layout, 0, _vp(g.bptr), _vp(g.bcol), _vp(mg))
+        nbytes = ctypes.c_int64()
+        if coarse is None:
+            _hip.call('fe_lobpcg_workspace', gctx.h, ctypes.byref(ks), ctypes.byref(opts),
+                      ctypes.byref(nbytes))
+        else:
+            _hip.call('fe_lobpcg_coarse_workspace', gctx.h, ctypes.byref(ks), ctypes.byref(cs),
+                      ctypes.byref(opts), ctypes.byref(nbytes))
         work = torch.empty(nbytes.value, dtype=torch.uint8, device=ctx.torch_device)
-        _hip.call('fe_lobpcg_coarse', ctx.h, ctypes.byref(ks), ctypes.byref(ms), _vp(T), _vp(fixed),
-                  ctypes.byref(cs), _vp(Ainv), coarse.ld * coarse.ld, _vp(X), _vp(lam), _vp(res),
-                  _vp(iters), _vp(vstat), _vp(work), work.numel(), ctypes.byref(opts),
-                  ctypes.byref(stats))
+        sl = slice(v0 * block, v1 * block)
+        groups.append(dict(ctx=gctx, opts=opts, ks=ks, ms=ms, T=tg, X=X[v0:v1], lam=lam[sl],
+                           res=res[sl], iters=iters[v0:v1], vstat=vstat[v0:v1], work=work,
+                           Ainv=None if coarse is None else Ainv[v0:v1],
+                           stats=_hip.FeLobpcgStats()))
+
+    def run(gr):
+        a = (ctypes.byref(gr['ks']), ctypes.byref(gr['ms']), _vp(gr['T']), _vp(fixed))
+        tail = (_vp(gr['X']), _vp(gr['lam']), _vp(gr['res']), _vp(gr['iters']), _vp(gr['vstat']),
+                _vp(gr['work']), gr['work'].numel(), ctypes.byref(gr['opts']), ctypes.byref(gr['stats']))
+        if coarse is None:
+            _hip.call('fe_lobpcg', gr['ctx'].h, *a, *tail)
+        else:
+            _hip.call('fe_lobpcg_coarse', gr['ctx'].h, *a, ctypes.byref(cs), _vp(gr['Ainv']),
+                      coarse.ld * coarse.ld, *tail)
+
+    if len(groups) == 1:
+        run(groups[0])
+    else:
+        # everything the groups read is ready before any stream starts; ctypes releases the GIL
+        # for the calls, so the host loops (each polling its own stream) run side by side
+        torch.cuda.synchronize(ctx.device)
+        import threading
+        errors = []
+
+        def guarded(gr):
+            try:
+                run(gr)
+            except Exception as exc:  # re-raised below, after every call returned
+                errors.append(exc)
+
+        ths = [threading.Thread(target=guarded, args=(gr,)) for gr in groups[1:]]
+        for th in ths:
+            th.start()
+        guarded(groups[0])
+        for th in ths:
+            th.join()
+        torch.cuda.synchronize(ctx.device)
+        if errors:
+            raise errors[0]
+    stats = groups[0]['stats']
+    for gr in groups[1:]:
+        q = gr['stats']
+        stats.max_iterations = max(stats.max_iterations, q.max_iterations)
+        stats.unconverged += q.unconverged
+        stats.failed += q.failed
+        if q.status != 0 and (stats.status == 0 or q.status == _hip.FE_ERR_BREAKDOWN):
+            stats.status = q.status
     lam_h = lam.cpu().numpy().reshape(nvar, block)
     res_h = res.cpu().numpy().reshape(nvar, block)
     info = {'iterations': iters.cpu().numpy(), 'variant_status': vstat.cpu().numpy(),

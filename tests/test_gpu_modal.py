@@ -105,6 +105,11 @@ def test_lobpcg_two_level_vs_eigsh(ctx):
     v1, _, i1 = modal.lobpcg(KDs, MD, 12, block=16, fixed=fixed, tol=1e-7, max_iter=3000)
     _say(f'  iterations two-level {i2["iterations"]}, block Jacobi {i1["iterations"]}')
     assert i2['status'] == 0 and i1['status'] == 0
+    # the variants split over two concurrent calls on two streams: the same bits
+    v2s, x2s, i2s = modal.lobpcg(KDs, MD, 12, block=16, fixed=fixed, tol=1e-7, max_iter=3000,
+                                 coarse=coarse, streams=2)
+    assert np.array_equal(v2s, v2) and torch.equal(x2s, x2)
+    assert np.array_equal(i2s['iterations'], i2['iterations']) and i2s['status'] == 0
     assert np.all(i2['iterations'] < i1['iterations'])
     for v, pr in enumerate(prs):
         Ko, Mo = _oracle_pencil(pr)
@@ -114,8 +119,8 @@ def test_lobpcg_two_level_vs_eigsh(ctx):
 
 @pytest.mark.timeout(400)
 def test_modal48_against_golden_eigsh(ctx):
-    """The full 64-variant batch of the 48^3 modal workload (seeds 5000-5063, 352,947 DOF each) in one
-    batched LOBPCG call, as the bench runs it: every variant must reach the residual tolerance, and the
+    """The full 64-variant batch of the 48^3 modal workload (seeds 5000-5063, 352,947 DOF each) in
+    batched LOBPCG over two streams, as the bench runs it: every variant must reach the residual tolerance, and the
     variants whose FP64 eigsh values the oracle wrote (tests/golden/make_modal_golden.py) must match
     them within 1e-8 relative in all 20 lowest eigenvalues."""
     path = os.path.join(GOLDEN, 'modal48_eigs.npz')
@@ -135,7 +140,8 @@ def test_modal48_against_golden_eigsh(ctx):
         KDs.append(device.apply_dirichlet(s.stiffness_bsr(), fixed))
     coarse = s.coarse_space(pr0.fixed_dofs)
     tol = 1e-6
-    vals, _, info = modal.lobpcg(KDs, MD, 20, fixed=fixed, tol=tol, max_iter=3000, coarse=coarse)
+    vals, _, info = modal.lobpcg(KDs, MD, 20, fixed=fixed, tol=tol, max_iter=3000, coarse=coarse,
+                                 streams=2)
     its = info['iterations']
     _say(f'  iterations {its.min()}..{its.max()}, status {info["status"]}, max residual '
          f'{info["resid"].max():.2e}')
@@ -191,3 +197,7 @@ def test_lobpcg_batched_variants_share_mass(ctx):
                                  iteration_layout=_hip.FE_LAYOUT_PLANES)
     assert np.array_equal(vp_, vals) and torch.equal(xp_, vecs)
     assert np.array_equal(ip_['iterations'], info['iterations'])
+    # three variants over two streams (2 + 1): the same bits
+    vs_, xs_, is_ = modal.lobpcg(KDs, MD, 8, block=12, fixed=fixed, tol=1e-7, max_iter=3000, streams=2)
+    assert np.array_equal(vs_, vals) and torch.equal(xs_, vecs)
+    assert np.array_equal(is_['iterations'], info['iterations'])
