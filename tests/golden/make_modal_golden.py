@@ -7,7 +7,7 @@ the 20 lowest eigenvalues of the clamped pencil by scipy eigsh in shift-invert m
 
 Seeds already present in the output file are kept (a seed computed again replaces its row), so
 variants can be computed by several processes into part files and merged afterwards.  Takes about an
-hour and several GB per variant (sparse LU of the 346k-DOF free stiffness).
+two hours and about 27 GB per variant (sparse LU of the 346k-DOF free stiffness).
 """
 import os
 import sys
