@@ -610,6 +610,130 @@ __global__ __launch_bounds__(kThreads) void k_lob_gsum(LobParams p, const LobSta
 }
 
 // --------------------------------------------------------------------------------------------------
+// K3' / K4': the Gram matrices of S = [X W P] between refreshes.  X = S_old C and P = S_old C0 (C0:
+// C with its X rows zeroed) are combinations of the previous basis, so their blocks X^T K X,
+// X^T K P, P^T K P (and the M ones) are C^T G_old C, C^T G_old C0, C0^T G_old C0, which k_lob_rr
+// forms from the previous Gram (Gp: [2][2m][2m], rows and columns X then P).  Only the columns of
+// the new W are summed over the rows here: S^T (K W) and S^T (M W), reading S, KW and MW -- five
+// blocks instead of the nine (S and both images) of the full Gram.  Same MFMA layout as k_lob_gram;
+// wavefronts 0, 1 form K W, 2, 3 M W, each taking three 16-column tiles of S against the two tiles
+// of W's image.  Every kRefresh iterations (when the images of X and P are recomputed) the full
+// Gram runs instead, so the recurrence restarts from explicit products.
+// --------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void gram_rows_w(const double* __restrict__ base, int ns, int m, int half,
+                                            const int (&bS)[3], int bY, int64_t nm, int64_t lo,
+                                            int64_t hi, double* __restrict__ out) {
+  constexpr int NI = 3, NJ = 2;
+  const int lane = threadIdx.x & 63, rr = lane >> 4, cl = lane & 15;
+  int64_t offS[NI], offY[NJ];
+  bool okS[NI], okY[NJ];
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    const int c = (half * NI + i) * kGramTile + cl;
+    okS[i] = c < ns;
+    offS[i] = okS[i] ? bS[c / m] * nm + c % m : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int c = j * kGramTile + cl;
+    okY[j] = c < m;
+    offY[j] = okY[j] ? (int64_t)bY * nm + c : 0;
+  }
+  GramAcc acc[NI][NJ];
+#pragma unroll
+  for (int i = 0; i < NI; ++i)
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) acc[i][j] = GramAcc{0.0, 0.0, 0.0, 0.0};
+  double sa[kGramSteps][NI], yb[kGramSteps][NJ], sn[kGramSteps][NI], yn[kGramSteps][NJ];
+  auto load = [&](int64_t r0, double(&sv)[kGramSteps][NI], double(&yv)[kGramSteps][NJ])
+                  __attribute__((always_inline)) {
+#pragma unroll
+    for (int k = 0; k < kGramSteps; ++k) {
+      const int64_t row = r0 + 4 * k + rr;
+      const bool rok = row < hi;
+      const int64_t ro = row * m;
+#pragma unroll
+      for (int i = 0; i < NI; ++i) sv[k][i] = (rok && okS[i]) ? base[offS[i] + ro] : 0.0;
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) yv[k][j] = (rok && okY[j]) ? base[offY[j] + ro] : 0.0;
+    }
+  };
+  load(lo, sa, yb);
+  for (int64_t r0 = lo; r0 < hi; r0 += 4 * kGramSteps) {
+    load(r0 + 4 * kGramSteps, sn, yn);
+#pragma unroll
+    for (int k = 0; k < kGramSteps; ++k)
+#pragma unroll
+      for (int i = 0; i < NI; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(sa[k][i], yb[k][j], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int k = 0; k < kGramSteps; ++k) {
+#pragma unroll
+      for (int i = 0; i < NI; ++i) sa[k][i] = sn[k][i];
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) yb[k][j] = yn[k][j];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NI; ++i)
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const int col = j * kGramTile + cl;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int row = (half * NI + i) * kGramTile + 4 * c + rr;
+        if (row < ns && col < m) out[row * m + col] = acc[i][j][c];
+      }
+    }
+}
+
+__global__ __launch_bounds__(kGramThreads) void k_lob_gram_w(LobParams p, const LobState* st,
+                                                             const double* blk,
+                                                             double* __restrict__ gpart) {
+  const int v = blockIdx.y, chunk = blockIdx.x;
+  if (st[v].done) return;
+  const int m = p.m, ns = st[v].ns, cur = st[v].cur;
+  const int wave = threadIdx.x >> 6, img = wave >> 1, half = wave & 1;  // img 0: K, 1: M
+  if (half * 3 * kGramTile >= ns) return;
+  const double* base = blk + (int64_t)v * kNumBlocks * p.nm;
+  const int bS[3] = {B_X + cur, B_W, B_P + cur};
+  const int64_t per = ceil_div(ceil_div(p.n, (int64_t)p.nchunk_g), 4 * kGramSteps) * 4 * kGramSteps;
+  const int64_t lo = min(p.n, (int64_t)chunk * per), hi = min(p.n, lo + per);
+  const int NS = 3 * m;
+  double* out = gpart + (((int64_t)v * p.nchunk_g + chunk) * 2 + img) * NS * m;
+  gram_rows_w(base, ns, m, half, bS, img ? B_MW : B_KW, p.nm, lo, hi, out);
+}
+
+// G[v][2][NS][NS] from the W-column partials (fixed chunk order) and the recurrence blocks Gp.
+__global__ __launch_bounds__(kThreads) void k_lob_gsum_w(LobParams p, const LobState* st,
+                                                         const double* __restrict__ gpart,
+                                                         const double* __restrict__ Gp,
+                                                         double* __restrict__ G) {
+  const int v = blockIdx.y;
+  if (st[v].done) return;
+  const int m = p.m, NS = 3 * m, ns = st[v].ns;
+  const int e = blockIdx.x * kThreads + threadIdx.x;
+  if (e >= 2 * NS * NS) return;
+  const int mat = e / (NS * NS), a = (e % (NS * NS)) / NS, b = e % NS;
+  if (a >= ns || b >= ns) return;
+  const bool aw = a >= m && a < 2 * m, bw = b >= m && b < 2 * m;
+  double g;
+  if (aw || bw) {
+    const int r = bw ? a : b, c = bw ? b - m : a - m;  // (row of S, column of W)
+    const double* src = gpart + ((int64_t)v * p.nchunk_g * 2 + mat) * NS * m + r * m + c;
+    g = 0.0;
+    for (int k = 0; k < p.nchunk_g; ++k) g += src[(int64_t)k * 2 * NS * m];
+  } else {
+    const int ia = a < m ? a : a - m, ib = b < m ? b : b - m;  // X: 0..m-1, P: m..2m-1
+    const int lo_ = ia < ib ? ia : ib, hi_ = ia < ib ? ib : ia;
+    g = Gp[(((int64_t)v * 2 + mat) * 2 * m + lo_) * 2 * m + hi_];
+  }
+  G[((int64_t)v * 2 + mat) * NS * NS + a * NS + b] = g;
+}
+
+// --------------------------------------------------------------------------------------------------
 // Parallel cyclic Jacobi on an r x r symmetric matrix in LDS (row-major, leading dimension r).
 // On return the diagonal of A holds the eigenvalues and the columns of V the eigenvectors.
 // Round-robin pairing: every step rotates r/2 disjoint planes at once (rows, then columns).
@@ -748,7 +872,8 @@ __global__ __launch_bounds__(kRRThreads) void k_lob_rr(LobParams p, LobState* st
                                                        const double* __restrict__ G,
                                                        double* __restrict__ Zg,
                                                        double* __restrict__ Tg,
-                                                       double* __restrict__ Cg) {
+                                                       double* __restrict__ Cg,
+                                                       double* __restrict__ Gp) {
   extern __shared__ double lds[];
   const int v = blockIdx.x;
   LobState* s = st + v;
@@ -890,6 +1015,37 @@ __global__ __launch_bounds__(kRRThreads) void k_lob_rr(LobParams p, LobState* st
     double c = 0.0;
     for (int k = 0; k < rk; ++k) c += Z[a * NS + k] * sV[k * rk + s_sel[j]];
     C[idx] = c;
+  }
+  __syncthreads();
+  // Gram blocks of the next basis' X' = S C and P' = S C0 (C0: the rows of X zeroed), for the
+  // iterations that sum only the new W's columns: T = G C, T0 = G C0 (into T1), then
+  // Gp = [C C0]^T [T T0] (upper triangle used), for G_K and G_M
+  const int m2 = 2 * m;
+  for (int mat = 0; mat < 2; ++mat) {
+    const double* Gm = mat ? GM : GK;
+    for (int idx = tid; idx < ns * m; idx += blockDim.x) {
+      const int a = idx / m, j = idx % m;
+      double t = 0.0, t0 = 0.0;
+      for (int b = 0; b < ns; ++b) {
+        const double gab = 0.5 * (Gm[a * NS + b] + Gm[b * NS + a]) * C[b * m + j];
+        t += gab;
+        if (b >= m) t0 += gab;
+      }
+      T1[idx] = t;
+      T1[ns * m + idx] = t0;
+    }
+    __syncthreads();
+    double* out = Gp + ((int64_t)v * 2 + mat) * m2 * m2;
+    for (int idx = tid; idx < m2 * m2; idx += blockDim.x) {
+      const int i = idx / m2, j = idx % m2;
+      if (j < i) continue;
+      const double* Tj = T1 + (j < m ? 0 : ns * m);
+      const int jc = j < m ? j : j - m, ic = i < m ? i : i - m;
+      double g = 0.0;
+      for (int a = (i < m ? 0 : m); a < ns; ++a) g += C[a * m + ic] * Tj[a * m + jc];
+      out[idx] = g;
+    }
+    __syncthreads();
   }
 }
 
@@ -1070,6 +1226,7 @@ struct LobWork {
   double* C;
   double* Z;
   double* T1;
+  double* Gp;  // [nvar][2][2m][2m] Gram blocks of the next X and P (upper triangles)
   double* rc;  // two-level: [nvar][ld][m] restricted residuals and coarse solutions
   double* zc;
   int64_t bytes;
@@ -1116,6 +1273,7 @@ LobWork layout_lob(char* base, const LobParams& p, int64_t ld = 0) {
   w.C = (double*)take(nv * NS * p.m * (int64_t)sizeof(double));
   w.Z = (double*)take(nv * NS * NS * (int64_t)sizeof(double));
   w.T1 = (double*)take(nv * NS * NS * (int64_t)sizeof(double));
+  w.Gp = (double*)take(nv * 2 * 4 * (int64_t)p.m * p.m * (int64_t)sizeof(double));
   if (ld > 0) {
     w.rc = (double*)take(nv * ld * p.m * (int64_t)sizeof(double));
     w.zc = (double*)take(nv * ld * p.m * (int64_t)sizeof(double));
@@ -1199,11 +1357,18 @@ int run_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
   const size_t rr_lds = 2 * (size_t)NS * NS * sizeof(double);
   FE_CHECK_HIP(hipFuncSetAttribute((const void*)k_lob_rr, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)rr_lds));
-  auto rr_update = [&]() -> int {
-    hipLaunchKernelGGL(k_lob_gram, ggram, dim3(kGramThreads), 0, st, p, w.st, w.blk, w.gpart);
-    hipLaunchKernelGGL(k_lob_gsum, ggsum, dim3(kThreads), 0, st, p, w.st, w.gpart, w.G);
+  // full: the Gram of the whole basis from its images; otherwise only the new W's columns, the
+  // blocks of X and P coming from the previous Rayleigh-Ritz step
+  auto rr_update = [&](bool full) -> int {
+    if (full) {
+      hipLaunchKernelGGL(k_lob_gram, ggram, dim3(kGramThreads), 0, st, p, w.st, w.blk, w.gpart);
+      hipLaunchKernelGGL(k_lob_gsum, ggsum, dim3(kThreads), 0, st, p, w.st, w.gpart, w.G);
+    } else {
+      hipLaunchKernelGGL(k_lob_gram_w, ggram, dim3(kGramThreads), 0, st, p, w.st, w.blk, w.gpart);
+      hipLaunchKernelGGL(k_lob_gsum_w, ggsum, dim3(kThreads), 0, st, p, w.st, w.gpart, w.Gp, w.G);
+    }
     hipLaunchKernelGGL(k_lob_rr, dim3(nv), dim3(kRRThreads), rr_lds, st, p, w.st, w.rpart, w.G, w.Z,
-                       w.T1, w.C);
+                       w.T1, w.C, w.Gp);
     launch_update(st, p, w.st, w.blk, w.C);
     hipLaunchKernelGGL(k_lob_flip, gvar, dim3(64), 0, st, p, w.st);
     FE_CHECK_LAUNCH();
@@ -1212,7 +1377,7 @@ int run_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
   // start: K X, M X, Rayleigh-Ritz on X alone
   spmm(1);
   FE_CHECK_LAUNCH();
-  int rc = rr_update();
+  int rc = rr_update(true);
   if (rc) return rc;
   const int chunk = o->check_every > 0 ? o->check_every : 8;
   LobStatus* pinned = (LobStatus*)((char*)h->pinned + 1024);
@@ -1232,7 +1397,7 @@ int run_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
                          w.st, w.blk, w.rpart);
       if (cc.C) launch_lob_coarse<D, D == 3 ? 6 : D == 2 ? 3 : 1>(st, p, cc, w);
       spmm(0);
-      int rc2 = rr_update();
+      int rc2 = rr_update(host_it % kRefresh == kRefresh - 1);
       if (rc2) return rc2;
     }
     hipLaunchKernelGGL(k_lob_status, dim3(1), dim3(kThreads), 0, st, p, w.st, w.status);
