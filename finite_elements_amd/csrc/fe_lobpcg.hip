@@ -482,6 +482,7 @@ constexpr int kGramTile = 16;
 constexpr int kGramTiles = kMaxBasis / kGramTile;  // 6
 constexpr int kGramThreads = 256;
 constexpr int kGramSteps = 3;
+constexpr int kGramStepsW = 2;  // the W-column Gram: 2 steps in flight keep it at 3 waves per SIMD
 
 typedef double GramAcc __attribute__((ext_vector_type(4)));
 
@@ -644,11 +645,11 @@ __device__ __forceinline__ void gram_rows_w(const double* __restrict__ base, int
   for (int i = 0; i < NI; ++i)
 #pragma unroll
     for (int j = 0; j < NJ; ++j) acc[i][j] = GramAcc{0.0, 0.0, 0.0, 0.0};
-  double sa[kGramSteps][NI], yb[kGramSteps][NJ], sn[kGramSteps][NI], yn[kGramSteps][NJ];
-  auto load = [&](int64_t r0, double(&sv)[kGramSteps][NI], double(&yv)[kGramSteps][NJ])
+  double sa[kGramStepsW][NI], yb[kGramStepsW][NJ], sn[kGramStepsW][NI], yn[kGramStepsW][NJ];
+  auto load = [&](int64_t r0, double(&sv)[kGramStepsW][NI], double(&yv)[kGramStepsW][NJ])
                   __attribute__((always_inline)) {
 #pragma unroll
-    for (int k = 0; k < kGramSteps; ++k) {
+    for (int k = 0; k < kGramStepsW; ++k) {
       const int64_t row = r0 + 4 * k + rr;
       const bool rok = row < hi;
       const int64_t ro = row * m;
@@ -659,17 +660,17 @@ __device__ __forceinline__ void gram_rows_w(const double* __restrict__ base, int
     }
   };
   load(lo, sa, yb);
-  for (int64_t r0 = lo; r0 < hi; r0 += 4 * kGramSteps) {
-    load(r0 + 4 * kGramSteps, sn, yn);
+  for (int64_t r0 = lo; r0 < hi; r0 += 4 * kGramStepsW) {
+    load(r0 + 4 * kGramStepsW, sn, yn);
 #pragma unroll
-    for (int k = 0; k < kGramSteps; ++k)
+    for (int k = 0; k < kGramStepsW; ++k)
 #pragma unroll
       for (int i = 0; i < NI; ++i)
 #pragma unroll
         for (int j = 0; j < NJ; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(sa[k][i], yb[k][j], acc[i][j], 0, 0, 0);
 #pragma unroll
-    for (int k = 0; k < kGramSteps; ++k) {
+    for (int k = 0; k < kGramStepsW; ++k) {
 #pragma unroll
       for (int i = 0; i < NI; ++i) sa[k][i] = sn[k][i];
 #pragma unroll
@@ -689,7 +690,7 @@ __device__ __forceinline__ void gram_rows_w(const double* __restrict__ base, int
     }
 }
 
-__global__ __launch_bounds__(kGramThreads) void k_lob_gram_w(LobParams p, const LobState* st,
+__global__ __launch_bounds__(kGramThreads, 3) void k_lob_gram_w(LobParams p, const LobState* st,
                                                              const double* blk,
                                                              double* __restrict__ gpart) {
   const int v = blockIdx.y, chunk = blockIdx.x;
