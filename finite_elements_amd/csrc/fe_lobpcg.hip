@@ -1054,8 +1054,7 @@ __global__ __launch_bounds__(kRRThreads) void k_lob_rr(LobParams p, LobState* st
 // K6, one thread per DOF row holding all MB columns: the coefficients, uniform over the workgroup,
 // are read as scalars (the constant address space), so no coefficient goes through LDS.  Per image
 // (basis, K-image, M-image) the steps are W, P (their sum is P' = W C_W + P C_P, stored before X is
-// added) and X (X' = P' + X C_X); W and P are skipped while they are not in the basis.  The row of
-// the next step's block is loaded while the current one is multiplied.
+// added) and X (X' = P' + X C_X); W and P are skipped while they are not in the basis.
 using ConstF64 = const __attribute__((address_space(4))) double*;
 
 template <int MB>
@@ -1094,13 +1093,14 @@ __global__ __launch_bounds__(kThreads) void k_lob_update_rows(LobParams p, const
     const int b = k == 0 ? B_W + which : k == 1 ? B_P + 2 * which + cur : B_X + 2 * which + cur;
     return base + (int64_t)b * p.nm;
   };
-  double acc[MB], y[MB], yn[MB];
+  // no register prefetch of the next step's row: 125 instead of 172 VGPRs (block 28) doubles the
+  // waves per SIMD, which hides the loads better than the prefetch did
+  double acc[MB], y[MB];
 #pragma unroll
   for (int j = 0; j < MB; ++j) acc[j] = 0.0;
-  upd_load<MB>(src_of(0), y);
 #pragma unroll 1
   for (int s = 0; s < nsteps; ++s) {
-    if (s + 1 < nsteps) upd_load<MB>(src_of(s + 1), yn);
+    upd_load<MB>(src_of(s), y);
     const int which = s / nk, k = kind_of(s);
     if (k == 2) upd_store<MB>(base + (int64_t)(B_P + 2 * which + 1 - cur) * p.nm, acc);
     const ConstF64 Ck = C + (k == 2 ? 0 : k == 0 ? MB * MB : 2 * MB * MB);
@@ -1113,8 +1113,6 @@ __global__ __launch_bounds__(kThreads) void k_lob_update_rows(LobParams p, const
 #pragma unroll
       for (int j = 0; j < MB; ++j) acc[j] = 0.0;
     }
-#pragma unroll
-    for (int a = 0; a < MB; ++a) y[a] = yn[a];
   }
 }
 
