@@ -340,14 +340,15 @@ __global__ __launch_bounds__(kThreads) void k_lob_spmm(LobParams p, const int32_
   }
 }
 
-// The same product for the block-contiguous layout with the workgroup's K and M blocks staged in
-// LDS.  Its rows' blocks are one contiguous range of both value arrays, so a 128-thread workgroup
-// loads them with coalesced 8-byte loads (each block once) instead of every lane of a row loading
-// each of the 2 d^2 values of every block (d^2 scattered lines per wavefront load).  Blocks beyond
-// the LDS capacity (rows of unusually high degree) are read from HBM as before; the per-row sum
-// order is the same, so the result is bitwise that of k_lob_spmm.
+// The same product for the block-contiguous layout with the workgroup's K blocks staged in LDS.
+// Its rows' blocks are one contiguous range of the value array, so a 128-thread workgroup loads them
+// with coalesced 8-byte loads (each block once) instead of every lane of a row loading each value of
+// every block (d^2 scattered lines per wavefront load).  M, one matrix shared by every variant, is
+// read directly (it stays in the caches); staging it too doubled the LDS per workgroup and cost more
+// in occupancy than it saved.  Blocks beyond the LDS capacity (rows of unusually high degree) are
+// read from HBM; the per-row sum order is the same, so the result is bitwise that of k_lob_spmm.
 constexpr int kSpmmThreads = 128;
-constexpr int kSpmmLdsVals = 2448;  // per matrix: 272 blocks of 3 x 3 (19.1 KB); K and M: 38.3 KB
+constexpr int kSpmmLdsVals = 2448;  // 272 blocks of 3 x 3 (19.1 KB)
 
 template <int D>
 __device__ __forceinline__ void spmm_block(const double* __restrict__ kb, const double* __restrict__ mb,
@@ -390,7 +391,7 @@ __global__ __launch_bounds__(kSpmmThreads) void k_lob_spmm_lds(LobParams p, cons
                                                                double* blk, int src) {
   constexpr int DD = D * D;
   constexpr int kCap = kSpmmLdsVals / DD;
-  __shared__ double sK[kCap * DD], sM[kCap * DD];
+  __shared__ double sK[kCap * DD];
   __shared__ int32_t sCol[kCap];
   __shared__ int s_skip, s_cur;
   const int v = blockIdx.y;
@@ -410,7 +411,6 @@ __global__ __launch_bounds__(kSpmmThreads) void k_lob_spmm_lds(LobParams p, cons
   const double* M = Mv + (int64_t)v * m_stride;
   for (int t = threadIdx.x; t < nst * DD; t += kSpmmThreads) {
     sK[t] = K[(int64_t)qa * DD + t];
-    sM[t] = M[(int64_t)qa * DD + t];
   }
   for (int t = threadIdx.x; t < nst; t += kSpmmThreads) sCol[t] = bcol[qa + t];
   __syncthreads();
@@ -434,7 +434,7 @@ __global__ __launch_bounds__(kSpmmThreads) void k_lob_spmm_lds(LobParams p, cons
     const int l = q - qa;
     double y[D][4];
     spmm_load_y<D>(Y, sCol[l], m, j0, y);
-    spmm_block<D>(sK + l * DD, sM + l * DD, y, ak, am);
+    spmm_block<D>(sK + l * DD, M + (int64_t)q * DD, y, ak, am);
   }
   for (int32_t q = q0 > qs ? q0 : qs; q < q1; ++q) {
     double y[D][4];
@@ -1338,7 +1338,7 @@ int run_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
                                   p.nm * sizeof(double), st));
   const double* Kv = (const double*)K->vals;
   const double* Mv = (const double*)M->vals;
-  // block SpMM: K and M staged in LDS for the block-contiguous layout
+  // block SpMM: K staged in LDS for the block-contiguous layout
   const int npg = (LAYOUT == FE_LAYOUT_BSR ? kSpmmThreads : kThreads) / (p.m / 4);
   const dim3 gspmm((unsigned)ceil_div(p.nb, npg), nv);
   auto spmm = [&](int src) {
