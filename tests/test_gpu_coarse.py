@@ -33,13 +33,15 @@ def _setup(pr, target):
     return s, KD, b, dc
 
 
-@pytest.mark.parametrize('case', ['cube', 'cantilever', 'magnetic'])
+@pytest.mark.parametrize('case', ['cube', 'cantilever', 'magnetic', 'cube16'])
 def test_coarse_operator_and_inverse(ctx, case):
+    """cube16: 216 aggregates, 1,296 coarse DOF -- 21 panels of the blocked Gauss-Jordan inverse."""
     _say(f'coarse operator {case}')
     pr = {'cube': lambda: meshgen.cube_elasticity(n=6),
           'cantilever': lambda: meshgen.cantilever_2d(),
-          'magnetic': lambda: meshgen.magnetostatic_2d(n=30)}[case]()
-    s, KD, b, dc = _setup(pr, 27 if case == 'cube' else 16)
+          'magnetic': lambda: meshgen.magnetostatic_2d(n=30),
+          'cube16': lambda: meshgen.cube_elasticity(n=16)}[case]()
+    s, KD, b, dc = _setup(pr, 27 if case in ('cube', 'cube16') else 16)
     A = _host_matrix(KD)
     ref, P = oc.coarse_operator(dc.host, A)
     n = ref.shape[0]

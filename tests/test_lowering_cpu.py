@@ -111,7 +111,9 @@ def test_elastic_lowering_quirks():
         prob, o_els[3], positions, prob.node_to_index())
 
 
-def test_continuity_conditions_rejected():
+def test_continuity_conditions_need_a_scalar_problem():
+    """The reference writes node indices as DOF rows (analysis.py:190-197), which only means
+    something for one DOF per node: vector problems are refused."""
     m, pts, els, _ = _two_groups_tri()
     an = FiniteElementAnalysis(m, [], [], [], [], [conditions.ContinuityCondition(pts[0], pts[1], 1)],
                                [], [], [], plane_strain=False, plane_stress=True)
