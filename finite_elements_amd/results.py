@@ -136,12 +136,86 @@ class ElasticityResults(Result):
         return float(np.sum(self.element_fields()[2]))
 
 
+    def _component(self, which: str, k: int):
+        """{element: component k of the element's strain or stress vector}."""
+        arr = self.element_fields()[0 if which == 'strain' else 1][:, k]
+        return {el: float(v) for el, v in zip(_elements(self.mesh), arr)}
+
+    def displacement_per_node_x(self):
+        """[u_x of every node] in node order (results.py:618-626)."""
+        return self.displacements_array()[:, 0].tolist()
+
+    def displacement_per_node_y(self):
+        """[u_y of every node] in node order (results.py:628-636)."""
+        return self.displacements_array()[:, 1].tolist()
+
+
 class ElasticityResults2D(ElasticityResults):
-    pass
+    """Plane strain / stress components per element (finite_elements/results.py:930-1466): the
+    strain and stress vectors are (xx, yy, xy)."""
+
+    def axial_strain_x(self):
+        return self._component('strain', 0)
+
+    def axial_strain_y(self):
+        return self._component('strain', 1)
+
+    def shear_strain_xy(self):
+        return self._component('strain', 2)
+
+    def axial_stress_x(self):
+        return self._component('stress', 0)
+
+    def axial_stress_y(self):
+        return self._component('stress', 1)
+
+    def shear_stress_xy(self):
+        return self._component('stress', 2)
 
 
 class ElasticityResults3D(ElasticityResults):
-    pass
+    """Solid components per element (finite_elements/results.py:1468-1740): the strain and stress
+    vectors are (xx, yy, zz, xy, yz, zx)."""
+
+    def axial_strain_x(self):
+        return self._component('strain', 0)
+
+    def axial_strain_y(self):
+        return self._component('strain', 1)
+
+    def axial_strain_z(self):
+        return self._component('strain', 2)
+
+    def shear_strain_xy(self):
+        return self._component('strain', 3)
+
+    def shear_strain_yz(self):
+        return self._component('strain', 4)
+
+    def shear_strain_zx(self):
+        return self._component('strain', 5)
+
+    def axial_stress_x(self):
+        return self._component('stress', 0)
+
+    def axial_stress_y(self):
+        return self._component('stress', 1)
+
+    def axial_stress_z(self):
+        return self._component('stress', 2)
+
+    def shear_stress_xy(self):
+        return self._component('stress', 3)
+
+    def shear_stress_yz(self):
+        return self._component('stress', 4)
+
+    def shear_stress_zx(self):
+        return self._component('stress', 5)
+
+    def displacement_per_node_z(self):
+        """[u_z of every node] in node order (results.py:1585-1593)."""
+        return self.displacements_array()[:, 2].tolist()
 
 
 class MagneticResults(Result):

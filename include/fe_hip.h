@@ -238,8 +238,10 @@ int fe_assemble_gather(fe_handle* h, const double* ke, const int32_t* gather, co
 /* Fused assembly: each workgroup owns a node range, computes the element-matrix row blocks of its
    nodes' incidences from coordinates into LDS and reduces them per block entry in ascending element
    order.  No element matrix touches HBM; no atomics; bitwise deterministic.  what = FE_STIFFNESS or
-   FE_MASS; layout = FE_LAYOUT_CSR or FE_LAYOUT_BSR.  Replaces k_matrix_data + csr_matrix
-   (finite_elements/analysis.py:324-339, 661) and m_matrix_data (357-365). */
+   FE_MASS; layout = FE_LAYOUT_CSR, FE_LAYOUT_BSR or FE_LAYOUT_PLANES.  Replaces k_matrix_data +
+   csr_matrix (finite_elements/analysis.py:324-339, 661) and m_matrix_data (357-365). */
+int fe_assemble_fused(fe_handle* h, const fe_mesh* mesh, const fe_graph* graph, int what, int layout,
+                      double* vals);
 /* Per-element fields of a solution u (device, interleaved DOFs), component-major out[k * n_elem + e]
    (replaces ElasticityResults.strain / stress / energy_per_element, finite_elements/results.py:689-830,
    and MagneticResults' field, results.py:121-152).  Elasticity: strain and stress in Voigt order (tri
@@ -247,8 +249,6 @@ int fe_assemble_gather(fe_handle* h, const double* ke, const int32_t* gather, co
    three may be NULL.  Magnetic: B = (Bx, By) in strain (2 components), stress and energy NULL. */
 int fe_element_fields(fe_handle* h, const fe_mesh* mesh, const double* u, double* strain,
                       double* stress, double* energy);
-int fe_assemble_fused(fe_handle* h, const fe_mesh* mesh, const fe_graph* graph, int what, int layout,
-                      double* vals);
 /* Permute node-chunk values between the CSR and BSR layouts (out-of-place). */
 int fe_relayout(fe_handle* h, const int32_t* bptr, int64_t n_nodes, int dof, int to_layout,
                 const double* src, double* dst);

@@ -276,3 +276,39 @@ def test_gmsh_beams_and_cantilever_trend(ctx, golden):
         errs.append(abs(x[2 * tip + 1] - delta))
     # the tip deflection approaches the Euler-Bernoulli value as the mesh is refined
     assert errs[-1] < errs[0]
+
+
+@pytest.mark.parametrize('name', ['beam3d_0.5', 'beam3d_1'])
+def test_gmsh_beam3d_script(ctx, name):
+    """The reference's scripts/Elasticity/beam3d_example_2.py on its own Gmsh input files: tets read
+    from the .msh (file node order), x = 0 clamped, -1000 in z on every node of the edge x = 10,
+    z = 2; solve() against the oracle's direct FP64 solve, and the ElasticityResults3D accessors."""
+    import os
+    from finite_elements_amd import gmsh, results
+    from oracle import vectorized as vz
+    print(f'gmsh 3D beam {name}', flush=True)
+    path = os.path.join(os.path.dirname(__file__), 'golden', 'gmsh', f'{name}.msh')
+    mats = {'E': 30e6, 'nu': 0.25, 'rho': 2.7}
+    am = gmsh.array_mesh_from_msh(path, 'tet_elastic', mats)
+    c = am.coords
+    nodes = am.nodes
+    loaded = np.nonzero((c[:, 0] == 10) & (c[:, 2] == 2))[0]
+    clamped = np.nonzero(c[:, 0] == 0)[0]
+    nl = [loads.NodeLoad(nodes[i], -1000, 3) for i in loaded]
+    bcs = [conditions.NodeBoundaryCondition(nodes[i], 0, d) for i in clamped for d in (1, 2, 3)]
+    an = FiniteElementAnalysis(am, [], [], nl, [], [], bcs, [], [], plane_strain=False, plane_stress=True)
+    x = np.asarray(an.solve(tol=1e-10).result_vector)
+    n = 3 * c.shape[0]
+    K = vz.assemble_csr('tet_elastic', c, am.connectivity, mats)
+    f = np.zeros(n)
+    f[3 * loaded + 2] = -1000.0
+    fixed = (3 * clamped[:, None] + np.arange(3)).reshape(-1)
+    u_ref = vz.reduced_solve(K, f, fixed)
+    err = np.linalg.norm(x[:n] - u_ref) / np.linalg.norm(u_ref)
+    print(f'  {c.shape[0]} nodes, {am.connectivity.shape[0]} tets: {an.last_solve_info["iterations"]} it, '
+          f'error vs direct {err:.2e}, tip w {u_ref[3 * loaded + 2].mean():.4e}', flush=True)
+    assert err < 1e-8
+    assert np.all(u_ref[3 * loaded + 2] < 0)
+    r = results.ElasticityResults3D(an.mesh, list(x), an.plane_strain, an.plane_stress)
+    assert r.displacement_per_node_z() == x[2:n:3].tolist()
+

@@ -110,3 +110,18 @@ def test_result_properties(ctx):
         d = an.dimension
         assert r.displacements_per_element[els[0]] == [float(x[an.mesh.node_to_index[p] * d + c])
                                                         for p in els[0].points for c in range(d)]
+        # component accessors of the 2D / 3D classes (the reference's index of each component)
+        names = (['axial_strain_x', 'axial_strain_y', 'shear_strain_xy'] if d == 2 else
+                 ['axial_strain_x', 'axial_strain_y', 'axial_strain_z', 'shear_strain_xy',
+                  'shear_strain_yz', 'shear_strain_zx'])
+        for k, name in enumerate(names):
+            for which, ref_arr in (('strain', f_ref), ('stress', s_ref)):
+                comp = getattr(r, name.replace('strain', which))()
+                got = np.array([comp[el] for el in els])
+                scale = np.maximum(np.abs(ref_arr).max(axis=1), 1e-300)
+                assert np.max(np.abs(got - ref_arr[:, k]) / scale) < 1e-12
+        u = x[:n].reshape(-1, d)
+        assert r.displacement_per_node_x() == u[:, 0].tolist()
+        assert r.displacement_per_node_y() == u[:, 1].tolist()
+        if d == 3:
+            assert r.displacement_per_node_z() == u[:, 2].tolist()
