@@ -938,8 +938,10 @@ int pcg_common(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M
   }
   std::vector<fe_pcg_seg_stats> seg((size_t)nseg);
   if (ntiles > 0) {
-    FE_CHECK_HIP(hipMemcpy(seg.data(), w.seg_stats, nseg * sizeof(fe_pcg_seg_stats),
-                           hipMemcpyDeviceToHost));
+    // on the solver's stream: a plain hipMemcpy would also wait for every other blocking stream
+    FE_CHECK_HIP(hipMemcpyAsync(seg.data(), w.seg_stats, nseg * sizeof(fe_pcg_seg_stats),
+                                hipMemcpyDeviceToHost, h->stream));
+    FE_CHECK_HIP(hipStreamSynchronize(h->stream));
   } else {
     for (auto& q : seg) q = fe_pcg_seg_stats{};
   }
@@ -963,8 +965,9 @@ int pcg_common(fe_handle* h, const fe_bsr* A64, const fe_bsr* A32, const void* M
     rc = dispatch_dof<double>(h, pb64, w, &o64, &s64);
     if (rc) return rc;
     std::vector<fe_pcg_seg_stats> seg64((size_t)nseg);
-    FE_CHECK_HIP(hipMemcpy(seg64.data(), w.seg_stats, nseg * sizeof(fe_pcg_seg_stats),
-                           hipMemcpyDeviceToHost));
+    FE_CHECK_HIP(hipMemcpyAsync(seg64.data(), w.seg_stats, nseg * sizeof(fe_pcg_seg_stats),
+                                hipMemcpyDeviceToHost, h->stream));
+    FE_CHECK_HIP(hipStreamSynchronize(h->stream));
     for (int g = 0; g < nseg; ++g) {
       if (seg[g].status == FE_OK) continue;
       fe_pcg_seg_stats q = seg64[g];
