@@ -454,6 +454,10 @@ def plates_bench(count=4096, precision='fp64', precond='two_level', compare=True
         dc = mb.coarse_space()
         log(f'coarse space: {dc.host.n_agg} aggregates, {dc.n_coarse} coarse DOF in {dc.n_seg} blocks '
             f'of at most {dc.host.max_block}')
+    # the solver's chunk schedule, autotuned like the assembly's (untimed)
+    best_ce, ce_times = mb.tune_solver(precision=precision, precond=precond)
+    log(f'solver autotune: {best_ce} iterations per chunk; ' +
+        ', '.join(f'{k}: {v * 1e3:.1f} ms' for k, v in ce_times.items()))
 
     def run(prec, pc):
         mb.solve(tol=1e-8, precision=prec, precond=pc, graph=True, max_iter=20000)
@@ -480,7 +484,9 @@ def plates_bench(count=4096, precision='fp64', precond='two_level', compare=True
            'unit': 'plates/s', 'seconds': head['seconds'], 'plates': count, 'precision': precision,
            'precond': precond, 'dofs': int(mb.sysm.n), 'iterations_min': head['iterations_min'],
            'iterations_max': head['iterations_max'], 'not_converged': head['not_converged'],
-           'fallbacks': head['fallbacks'], 'max_relres': head['max_relres']}
+           'fallbacks': head['fallbacks'], 'max_relres': head['max_relres'],
+           'solver_schedule': {'check_every': int(best_ce),
+                               'candidates_ms': {str(k): v * 1e3 for k, v in ce_times.items()}}}
     if compare:
         other = run('mixed' if precision == 'fp64' else 'fp64', precond)
         res[f'{other["precision"]}_{precond}'] = other

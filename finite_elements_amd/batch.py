@@ -65,6 +65,7 @@ class MeshBatch:
             self.schedule = best
         self._fixed = None
         self._solver = None
+        self.check_every = 16  # iterations per chunk (one status poll, one graph replay)
 
     @property
     def n_meshes(self) -> int:
@@ -87,8 +88,10 @@ class MeshBatch:
         return self.sysm.coarse_space(self.fixed_dofs, node_offsets=self.node_offsets, max_block=max_block)
 
     def solve(self, tol: float = 1e-8, precision: str = 'fp64', precond: str = 'block_jacobi',
-              graph: bool = True, max_iter: Optional[int] = None, check_every: int = 16):
+              graph: bool = True, max_iter: Optional[int] = None, check_every: Optional[int] = None):
         """Assemble and solve every mesh; returns (u on the device, info with per-mesh arrays).
+
+        check_every: iterations per chunk (default: ``self.check_every``, set by ``tune_solver``).
 
         precond 'block_jacobi', 'jacobi' or 'two_level' (block Jacobi plus each mesh's own coarse
         correction over its aggregated rigid-body modes).  precision 'mixed' iterates in FP32 inside
@@ -97,10 +100,39 @@ class MeshBatch:
         PCG; 'fp64' iterates in FP64 throughout."""
         K, KD, b = self.assemble()
         coarse = self.coarse_space() if precond == 'two_level' else None
+        ce = self.check_every if check_every is None else int(check_every)
         solver = self.sysm.solver(KD, precond, precision, segments=self.node_offsets, graph=graph,
-                                  check_every=check_every, coarse=coarse)
+                                  check_every=ce, coarse=coarse)
         u, info = solver.solve(b, tol=tol, max_iter=max_iter)
         return u, info
+
+    def tune_solver(self, candidates=(8, 16, 32), tol: float = 1e-8, precision: str = 'fp64',
+                    precond: str = 'two_level', graph: bool = True, reps: int = 2,
+                    max_iter: Optional[int] = None):
+        """Autotune the batched solve's chunk schedule: time the whole solve (assembly, set-up and
+        iterations) for every chunk length -- iterations per launch group, between two status polls
+        and one HIP-graph replay; longer chunks poll less, shorter ones stop closer to the last
+        plate's convergence -- after one warm-up each, keep the fastest in ``self.check_every``.
+        Every timed solve must converge.  Returns (best, {check_every: seconds})."""
+        import time
+        times = {}
+        for ce in candidates:
+            self.solve(tol=tol, precision=precision, precond=precond, graph=graph, max_iter=max_iter,
+                       check_every=ce)
+            best = None
+            for _ in range(reps):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                _, info = self.solve(tol=tol, precision=precision, precond=precond, graph=graph,
+                                     max_iter=max_iter, check_every=ce)
+                torch.cuda.synchronize()
+                el = time.perf_counter() - t0
+                if info['status'] != 0:
+                    raise RuntimeError(f'tune_solver: the solve with check_every={ce} did not converge')
+                best = el if best is None else min(best, el)
+            times[int(ce)] = best
+        self.check_every = min(times, key=times.get)
+        return self.check_every, times
 
     def split(self, u: torch.Tensor) -> List[np.ndarray]:
         """Per-mesh solution vectors (host) from the batched solution."""

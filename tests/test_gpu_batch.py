@@ -176,3 +176,19 @@ def test_batched_two_level(ctx, precision):
     for pr, up in zip(plates, mb.split(u)):
         ref_u, _ = _oracle_solution(pr)
         assert np.linalg.norm(up - ref_u) <= 1e-6 * np.linalg.norm(ref_u)
+
+
+def test_solver_chunk_autotune(ctx):
+    """The batched solver's chunk schedule is autotuned like the assembly's: every candidate chunk
+    length solves every plate, the fastest is kept, and later solves use it."""
+    _say('batched solver chunk autotune')
+    plates = _small_plates(200, 11)[:10]
+    mb = MeshBatch(plates, tune=False)
+    mb.coarse_space(max_block=60)
+    best, times = mb.tune_solver(candidates=(4, 8, 16), reps=1)
+    _say(f'  {len(plates)} plates: seconds per solve {times}, best {best}')
+    assert set(times) == {4, 8, 16} and best == min(times, key=times.get)
+    assert mb.check_every == best
+    _, info = mb.solve(tol=1e-8, precision='fp64', precond='two_level')
+    seg = info['segments']
+    assert np.all(seg['status'] == 0) and np.all(seg['relres'] <= 1e-8)
