@@ -25,9 +25,11 @@ class FESystem:
     """
 
     def __init__(self, kind: str, coords, conn, materials: dict, plane_mode: int = 0,
-                 device_index: int = 0, with_gather: bool = False, max_inc: Optional[int] = None):
+                 device_index: int = 0, with_gather: bool = False, max_inc: Optional[int] = None,
+                 ctx: Optional[Context] = None):
+        """ctx: the handle (and stream) the system's work runs on; default: the device's main one."""
         device.require_gpu()
-        self.ctx = Context.get(device_index)
+        self.ctx = Context.get(device_index) if ctx is None else ctx
         self.kind = kind
         _, self.npe, self.dof = device.KIND_INFO[kind]
         coords = np.asarray(coords, dtype=np.float64)

@@ -192,3 +192,34 @@ def test_solver_chunk_autotune(ctx):
     _, info = mb.solve(tol=1e-8, precision='fp64', precond='two_level')
     seg = info['segments']
     assert np.all(seg['status'] == 0) and np.all(seg['relres'] <= 1e-8)
+
+
+def test_split_batch_precision_classes(ctx):
+    """SplitBatch: the long thin plates (aspect ratio >= 4) in FP64, the others in FP32 inside FP64
+    reliable updates, two batches on two streams solved concurrently.  Every plate against its
+    direct solve; the FP64 class bitwise equal to the same plates batched and solved alone in FP64
+    (one launch per class, same per-segment arithmetic)."""
+    from finite_elements_amd.batch import SplitBatch, aspect_ratio
+    _say('split batch (precision classes, two streams)')
+    plates = _small_plates(400, 7)[:16]
+    asp = np.array([aspect_ratio(pr) for pr in plates])
+    assert (asp < 4).any() and (asp >= 4).any()
+    sb = SplitBatch(plates, fp64_aspect=4.0, tune=False)
+    sb.coarse_space(max_block=60)
+    u, info = sb.solve(tol=1e-10, precond='two_level')
+    seg = info['segments']
+    _say(f'  {len(plates)} plates: {int((info["precision"] == "mixed").sum())} mixed, '
+         f'{int((info["precision"] == "fp64").sum())} FP64, iterations {seg["iterations"].max()}, '
+         f'fallbacks {info["fallbacks"]}')
+    assert info['status'] == 0 and np.all(seg['status'] == 0) and np.all(seg['relres'] <= 1e-10)
+    assert list(info['precision']) == ['fp64' if a >= 4 else 'mixed' for a in asp]
+    parts = sb.split(u)
+    for pr, x in zip(plates, parts):
+        ref, _ = _oracle_solution(pr)
+        assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-6
+    hi = [pr for pr, a in zip(plates, asp) if a >= 4]
+    mb = MeshBatch(hi, tune=False)
+    mb.coarse_space(max_block=60)
+    u64, _ = mb.solve(tol=1e-10, precision='fp64', precond='two_level')
+    for a, b in zip(mb.split(u64), [x for x, a in zip(parts, asp) if a >= 4]):
+        assert np.array_equal(a, b)
