@@ -32,10 +32,12 @@ def test_concatenate_is_block_diagonal():
 
 def test_aspect_ratio_classes():
     """The precision classes of SplitBatch come from each mesh's bounding box: a plate of nx x ny
-    cells of one size has aspect ratio nx / ny (the 4096 plates: 2,063 below 4, 2,033 at or above)."""
+    square cells has aspect ratio max(nx, ny) / min(nx, ny) (the 4096 plates: 2,063 below 4, 2,033
+    at or above)."""
     from finite_elements_amd.batch import aspect_ratio
     plates = meshgen.plates(count=64, seed=3)
     for pr in plates:
-        assert np.isclose(aspect_ratio(pr), pr.meta['nx'] / pr.meta['ny'])
+        nx, ny = pr.meta['nx'], pr.meta['ny']
+        assert np.isclose(aspect_ratio(pr), max(nx, ny) / min(nx, ny))
     cube = meshgen.cube_elasticity(n=4)
     assert aspect_ratio(cube) == 1.0
