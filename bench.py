@@ -493,7 +493,7 @@ def plates_bench(count=4096, precision='fp64', precond='two_level', compare=True
         res[f'{other["precision"]}_{precond}'] = other
         res['fp64_block_jacobi'] = run('fp64', 'block_jacobi')
         # the precision classes: long thin plates in FP64, the rest mixed, two concurrent batches
-        split = split_plates_run(plates, precond, count)
+        split = split_plates_run(plates, precond, count, mb.schedule)
         res['split_mixed_fp64'] = split
         if split['plates_per_s'] > res['value']:
             # the faster method is the headline; the one-batch FP64 run stays beside it
@@ -509,13 +509,14 @@ def plates_bench(count=4096, precision='fp64', precond='two_level', compare=True
     return res
 
 
-def split_plates_run(plates, precond, count):
+def split_plates_run(plates, precond, count, schedule=None):
     """SplitBatch of the plates (aspect ratio >= 4 in FP64, the others mixed), the two classes solved
-    concurrently on two streams; timed like the one-batch runs (assembly + elimination + set-up +
-    solve, after one untimed solve).  Every plate must reach the tolerance."""
+    concurrently on two streams, with the assembly schedule the one-batch autotuner picked; timed
+    like the one-batch runs (assembly + elimination + set-up + solve, after one untimed solve).
+    Every plate must reach the tolerance."""
     import torch
     from finite_elements_amd.batch import SplitBatch
-    sb = SplitBatch(plates, fp64_aspect=4.0, tune=True)
+    sb = SplitBatch(plates, fp64_aspect=4.0, schedule=schedule)
     if precond == 'two_level':
         sb.coarse_space()
     sb.solve(tol=1e-8, precond=precond, max_iter=20000)
