@@ -61,11 +61,22 @@ def fused_assembly_bytes(sysm):
     # in place of the connectivity)
     g = sysm.graph
     n_welem = int(g.welem.numel())
+    vals = 8 * p.nnzb * d * d
+    compulsory = coords + conn + mats + vals
+    if g.producers > 0:
+        # the wave-specialised kernel: per range one map image (fixed size, the schedule's maxima:
+        # every range's image is loaded whole) and header, per element-list entry its id and vertex
+        # positions, the node lists (coordinates are loaded through them)
+        import ctypes
+        from finite_elements_amd import _hip
+        nb = ctypes.c_int64()
+        gs = g.struct()
+        _hip.call('fe_ws_map_bytes', ctypes.byref(gs), ctypes.byref(nb))
+        graph = g.n_blk * (nb.value + 48) + 12 * n_welem + 4 * int(g.wnode.numel())
+        return coords + mats + graph + vals, compulsory
     graph = (4 * (N + 1) * 2 + 4 * N + 4 * (p.nnzb + 1) + 2 * p.n_inc * npe + 2 * p.n_inc
              + 48 * g.n_blk + 4 * n_welem)
     welem_conn = 4 * npe * n_welem
-    vals = 8 * p.nnzb * d * d
-    compulsory = coords + conn + mats + vals
     return coords + welem_conn + mats + graph + vals, compulsory
 
 
@@ -351,7 +362,7 @@ def summarize_pmc(fetch_dir, write_dir):
            'kernels': {}}
     for name in sorted(fetch):
         if not any(k in name for k in ('k_cg_spmv', 'k_cg_vec', 'k_cg_reduce', 'k_assemble_fused',
-                                        'k_assemble_pipe', 'k_ru_', 'k_spmv_bsr')):
+                                        'k_assemble_pipe', 'k_assemble_ws', 'k_ru_', 'k_spmv_bsr')):
             continue
         # dispatches that did the work (launches of finished solves return at once)
         fa = np.array(fetch[name])
@@ -634,8 +645,9 @@ def main():
     ap.add_argument('--no-learned', action='store_true')
     ap.add_argument('--weights', default=None)
     ap.add_argument('--no-tune', action='store_true', help='keep the default assembly schedule')
-    ap.add_argument('--schedule', nargs=2, type=int, metavar=('THREADS', 'MAX_INC'),
-                    help='use this assembly schedule instead of the autotuner')
+    ap.add_argument('--schedule', nargs='+', type=int, metavar='N',
+                    help='THREADS MAX_INC [PRODUCERS]: use this assembly schedule instead of the '
+                         'autotuner')
     ap.add_argument('--max-iter', type=int, default=40000,
                     help='PCG iteration cap (lowered only for short counter-collection runs)')
     ap.add_argument('--copy-calibration', action='store_true',
@@ -688,29 +700,35 @@ def main():
         copy_calibration()
     # ---- assembly: autotune the row-block schedule, then time the best ----
     if args.schedule:
-        sysm.set_schedule(args.schedule[1], args.schedule[0])
-        best, tune_times = (args.schedule[0], args.schedule[1]), {}
+        sched = (tuple(args.schedule) + (0,))[:3]
+        sysm.set_schedule(sched[1], sched[0], sched[2])
+        best, tune_times = sched, {}
     elif args.no_tune:
-        best, tune_times = (sysm.graph.threads, sysm.graph.max_inc), {}
+        best, tune_times = (sysm.graph.threads, sysm.graph.max_inc, sysm.graph.producers), {}
     else:
         best, tune_times = sysm.tune_assembly()
-    log(f'assembly autotune: best (threads, max_inc) = {best}; ' +
+    log(f'assembly autotune: best (threads, max_inc, producers) = {best}; ' +
         ', '.join(f'{k}:{v:.3f}ms' for k, v in sorted(tune_times.items())))
     asm_ms = time_assembly(sysm)
     asm_bytes, asm_compulsory = fused_assembly_bytes(sysm)
     g_ = sysm.graph
     persistent = (g_.welem_conn is not None and 0 < g_.max_nodes < best[0] and g_.max_inc <= best[0]
                   and g_.max_elem <= best[0] and g_.n_blk >= 8)
-    asm_kernel = 'k_assemble_pipe' if persistent else 'k_assemble_fused'
+    if best[2] > 0:
+        asm_kernel = 'k_assemble_ws'
+        asm_targs = f'{best[0]}, {best[2]}'
+    else:
+        asm_kernel = 'k_assemble_pipe' if persistent else 'k_assemble_fused'
+        asm_targs = f'{best[0]}'
     asm = {'kernel': asm_kernel,
-           'schedule': {'threads': best[0], 'max_inc': best[1]},
+           'schedule': {'threads': best[0], 'max_inc': best[1], 'producers': best[2]},
            'ms': asm_ms, 'bytes': asm_bytes,
            'gbps': asm_bytes / asm_ms / 1e6, 'frac': asm_bytes / asm_ms / 1e6 / HBM_PEAK_GBS,
            'compulsory_bytes': asm_compulsory,
            'compulsory_gbps': asm_compulsory / asm_ms / 1e6,
            'compulsory_frac': asm_compulsory / asm_ms / 1e6 / HBM_PEAK_GBS,
            'traffic': measured_traffic(f"{asm_kernel}<{sysm.mesh.kind_id}, {_hip.FE_STIFFNESS}, "
-                                       f"{_hip.FE_LAYOUT_PLANES}, {best[0]}>")}
+                                       f"{_hip.FE_LAYOUT_PLANES}, {asm_targs}>")}
     log(f'assembly {asm_ms:.3f} ms, {asm["gbps"]:.0f} GB/s algorithmic')
 
     model = None

@@ -54,9 +54,11 @@ class FeGraph(ctypes.Structure):
     _fields_ = [('n_nodes', _i64), ('nnzb', _i64), ('n_inc', _i64), ('n_blk', _i64),
                 ('max_inc', _i32), ('threads', _i32), ('bptr', _vp), ('bcol', _vp), ('inc_ptr', _vp),
                 ('inc', _vp), ('bseg_ptr', _vp), ('bseg', _vp), ('blk', _vp), ('max_elem', _i32),
-                ('pad', _i32), ('welem_ptr', _vp), ('welem', _vp), ('inc_sa', _vp), ('bdiag', _vp),
-                ('wg_hdr', _vp), ('welem_conn', _vp), ('n_welem', _i64), ('max_nodes', _i32),
-                ('pad2', _i32)]
+                ('producers', _i32), ('welem_ptr', _vp), ('welem', _vp), ('inc_sa', _vp),
+                ('bdiag', _vp), ('wg_hdr', _vp), ('welem_conn', _vp), ('n_welem', _i64),
+                ('max_nodes', _i32), ('max_nbq', _i32), ('max_nseg', _i32), ('max_wnode', _i32),
+                ('wnode', _vp), ('welem_local', _vp), ('wcode', _vp), ('brow', _vp),
+                ('wmaps', _vp)]
 
 
 class FeBsr(ctypes.Structure):
@@ -119,6 +121,8 @@ SIGNATURES = {
     'fe_pattern_destroy': (_int, [_vp]),
     'fe_assemble_gather': (_int, [_vp, _vp, _vp, _vp, _i64, _vp]),
     'fe_assemble_fused': (_int, [_vp, _P(FeMesh), _P(FeGraph), _int, _int, _vp]),
+    'fe_ws_map_bytes': (_int, [_P(FeGraph), _P(_i64)]),
+    'fe_ws_pack': (_int, [_vp, _P(FeGraph), _vp]),
     'fe_relayout': (_int, [_vp, _vp, _i64, _int, _int, _vp, _vp]),
     'fe_apply_dirichlet_bsr': (_int, [_vp, _P(FeBsr), _vp, _vp]),
     'fe_dirichlet_rhs': (_int, [_vp, _i64, _vp, _vp, _vp, _vp, _vp]),
@@ -128,6 +132,9 @@ SIGNATURES = {
     'fe_bsr_convert': (_int, [_vp, _P(FeBsr), _P(FeBsr)]),
     'fe_element_fields': (_int, [_vp, _P(FeMesh), _vp, _vp, _vp, _vp]),
     'fe_schedule_elements': (_int, [_vp, _vp, _int, _i64, _vp, _i64, _vp, _vp, _vp, _P(_i32)]),
+    'fe_schedule_ranges': (_int, [_vp, _i64, _int, _int, _vp, _P(_i64), _P(_i32)]),
+    'fe_schedule_nodes': (_int, [_vp, _int, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _P(_i32)]),
+    'fe_schedule_codes': (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp, _vp]),
     'fe_bsr_repack': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp]),
     'fe_bsr_fold': (_int, [_vp, _P(FeBsr), _P(FeBsr), _vp, _vp, _vp]),
     'fe_precond_setup': (_int, [_vp, _P(FeBsr), _int, _vp]),

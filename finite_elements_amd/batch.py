@@ -63,7 +63,7 @@ class MeshBatch:
         self.fixed_values = vals
         self.f = f
         self.tune_times = None
-        self.schedule = None  # (threads, max_inc) picked by the autotuner
+        self.schedule = None  # (threads, max_inc, producers) picked by the autotuner
         if tune:
             best, self.tune_times = self.sysm.tune_assembly()
             self.schedule = best
@@ -163,8 +163,8 @@ class SplitBatch:
 
     def __init__(self, problems: Sequence[Problem], fp64_aspect: float = 4.0, device_index: int = 0,
                  tune: bool = True, schedule=None):
-        """schedule: (threads, max_inc) of the fused assembly to use in both classes instead of
-        autotuning each (e.g. the schedule a MeshBatch of all the meshes picked)."""
+        """schedule: (threads, max_inc[, producers]) of the fused assembly to use in both classes
+        instead of autotuning each (e.g. the schedule a MeshBatch of all the meshes picked)."""
         from .device import Context
         self.problems = list(problems)
         asp = np.array([aspect_ratio(pr) for pr in self.problems])
@@ -179,9 +179,9 @@ class SplitBatch:
                     mb = MeshBatch([self.problems[i] for i in idx], device_index,
                                    tune and schedule is None, ctx=ctx)
                     if schedule is not None:
-                        threads, max_inc = schedule
-                        mb.sysm.set_schedule(int(max_inc), int(threads))
-                        mb.schedule = (int(threads), int(max_inc))
+                        threads, max_inc, prods = (tuple(schedule) + (0,))[:3]
+                        mb.sysm.set_schedule(int(max_inc), int(threads), int(prods))
+                        mb.schedule = (int(threads), int(max_inc), int(prods))
                     self.batches[name] = mb
                 ctx.stream.synchronize()
 

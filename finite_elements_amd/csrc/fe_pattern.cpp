@@ -172,23 +172,22 @@ int build(fe_pattern& p, const int32_t* conn, int with_gather) {
 // element slots of one workgroup: 14 bits of the packed (slot << 2 | local vertex) incidence code
 constexpr size_t kMaxElemSlots = 16383;
 
-extern "C" int fe_pattern_schedule(fe_pattern* p, int max_inc_per_block) {
-  if (!p) return fe_set_error(FE_ERR_ARG, "fe_pattern_schedule: null pattern");
-  if (max_inc_per_block < 1 || max_inc_per_block > 4096)
-    return fe_set_error(FE_ERR_ARG, "fe_pattern_schedule: max_inc_per_block out of range");
-  const int kMaxNodes = 256;
-  p->blk.clear();
-  p->blk.push_back(0);
-  int32_t cur_inc = 0, cur_nodes = 0, max_inc = 0;
-  for (int64_t i = 0; i < p->n_nodes; ++i) {
-    const int32_t ni = p->inc_ptr[i + 1] - p->inc_ptr[i];
+// Greedy row-block schedule: consecutive nodes while a range holds at most max_inc incidences and
+// max_nodes nodes.  Appends the range starts and the end to blk; returns the largest range's
+// incidence count in *biggest.
+static int greedy_ranges(const int32_t* inc_ptr, int64_t n_nodes, int max_inc_per_block, int max_nodes,
+                         std::vector<int32_t>& blk, int32_t* biggest, const char* who) {
+  blk.clear();
+  blk.push_back(0);
+  int32_t cur_inc = 0, cur_nodes = 0, mx = 0;
+  for (int64_t i = 0; i < n_nodes; ++i) {
+    const int32_t ni = inc_ptr[i + 1] - inc_ptr[i];
     if (ni > max_inc_per_block)
-      return fe_set_error(FE_ERR_ARG,
-                          "fe_pattern_schedule: node %lld has %d incidences > max_inc_per_block %d",
-                          (long long)i, ni, max_inc_per_block);
-    if (cur_nodes > 0 && (cur_inc + ni > max_inc_per_block || cur_nodes >= kMaxNodes)) {
-      p->blk.push_back((int32_t)i);
-      max_inc = std::max(max_inc, cur_inc);
+      return fe_set_error(FE_ERR_ARG, "%s: node %lld has %d incidences > max_inc %d", who, (long long)i,
+                          ni, max_inc_per_block);
+    if (cur_nodes > 0 && (cur_inc + ni > max_inc_per_block || cur_nodes >= max_nodes)) {
+      blk.push_back((int32_t)i);
+      mx = std::max(mx, cur_inc);
       cur_inc = 0;
       cur_nodes = 0;
     }
@@ -196,10 +195,34 @@ extern "C" int fe_pattern_schedule(fe_pattern* p, int max_inc_per_block) {
     cur_nodes++;
   }
   if (cur_nodes > 0) {
-    p->blk.push_back((int32_t)p->n_nodes);
-    max_inc = std::max(max_inc, cur_inc);
+    blk.push_back((int32_t)n_nodes);
+    mx = std::max(mx, cur_inc);
   }
-  p->max_inc = max_inc;
+  *biggest = mx;
+  return FE_OK;
+}
+
+extern "C" int fe_pattern_schedule(fe_pattern* p, int max_inc_per_block) {
+  if (!p) return fe_set_error(FE_ERR_ARG, "fe_pattern_schedule: null pattern");
+  if (max_inc_per_block < 1 || max_inc_per_block > 4096)
+    return fe_set_error(FE_ERR_ARG, "fe_pattern_schedule: max_inc_per_block out of range");
+  int32_t mx = 0;
+  const int rc = greedy_ranges(p->inc_ptr.data(), p->n_nodes, max_inc_per_block, 256, p->blk, &mx,
+                               "fe_pattern_schedule");
+  if (rc != FE_OK) return rc;
+  p->max_inc = mx;
+  return FE_OK;
+}
+
+extern "C" int fe_schedule_ranges(const int32_t* inc_ptr, int64_t n_nodes, int max_inc, int max_nodes,
+                                  int32_t* blk, int64_t* n_blk, int32_t* biggest) {
+  if (!inc_ptr || !blk || !n_blk || !biggest) return fe_set_error(FE_ERR_ARG, "fe_schedule_ranges: null argument");
+  if (max_inc < 1 || max_nodes < 1) return fe_set_error(FE_ERR_ARG, "fe_schedule_ranges: bad limits");
+  std::vector<int32_t> b;
+  const int rc = greedy_ranges(inc_ptr, n_nodes, max_inc, max_nodes, b, biggest, "fe_schedule_ranges");
+  if (rc != FE_OK) return rc;
+  std::copy(b.begin(), b.end(), blk);
+  *n_blk = (int64_t)b.size() - 1;
   return FE_OK;
 }
 
@@ -238,6 +261,79 @@ extern "C" int fe_schedule_elements(const int32_t* inc_ptr, const int32_t* inc, 
     mx = std::max(mx, (int32_t)ids.size());
   }
   *max_elem = mx;
+  return FE_OK;
+}
+
+extern "C" int fe_schedule_nodes(const int32_t* welem_conn, int npe, int64_t n_welem,
+                                 const int32_t* welem_ptr, int64_t n_blk, int64_t n_nodes,
+                                 int32_t* wnode_ptr, int32_t* wnode, uint16_t* welem_local,
+                                 int32_t* max_wnode) {
+  if (!welem_ptr || !wnode_ptr || !max_wnode || (n_welem > 0 && (!welem_conn || !wnode || !welem_local)))
+    return fe_set_error(FE_ERR_ARG, "fe_schedule_nodes: null argument");
+  if (npe < 2 || npe > 4) return fe_set_error(FE_ERR_ARG, "fe_schedule_nodes: npe must be 2..4");
+  if (n_blk < 0 || welem_ptr[0] != 0 || welem_ptr[n_blk] != n_welem)
+    return fe_set_error(FE_ERR_ARG, "fe_schedule_nodes: element lists must cover 0..n_welem");
+  std::vector<int32_t> ids;
+  int64_t out = 0;
+  int32_t mx = 0;
+  wnode_ptr[0] = 0;
+  for (int64_t w = 0; w < n_blk; ++w) {
+    const int32_t e0 = welem_ptr[w], e1 = welem_ptr[w + 1];
+    if (e1 < e0) return fe_set_error(FE_ERR_ARG, "fe_schedule_nodes: element list offsets must not decrease");
+    ids.clear();
+    for (int b = 0; b < npe; ++b)
+      for (int32_t e = e0; e < e1; ++e) {
+        const int32_t n = welem_conn[(int64_t)b * n_welem + e];
+        if (n < 0 || n >= n_nodes) return fe_set_error(FE_ERR_ARG, "fe_schedule_nodes: node id out of range");
+        ids.push_back(n);
+      }
+    std::sort(ids.begin(), ids.end());
+    ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+    if (ids.size() > 65535)
+      return fe_set_error(FE_ERR_ARG, "fe_schedule_nodes: %zu nodes in one workgroup > 65535", ids.size());
+    std::copy(ids.begin(), ids.end(), wnode + out);
+    for (int32_t e = e0; e < e1; ++e)
+      for (int b = 0; b < 4; ++b) {
+        uint16_t loc = 0;
+        if (b < npe) {
+          const int32_t n = welem_conn[(int64_t)b * n_welem + e];
+          loc = (uint16_t)(std::lower_bound(ids.begin(), ids.end(), n) - ids.begin());
+        }
+        welem_local[(int64_t)e * 4 + b] = loc;
+      }
+    out += (int64_t)ids.size();
+    wnode_ptr[w + 1] = (int32_t)out;
+    mx = std::max(mx, (int32_t)ids.size());
+  }
+  *max_wnode = mx;
+  return FE_OK;
+}
+
+static inline bool bcol_is_row(const int32_t* bcol, int32_t q, int32_t i) { return bcol[q] == i; }
+
+extern "C" int fe_schedule_codes(const int32_t* bptr, const int32_t* bcol, const int32_t* bseg_ptr,
+                                 const uint16_t* bseg, const int32_t* inc_ptr, const uint16_t* inc_sa,
+                                 int64_t n_nodes, const int32_t* blk, int64_t n_blk, uint16_t* code,
+                                 uint16_t* brow) {
+  if (!bptr || !bcol || !bseg_ptr || !inc_ptr || !blk || !code || !brow || (n_nodes > 0 && (!bseg || !inc_sa)))
+    return fe_set_error(FE_ERR_ARG, "fe_schedule_codes: null argument");
+  if (n_blk < 0 || blk[0] != 0 || blk[n_blk] != n_nodes)
+    return fe_set_error(FE_ERR_ARG, "fe_schedule_codes: schedule must cover nodes 0..n_nodes");
+  for (int64_t w = 0; w < n_blk; ++w) {
+    if (blk[w + 1] < blk[w]) return fe_set_error(FE_ERR_ARG, "fe_schedule_codes: schedule offsets must not decrease");
+    if (blk[w + 1] - blk[w] > 32767)
+      return fe_set_error(FE_ERR_ARG, "fe_schedule_codes: a range of more than 32767 nodes");
+    for (int32_t i = blk[w]; i < blk[w + 1]; ++i)
+      for (int32_t q = bptr[i]; q < bptr[i + 1]; ++q) {
+        brow[q] = (uint16_t)((i - blk[w]) | (bcol_is_row(bcol, q, i) ? 0x8000 : 0));
+        for (int32_t c = bseg_ptr[q]; c < bseg_ptr[q + 1]; ++c) {
+          const int32_t sa = inc_sa[inc_ptr[i] + (bseg[c] >> 2)];
+          if ((sa >> 2) >= 4096)
+            return fe_set_error(FE_ERR_ARG, "fe_schedule_codes: element slot %d >= 4096", sa >> 2);
+          code[c] = (uint16_t)(((sa >> 2) << 4) | ((sa & 3) << 2) | (bseg[c] & 3));
+        }
+      }
+  }
   return FE_OK;
 }
 

@@ -158,24 +158,20 @@ class Pattern:
 
     @staticmethod
     def schedule_for(inc_ptr: np.ndarray, max_inc: int, max_nodes: int = 256):
-        """Row-block schedule (same greedy rule as fe_pattern_schedule) for a different max_inc."""
+        """Row-block schedule (the greedy rule of fe_pattern_schedule, fe_schedule_ranges) for a
+        different max_inc: (blk, largest incidence count of a range)."""
+        inc_ptr = np.ascontiguousarray(np.asarray(inc_ptr, dtype=np.int32))
+        n = inc_ptr.size - 1
         counts = np.diff(inc_ptr)
         if counts.size and counts.max() > max_inc:
             raise ValueError(f'a node has {counts.max()} incidences > max_inc {max_inc}')
-        blk = [0]
-        cur = nodes = 0
-        biggest = 0
-        for i, c in enumerate(counts.tolist()):
-            if nodes and (cur + c > max_inc or nodes >= max_nodes):
-                blk.append(i)
-                biggest = max(biggest, cur)
-                cur = nodes = 0
-            cur += c
-            nodes += 1
-        if nodes:
-            blk.append(len(counts))
-            biggest = max(biggest, cur)
-        return np.asarray(blk, dtype=np.int32), biggest
+        blk = np.empty(n + 1, dtype=np.int32)
+        n_blk = ctypes.c_int64()
+        biggest = ctypes.c_int32()
+        _hip.call('fe_schedule_ranges', inc_ptr.ctypes.data_as(ctypes.c_void_p), n, int(max_inc),
+                  int(max_nodes), blk.ctypes.data_as(ctypes.c_void_p), ctypes.byref(n_blk),
+                  ctypes.byref(biggest))
+        return blk[:n_blk.value + 1].copy(), int(biggest.value)
 
 
 # ------------------------------------------------------------------------------------------------
@@ -271,10 +267,49 @@ class DeviceGraph:
         self._set_welem_conn()
 
     def _set_welem_conn(self):
+        self.wnode = self.welem_local = self.wcode = self.brow = self.wmaps = None
+        self.max_wnode = 0
         if self._conn is None or self._welem_host.size == 0:
             self.welem_conn = None
             return
-        self.welem_conn = self.ctx.tensor(np.ascontiguousarray(self._conn[self._welem_host].T))
+        wconn = np.ascontiguousarray(self._conn[self._welem_host].T)
+        self.welem_conn = self.ctx.tensor(wconn)
+        if self.producers > 0:
+            self._set_ws_lists(wconn)
+
+    def _set_ws_lists(self, wconn):
+        """The wave-specialised kernel's node lists and resolved block maps for this schedule."""
+        # node lists of the element lists (fe_schedule_nodes) and their slices in header words 10, 11
+        npe, n_welem = wconn.shape
+        wptr = self._welem_ptr_host
+        wnode_ptr = np.empty(self.n_blk + 1, dtype=np.int32)
+        wnode = np.empty(max(npe * n_welem, 1), dtype=np.int32)
+        local = np.empty((n_welem, 4), dtype=np.uint16)
+        mx = ctypes.c_int32()
+        _hip.call('fe_schedule_nodes', wconn.ctypes.data_as(ctypes.c_void_p), npe, n_welem,
+                  wptr.ctypes.data_as(ctypes.c_void_p), self.n_blk, self.n_nodes,
+                  wnode_ptr.ctypes.data_as(ctypes.c_void_p), wnode.ctypes.data_as(ctypes.c_void_p),
+                  local.ctypes.data_as(ctypes.c_void_p), ctypes.byref(mx))
+        self.max_wnode = int(mx.value)
+        self.wnode = self.ctx.tensor(wnode[:int(wnode_ptr[-1])])
+        self.welem_local = self.ctx.tensor(local.view(np.int16))
+        hdr = self._hdr_host.copy()
+        hdr[:, 10], hdr[:, 11] = wnode_ptr[:-1], np.diff(wnode_ptr)
+        self.wg_hdr = self.ctx.tensor(hdr.reshape(-1))
+        # the block maps resolved against this schedule's element lists (fe_schedule_codes)
+        p = self.pattern
+        code = np.empty(max(p.bseg.size, 1), dtype=np.uint16)
+        brow = np.empty(max(p.nnzb, 1), dtype=np.uint16)
+        blk = self._blk_host
+        _hip.call('fe_schedule_codes', p.bptr.ctypes.data_as(ctypes.c_void_p),
+                  p.bcol.ctypes.data_as(ctypes.c_void_p),
+                  p.bseg_ptr.ctypes.data_as(ctypes.c_void_p), p.bseg.ctypes.data_as(ctypes.c_void_p),
+                  p.inc_ptr.ctypes.data_as(ctypes.c_void_p),
+                  self._inc_sa_host.ctypes.data_as(ctypes.c_void_p), p.n_nodes,
+                  blk.ctypes.data_as(ctypes.c_void_p), self.n_blk,
+                  code.ctypes.data_as(ctypes.c_void_p), brow.ctypes.data_as(ctypes.c_void_p))
+        self.wcode = self.ctx.tensor(code.view(np.int16))
+        self.brow = self.ctx.tensor(brow.view(np.int16))
 
     def padded(self) -> 'PaddedGraph':
         """The padded pattern of the same rows (row_align = 4), built once."""
@@ -282,9 +317,10 @@ class DeviceGraph:
             self._padded = PaddedGraph(self)
         return self._padded
 
-    def set_schedule(self, blk: np.ndarray, max_inc: int, threads: int = 256):
+    def set_schedule(self, blk: np.ndarray, max_inc: int, threads: int = 256, producers: int = 0):
         """Row-block schedule of the fused assembly and its element schedule (the distinct elements
-        of every workgroup and each incidence's slot among them)."""
+        of every workgroup and each incidence's slot among them).  ``producers`` > 0 selects the
+        wave-specialised persistent kernel with that many producer wavefronts."""
         blk = np.ascontiguousarray(np.asarray(blk, dtype=np.int32))
         wptr, welem, inc_sa, max_elem = element_schedule(self.pattern, blk)
         self.blk = self.ctx.tensor(blk)
@@ -307,7 +343,14 @@ class DeviceGraph:
             hdr[:, 6], hdr[:, 7] = B0, B1 - B0
             hdr[:, 8], hdr[:, 9] = p.bseg_ptr[B0], p.bseg_ptr[B1] - p.bseg_ptr[B0]
         self.wg_hdr = self.ctx.tensor(hdr.reshape(-1) if hdr.size else np.zeros(12, np.int32))
+        self._hdr_host = hdr
+        self._welem_ptr_host = wptr
+        self._blk_host = blk
+        self._inc_sa_host = np.ascontiguousarray(inc_sa if inc_sa.size else np.zeros(1, np.uint16))
         self.max_nodes = int(np.diff(blk).max()) if self.n_blk else 0
+        self.max_nbq = int(hdr[:, 7].max()) if self.n_blk else 0
+        self.max_nseg = int(hdr[:, 9].max()) if self.n_blk else 0
+        self.producers = int(producers)
         self._welem_host = welem
         self._set_welem_conn()
 
@@ -315,10 +358,26 @@ class DeviceGraph:
         return _hip.FeGraph(self.n_nodes, self.nnzb, int(self.inc.numel()), self.n_blk, self.max_inc,
                             self.threads,
                             _vp(self.bptr), _vp(self.bcol), _vp(self.inc_ptr), _vp(self.inc),
-                            _vp(self.bseg_ptr), _vp(self.bseg), _vp(self.blk), self.max_elem, 0,
-                            _vp(self.welem_ptr), _vp(self.welem), _vp(self.inc_sa), _vp(self.bdiag),
-                            _vp(self.wg_hdr), _vp(self.welem_conn), int(self._welem_host.size),
-                            self.max_nodes, 0)
+                            _vp(self.bseg_ptr), _vp(self.bseg), _vp(self.blk), self.max_elem,
+                            self.producers, _vp(self.welem_ptr), _vp(self.welem), _vp(self.inc_sa),
+                            _vp(self.bdiag), _vp(self.wg_hdr), _vp(self.welem_conn),
+                            int(self._welem_host.size), self.max_nodes, self.max_nbq, self.max_nseg,
+                            self.max_wnode, _vp(self.wnode), _vp(self.welem_local), _vp(self.wcode),
+                            _vp(self.brow), _vp(self.wmaps))
+
+    def ws_maps(self):
+        """The map images of the wave-specialised assembly for this schedule (fe_ws_pack), built on
+        the device on first use."""
+        if self.wmaps is None and self.wcode is not None:
+            nb = ctypes.c_int64()
+            gs = self.struct()
+            _hip.call('fe_ws_map_bytes', ctypes.byref(gs), ctypes.byref(nb))
+            n4 = max(self.n_blk * nb.value // 16, 1)
+            with torch.cuda.stream(self.ctx.stream):  # zeroed on the stream fe_ws_pack runs on
+                buf = torch.zeros((n4, 4), dtype=torch.int32, device=self.ctx.torch_device)
+            _hip.call('fe_ws_pack', self.ctx.h, ctypes.byref(gs), _vp(buf))
+            self.wmaps = buf
+        return self.wmaps
 
 
 def pad_pattern(bptr: np.ndarray, bcol: np.ndarray, align: int = 4):
@@ -490,6 +549,8 @@ def assemble_fused(mesh: DeviceMesh, graph: DeviceGraph, what: str = 'stiffness'
     n = graph.nnzb * mesh.dof * mesh.dof
     vals = ctx.empty(n) if out is None else out
     w = _hip.FE_STIFFNESS if what == 'stiffness' else _hip.FE_MASS
+    if graph.producers > 0:
+        graph.ws_maps()
     ms, gs = mesh.struct(), graph.struct()
     _hip.call('fe_assemble_fused', ctx.h, ctypes.byref(ms), ctypes.byref(gs), w, layout, _vp(vals))
     return vals

@@ -58,28 +58,33 @@ class FESystem:
                                self.mesh.plane_mode)
 
     # ---------------------------------------------------------------------------------------------
-    def set_schedule(self, max_inc: int, threads: int = 256):
+    def set_schedule(self, max_inc: int, threads: int = 256, producers: int = 0):
         """Re-tile the fused assembly: row-block schedule with at most ``max_inc`` node-element
-        incidences per workgroup, ``threads`` lanes per workgroup (64, 128, 256, 512 or 1024)."""
+        incidences per workgroup, ``threads`` lanes per workgroup (64, 128, 256, 512 or 1024) and,
+        when ``producers`` > 0, the wave-specialised kernel with that many producer wavefronts
+        (1024 threads: 4, 6 or 8; 512 threads: 2 or 4)."""
         blk, biggest = Pattern.schedule_for(self.pattern.inc_ptr, max_inc)
-        self.graph.set_schedule(blk, biggest, threads)
+        self.graph.set_schedule(blk, biggest, threads, producers)
 
     def tune_assembly(self, candidates=None, reps: int = 5, layout: int = _hip.FE_LAYOUT_PLANES):
-        """Autotuner of the fused assembly: time every (threads, max_inc) candidate with HIP events
-        on the device stream and keep the fastest.  Returns (best, {candidate: ms})."""
+        """Autotuner of the fused assembly: time every (threads, max_inc, producers) candidate with
+        HIP events on the device stream and keep the fastest.  Returns (best, {candidate: ms})."""
         if candidates is None:
             base = int(np.diff(self.pattern.inc_ptr).max()) if self.n_nodes else 1
-            cands = []
-            for threads in (128, 256, 512, 1024):
-                for mi in (48, 64, 96, 128, 192, 256, 384, 512, 768, 1024, 1536):
-                    if mi >= base and mi <= 8 * threads:
-                        cands.append((threads, mi))
-            candidates = cands
+            # one-range-per-workgroup and persistent forms, and the wave-specialised form (a
+            # schedule whose element or node lists outgrow its producers, or whose slots outgrow
+            # the LDS, is refused by the launch and skipped)
+            cands = [(256, 128, 0), (256, 192, 0), (256, 256, 0), (512, 256, 0), (512, 384, 0),
+                     (512, 512, 0), (1024, 512, 0), (1024, 768, 0), (1024, 1024, 0), (1024, 1536, 0),
+                     (512, 128, 2), (512, 192, 4), (512, 256, 4), (1024, 384, 6), (1024, 384, 8),
+                     (1024, 512, 8)]
+            candidates = [c for c in cands if c[1] >= base]
         out = self.stiffness(layout)
         times = {}
-        for threads, mi in candidates:
+        for cand in candidates:
+            threads, mi, prods = (tuple(cand) + (0,))[:3]
             try:
-                self.set_schedule(mi, threads)
+                self.set_schedule(mi, threads, prods)
                 self.stiffness(layout, out=out)
             except (ValueError, RuntimeError):
                 continue
@@ -90,9 +95,9 @@ class FESystem:
                 self.stiffness(layout, out=out)
             stop.record()
             stop.synchronize()
-            times[(threads, mi)] = start.elapsed_time(stop) / reps
+            times[(threads, mi, prods)] = start.elapsed_time(stop) / reps
         best = min(times, key=times.get)
-        self.set_schedule(best[1], best[0])
+        self.set_schedule(best[1], best[0], best[2])
         return best, times
 
     def stiffness(self, layout: int = _hip.FE_LAYOUT_BSR, out=None) -> torch.Tensor:

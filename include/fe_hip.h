@@ -104,20 +104,34 @@ typedef struct {
   const uint16_t* bseg;
   const int32_t* blk;
   int32_t max_elem; /* largest welem count of one workgroup */
-  int32_t pad;
+  int32_t producers; /* > 0: the wave-specialised persistent assembly with this many producer waves
+                        (needs wg_hdr, welem_conn, max_nbq and max_nseg; 0: not used) */
   const int32_t* welem_ptr;
   const int32_t* welem;
   const uint16_t* inc_sa;
   const int32_t* bdiag; /* per node, the block index of its diagonal entry (-1: node without elements) */
   /* Optional (NULL: derived on the device from the arrays above).  wg_hdr: per workgroup, 12 int32
-     {n0, n1, E0, nel, I0, ninc, B0, nbq, c0, nseg, 0, 0}: its node range [n0, n1), welem list start
-     and length, first incidence and count, first block and count, first bseg entry and count.
+     {n0, n1, E0, nel, I0, ninc, B0, nbq, c0, nseg, N0, nnode}: its node range [n0, n1), welem list
+     start and length, first incidence and count, first block and count, first bseg entry and count,
+     and its wnode list's start and length (0, 0 without node lists).
      welem_conn: the node ids of the welem list's elements, [npe][n_welem] (n_welem = welem_ptr[n_blk]). */
   const int32_t* wg_hdr;
   const int32_t* welem_conn;
   int64_t n_welem;
   int32_t max_nodes; /* largest node count of one workgroup's range (0: not given) */
-  int32_t pad2;
+  int32_t max_nbq;   /* largest block count of one workgroup's range (0: not given) */
+  int32_t max_nseg;  /* largest bseg entry count of one workgroup's range (0: not given) */
+  int32_t max_wnode; /* longest node list of one workgroup (fe_schedule_nodes; 0: not given) */
+  /* For the wave-specialised form: the node lists of the element lists (wnode, the workgroup's slice
+     in header words 10 and 11: start and length) and per element-list entry the positions of its
+     vertices in its workgroup's node list ([n_welem][4]). */
+  const int32_t* wnode;
+  const uint16_t* welem_local;
+  /* For the wave-specialised form: the schedule-resolved block maps of fe_schedule_codes, and one
+     map image per range written from them by fe_ws_pack (n_blk images of fe_ws_map_bytes bytes). */
+  const uint16_t* wcode;
+  const uint16_t* brow;
+  const void* wmaps;
 } fe_graph;
 
 /* Block sparse matrix (node-block rows) on the device; precision 0 = FP64, 1 = FP32; layout
@@ -225,9 +239,33 @@ int fe_pattern_destroy(fe_pattern* p);
    capacity n_inc) and per incidence its packed (slot << 2 | a) code (inc_sa: n_inc entries);
    max_elem receives the largest element count of one workgroup.  Host arrays; no counterpart in the
    reference (its assembly is a host loop). */
+/* Greedy row-block schedule of consecutive nodes (the rule fe_pattern_schedule applies): a range
+   takes nodes while it holds at most max_inc incidences and max_nodes nodes.  blk (capacity
+   n_nodes + 1) receives the range starts and the end, n_blk the range count, biggest the largest
+   range's incidence count.  Host arrays; no counterpart in the reference. */
+int fe_schedule_ranges(const int32_t* inc_ptr, int64_t n_nodes, int max_inc, int max_nodes,
+                       int32_t* blk, int64_t* n_blk, int32_t* biggest);
 int fe_schedule_elements(const int32_t* inc_ptr, const int32_t* inc, int npe, int64_t n_nodes,
                          const int32_t* blk, int64_t n_blk, int32_t* welem_ptr, int32_t* welem,
                          uint16_t* inc_sa, int32_t* max_elem);
+/* Node lists of a schedule's element lists (for the wave-specialised assembly): per workgroup w the
+   ascending distinct nodes of its elements, wnode[wnode_ptr[w] .. wnode_ptr[w+1]) (wnode capacity
+   npe * n_welem), and per element-list entry e the positions of its vertices in that list,
+   welem_local[e * 4 + b] (entries b >= npe are 0).  welem_conn is [npe][n_welem]; max_wnode receives
+   the longest list.  Host arrays; no counterpart in the reference. */
+int fe_schedule_nodes(const int32_t* welem_conn, int npe, int64_t n_welem, const int32_t* welem_ptr,
+                      int64_t n_blk, int64_t n_nodes, int32_t* wnode_ptr, int32_t* wnode,
+                      uint16_t* welem_local, int32_t* max_wnode);
+/* Schedule-resolved block maps (for the wave-specialised assembly): per bseg entry c of block (i, q),
+   code[c] = (slot << 4) | (a << 2) | b with slot the element's position in its workgroup's element
+   list, a the local vertex of node i and b that of the column node (the bseg and inc_sa codes
+   combined, so the block sum reads a record without going through the incidence list); per block
+   q, brow[q] = its row node's position in its workgroup's node range, | 0x8000 for the diagonal
+   block.  Host arrays (code: bseg's length; brow: nnzb); no counterpart in the reference. */
+int fe_schedule_codes(const int32_t* bptr, const int32_t* bcol, const int32_t* bseg_ptr,
+                      const uint16_t* bseg, const int32_t* inc_ptr, const uint16_t* inc_sa,
+                      int64_t n_nodes, const int32_t* blk, int64_t n_blk, uint16_t* code,
+                      uint16_t* brow);
 
 /* ---- assembly ------------------------------------------------------------------------------------ */
 /* Two-phase assembly from element matrices: vals[p] = sum over the sorted contribution segment of p.
@@ -242,6 +280,12 @@ int fe_assemble_gather(fe_handle* h, const double* ke, const int32_t* gather, co
    csr_matrix (finite_elements/analysis.py:324-339, 661) and m_matrix_data (357-365). */
 int fe_assemble_fused(fe_handle* h, const fe_mesh* mesh, const fe_graph* graph, int what, int layout,
                       double* vals);
+/* Map images of the wave-specialised assembly: the size of one range's image for this schedule
+   (fe_ws_map_bytes, host only), and the images of all n_blk ranges written into out (device memory,
+   zeroed by the caller, n_blk * that size bytes; fe_ws_pack reads the device arrays of g).  No
+   counterpart in the reference. */
+int fe_ws_map_bytes(const fe_graph* graph, int64_t* bytes);
+int fe_ws_pack(fe_handle* h, const fe_graph* graph, void* out);
 /* Per-element fields of a solution u (device, interleaved DOFs), component-major out[k * n_elem + e]
    (replaces ElasticityResults.strain / stress / energy_per_element, finite_elements/results.py:689-830,
    and MagneticResults' field, results.py:121-152).  Elasticity: strain and stress in Voigt order (tri

@@ -140,6 +140,81 @@ def test_persistent_assembly_equals_one_range_per_workgroup(ctx, kind, max_inc, 
         g.welem_conn = saved
 
 
+def _perturbed_large(kind, rng):
+    """Meshes with hundreds to thousands of node ranges at these schedules, so that every persistent
+    workgroup walks several ranges (both LDS slots, the register prefetch and the round loop)."""
+    if kind == 'tet_elastic':
+        c, t = meshgen.kuhn_tets(24)
+        c = c + rng.uniform(-0.01, 0.01, c.shape)
+    else:
+        c, t = meshgen.structured_tri(200, 150, 2.0, 1.5)
+        c = c + rng.uniform(-0.002, 0.002, c.shape)
+    n = t.shape[0]
+    if kind == 'tri_magnetic':
+        return c, t, {'mu': 4e-7 * np.pi * rng.uniform(1, 1000, n)}
+    mats = {'E': rng.uniform(5e10, 3e11, n), 'nu': rng.uniform(0.2, 0.45, n), 'rho': rng.uniform(1e3, 8e3, n)}
+    if kind == 'tri_elastic':
+        mats['thickness'] = rng.uniform(0.005, 0.5, n)
+    return c, t, mats
+
+
+@pytest.mark.parametrize('kind', ['tet_elastic', 'tri_elastic', 'tri_magnetic'])
+@pytest.mark.parametrize('threads,producers,max_inc', [(512, 2, 64), (512, 4, 256), (1024, 4, 96),
+                                                       (1024, 6, 384), (1024, 8, 512)])
+def test_wave_specialised_assembly_equals_one_range_per_workgroup(ctx, kind, threads, producers, max_inc):
+    """The wave-specialised persistent assembly (producer and consumer wavefronts, two LDS slots)
+    writes the same bits as the one-range-per-workgroup kernel: stiffness and mass, every layout."""
+    _say(f'wave-specialised assembly {kind} threads={threads} producers={producers} max_inc={max_inc}')
+    rng = np.random.default_rng(23)
+    c, t, mats = _perturbed_large(kind, rng)
+    s = FESystem(kind, c, t, mats, plane_mode=_hip.FE_PLANE_STRAIN if kind == 'tri_elastic' else 0)
+    s.set_schedule(max_inc, threads, producers)
+    g = s.graph
+    # more ranges than the grid has workgroups (one of 1024 threads or two of 512 per CU on 256 CUs):
+    # the workgroups walk several ranges
+    assert g.producers == producers and g.n_blk > 256 * (1024 // threads), g.n_blk
+    whats = ['stiffness'] + ([] if kind == 'tri_magnetic' else ['mass'])
+    layouts = [_hip.FE_LAYOUT_CSR, _hip.FE_LAYOUT_BSR, _hip.FE_LAYOUT_PLANES]
+    got = {(w, l): (s.stiffness(l) if w == 'stiffness' else s.mass(l)).cpu().numpy()
+           for w in whats for l in layouts}
+    g.producers = 0
+    saved = g.welem_conn
+    g.welem_conn = None
+    try:
+        for w in whats:
+            for l in layouts:
+                ref = (s.stiffness(l) if w == 'stiffness' else s.mass(l)).cpu().numpy()
+                assert np.array_equal(got[(w, l)], ref), (w, l)
+    finally:
+        g.welem_conn = saved
+        g.producers = producers
+    # a schedule that does not fit the producers' registers is refused, not run another way
+    s.set_schedule(max_inc, threads, producers)
+    g.max_elem = 64 * producers + 1
+    with pytest.raises(ValueError, match='wave-specialised'):
+        s.stiffness(_hip.FE_LAYOUT_PLANES)
+
+
+def test_wave_specialised_assembly_full_size(ctx):
+    """At 64^3 (823,875 DOF, 12,481 ranges of at most 512 incidences) the wave-specialised kernel
+    writes the same bits as the one-range-per-workgroup kernel."""
+    _say('wave-specialised assembly at 64^3')
+    pr = meshgen.cube_elasticity(n=64)
+    s = FESystem(pr.kind, pr.coords, pr.conn, pr.materials)
+    s.set_schedule(512, 1024, 8)
+    a = s.stiffness(_hip.FE_LAYOUT_PLANES)
+    g = s.graph
+    g.producers = 0
+    saved = g.welem_conn
+    g.welem_conn = None
+    try:
+        b = s.stiffness(_hip.FE_LAYOUT_PLANES)
+    finally:
+        g.welem_conn = saved
+        g.producers = 8
+    assert torch.equal(a, b)
+
+
 def test_autotuner_picks_a_valid_schedule(ctx):
     _say('assembly autotuner')
     c, t = meshgen.kuhn_tets(8)

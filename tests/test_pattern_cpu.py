@@ -134,6 +134,51 @@ def test_element_schedule(name, kind, coords, conn, max_inc):
     assert np.all(conn[e, a] == i) and np.all(conn[e, b] == pat.bcol[owner])
 
 
+@pytest.mark.parametrize('name,kind,coords,conn', _cases(), ids=[c[0] for c in _cases()])
+@pytest.mark.parametrize('max_inc', [64, 256])
+def test_node_lists_of_the_schedule(name, kind, coords, conn, max_inc):
+    """fe_schedule_nodes (the wave-specialised assembly's node lists): each workgroup's list is the
+    ascending set of its elements' nodes, every element vertex resolves through its local position
+    to its own node, positions past the element's vertex count are 0, and the header words 10 and
+    11 hold each list's slice."""
+    from finite_elements_amd.device import Context, DeviceGraph
+    pat = Pattern(conn, DOF[kind], coords.shape[0])
+    cap = max(max_inc, int(np.diff(pat.inc_ptr).max()))
+    blk, biggest = Pattern.schedule_for(pat.inc_ptr, cap)
+    g = DeviceGraph.__new__(DeviceGraph)  # host half only: the arrays are built before any upload
+    g.pattern, g.n_nodes = pat, pat.n_nodes
+    g.ctx = type('HostCtx', (), {'tensor': staticmethod(lambda a: a)})()
+    g._conn = np.ascontiguousarray(conn.astype(np.int32))
+    g.set_schedule(blk, biggest, 1024, 8)
+    npe = conn.shape[1]
+    hdr = g.wg_hdr.reshape(-1, 12)
+    wptr = g._welem_ptr_host
+    local = g.welem_local.view(np.uint16).reshape(-1, 4)
+    assert g.max_wnode == int(hdr[:, 11].max())
+    for w in range(blk.size - 1):
+        n0, nn = hdr[w, 10], hdr[w, 11]
+        nodes = g.wnode[n0:n0 + nn]
+        elems = g._welem_host[wptr[w]:wptr[w + 1]]
+        assert np.array_equal(nodes, np.unique(conn[elems]))
+        loc = local[wptr[w]:wptr[w + 1]].astype(np.int64)
+        assert np.array_equal(nodes[loc[:, :npe]], conn[elems])
+        assert np.all(loc[:, npe:] == 0)
+    # fe_schedule_codes: every contribution's resolved code names the element and the two local
+    # vertices that the block's row and column nodes are, and brow is the row's place in its range
+    code = g.wcode.view(np.uint16).astype(np.int64)
+    brow = g.brow.view(np.uint16).astype(np.int64)
+    N = pat.n_nodes
+    rows = np.repeat(np.arange(N), np.diff(pat.bptr))
+    w_of_node = np.repeat(np.arange(blk.size - 1), np.diff(blk))
+    assert np.array_equal(brow & 0x7fff, rows - blk[w_of_node[rows]])
+    assert np.array_equal(brow >> 15, (pat.bcol == rows).astype(np.int64))
+    owner = np.repeat(np.arange(pat.nnzb), np.diff(pat.bseg_ptr))
+    i = rows[owner]
+    e = g._welem_host[wptr[w_of_node[i]] + (code >> 4)]
+    a, b = (code >> 2) & 3, code & 3
+    assert np.all(conn[e, a] == i) and np.all(conn[e, b] == pat.bcol[owner])
+
+
 def test_schedule_covers_nodes():
     c, t = meshgen.kuhn_tets(6)
     pat = Pattern(t, 3, c.shape[0])
