@@ -3,7 +3,8 @@
 //
 // Variants share one sparsity pattern (bptr/bcol); each has its own K values (and optionally its own
 // M values and block-Jacobi preconditioner).  Per variant and iteration (Knyazev's LOBPCG):
-//   K1 k_lob_resid   R = K X - M X diag(lambda), W = T R, column norms of R and M X (partials)
+//   K1 k_lob_resid   R = K X - M X diag(lambda), W = T R, column norms of R and M X (partials);
+//                    with the two-level preconditioner k_lob_resid_agg, which also forms P^T R
 //   K2 k_lob_spmm    K W and M W in one pass over the shared pattern
 //   K3 k_lob_gram    partial Gram matrices S^T K S, S^T M S of the basis S = [X W P]
 //   K4 k_lob_gsum    fixed-order sum of the partials
@@ -129,57 +130,104 @@ __global__ __launch_bounds__(kThreads) void k_lob_resid(LobParams p, const doubl
 }
 
 // --------------------------------------------------------------------------------------------------
-// K1b (two-level preconditioner): W += P A_c^-1 P^T R, for every column of the block.  R is
-// recomputed from K X, M X and lambda; P rows of constrained DOFs are zero, so W stays zero there.
+// K1b (two-level preconditioner): W += P A_c^-1 P^T R, for every column of the block.  P rows of
+// constrained DOFs are zero, so W stays zero there.
 // --------------------------------------------------------------------------------------------------
-// r_c = P^T R: one workgroup per (aggregate, variant); thread (group g, column j) sums over the
-// aggregate's nodes g, g + npg, ..., then the groups are added in a fixed order.
+// K1 and K1b's restriction in one pass (two-level preconditioner): chunk b of the residual partials
+// takes aggregates [b n_agg / chunks, (b+1) n_agg / chunks) and walks their nodes (every node lies in
+// one aggregate), forming R, W = T R and the column norms as k_lob_resid does and r_c = P^T R (thread
+// (group g, column j) sums an aggregate's nodes g, g + npg, ..., then the groups are added in a
+// fixed order), so K X and M X are read once per iteration.
 template <int D, int NC>
-__global__ __launch_bounds__(kThreads) void k_lob_crestrict(LobParams p, fe_coarse C, int64_t ld,
-                                                            const LobState* st, const double* blk,
+__global__ __launch_bounds__(kThreads) void k_lob_resid_agg(LobParams p, fe_coarse C, int64_t ld,
+                                                            const double* __restrict__ T, int64_t t_stride,
+                                                            const uint8_t* __restrict__ fixed,
+                                                            const LobState* st, double* blk,
+                                                            double* __restrict__ rpart,
                                                             double* __restrict__ rc) {
   const int v = blockIdx.y;
-  const int64_t agg = blockIdx.x;
   __shared__ int s_skip, s_cur;
   __shared__ double s_lam[kMaxBlock];
+  __shared__ int s_lock[kMaxBlock];
+  __shared__ double s_rr[kThreads], s_bb[kThreads];
   __shared__ double s_acc[kThreads][NC];
   if (threadIdx.x == 0) {
     s_skip = st[v].done;
     s_cur = st[v].cur;
   }
-  if (threadIdx.x < p.m) s_lam[threadIdx.x] = st[v].lambda[threadIdx.x];
+  if (threadIdx.x < p.m) {
+    s_lam[threadIdx.x] = st[v].lambda[threadIdx.x];
+    s_lock[threadIdx.x] = !st[v].init && threadIdx.x < p.n_wanted &&
+                          st[v].resid[threadIdx.x] <= 0.1 * p.tol;
+  }
   __syncthreads();
   if (s_skip) return;
-  const int m = p.m, npg = kThreads / m;
+  const int m = p.m;
+  const int npg = kThreads / m;
   const int g = threadIdx.x / m, j = threadIdx.x % m;
-  const double* KX = lob_block(const_cast<double*>(blk), p, v, B_KX + s_cur);
-  const double* MX = lob_block(const_cast<double*>(blk), p, v, B_MX + s_cur);
-  double acc[NC];
+  const double* KX = lob_block(blk, p, v, B_KX + s_cur);
+  const double* MX = lob_block(blk, p, v, B_MX + s_cur);
+  double* W = lob_block(blk, p, v, B_W);
+  const double* Tv = T ? T + (int64_t)v * t_stride : nullptr;
+  const int64_t alo = C.n_agg * blockIdx.x / gridDim.x, ahi = C.n_agg * (blockIdx.x + 1) / gridDim.x;
+  const double lam = s_lam[j];
+  double rr = 0.0, bb = 0.0;
+  for (int64_t agg = alo; agg < ahi; ++agg) {
+    double acc[NC];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) acc[c] = 0.0;
-  if (g < npg) {
-    const double lam = s_lam[j];
-    const int32_t a0 = C.agg_ptr[agg], a1 = C.agg_ptr[agg + 1];
-    for (int32_t k = a0 + g; k < a1; k += npg) {
-      const int64_t i = C.agg_nodes[k];
+    for (int c = 0; c < NC; ++c) acc[c] = 0.0;
+    if (g < npg) {
+      const int32_t a0 = C.agg_ptr[agg], a1 = C.agg_ptr[agg + 1];
+      for (int32_t k = a0 + g; k < a1; k += npg) {
+        const int64_t i = C.agg_nodes[k];
+        double R[D];
 #pragma unroll
-      for (int e = 0; e < D; ++e) {
-        const int64_t at = (i * D + e) * m + j;
-        const double r = KX[at] - lam * MX[at];
-        const double* Pr = C.P + (i * D + e) * NC;
+        for (int e = 0; e < D; ++e) {
+          const int64_t at = (i * D + e) * m + j;
+          const double mx = MX[at];
+          R[e] = KX[at] - lam * mx;
+          rr += R[e] * R[e];
+          bb += mx * mx;
+          const double* Pr = C.P + (i * D + e) * NC;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) acc[c] += Pr[c] * r;
+          for (int c = 0; c < NC; ++c) acc[c] += Pr[c] * R[e];
+        }
+#pragma unroll
+        for (int e = 0; e < D; ++e) {
+          double w = R[e];
+          if (Tv) {
+            w = 0.0;
+#pragma unroll
+            for (int f = 0; f < D; ++f) w += Tv[(i * D + e) * D + f] * R[f];
+          }
+          if ((fixed && fixed[i * D + e]) || s_lock[j]) w = 0.0;
+          W[(i * D + e) * m + j] = w;
+        }
       }
     }
-  }
 #pragma unroll
-  for (int c = 0; c < NC; ++c) s_acc[threadIdx.x][c] = acc[c];
+    for (int c = 0; c < NC; ++c) s_acc[threadIdx.x][c] = acc[c];
+    __syncthreads();
+    if (threadIdx.x < m * NC) {
+      const int jj = threadIdx.x % m, c = threadIdx.x / m;
+      double sum = 0.0;
+      for (int q = 0; q < npg; ++q) sum += s_acc[q * m + jj][c];
+      rc[((int64_t)v * ld + agg * NC + c) * m + jj] = sum;
+    }
+    __syncthreads();
+  }
+  s_rr[threadIdx.x] = rr;
+  s_bb[threadIdx.x] = bb;
   __syncthreads();
-  if (threadIdx.x < m * NC) {
-    const int jj = threadIdx.x % m, c = threadIdx.x / m;
-    double sum = 0.0;
-    for (int q = 0; q < npg; ++q) sum += s_acc[q * m + jj][c];
-    rc[((int64_t)v * ld + agg * NC + c) * m + jj] = sum;
+  if (threadIdx.x < m) {
+    double a = 0.0, b = 0.0;
+    for (int q = 0; q < npg; ++q) {
+      a += s_rr[q * m + threadIdx.x];
+      b += s_bb[q * m + threadIdx.x];
+    }
+    double* out = rpart + ((int64_t)v * gridDim.x + blockIdx.x) * 2 * m;
+    out[threadIdx.x] = a;
+    out[m + threadIdx.x] = b;
   }
 }
 
@@ -1305,11 +1353,11 @@ struct LobCoarse {
   int64_t ld;
 };
 
+// The two-level preconditioner's part of an iteration after the residual pass (k_lob_resid_agg,
+// which also formed r_c = P^T R): z_c = A_c^-1 r_c, then W += P z_c.
 template <int D, int NC>
 void launch_lob_coarse(hipStream_t st, const LobParams& p, const LobCoarse& cc, const LobWork& w) {
   const int nv = p.nvar;
-  hipLaunchKernelGGL((k_lob_crestrict<D, NC>), dim3((unsigned)cc.C->n_agg, nv), dim3(kThreads), 0, st, p,
-                     *cc.C, cc.ld, w.st, w.blk, w.rc);
   hipLaunchKernelGGL(k_lob_cgemm, dim3((unsigned)(cc.ld / kCgRows), nv), dim3(kThreads), 0, st, p, cc.ld,
                      w.st, cc.Ainv, cc.ainv_stride, w.rc, w.zc);
   hipLaunchKernelGGL((k_lob_cprolong<D, NC>), dim3((unsigned)ceil_div(p.n, kThreads / (p.m / 4)), nv), dim3(kThreads),
@@ -1392,9 +1440,15 @@ int run_lobpcg(fe_handle* h, const fe_bsr* K, const fe_bsr* M, const double* T,
         spmm(1);
         spmm(2);
       }
-      hipLaunchKernelGGL((k_lob_resid<D>), gres, dim3(kThreads), 0, st, p, T, o->t_stride, fixed,
-                         w.st, w.blk, w.rpart);
-      if (cc.C) launch_lob_coarse<D, D == 3 ? 6 : D == 2 ? 3 : 1>(st, p, cc, w);
+      constexpr int NC = D == 3 ? 6 : D == 2 ? 3 : 1;
+      if (cc.C) {
+        hipLaunchKernelGGL((k_lob_resid_agg<D, NC>), gres, dim3(kThreads), 0, st, p, *cc.C, cc.ld, T,
+                           o->t_stride, fixed, w.st, w.blk, w.rpart, w.rc);
+        launch_lob_coarse<D, NC>(st, p, cc, w);
+      } else {
+        hipLaunchKernelGGL((k_lob_resid<D>), gres, dim3(kThreads), 0, st, p, T, o->t_stride, fixed,
+                           w.st, w.blk, w.rpart);
+      }
       spmm(0);
       int rc2 = rr_update(host_it % kRefresh == kRefresh - 1);
       if (rc2) return rc2;
