@@ -588,8 +588,13 @@ class FiniteElementAnalysis(FiniteElements):
         ``result_vector`` = n DOF values followed by one Lagrange multiplier per collapsed condition,
         in the reference's condition order, then one per continuity condition.  Continuity
         conditions u(node1) = value u(node2) are eliminated (``ContinuityFold``): the folded matrix
-        T^T K T is formed on the GPU and node1 is recovered from node2 after the solve.  method: 'gpu' (PCG on the MI355X).  precision: 'fp64' or
-        'mixed' (FP32 SpMV/preconditioner inside FP64 reliable updates).  precond: 'jacobi',
+        T^T K T is formed on the GPU and node1 is recovered from node2 after the solve.
+        method: 'gpu' (PCG on the MI355X).  tol: the true FP64 relative residual to reach; the
+        reference's direct ``spsolve`` differs from the result by at most about (condition number)
+        x tol, so a caller who needs its digits asks for a tolerance near the direct solve's own
+        residual (on the reference's example scripts 1e-12 to 1e-9 give 4e-16 to 4e-12 relative
+        error).  precision: 'fp64' or 'mixed' (FP32 SpMV/preconditioner inside FP64 reliable
+        updates).  precond: 'jacobi',
         'block_jacobi', 'two_level' (block Jacobi plus a coarse correction over aggregated rigid-body
         modes), 'learned' (the learned blocks in place of block Jacobi; needs ``learned``, a trained
         ``LearnedPreconditioner``) or 'learned_two_level' (the learned blocks plus the coarse
