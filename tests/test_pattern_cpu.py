@@ -190,6 +190,25 @@ def test_schedule_covers_nodes():
     assert np.array_equal(blk2, pat.blk) and biggest == pat.max_inc
     with pytest.raises(ValueError):
         Pattern.schedule_for(pat.inc_ptr, 5)
+    # fe_schedule_ranges against the greedy rule written out: a range takes consecutive nodes while
+    # it stays within max_inc incidences and max_nodes nodes
+    g = np.load(os.path.join(GOLDEN, 'gmsh_meshes.npz'))
+    for conn, n in ((t, c.shape[0]), (g['cube_tetrahedra'], g['cube_coords'].shape[0])):
+        p = Pattern(conn, 3, n)
+        counts = np.diff(p.inc_ptr).tolist()
+        for max_inc, max_nodes in ((64, 256), (96, 7), (300, 256)):
+            want, cur, nodes, big = [0], 0, 0, 0
+            for i, k in enumerate(counts):
+                if nodes and (cur + k > max_inc or nodes >= max_nodes):
+                    want.append(i)
+                    big = max(big, cur)
+                    cur = nodes = 0
+                cur += k
+                nodes += 1
+            want.append(len(counts))
+            big = max(big, cur)
+            got, got_big = Pattern.schedule_for(p.inc_ptr, max_inc, max_nodes)
+            assert np.array_equal(got, want) and got_big == big
 
 
 @pytest.mark.parametrize('case,expect', [
