@@ -787,7 +787,8 @@ __global__ __launch_bounds__(kThreads) void k_lob_gsum_w(LobParams p, const LobS
 // On return the diagonal of A holds the eigenvalues and the columns of V the eigenvectors.
 // Round-robin pairing: every step rotates r/2 disjoint planes at once (rows, then columns).
 // --------------------------------------------------------------------------------------------------
-__device__ void jacobi_eig(double* A, double* V, int r, int max_sweeps) {
+// Stops when the off-diagonal squared norm is at most off2 times the diagonal's.
+__device__ void jacobi_eig(double* A, double* V, int r, int max_sweeps, double off2 = 1e-30) {
   __shared__ double s_c[kMaxBasis / 2 + 1], s_s[kMaxBasis / 2 + 1];
   __shared__ int s_p[kMaxBasis / 2 + 1], s_q[kMaxBasis / 2 + 1];
   __shared__ double s_off[kRRThreads / 64], s_dia[kRRThreads / 64];
@@ -821,7 +822,7 @@ __device__ void jacobi_eig(double* A, double* V, int r, int max_sweeps) {
         o2 += s_off[w];
         d2 += s_dia[w];
       }
-      s_stop = !(o2 > 1e-30 * d2);
+      s_stop = !(o2 > off2 * d2);
     }
     __syncthreads();
     if (s_stop) break;
@@ -894,6 +895,9 @@ __device__ void jacobi_eig(double* A, double* V, int r, int max_sweeps) {
 }
 
 constexpr int kJacobiSweeps = 30;
+// The Rayleigh-Ritz problems stop at an off-diagonal norm of 1e-12 of the diagonal's: the Ritz values
+// then carry errors of order 1e-24, far below what the residual tolerance resolves.
+constexpr double kRROff2 = 1e-24;
 
 // Batched symmetric eigen-decomposition: one workgroup per matrix.
 __global__ __launch_bounds__(kRRThreads) void k_sym_eig(int r, const double* __restrict__ A,
@@ -984,7 +988,7 @@ __global__ __launch_bounds__(kRRThreads) void k_lob_rr(LobParams p, LobState* st
     sA[idx] = 0.5 * (GM[a * NS + b] + GM[b * NS + a]) * s_scale[a] * s_scale[b];
   }
   __syncthreads();
-  jacobi_eig(sA, sV, ns, kJacobiSweeps);
+  jacobi_eig(sA, sV, ns, kJacobiSweeps, kRROff2);
   if (tid == 0) {
     double mx = 0.0;
     for (int k = 0; k < ns; ++k) mx = fmax(mx, sA[k * ns + k]);
@@ -1041,7 +1045,7 @@ __global__ __launch_bounds__(kRRThreads) void k_lob_rr(LobParams p, LobState* st
     }
   }
   __syncthreads();
-  jacobi_eig(sA, sV, rk, kJacobiSweeps);
+  jacobi_eig(sA, sV, rk, kJacobiSweeps, kRROff2);
   // select the m smallest (largest) Ritz values in order: thread k ranks value k (ties by index)
   if (tid < rk) {
     const double t = sA[tid * rk + tid];
