@@ -516,25 +516,45 @@ struct Vec4<double> {
 };
 
 // One wavefront per row; 4-wide vector loads of the row (rows start 64-aligned: ld % 64 == 0).
+// A lane adds its 4-wide pieces c = 4 lane + 256 j in order of j; the loads of kGemvBatch of them are
+// issued together, the first batch before the state test (whose latency it overlaps).  Pieces past
+// the row's end load the row's last piece again (a cache hit) and are not added: unconditional loads
+// keep the batch in flight together.
+constexpr int kGemvBatch = 8;
 template <typename T>
 __global__ __launch_bounds__(256) void k_coarse_gemv(int64_t n, int64_t ld, const T* __restrict__ Ainv,
                                                      const T* __restrict__ x, T* __restrict__ y,
                                                      const int32_t* state, int mode) {
   using V = typename Vec4<T>::type;
   __shared__ int s_skip;
-  if (threadIdx.x == 0) s_skip = state && !coarse_active(state, mode);
-  __syncthreads();
-  if (s_skip) return;
   const int64_t row = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
-  if (row >= n) return;
-  const T* a = Ainv + row * ld;
-  const int64_t n4 = n & ~(int64_t)3;
+  const bool live = row < n;
+  const T* a = Ainv + (live ? row : 0) * ld;
+  const int64_t n4 = live ? n & ~(int64_t)3 : 0;
+  V av[kGemvBatch], xv[kGemvBatch];
+  auto load = [&](int64_t c0) {
+    if (n4 == 0) return;  // uniform over the wavefront
+#pragma unroll
+    for (int j = 0; j < kGemvBatch; ++j) {
+      const int64_t c = min(c0 + 256 * j, n4 - 4);
+      av[j] = *reinterpret_cast<const V*>(a + c);
+      xv[j] = V{x[c], x[c + 1], x[c + 2], x[c + 3]};
+    }
+  };
+  load(4 * lane);
+  if (threadIdx.x == 0) s_skip = state && !coarse_active(state, mode);
+  __syncthreads();
+  if (s_skip || !live) return;
   double acc = 0.0;
-  for (int64_t c = 4 * lane; c < n4; c += 256) {
-    const V av = *reinterpret_cast<const V*>(a + c);
-    acc += (double)av.x * (double)x[c] + (double)av.y * (double)x[c + 1] + (double)av.z * (double)x[c + 2] +
-           (double)av.w * (double)x[c + 3];
+  for (int64_t c0 = 4 * lane; c0 < n4; c0 += 256 * kGemvBatch) {
+    if (c0 != 4 * lane) load(c0);
+#pragma unroll
+    for (int j = 0; j < kGemvBatch; ++j) {
+      if (c0 + 256 * j < n4)
+        acc += (double)av[j].x * (double)xv[j].x + (double)av[j].y * (double)xv[j].y +
+               (double)av[j].z * (double)xv[j].z + (double)av[j].w * (double)xv[j].w;
+    }
   }
   for (int64_t c = n4 + lane; c < n; c += 64) acc += (double)a[c] * (double)x[c];
   acc = wave_sum(acc);
@@ -587,7 +607,9 @@ __device__ inline void rigid_moments(const double x[3], const double (&rv)[D], d
 
 // r_c = P^T r with P = Z(x^) T: one workgroup per aggregate sums the moments w, then r_c = T^T w.
 // NT threads per workgroup, sized to the aggregates (1,000-node aggregates of one large system,
-// ~100-node aggregates of a batch of small meshes).
+// ~100-node aggregates of a batch of small meshes).  A thread adds its nodes t, t + NT, ... in that
+// order; two of them are loaded at a time.  The aggregate's bounds and T columns are loaded before the
+// state test, whose latency they overlap.
 template <typename T, int D, int NC, int NT>
 __global__ __launch_bounds__(NT) void k_coarse_restrict_rm(fe_coarse C, const T* __restrict__ xh,
                                                                          const T* __restrict__ Tm,
@@ -596,10 +618,19 @@ __global__ __launch_bounds__(NT) void k_coarse_restrict_rm(fe_coarse C, const T*
                                                                          const int32_t* state,
                                                                          int state_stride, int mode) {
   __shared__ double red[NC][NT / 64];
-  __shared__ double s_w[NC];
   __shared__ int s_skip;
   const int g = blockIdx.x;
-  if (threadIdx.x == 0) {
+  const int tid = threadIdx.x;
+  const int32_t a0 = C.agg_ptr[g], a1 = C.agg_ptr[g + 1];
+  // T's column tid for the threads tid < NC of the first wavefront (the lanes above load column 0:
+  // unconditional loads stay in flight together)
+  T tcol[NC];
+  if (tid < 64) {
+    const int c = tid < NC ? tid : 0;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) tcol[k] = Tm[((int64_t)g * NC + k) * NC + c];
+  }
+  if (tid == 0) {
     const int64_t sg = C.n_seg ? C.seg_of_agg[g] : 0;  // the aggregate's segment
     s_skip = state && !coarse_active(state + sg * state_stride, mode);
   }
@@ -608,34 +639,61 @@ __global__ __launch_bounds__(NT) void k_coarse_restrict_rm(fe_coarse C, const T*
   double acc[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) acc[c] = 0.0;
-  for (int32_t t = C.agg_ptr[g] + threadIdx.x; t < C.agg_ptr[g + 1]; t += blockDim.x) {
-    const int64_t i = C.agg_nodes[t];
-    const int bits = C.fixed_bits[i];
-    const double x[3] = {(double)xh[i * 3], (double)xh[i * 3 + 1], (double)xh[i * 3 + 2]};
-    double rv[D];
+  for (int32_t t = a0 + tid; t < a1; t += 2 * NT) {
+    // the second node of the pair is the first again, with every DOF read as fixed, when t + NT is
+    // past the aggregate: its moments are then exact zeros, which leave the sums' bits unchanged, and
+    // the pair's loads are unconditional, in one block, in flight together
+    const bool two = t + NT < a1;
+    int32_t n0 = C.agg_nodes[t];
+    int32_t n1 = C.agg_nodes[two ? t + NT : t];
+    // both node ids in flight before either is used: the empty asm is their first use (the compiler
+    // would otherwise widen the first id, and wait for it, before issuing the second load)
+    asm volatile("" : "+v"(n0), "+v"(n1));
+    const int64_t i0 = n0, i1 = n1;
+    int braw[2];
+    T xr[2][3], rr[2][D];
 #pragma unroll
-    for (int e = 0; e < D; ++e) rv[e] = ((bits >> e) & 1) ? 0.0 : (double)r[i * D + e];
-    rigid_moments<D>(x, rv, acc);
+    for (int h = 0; h < 2; ++h) {
+      const int64_t i = h ? i1 : i0;
+      braw[h] = C.fixed_bits[i];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) xr[h][c] = xh[i * 3 + c];
+#pragma unroll
+      for (int e = 0; e < D; ++e) rr[h][e] = r[i * D + e];
+    }
+    __builtin_amdgcn_sched_barrier(0);  // both nodes' data in flight before any is used
+    double x[2][3], rv[2][D];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int bits = (h && !two) ? 7 : braw[h];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) x[h][c] = (double)xr[h][c];
+#pragma unroll
+      for (int e = 0; e < D; ++e) rv[h][e] = ((bits >> e) & 1) ? 0.0 : (double)rr[h][e];
+    }
+    rigid_moments<D>(x[0], rv[0], acc);
+    rigid_moments<D>(x[1], rv[1], acc);
   }
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int lane = tid & 63, wid = tid >> 6;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     acc[c] = wave_sum(acc[c]);
     if (lane == 0) red[c][wid] = acc[c];
   }
   __syncthreads();
-  if (threadIdx.x < NC) {
-    double v = 0.0;
-    for (int q = 0; q < NT / 64; ++q) v += red[threadIdx.x][q];
-    s_w[threadIdx.x] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x < NC) {
-    const int c = threadIdx.x;
+  if (tid < NC) {
+    // every one of the NC threads sums all NC moments (same order), then applies its column of T
+    double wsum[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      double v = 0.0;
+      for (int q = 0; q < NT / 64; ++q) v += red[k][q];
+      wsum[k] = v;
+    }
     double v = 0.0;
 #pragma unroll
-    for (int k = 0; k < NC; ++k) v += (double)Tm[((int64_t)g * NC + k) * NC + c] * s_w[k];
-    rc[(int64_t)g * NC + c] = (T)v;
+    for (int k = 0; k < NC; ++k) v += (double)tcol[k] * wsum[k];
+    rc[(int64_t)g * NC + tid] = (T)v;
   }
 }
 
@@ -648,14 +706,21 @@ __global__ __launch_bounds__(256) void k_coarse_prolong_rm(fe_coarse C, const T*
                                                            const int32_t* state, int state_stride,
                                                            int mode) {
   __shared__ int s_skip;
-  // one system: one state for the workgroup; a batch: the node's segment (ranges of a workgroup may
-  // belong to several)
+  // the node's aggregate, coordinates and fixed flags are loaded before the state test (whose
+  // latency they overlap); one system: one state for the workgroup; a batch: the node's segment
+  // (ranges of a workgroup may belong to several)
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const bool live = i < C.n_nodes;
+  const int64_t il = live ? i : 0;
+  const int64_t g = C.node_agg[il];
+  const int bits = C.fixed_bits[il];
+  const T x = xh[il * 3], yy = xh[il * 3 + 1], zz = xh[il * 3 + 2];
+  T uo[D];
+#pragma unroll
+  for (int e = 0; e < D; ++e) uo[e] = u[il * D + e];
   if (threadIdx.x == 0) s_skip = !C.n_seg && state && !coarse_active(state, mode);
   __syncthreads();
-  if (s_skip) return;
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= C.n_nodes) return;
-  const int64_t g = C.node_agg[i];
+  if (s_skip || !live) return;
   if (C.n_seg && state && !coarse_active(state + (int64_t)C.seg_of_agg[g] * state_stride, mode)) return;
   const T* Tg = Tm + g * NC * NC;
   const T* zg = z + g * NC;
@@ -667,7 +732,6 @@ __global__ __launch_bounds__(256) void k_coarse_prolong_rm(fe_coarse C, const T*
     for (int c = 0; c < NC; ++c) v += Tg[k * NC + c] * zg[c];
     y[k] = v;
   }
-  const T x = xh[i * 3], yy = xh[i * 3 + 1], zz = xh[i * 3 + 2];
   T du[D];
   if constexpr (D == 3) {
     du[0] = y[0] + zz * y[4] - yy * y[5];
@@ -679,10 +743,9 @@ __global__ __launch_bounds__(256) void k_coarse_prolong_rm(fe_coarse C, const T*
   } else {
     du[0] = y[0];
   }
-  const int bits = C.fixed_bits[i];
 #pragma unroll
   for (int e = 0; e < D; ++e)
-    if (!((bits >> e) & 1)) u[i * D + e] += du[e];
+    if (!((bits >> e) & 1)) u[i * D + e] = uo[e] + du[e];
 }
 
 template <typename T, int D, int NC>

@@ -76,18 +76,39 @@ inline int tile_rows_dof(int d) { return kSpmvPasses * (kBlock / lanes_for_dof(d
 __device__ inline int tile_id() { return (int)xcd_remap(blockIdx.x, gridDim.x); }
 
 __device__ inline bool seg_running(const PcgState* s) { return !s->done && !s->need_ru; }
+
+// The running test with both words loaded before either is tested (short-circuit evaluation would
+// make them two dependent round trips; the empty asm keeps the loads together).
+__device__ inline bool seg_running_now(const PcgState* s) {
+  const int32_t dn = s->done, nr = s->need_ru;
+  asm volatile("" ::"s"(dn), "s"(nr));
+  return (dn | nr) == 0;
+}
 __device__ inline bool seg_updating(const PcgState* s) { return !s->done && s->need_ru; }
 
 // Sum NV partial arrays (each ntiles long) over the tiles [t0, t1) of one segment; result in thread 0.
+// A thread adds its tiles t0 + tid + j kRed in order of j; the loads of kSumBatch consecutive j are
+// issued together (a 64^3 cube has 2,152 tiles: one round of loads instead of nine).
+constexpr int kSumBatch = 8;
 template <int NV>
 __device__ inline void seg_sum(const double* __restrict__ partials, int ntiles, int t0, int t1,
                                double (&v)[NV]) {
   __shared__ double red[NV][kRed / 64];
 #pragma unroll
   for (int k = 0; k < NV; ++k) v[k] = 0.0;
-  for (int t = t0 + (int)threadIdx.x; t < t1; t += kRed)
+  for (int t = t0 + (int)threadIdx.x; t < t1; t += kSumBatch * kRed) {
+    double a[kSumBatch][NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] += partials[(int64_t)k * ntiles + t];
+    for (int j = 0; j < kSumBatch; ++j)
+#pragma unroll
+      for (int k = 0; k < NV; ++k)
+        a[j][k] = t + j * kRed < t1 ? partials[(int64_t)k * ntiles + t + j * kRed] : 0.0;
+#pragma unroll
+    for (int j = 0; j < kSumBatch; ++j)
+      if (t + j * kRed < t1)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] += a[j][k];
+  }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
@@ -119,7 +140,10 @@ __device__ inline T precond_dof(const T* __restrict__ M, int64_t node, int d, in
 // --------------------------------------------------------------------------------------------------
 // iteration kernels
 // --------------------------------------------------------------------------------------------------
-template <typename T, int D, int PC>
+// SPEC (one segment, whose tiles all run until the solve ends): the tile's vectors and preconditioner
+// rows are loaded before the state test, so that its latency overlaps theirs; a launch that finds the
+// segment finished then drops them.  A batch (SPEC false) loads only for running segments.
+template <typename T, int D, int PC, bool SPEC>
 __global__ __launch_bounds__(kVecNodes * D) void k_cg_vec(const Tile* __restrict__ tiles,
                                                           double* __restrict__ x, T* __restrict__ p,
                                                           T* __restrict__ s, T* __restrict__ r,
@@ -128,46 +152,83 @@ __global__ __launch_bounds__(kVecNodes * D) void k_cg_vec(const Tile* __restrict
                                                           const PcgState* st) {
   constexpr int kPasses = tile_rows<lanes_for<D>()>() / kVecNodes;
   static_assert(kPasses * kVecNodes == tile_rows<lanes_for<D>()>(), "tile = whole passes");
+  constexpr int NM = PC == FE_PC_JACOBI ? 1 : D;  // preconditioner entries per DOF
   __shared__ T s_r[kPasses][kVecNodes * D];
   __shared__ int s_skip;
   __shared__ double s_ab[2];
   const int tid_ = tile_id();
   const Tile tl = tiles[tid_];
+  const int64_t kend = (int64_t)tl.row1 * D;
+  const int d = threadIdx.x % D;
+  // the tile's passes unrolled: every pass's loads are issued before the one barrier that publishes
+  // the new residuals for the block-Jacobi product
+  T uk[kPasses], pk[kPasses], wk[kPasses], sk[kPasses], rk[kPasses], mk[kPasses][NM];
+  double xk[kPasses];
+  auto load = [&]() {
+#pragma unroll
+    for (int ps = 0; ps < kPasses; ++ps) {
+      const int64_t k = ((int64_t)tl.row0 + ps * kVecNodes) * D + threadIdx.x;
+      uk[ps] = pk[ps] = wk[ps] = sk[ps] = rk[ps] = T(0);
+      xk[ps] = 0.0;
+#pragma unroll
+      for (int j = 0; j < NM; ++j) mk[ps][j] = T(0);
+      if (k < kend) {
+        uk[ps] = u[k];
+        pk[ps] = p[k];
+        wk[ps] = w[k];
+        sk[ps] = s[k];
+        rk[ps] = r[k];
+        xk[ps] = x[k];
+        const T* m = PC == FE_PC_JACOBI ? M + k : M + (k / D) * D * D + d * D;
+#pragma unroll
+        for (int j = 0; j < NM; ++j) mk[ps][j] = m[j];
+      }
+    }
+  };
+  if constexpr (SPEC) load();
   if (threadIdx.x == 0) {
     const PcgState* ss = st + tl.seg;
-    s_skip = !seg_running(ss);
-    s_ab[0] = ss->alpha;
-    s_ab[1] = ss->beta;
+    const double al = ss->alpha, be = ss->beta;
+    s_skip = !seg_running_now(ss);
+    s_ab[0] = al;
+    s_ab[1] = be;
   }
   __syncthreads();
   if (s_skip) return;
+  if constexpr (!SPEC) load();
   const double alpha = s_ab[0];
   const T alpha_t = (T)alpha, beta = (T)s_ab[1];
-  const int64_t kend = (int64_t)tl.row1 * D;
-  // the tile's passes unrolled: every pass's loads are issued before the one barrier that publishes
-  // the new residuals for the block-Jacobi product
-  T rk[kPasses];
 #pragma unroll
   for (int ps = 0; ps < kPasses; ++ps) {
     const int64_t k = ((int64_t)tl.row0 + ps * kVecNodes) * D + threadIdx.x;
-    rk[ps] = T(0);
+    T rn = T(0);
     if (k < kend) {
-      const T pk = u[k] + beta * p[k];
-      const T sk = w[k] + beta * s[k];
-      p[k] = pk;
-      s[k] = sk;
-      x[k] += alpha * (double)pk;
-      rk[ps] = r[k] - alpha_t * sk;
-      r[k] = rk[ps];
+      const T pn = uk[ps] + beta * pk[ps];
+      const T sn = wk[ps] + beta * sk[ps];
+      p[k] = pn;
+      s[k] = sn;
+      x[k] = xk[ps] + alpha * (double)pn;
+      rn = rk[ps] - alpha_t * sn;
+      r[k] = rn;
     }
-    s_r[ps][threadIdx.x] = rk[ps];
+    rk[ps] = rn;
+    s_r[ps][threadIdx.x] = rn;
   }
   __syncthreads();
-  const int d = threadIdx.x % D;
 #pragma unroll
   for (int ps = 0; ps < kPasses; ++ps) {
     const int64_t k = ((int64_t)tl.row0 + ps * kVecNodes) * D + threadIdx.x;
-    if (k < kend) u[k] = precond_dof<T, D, PC>(M, k / D, d, k, &s_r[ps][threadIdx.x - d], rk[ps]);
+    if (k >= kend) continue;
+    T un;
+    if (PC == FE_PC_JACOBI) {
+      un = mk[ps][0] * rk[ps];
+    } else {
+      const T* rn = &s_r[ps][threadIdx.x - d];
+      un = T(0);
+#pragma unroll
+      for (int j = 0; j < NM; ++j) un += mk[ps][j] * rn[j];
+    }
+    u[k] = un;
   }
 }
 
@@ -183,7 +244,20 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(const Tile* __restrict__ til
   __shared__ int s_skip;
   const int tid_ = tile_id();
   const Tile tl = tiles[tid_];
-  if (threadIdx.x == 0) s_skip = !force && !seg_running(st + tl.seg);
+  constexpr int kRowsPerPass = kBlock / G;
+  constexpr int kPasses = tile_rows<lanes_for<D>()>() / kRowsPerPass;
+  // VEC: every pass's row bounds are loaded before the state test (their latency overlaps it)
+  int32_t rs[VEC ? kPasses : 1], re[VEC ? kPasses : 1];
+  if constexpr (VEC) {
+#pragma unroll
+    for (int pass = 0; pass < kPasses; ++pass) {
+      const int64_t i = tl.row0 + pass * kRowsPerPass + threadIdx.x / G;
+      const bool ok = i < tl.row1;
+      rs[pass] = ok ? bptr[i] : 0;
+      re[pass] = ok ? bptr[i + 1] : 0;
+    }
+  }
+  if (threadIdx.x == 0) s_skip = !seg_running_now(st + tl.seg) && !force;
   __syncthreads();
   if (s_skip) return;
   const int lane = threadIdx.x % G;
@@ -200,15 +274,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(const Tile* __restrict__ til
   if constexpr (VEC) {
     // a tile is a fixed number of passes: unrolled, so that the loads of every pass are in flight
     // together
-    constexpr int kRowsPerPass = kBlock / G;
-    constexpr int kPasses = tile_rows<lanes_for<D>()>() / kRowsPerPass;
     static_assert(kPasses * kRowsPerPass == tile_rows<lanes_for<D>()>(), "tile = whole passes");
 #pragma unroll
     for (int pass = 0; pass < kPasses; ++pass) {
       const int64_t i = tl.row0 + pass * kRowsPerPass + threadIdx.x / G;
       const bool ok = i < tl.row1;
       T acc[D];
-      bsr_row_vec<T, D, G>(bptr, bcol, vals, nnzb, u, i, lane, acc, ok);
+      bsr_row_vec_range<T, D, G>(bcol, vals, nnzb, u, rs[pass], re[pass], lane, acc);
       if (ok && lane < D) row_out(i, acc);
     }
   } else {
@@ -230,29 +302,40 @@ __global__ __launch_bounds__(kRed) void k_cg_reduce(const int32_t* __restrict__ 
                                                     const double* __restrict__ partials,
                                                     PcgState* st, int last) {
   PcgState* s = st + blockIdx.x;
+  // the tile range, the running test and the fields the update reads are loaded together (only
+  // thread 0 writes the state, after the sums); the test is uniform
   const int t0 = seg_tiles[blockIdx.x], t1 = seg_tiles[blockIdx.x + 1];
-  if (!seg_running(s)) return;  // uniform: only thread 0 writes the state, after the sums
+  const int32_t done = s->done, need_ru = s->need_ru;
+  const double gamma0 = s->gamma, alpha0 = s->alpha, tol2bb = s->tol2bb;
+  const double ru_delta2 = s->ru_delta2, rr_ref = s->rr_ref;
+  const int32_t iter = s->iter + 1, max_iter = s->max_iter, mixed = s->mixed;
+  int32_t ru_pending = s->ru_pending;
+  // an empty asm that takes the values keeps their loads ahead of the exit (the compiler would
+  // otherwise sink each one to its first use, a memory round trip apiece)
+  asm volatile("" ::"s"(t0), "s"(t1), "s"(gamma0), "s"(alpha0), "s"(tol2bb), "s"(ru_delta2), "s"(rr_ref),
+               "s"(iter), "s"(max_iter), "s"(mixed), "s"(ru_pending), "s"(ntiles), "s"(partials));
+  if (done | need_ru) return;
   double v[3];
   seg_sum<3>(partials, ntiles, t0, t1, v);
   if (threadIdx.x != 0) return;
   const double g = v[0], dl = v[1];
-  s->iter += 1;
+  s->iter = iter;
   s->rr = v[2];
   if (!isfinite(g) || !isfinite(dl) || !isfinite(v[2]) || g < 0.0) {
     s->done = 3;
     return;
   }
-  s->gamma_prev = s->gamma;
+  s->gamma_prev = gamma0;
   s->gamma = g;
   // convergence claims and the iteration limit hand over to the true-residual check at the end of
   // the chunk; (mixed precision) the reliable-update threshold marks the update pending
-  if (v[2] <= s->tol2bb || s->iter >= s->max_iter) {
+  if (v[2] <= tol2bb || iter >= max_iter) {
     s->need_ru = 1;
     return;
   }
-  if (s->mixed && v[2] < s->ru_delta2 * s->rr_ref) s->ru_pending = 1;
-  const double beta = g / s->gamma_prev;
-  const double den = dl - beta * g / s->alpha;
+  if (mixed && v[2] < ru_delta2 * rr_ref) s->ru_pending = ru_pending = 1;
+  const double beta = g / gamma0;
+  const double den = dl - beta * g / alpha0;
   if (!(den > 0.0)) {
     // the recurrences lost positivity (rounding, most often in FP32): restart from the true
     // residual; a genuine breakdown shows up there as p.Ap <= 0
@@ -264,7 +347,7 @@ __global__ __launch_bounds__(kRed) void k_cg_reduce(const int32_t* __restrict__ 
   s->beta = beta;
   s->alpha = g / den;
   // the update replaces r, u, w, s and recomputes beta and alpha from gamma_prev, as at a hand-over
-  if (last && s->ru_pending) s->need_ru = 1;
+  if (last && ru_pending) s->need_ru = 1;
 }
 
 // --------------------------------------------------------------------------------------------------
@@ -723,7 +806,12 @@ int run_pcg(fe_handle* h, const Problem& pb, const Work& w, const fe_pcg_opts* o
   // one chunk of iterations + the reliable update + the status copy into pinned slot `slot`
   auto chunk_body = [&](int slot) -> int {
     for (int it = 0; it < chunk; ++it) {
-      hipLaunchKernelGGL((k_cg_vec<T, D, PC>), gt, bvec, 0, cs, w.tiles, pb.x, p, sv, r, u, wv, M, w.st);
+      if (nseg == 1)
+        hipLaunchKernelGGL((k_cg_vec<T, D, PC, true>), gt, bvec, 0, cs, w.tiles, pb.x, p, sv, r, u, wv, M,
+                           w.st);
+      else
+        hipLaunchKernelGGL((k_cg_vec<T, D, PC, false>), gt, bvec, 0, cs, w.tiles, pb.x, p, sv, r, u, wv,
+                           M, w.st);
       if (pb.C) {
         int rc3 = fe_coarse_correction(cs, pb.C, (int)(sizeof(T) == 4), pb.Ainv, r, u, w.rc, w.zc,
                                        &w.st[0].done, kStateStride, 0);

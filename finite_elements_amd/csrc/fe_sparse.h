@@ -91,16 +91,16 @@ struct Vec16<double> {
 // takes W consecutive blocks per step, one 16-byte load per value plane and one for the columns.
 // The summation order depends on the row alone (not on where the row sits in the arrays).
 // active false: an empty row (the lanes still take part in the group's shuffles).
+// The row's block range [s, e) as read from bptr (by the caller, e.g. ahead of a barrier).
 template <typename T, int D, int G>
-__device__ inline void bsr_row_vec(const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
-                                   const T* __restrict__ vals, int64_t nnzb, const T* __restrict__ x,
-                                   int64_t i, int lane, T (&acc)[D], bool active = true) {
+__device__ inline void bsr_row_vec_range(const int32_t* __restrict__ bcol, const T* __restrict__ vals,
+                                         int64_t nnzb, const T* __restrict__ x, int32_t s, int32_t e,
+                                         int lane, T (&acc)[D]) {
   using V = typename Vec16<T>::V;
   using I = typename Vec16<T>::I;
   constexpr int W = Vec16<T>::W;
 #pragma unroll
   for (int d = 0; d < D; ++d) acc[d] = T(0);
-  const int32_t s = active ? bptr[i] : 0, e = active ? bptr[i + 1] : 0;
   for (int32_t q = s + lane * W; q < e; q += G * W) {
     const I cols = *reinterpret_cast<const I*>(bcol + q);
     V a[D * D];
@@ -122,6 +122,14 @@ __device__ inline void bsr_row_vec(const int32_t* __restrict__ bptr, const int32
   for (int off = G / 2; off > 0; off >>= 1)
 #pragma unroll
     for (int d = 0; d < D; ++d) acc[d] += __shfl_xor(acc[d], off, G);
+}
+
+template <typename T, int D, int G>
+__device__ inline void bsr_row_vec(const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
+                                   const T* __restrict__ vals, int64_t nnzb, const T* __restrict__ x,
+                                   int64_t i, int lane, T (&acc)[D], bool active = true) {
+  const int32_t s = active ? bptr[i] : 0, e = active ? bptr[i + 1] : 0;
+  bsr_row_vec_range<T, D, G>(bcol, vals, nnzb, x, s, e, lane, acc);
 }
 
 // Whether A is on a padded pattern the vector row product may walk.
