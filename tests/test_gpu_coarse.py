@@ -85,6 +85,49 @@ def test_coarse_operator_and_inverse(ctx, case):
         dc.xhat, dc.xhat32, dc.T, dc.T32 = saved
 
 
+@pytest.mark.parametrize('case', ['cube20', 'magnetic80', 'cantilever160'])
+def test_coarse_correction_large_aggregates(ctx, case):
+    """Aggregates of 600+ nodes in one system (the restriction's 1,024-thread form, two nodes per
+    thread for some): the correction u += P A_c^-1 P^T r against the oracle and against the stored-P
+    form in both precisions, on 3, 1 and 2 DOF per node."""
+    _say(f'coarse correction, large aggregates: {case}')
+    # 8 aggregates of 1,000-1,331 nodes (two per thread for some); 9 of 729 (coarse dimension 9: a
+    # column past the last 4-wide piece); 5 of 1,312-1,353
+    pr, target = {'cube20': lambda: (meshgen.cube_elasticity(n=20), 729),
+                  'magnetic80': lambda: (meshgen.magnetostatic_2d(n=80), 900),
+                  'cantilever160': lambda: (meshgen.cantilever_2d(nx=160, ny=40), 900)}[case]()
+    s, KD, b, dc = _setup(pr, target)
+    n_nodes, n_agg = dc.host.node_agg.size, dc.host.n_agg
+    assert dc.n_seg == 0 and n_nodes // n_agg >= 600, (n_nodes, n_agg)
+    A = _host_matrix(KD)
+    ref, P = oc.coarse_operator(dc.host, A)
+    rng = np.random.default_rng(7)
+    r = rng.standard_normal(A.shape[0])
+    want = P @ np.linalg.solve(ref, P.T @ r)
+    inv64, inv32 = dc.operator(KD)
+    u = ctx.tensor(np.zeros(A.shape[0]))
+    dc.apply(inv64, ctx.tensor(r), u)
+    u32 = torch.zeros(A.shape[0], dtype=torch.float32, device=ctx.torch_device)
+    dc.apply(inv32, ctx.tensor(r).float(), u32)
+    scale = np.abs(want).max()
+    e64 = np.abs(u.cpu().numpy() - want).max() / scale
+    e32 = np.abs(u32.cpu().numpy() - want).max() / scale
+    _say(f'  {n_agg} aggregates of ~{n_nodes // n_agg} nodes: rel err fp64 {e64:.2e}, fp32 {e32:.2e}')
+    assert e64 <= 1e-10 and e32 <= 2e-3
+    saved = (dc.xhat, dc.xhat32, dc.T, dc.T32)
+    dc.xhat = dc.xhat32 = dc.T = dc.T32 = None
+    try:
+        p64, p32 = dc.operator(KD)
+        up = ctx.tensor(np.zeros(A.shape[0]))
+        dc.apply(p64, ctx.tensor(r), up)
+        up32 = torch.zeros(A.shape[0], dtype=torch.float32, device=ctx.torch_device)
+        dc.apply(p32, ctx.tensor(r).float(), up32)
+    finally:
+        dc.xhat, dc.xhat32, dc.T, dc.T32 = saved
+    assert np.abs(up.cpu().numpy() - u.cpu().numpy()).max() <= 1e-12 * scale
+    assert np.abs(up32.cpu().numpy() - u32.cpu().numpy()).max() <= 1e-5 * scale
+
+
 @pytest.mark.parametrize('precision', ['fp64', 'mixed'])
 def test_two_level_pcg_matches_direct_and_oracle_iterations(ctx, precision):
     _say(f'two-level pcg {precision}')
