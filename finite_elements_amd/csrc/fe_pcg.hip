@@ -77,6 +77,16 @@ __device__ inline int tile_id() { return (int)xcd_remap(blockIdx.x, gridDim.x); 
 
 __device__ inline bool seg_running(const PcgState* s) { return !s->done && !s->need_ru; }
 
+// This workgroup's tile: with one segment (n_rows >= 0, its node rows) the tiles are regular -- tile t
+// holds rows [t R, min((t + 1) R, n_rows)), the padding tiles none -- and are formed without a load.
+template <int D>
+__device__ inline Tile tile_of(const Tile* __restrict__ tiles, int t, int32_t n_rows) {
+  constexpr int R = tile_rows<lanes_for<D>()>();
+  if (n_rows < 0) return tiles[t];
+  const int64_t r0 = min((int64_t)t * R, (int64_t)n_rows);
+  return Tile{0, (int32_t)r0, (int32_t)min(r0 + R, (int64_t)n_rows), 0};
+}
+
 // The running test with both words loaded before either is tested (short-circuit evaluation would
 // make them two dependent round trips; the empty asm keeps the loads together).
 __device__ inline bool seg_running_now(const PcgState* s) {
@@ -149,15 +159,16 @@ __global__ __launch_bounds__(kVecNodes * D) void k_cg_vec(const Tile* __restrict
                                                           T* __restrict__ s, T* __restrict__ r,
                                                           T* __restrict__ u, const T* __restrict__ w,
                                                           const T* __restrict__ M,
-                                                          const PcgState* st) {
+                                                          const PcgState* st, int32_t n_rows) {
   constexpr int kPasses = tile_rows<lanes_for<D>()>() / kVecNodes;
   static_assert(kPasses * kVecNodes == tile_rows<lanes_for<D>()>(), "tile = whole passes");
   constexpr int NM = PC == FE_PC_JACOBI ? 1 : D;  // preconditioner entries per DOF
   __shared__ T s_r[kPasses][kVecNodes * D];
   __shared__ int s_skip;
   __shared__ double s_ab[2];
+  asm volatile("" ::"s"(n_rows), "s"(tiles), "s"(st));  // the arguments in one round of loads
   const int tid_ = tile_id();
-  const Tile tl = tiles[tid_];
+  const Tile tl = tile_of<D>(tiles, tid_, n_rows);
   const int64_t kend = (int64_t)tl.row1 * D;
   const int d = threadIdx.x % D;
   // the tile's passes unrolled: every pass's loads are issued before the one barrier that publishes
@@ -240,10 +251,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(const Tile* __restrict__ til
                                                     const T* __restrict__ vals,
                                                     const T* __restrict__ u, const T* __restrict__ r,
                                                     T* __restrict__ w, const PcgState* st,
-                                                    double* __restrict__ partials, int force) {
+                                                    double* __restrict__ partials, int force,
+                                                    int32_t n_rows) {
   __shared__ int s_skip;
+  asm volatile("" ::"s"(n_rows), "s"(tiles), "s"(st), "s"(force));  // the arguments in one round
   const int tid_ = tile_id();
-  const Tile tl = tiles[tid_];
+  const Tile tl = tile_of<D>(tiles, tid_, n_rows);
   constexpr int kRowsPerPass = kBlock / G;
   constexpr int kPasses = tile_rows<lanes_for<D>()>() / kRowsPerPass;
   // VEC: every pass's row bounds are loaded before the state test (their latency overlaps it)
@@ -757,13 +770,15 @@ int run_pcg(fe_handle* h, const Problem& pb, const Work& w, const fe_pcg_opts* o
   T* wv = (T*)w.w;
   T* p = (T*)w.p;
   T* sv = (T*)w.s;
+  // one segment: the kernels form their regular tiles from the row count instead of loading them
+  const int32_t one_rows = nseg == 1 ? (int32_t)A64->n_brows : -1;
   auto launch_spmv = [&](hipStream_t q, int force) {
     if (vec)
       hipLaunchKernelGGL((k_cg_spmv<T, D, GV, true>), gt, dim3(kBlock), 0, q, w.tiles, nt, nnzb_it,
-                         bptr_it, bcol_it, vit, u, r, wv, w.st, w.partials, force);
+                         bptr_it, bcol_it, vit, u, r, wv, w.st, w.partials, force, one_rows);
     else
       hipLaunchKernelGGL((k_cg_spmv<T, D, G, false>), gt, dim3(kBlock), 0, q, w.tiles, nt, nnzb_it,
-                         bptr_it, bcol_it, vit, u, r, wv, w.st, w.partials, force);
+                         bptr_it, bcol_it, vit, u, r, wv, w.st, w.partials, force, one_rows);
   };
   const dim3 bvec(kVecNodes * D);
   auto launch_ru = [&](hipStream_t q) -> int {
@@ -808,10 +823,10 @@ int run_pcg(fe_handle* h, const Problem& pb, const Work& w, const fe_pcg_opts* o
     for (int it = 0; it < chunk; ++it) {
       if (nseg == 1)
         hipLaunchKernelGGL((k_cg_vec<T, D, PC, true>), gt, bvec, 0, cs, w.tiles, pb.x, p, sv, r, u, wv, M,
-                           w.st);
+                           w.st, one_rows);
       else
         hipLaunchKernelGGL((k_cg_vec<T, D, PC, false>), gt, bvec, 0, cs, w.tiles, pb.x, p, sv, r, u, wv,
-                           M, w.st);
+                           M, w.st, one_rows);
       if (pb.C) {
         int rc3 = fe_coarse_correction(cs, pb.C, (int)(sizeof(T) == 4), pb.Ainv, r, u, w.rc, w.zc,
                                        &w.st[0].done, kStateStride, 0);
