@@ -120,7 +120,7 @@ def one_solve(sysm, method, fixed, g, f_t, tol=1e-8):
     method) and solve, with plain PCG as judge and fallback; returns (u, info)."""
     from finite_elements_amd.analysis import solve_reduced
     K = sysm.stiffness_bsr()
-    KD, b = sysm.prepare(K, fixed, g, f_t)
+    KD, b = sysm.prepare(K, fixed, g, f_t, coarse=method.get('coarse'))
     u, info = solve_reduced(sysm, KD, b, fixed, method['precond'], method['precision'], tol=tol,
                             max_iter=MAX_ITER, model=method.get('model'), coarse=method.get('coarse'))
     require_converged(info, tol, method['name'])
@@ -131,7 +131,7 @@ def profile_spmv(sysm, method, fixed, g, f_t, tol=1e-8):
     """The iteration's SpMV kernel timed by 20 back-to-back launches on the final state of one more
     solve of the method (outside every timed region); returns (ms per launch, launches)."""
     K = sysm.stiffness_bsr()
-    KD, b = sysm.prepare(K, fixed, g, f_t)
+    KD, b = sysm.prepare(K, fixed, g, f_t, coarse=method.get('coarse'))
     M = None
     if method.get('model') is not None:
         M = lambda: method['model'].preconditioner(sysm, KD, fixed)  # noqa: E731

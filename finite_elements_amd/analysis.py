@@ -627,7 +627,6 @@ class FiniteElementAnalysis(FiniteElements):
                 raise ValueError('the learned preconditioner does not take continuity conditions')
         fixed, g = sysm.dirichlet(fixed_dofs, fixed_vals)
         f_t = ctx.tensor(f)
-        KD, b = sysm.prepare(Kr, fixed, g, f_t if fold is None else ctx.tensor(fr))
         pc = precond
         model = None
         if precond in ('learned', 'learned_two_level'):
@@ -640,6 +639,7 @@ class FiniteElementAnalysis(FiniteElements):
         coarse = None
         if pc == 'two_level':
             coarse = sysm.coarse_space(fixed_dofs, graph=None if fold is None else Kr.graph)
+        KD, b = sysm.prepare(Kr, fixed, g, f_t if fold is None else ctx.tensor(fr), coarse=coarse)
         u, info = solve_reduced(sysm, KD, b, fixed, pc, precision, tol=tol, max_iter=max_iter,
                                 model=model, coarse=coarse)
         from . import device

@@ -668,6 +668,23 @@ class DeviceCoarse:
                   _vp(A32), _vp(scratch))
         return Ac, A32
 
+    def start_inverse(self, A64: BsrMatrix, ready: torch.cuda.Event):
+        """Queue A_c^-1 (and its FP32 copy) for the FP64 plane matrix A64 on the side stream of A64's
+        context, after the event ``ready``; returns (self, Ainv64, Ainv32, done event), or None when
+        A64 is not an FP64 plane matrix.  The coarse operator reads constrained rows and columns as
+        zero, so the assembled matrix gives the same A_c as its Dirichlet-eliminated copy: started
+        from it, the inverse runs beside the elimination."""
+        if A64.layout != _hip.FE_LAYOUT_PLANES or A64.precision != 0:
+            return None
+        side = A64.graph.ctx.side()
+        side.stream.wait_event(ready)
+        with torch.cuda.stream(side.stream):
+            inv64, inv32 = self.operator(A64, invert=True, ctx=side)
+        A64.vals.record_stream(side.stream)
+        done = torch.cuda.Event()
+        done.record(side.stream)
+        return (self, inv64, inv32, done)
+
     def apply(self, Ainv: torch.Tensor, r: torch.Tensor, u: torch.Tensor):
         """u += P Ainv P^T r (FP64 or FP32 by the dtype of r; a block-diagonal space takes its FP64
         blocks as Ainv at either precision)."""
@@ -702,6 +719,7 @@ class PcgSolver:
                  check_every: int = 16, ru_delta: float = 0.1, segments=None, graph: bool = False,
                  coarse: Optional['DeviceCoarse'] = None):
         self.ctx = A64.graph.ctx
+        started = getattr(A64, 'coarse_inverse', None)  # queued by FESystem.prepare
         if A64.layout != _hip.FE_LAYOUT_PLANES or A64.precision != 0:
             A64 = A64.convert(_hip.FE_LAYOUT_PLANES, torch.float64)
         self.A64 = A64
@@ -719,15 +737,19 @@ class PcgSolver:
                                  '(one block per segment), a single system the plain one')
             # the coarse operator and its inverse (part of the preconditioner set-up of every
             # solver) run on the side stream, beside the block preconditioner and the iteration
-            # matrix built below on the main stream
-            side = self.ctx.side()
-            ready = torch.cuda.Event()
-            ready.record(self.ctx.stream)
-            side.stream.wait_event(ready)
-            with torch.cuda.stream(side.stream):
-                self.Ainv64, self.Ainv32 = coarse.operator(A64, invert=True, ctx=side)
-            coarse_done = torch.cuda.Event()
-            coarse_done.record(side.stream)
+            # matrix built below on the main stream -- or were queued there already, from the
+            # assembled matrix, by FESystem.prepare
+            if started is not None and started[0] is coarse:
+                _, self.Ainv64, self.Ainv32, coarse_done = started
+            else:
+                side = self.ctx.side()
+                ready = torch.cuda.Event()
+                ready.record(self.ctx.stream)
+                side.stream.wait_event(ready)
+                with torch.cuda.stream(side.stream):
+                    self.Ainv64, self.Ainv32 = coarse.operator(A64, invert=True, ctx=side)
+                coarse_done = torch.cuda.Event()
+                coarse_done.record(side.stream)
         # block preconditioner: given, computed by a callable (e.g. a learned model's inference), or
         # the blocks of A64
         if M64 is None:

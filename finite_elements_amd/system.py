@@ -139,11 +139,19 @@ class FESystem:
             g[fixed_dofs] = np.asarray(values, dtype=np.float64)
         return self.ctx.tensor(mask), self.ctx.tensor(g)
 
-    def prepare(self, K: BsrMatrix, fixed: torch.Tensor, g: torch.Tensor, f: torch.Tensor):
-        """Dirichlet elimination: returns (K_D, b) with K_D SPD on the free DOFs."""
+    def prepare(self, K: BsrMatrix, fixed: torch.Tensor, g: torch.Tensor, f: torch.Tensor,
+                coarse=None):
+        """Dirichlet elimination: returns (K_D, b) with K_D SPD on the free DOFs.  ``coarse``: the
+        coarse space (``coarse_space``) of a two-level solve of K_D; its operator and inverse are then
+        queued on the side stream from K itself, to run beside the elimination, and handed to the
+        solver built for K_D."""
+        ready = torch.cuda.Event()
+        ready.record(self.ctx.stream)  # K is assembled
         kg = device.spmv(K, g)
         b = device.dirichlet_rhs(self.ctx, fixed, g, f, kg)
         KD = device.apply_dirichlet(K, fixed)
+        if coarse is not None:
+            KD.coarse_inverse = coarse.start_inverse(K, ready)
         return KD, b
 
     def coarse_space(self, fixed_dofs, target: Optional[int] = None, node_offsets=None,

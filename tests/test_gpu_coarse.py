@@ -128,6 +128,33 @@ def test_coarse_correction_large_aggregates(ctx, case):
     assert np.abs(up32.cpu().numpy() - u32.cpu().numpy()).max() <= 1e-5 * scale
 
 
+def test_inverse_queued_from_the_assembled_matrix(ctx):
+    """FESystem.prepare(coarse=...) queues the coarse operator and inverse from the assembled K on the
+    side stream, beside the elimination: the inverse is bitwise the one the solver forms from K_D
+    (constrained rows and columns are read as zero either way), and so is the solve."""
+    _say('coarse inverse queued from K')
+    pr = meshgen.cube_elasticity(n=12)
+    s = FESystem(pr.kind, pr.coords, pr.conn, pr.materials)
+    fixed, g = s.dirichlet(pr.fixed_dofs, pr.fixed_values)
+    dc = s.coarse_space(pr.fixed_dofs, target=64)
+    f = s.ctx.tensor(pr.f)
+    K = s.stiffness_bsr()
+    KD1, b1 = s.prepare(K, fixed, g, f)
+    KD2, b2 = s.prepare(K, fixed, g, f, coarse=dc)
+    assert getattr(KD1, 'coarse_inverse', None) is None and KD2.coarse_inverse[0] is dc
+    sol1 = s.solver(KD1, 'two_level', 'mixed', coarse=dc)
+    sol2 = s.solver(KD2, 'two_level', 'mixed', coarse=dc)
+    assert sol2.Ainv64 is KD2.coarse_inverse[1]
+    u1, i1 = sol1.solve(b1, tol=1e-8)
+    u2, i2 = sol2.solve(b2, tol=1e-8)
+    torch.cuda.synchronize()
+    n, ld = dc.n_coarse, dc.ld  # the FP32 copy holds the n x n inverse (its padding is not written)
+    assert torch.equal(sol1.Ainv64, sol2.Ainv64)
+    assert torch.equal(sol1.Ainv32.view(ld, ld)[:n, :n], sol2.Ainv32.view(ld, ld)[:n, :n])
+    assert torch.equal(b1, b2) and torch.equal(KD1.vals, KD2.vals)
+    assert i1['status'] == 0 and i1['iterations'] == i2['iterations'] and torch.equal(u1, u2)
+
+
 @pytest.mark.parametrize('precision', ['fp64', 'mixed'])
 def test_two_level_pcg_matches_direct_and_oracle_iterations(ctx, precision):
     _say(f'two-level pcg {precision}')
