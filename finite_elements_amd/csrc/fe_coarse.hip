@@ -127,41 +127,71 @@ __global__ __launch_bounds__(kAcThreads) void k_coarse_ac_rm(fe_coarse C, int64_
   double acc[NC * NC];
 #pragma unroll
   for (int k = 0; k < NC * NC; ++k) acc[k] = 0.0;
-  for (int32_t t = C.pair_ptr[pr] + threadIdx.x; t < C.pair_ptr[pr + 1]; t += kAcThreads) {
-    const int64_t q = C.pair_q[t];
-    const int64_t i = C.pair_row[t], j = bcol[q];
-    const int bi = C.fixed_bits[i], bj = C.fixed_bits[j];
-    double a[D][D];
+  // A thread adds its blocks t, t + kAcThreads, ... in that order, two per pass with every load of
+  // the pair in flight together: ids, then the columns, values, row bits and row coordinates, then
+  // the column bits and coordinates.  Past the pair's end the second block is the first again with
+  // every entry read as constrained: its product is exact zeros, which leave the sums unchanged.
+  const int32_t t_end = C.pair_ptr[pr + 1];
+  for (int32_t t = C.pair_ptr[pr] + threadIdx.x; t < t_end; t += 2 * kAcThreads) {
+    const bool two = t + kAcThreads < t_end;
+    const int32_t tt[2] = {t, two ? t + kAcThreads : t};
+    int32_t q0 = C.pair_q[tt[0]], q1 = C.pair_q[tt[1]];
+    int32_t r0 = C.pair_row[tt[0]], r1 = C.pair_row[tt[1]];
+    asm volatile("" : "+v"(q0), "+v"(q1), "+v"(r0), "+v"(r1));  // first use: all four in flight
+    const int64_t q[2] = {q0, q1}, i[2] = {r0, r1};
+    int32_t c0 = bcol[q[0]], c1 = bcol[q[1]];
+    double av[2][D * D], xi[2][3];
+    int bi[2];
 #pragma unroll
-    for (int e = 0; e < D; ++e)
+    for (int h = 0; h < 2; ++h) {
 #pragma unroll
-      for (int f = 0; f < D; ++f)
-        a[e][f] = (((bi >> e) | (bj >> f)) & 1) ? 0.0 : vals[(int64_t)(e * D + f) * nnzb + q];
-    const double xi[3] = {C.xhat[i * 3], C.xhat[i * 3 + 1], C.xhat[i * 3 + 2]};
-    const double xj[3] = {C.xhat[j * 3], C.xhat[j * 3 + 1], C.xhat[j * 3 + 2]};
-    double Zi[D][NC], Zj[D][NC];
-    rigid_modes_at<D, NC>(xi, Zi);
-    rigid_modes_at<D, NC>(xj, Zj);
-    // tz = A_ij Z_j, then acc += Z_i^T tz
-    double tz[D][NC];
+      for (int k = 0; k < D * D; ++k) av[h][k] = vals[(int64_t)k * nnzb + q[h]];
+      bi[h] = C.fixed_bits[i[h]];
 #pragma unroll
-    for (int e = 0; e < D; ++e)
+      for (int c = 0; c < 3; ++c) xi[h][c] = C.xhat[i[h] * 3 + c];
+    }
+    asm volatile("" : "+v"(c0), "+v"(c1));  // the columns first; the values stay in flight
+    const int64_t j[2] = {c0, c1};
+    double xj[2][3];
+    int bj[2];
 #pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        double v = 0.0;
+    for (int h = 0; h < 2; ++h) {
+      bj[h] = C.fixed_bits[j[h]];
 #pragma unroll
-        for (int f = 0; f < D; ++f) v += a[e][f] * Zj[f][c];
-        tz[e][c] = v;
-      }
+      for (int c = 0; c < 3; ++c) xj[h][c] = C.xhat[j[h] * 3 + c];
+    }
 #pragma unroll
-    for (int x = 0; x < NC; ++x)
+    for (int h = 0; h < 2; ++h) {
+      const int brow = (h && !two) ? 7 : bi[h];
+      double a[D][D];
 #pragma unroll
-      for (int y = 0; y < NC; ++y) {
-        double v = 0.0;
+      for (int e = 0; e < D; ++e)
 #pragma unroll
-        for (int e = 0; e < D; ++e) v += Zi[e][x] * tz[e][y];
-        acc[x * NC + y] += v;
-      }
+        for (int f = 0; f < D; ++f) a[e][f] = (((brow >> e) | (bj[h] >> f)) & 1) ? 0.0 : av[h][e * D + f];
+      double Zi[D][NC], Zj[D][NC];
+      rigid_modes_at<D, NC>(xi[h], Zi);
+      rigid_modes_at<D, NC>(xj[h], Zj);
+      // tz = A_ij Z_j, then acc += Z_i^T tz
+      double tz[D][NC];
+#pragma unroll
+      for (int e = 0; e < D; ++e)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          double v = 0.0;
+#pragma unroll
+          for (int f = 0; f < D; ++f) v += a[e][f] * Zj[f][c];
+          tz[e][c] = v;
+        }
+#pragma unroll
+      for (int x = 0; x < NC; ++x)
+#pragma unroll
+        for (int y = 0; y < NC; ++y) {
+          double v = 0.0;
+#pragma unroll
+          for (int e = 0; e < D; ++e) v += Zi[e][x] * tz[e][y];
+          acc[x * NC + y] += v;
+        }
+    }
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
